@@ -1,0 +1,597 @@
+"""FP64 CPU restatement of pyGIMLi's ERT forward + Jacobian path.
+
+TEST INFRASTRUCTURE ONLY (see oracle/__init__.py): the checker the HIP path
+is compared against; never the product path.
+
+Each function cites the reference code it restates (paths relative to the
+pyGIMLi repository).  The default forward is the singularity-removal (SR)
+operator DCSRMultiElectrodeModelling that ERTModelling(sr=True) builds
+(pygimli/physics/ert/ertModelling.py:87-106).  CHOLMOD (the reference's
+direct solver, an external SuiteSparse library) is replaced by scipy's
+SuperLU factorisation, an FP64 direct solver as well.
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+PI = np.pi
+TOLERANCE = 1e-12          # core/src/gimli.h:206
+MAX_DOUBLE = np.finfo(np.float64).max
+
+
+# --------------------------------------------------------------------------- a2
+# Bessel functions: Abramowitz & Stegun polynomial approximations 9.8.1-9.8.8
+# with the coefficients the reference uses (core/src/numericbase.h:80-180).
+
+def bessel_i0(x):
+    x = np.asarray(x, np.float64)
+    ax = np.abs(x)
+    out = np.empty_like(ax)
+    s = ax < 3.75
+    y = (x[s] / 3.75) ** 2
+    out[s] = 1.0 + y * (3.5156229 + y * (3.0899424 + y * (1.2067492 + y * (
+        0.2659732 + y * (0.360768e-1 + y * 0.45813e-2)))))
+    a = ax[~s]
+    y = 3.75 / a
+    out[~s] = (np.exp(a) / np.sqrt(a)) * (0.39894228 + y * (0.1328592e-1 + y * (
+        0.225319e-2 + y * (-0.157565e-2 + y * (0.916281e-2 + y * (-0.2057706e-1 + y * (
+            0.2635537e-1 + y * (-0.1647633e-1 + y * 0.392377e-2))))))))
+    return out
+
+
+def bessel_i1(x):
+    x = np.asarray(x, np.float64)
+    ax = np.abs(x)
+    out = np.empty_like(ax)
+    s = ax < 3.75
+    y = (x[s] / 3.75) ** 2
+    out[s] = ax[s] * (0.5 + y * (0.87890594 + y * (0.51498869 + y * (0.15084934 + y * (
+        0.2658733e-1 + y * (0.301532e-2 + y * 0.32411e-3))))))
+    a = ax[~s]
+    y = 3.75 / a
+    r = 0.2282967e-1 + y * (-0.2895312e-1 + y * (0.1787654e-1 - y * 0.420059e-2))
+    r = 0.39894228 + y * (-0.3988024e-1 + y * (-0.362018e-2 + y * (
+        0.163801e-2 + y * (-0.1031555e-1 + y * r))))
+    out[~s] = r * (np.exp(a) / np.sqrt(a))
+    return np.where(x < 0.0, -out, out)
+
+
+def bessel_k0(x):
+    x = np.asarray(x, np.float64)
+    out = np.empty_like(x)
+    s = x <= 2.0
+    xs = x[s]
+    y = xs * xs / 4.0
+    out[s] = (-np.log(xs / 2.0) * bessel_i0(xs)) + (-0.57721566 + y * (0.42278420 + y * (
+        0.23069756 + y * (0.3488590e-1 + y * (0.262698e-2 + y * (0.10750e-3 + y * 0.74e-5))))))
+    xl = x[~s]
+    y = 2.0 / xl
+    out[~s] = (np.exp(-xl) / np.sqrt(xl)) * (1.25331414 + y * (-0.7832358e-1 + y * (
+        0.2189568e-1 + y * (-0.1062446e-1 + y * (0.587872e-2 + y * (
+            -0.251540e-2 + y * 0.53208e-3))))))
+    return out
+
+
+def bessel_k1(x):
+    x = np.asarray(x, np.float64)
+    out = np.empty_like(x)
+    s = x <= 2.0
+    xs = x[s]
+    y = xs * xs / 4.0
+    out[s] = (np.log(xs / 2.0) * bessel_i1(xs)) + (1.0 / xs) * (1.0 + y * (0.15443144 + y * (
+        -0.67278579 + y * (-0.18156897 + y * (-0.1919402e-1 + y * (
+            -0.110404e-2 + y * (-0.4686e-4)))))))
+    xl = x[~s]
+    y = 2.0 / xl
+    out[~s] = (np.exp(-xl) / np.sqrt(xl)) * (1.25331414 + y * (0.23498619 + y * (
+        -0.3655620e-1 + y * (0.1504268e-1 + y * (-0.780353e-2 + y * (
+            0.325614e-2 + y * (-0.68245e-3)))))))
+    return out
+
+
+# --------------------------------------------------------------------------- a1
+def gauss_legendre(x1, x2, n):
+    """Newton iteration on P_n (core/src/numericbase.cpp:102-156, eps 3e-6)."""
+    x = np.zeros(n)
+    w = np.zeros(n)
+    eps = 3.0e-6
+    m = (n + 1.0) / 2.0
+    xm = 0.5 * (x2 + x1)
+    xl = 0.5 * (x2 - x1)
+    i = 1
+    while i <= m:
+        z = np.cos(PI * (i - 0.25) / (n + 0.5))
+        z1 = z + 2.0 * eps
+        pp = 0.0
+        while abs(z - z1) > eps:
+            p1, p2 = 1.0, 0.0
+            for j in range(1, n + 1):
+                p3 = p2
+                p2 = p1
+                p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j
+            pp = n * (z * p1 - p2) / (z * z - 1.0)
+            z1 = z
+            z = z1 - p1 / pp
+        x[i - 1] = xm - xl * z
+        x[n - i] = xm + xl * z
+        w[i - 1] = 2.0 * xl / ((1.0 - z * z) * pp * pp)
+        w[n - i] = w[i - 1]
+        i += 1
+    return x, w
+
+
+def gauss_laguerre(n):
+    """Newton iteration on L_n, alpha = 0 (core/src/numericbase.cpp:51-100,
+    eps 3e-11, at most 20 iterations)."""
+    x = np.zeros(n)
+    w = np.zeros(n)
+    eps = 3.0e-11
+    z = 0.0
+    for i in range(1, n + 1):
+        if i == 1:
+            z = 3.0 / (1.0 + 2.4 * n)
+        elif i == 2:
+            z = z + 15.0 / (1.0 + 2.5 * n)
+        else:
+            ai = i - 2
+            z = z + (1.0 + 2.55 * ai) / (1.9 * ai) * (z - x[ai - 1])
+        p1 = p2 = pp = 0.0
+        for _ in range(20):
+            p1, p2 = 1.0, 0.0
+            for j in range(1, n + 1):
+                p3 = p2
+                p2 = p1
+                p1 = ((2.0 * j - 1 - z) * p2 - (j - 1) * p3) / j
+            pp = n * (p1 - p2) / z
+            z1 = z
+            z = z1 - p1 / pp
+            if abs(z - z1) <= eps:
+                break
+        x[i - 1] = z
+        w[i - 1] = -1.0 / (pp * n * p2)
+    return x, w
+
+
+def kwave_from_range(rmin, rmax, n_leg, n_lag):
+    """core/src/bert/bertMisc.cpp:89-106."""
+    k0 = 1.0 / (2.0 * rmin)
+    x, w = gauss_legendre(0.0, 1.0, n_leg)
+    k_leg = k0 * x * x
+    w_leg = 2.0 * k0 * x * w / PI
+    x, w = gauss_laguerre(n_lag)
+    k_lag = k0 * (x + 1.0)
+    w_lag = k0 * np.exp(x) * w / PI
+    return np.concatenate([k_leg, k_lag]), np.concatenate([w_leg, w_lag])
+
+
+def kwave_list(dim, sensors):
+    """initKWaveList (core/src/bert/bertMisc.cpp:36-86)."""
+    if dim == 3:
+        return np.array([0.0]), np.array([1.0])
+    s = np.asarray(sensors, np.float64)
+    d = np.sqrt(((s[:, None, :] - s[None, :, :]) ** 2).sum(-1))
+    iu = np.triu_indices(len(s), 1)
+    rmin = d[iu].min() / 2.0
+    rmax = d[iu].max() * 2.0
+    n_leg = max(int(np.floor(6.0 * np.log10(rmax / rmin))), 4)
+    return kwave_from_range(rmin, rmax, n_leg, 4)
+
+
+# --------------------------------------------------------------------------- a2
+def exact_dc(points, src, k, surfaceZ, fallback):
+    """exactDCSolution(v, source, k, surfaceZ, fallback)
+    (core/src/bert/bertMisc.cpp:183-228), vectorised over points."""
+    v = np.asarray(points, np.float64)
+    src = np.asarray(src, np.float64)
+    r = np.sqrt(((v - src) ** 2).sum(-1))
+    dim = 2 if k > 0 else 3
+    mir = src.copy()
+    mir[dim - 1] = 2.0 * surfaceZ - src[dim - 1]
+    rm = np.sqrt(((v - mir) ** 2).sum(-1))
+    small = r < TOLERANCE
+    rs = np.where(small, 1.0, r)
+    if surfaceZ == -MAX_DOUBLE or np.isnan(surfaceZ):
+        out = 1.0 / (4.0 * PI * rs) if k == 0.0 else bessel_k0(rs * k) / (2.0 * PI)
+    else:
+        if k == 0.0:
+            out = (1.0 / rs + 1.0 / np.where(rm < TOLERANCE, 1.0, rm)) / (4.0 * PI)
+        elif ((src - mir) ** 2).sum() < TOLERANCE:  # RVector3 == (pos.h:319-322)
+            out = bessel_k0(rs * k) / PI
+        else:
+            out = (bessel_k0(rs * k) + bessel_k0(rm * k)) / (2.0 * PI)
+    return np.where(small, fallback, out)
+
+
+def geometric_factors(sensors, abmn, dim):
+    """geometricFactors (core/src/bert/bertMisc.cpp:131-176): 2D sensors with
+    y != 0 are taken as (x, z) and rotated into the y = 0 plane; then the
+    3D half-space potential with mirror plane z = 0."""
+    s = np.array(sensors, np.float64)
+    if dim == 2:
+        sw = s[:, 1] != 0.0
+        s[sw, 2] = s[sw, 1]
+        s[sw, 1] = 0.0
+    a, b, m, n = [np.asarray(abmn[:, i]) for i in range(4)]
+
+    def G(i, j):
+        ok = (i > -1) & (j > -1)
+        out = np.zeros(len(i))
+        for t in np.nonzero(ok)[0]:
+            out[t] = exact_dc(s[j[t]][None], s[i[t]], 0.0, 0.0, 1.0)[0]
+        return out
+
+    return 1.0 / (G(a, m) - G(b, m) - G(a, n) + G(b, n))
+
+
+# --------------------------------------------------------------------------- mesh-side set-up
+class Setup:
+    """Mesh/data dependent state of DCMultiElectrodeModelling
+    (updateMeshDependency_ dcfemmodelling.cpp:711-773, searchElectrodes_
+    :790-1073)."""
+
+    def __init__(self, mesh, sensors):
+        self.mesh = mesh
+        dim = mesh.dim()
+        self.dim = dim
+        bm = mesh.boundaryMarkers()
+        bc = mesh.boundaryCenters()
+        self.neumannDomain = not np.any(np.isin(bm, (-2, -3, -4)))
+        self.topography = False
+        self.surfaceZ = -MAX_DOUBLE
+        neu = np.nonzero(bm == -1)[0]
+        if len(neu):
+            self.surfaceZ = bc[neu[0], dim - 1]
+            if np.any(bc[neu, dim - 1] != self.surfaceZ):
+                self.topography = True
+        if self.neumannDomain:
+            self.topography = True
+            if dim == 2:
+                self.neumannDomain = False
+        ePos = np.array(sensors, np.float64)
+        if dim == 2:
+            zvar = np.ptp(ePos[:, 2]) > 0 or np.abs(ePos[:, 2]).max() > 0
+            yconst = not (np.ptp(ePos[:, 1]) > 0) and np.abs(ePos[:, 1]).max() < 1e-8
+            if zvar and yconst:
+                ePos[:, [1, 2]] = ePos[:, [2, 1]]
+        cand = list(mesh.findNodesIdxByMarker(-99))
+        nodes = mesh.nodes
+        self.elecNode = np.empty(len(ePos), np.int64)
+        for i, p in enumerate(ePos):
+            hit = None
+            for j, nid in enumerate(cand):
+                if np.linalg.norm(p - nodes[nid]) < 0.01:
+                    hit = j
+                    break
+            if hit is None:
+                raise NotImplementedError("electrode not on a marker -99 node")
+            self.elecNode[i] = cand.pop(hit)
+        self.elecPos = nodes[self.elecNode]
+        self.sourceCenter = self.elecPos.sum(0) / len(self.elecPos)
+        # cells around each electrode node, nearest neighbour distance
+        self.nodeCells = []
+        self.minRadius = np.empty(len(self.elecNode))
+        for i, nid in enumerate(self.elecNode):
+            cs = np.nonzero((mesh.cells == nid).any(axis=1))[0]
+            self.nodeCells.append(cs)
+            nb = np.unique(mesh.cells[cs].ravel())
+            nb = nb[nb != nid]
+            self.minRadius[i] = np.sqrt(((nodes[nb] - nodes[nid]) ** 2).sum(-1)).min()
+
+    def geom_mean_rho(self, i, rho):
+        """ElectrodeShapeNode::geomMeanCellAttributes (electrode.cpp:102-127)."""
+        v = rho[self.nodeCells[i]]
+        return float(np.exp(np.log(v).sum() / len(v)))
+
+    def primary(self, i, k):
+        """exactDCSolution(mesh, elec, k, surfaceZ, setSingValue=true)
+        (bertMisc.cpp:230-257) + ElectrodeShapeNode::setSingValue with
+        scale 0 (electrode.cpp:154-189)."""
+        u = exact_dc(self.mesh.nodes, self.elecPos[i], k, self.surfaceZ, 0.0)
+        r = self.minRadius[i]
+        if k > 0.0:
+            u[self.elecNode[i]] = bessel_k0(np.array([r / 6.0 * k]))[0] / PI
+        else:
+            u[self.elecNode[i]] = 1.0 / (2.0 * PI * r / 2.0)
+        return u
+
+
+# --------------------------------------------------------------------------- a4/a5
+def element_matrices(mesh):
+    """P1 element stiffness K_c = vol grad(N_i).grad(N_j) (ElementMatrix::
+    ux2uy2uz2, elementmatrix.cpp:798-1122) and the mass factor: M_c =
+    vol/((d+1)(d+2)) (1 + delta_ij) (ElementMatrix::u2, :682-796)."""
+    dim = mesh.dim()
+    p = mesh.nodes[mesh.cells][:, :, :dim]
+    D = p[:, 1:, :] - p[:, :1, :]              # (C, d, d) rows = edges
+    Dinv = np.linalg.inv(D)                     # columns = grads of N_1..N_d
+    g = np.transpose(Dinv, (0, 2, 1))           # (C, d, d) row j = grad N_{j+1}
+    g0 = -g.sum(axis=1, keepdims=True)
+    grads = np.concatenate([g0, g], axis=1)     # (C, d+1, d)
+    vol = np.abs(np.linalg.det(D)) / (2.0 if dim == 2 else 6.0)
+    K = vol[:, None, None] * np.einsum("cid,cjd->cij", grads, grads)
+    nv = dim + 1
+    mfac = vol / (nv * (nv + 1))
+    M = mfac[:, None, None] * (np.ones((nv, nv)) + np.eye(nv))[None]
+    return K, M, vol, grads
+
+
+def boundary_geometry(mesh):
+    """Boundary measure, centre and unit normal (Boundary::norm)."""
+    dim = mesh.dim()
+    p = mesh.nodes[mesh.bounds]
+    c = p.mean(axis=1)
+    if dim == 2:
+        t = p[:, 1, :] - p[:, 0, :]
+        size = np.sqrt((t ** 2).sum(-1))
+        nrm = np.stack([t[:, 1], -t[:, 0], np.zeros(len(t))], 1) / size[:, None]
+    else:
+        cr = np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0])
+        a2 = np.sqrt((cr ** 2).sum(-1))
+        size = a2 / 2.0
+        nrm = cr / a2[:, None]
+    return size, c, nrm
+
+
+def mixed_bc_alpha(mesh, source, k):
+    """mixedBoundaryCondition (dcfemmodelling.cpp:430-506), MIRRORSOURCE
+    branch, mirror plane 0, fabs(r.n)."""
+    dim = 2 if k > 0 else 3
+    smir = np.array(source, np.float64)
+    smir[dim - 1] = -source[dim - 1]
+    _, c, nrm = boundary_geometry(mesh)
+    r = source - c
+    rm = smir - c
+    ra = np.sqrt((r ** 2).sum(-1))
+    rma = np.sqrt((rm ** 2).sum(-1))
+    rn = np.abs((r * nrm).sum(-1))
+    rmn = np.abs((rm * nrm).sum(-1))
+    if k == 0:
+        return (rma * rma * rn / ra + ra * ra * rmn / rma) / (rma * ra * (ra + rma))
+    k0a = bessel_k0(ra * k)
+    k0m = bessel_k0(rma * k)
+    out = k * (rn / ra * bessel_k1(ra * k) + rmn / rma * bessel_k1(rma * k)) / (k0a + k0m)
+    return np.where((np.abs(k0a) < TOLERANCE) | (np.abs(k0m) < TOLERANCE), 0.0, out)
+
+
+def sparsity_pattern(mesh):
+    """SparseMatrix::buildSparsityPattern (sparsematrix.h:966-1032): all node
+    pairs sharing a cell, sorted column indices per row."""
+    n = mesh.nodeCount()
+    c = mesh.cells.astype(np.int64)
+    nv = c.shape[1]
+    rows = np.repeat(c, nv, axis=1).ravel()
+    cols = np.tile(c, (1, nv)).ravel()
+    A = sp.csr_matrix((np.ones(len(rows)), (rows, cols)), shape=(n, n))
+    A.sum_duplicates()
+    A.sort_indices()
+    return A.indptr.astype(np.int32), A.indices.astype(np.int32)
+
+
+def assemble(mesh, rho, k, source, geo=None):
+    """System matrix of one wavenumber as DCSRMultiElectrodeModelling::
+    calculateK assembles it (dcfemmodelling.cpp:2179-2182):
+    dcfemDomainAssembleStiffnessMatrix (:163-242) + dcfemBoundaryAssemble-
+    StiffnessMatrix (:243-292).  Returns scipy CSR with the reference's
+    pattern (full symmetric storage)."""
+    if geo is None:
+        geo = element_matrices(mesh)
+    K, M, vol, _ = geo
+    n = mesh.nodeCount()
+    rho = np.asarray(rho, np.float64)
+    use = np.abs(rho) > TOLERANCE
+    sig = np.where(use, 1.0 / np.where(use, rho, 1.0), 0.0)
+    Ae = (K + (k * k) * M) * sig[:, None, None] if k > 0.0 else K * sig[:, None, None]
+    c = mesh.cells.astype(np.int64)
+    nv = c.shape[1]
+    rows = [np.repeat(c, nv, axis=1).ravel()]
+    cols = [np.tile(c, (1, nv)).ravel()]
+    vals = [Ae.ravel()]
+    # mixed boundaries
+    bm = mesh.boundaryMarkers()
+    mix = np.nonzero(bm == -2)[0]
+    if len(mix):
+        size, _, _ = boundary_geometry(mesh)
+        alpha = mixed_bc_alpha(mesh, source, k)
+        lc = mesh.boundaryLeftCell()
+        if np.any(lc[mix] < 0):
+            raise RuntimeError("no cell found for a mixed boundary")
+        d = mesh.dim()
+        mb = size[mix] / (d * (d + 1))  # edge: len/6, triangle: area/12
+        coef = alpha[mix] / rho[lc[mix]]
+        Be = (mb * coef)[:, None, None] * (np.ones((d, d)) + np.eye(d))[None]
+        bb = mesh.bounds[mix].astype(np.int64)
+        rows.append(np.repeat(bb, d, axis=1).ravel())
+        cols.append(np.tile(bb, (1, d)).ravel())
+        vals.append(Be.ravel())
+    # one COO -> CSR conversion keeps explicit zeros, i.e. the full pattern
+    A = sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(n, n)).tocsr()
+    if np.any(np.isin(bm, (-3, -4))):
+        raise NotImplementedError("Dirichlet boundaries are not part of this path")
+    A.sum_duplicates()
+    A.sort_indices()
+    diag = A.diagonal()
+    if np.any(diag < TOLERANCE):
+        raise RuntimeError("zero diagonal (all cells around a node have rho = 0)")
+    return A.tocsr()
+
+
+def stiffness_matrix(mesh, a):
+    """RSparseMatrix::fillStiffnessMatrix(mesh, a) (sparsematrix.h:1043-1053)."""
+    K, _, _, _ = element_matrices(mesh)
+    return _scatter(mesh, K * np.asarray(a, np.float64)[:, None, None])
+
+
+def mass_matrix(mesh, a):
+    """RSparseMatrix::fillMassMatrix(mesh, a) (sparsematrix.h:1059-1069)."""
+    _, M, _, _ = element_matrices(mesh)
+    return _scatter(mesh, M * np.asarray(a, np.float64)[:, None, None])
+
+
+def _scatter(mesh, Ae):
+    n = mesh.nodeCount()
+    c = mesh.cells.astype(np.int64)
+    nv = c.shape[1]
+    rows = np.repeat(c, nv, axis=1).ravel()
+    cols = np.tile(c, (1, nv)).ravel()
+    A = sp.coo_matrix((Ae.ravel(), (rows, cols)), shape=(n, n)).tocsr()
+    A.sum_duplicates()
+    A.sort_indices()
+    return A
+
+
+# --------------------------------------------------------------------------- a13
+def map_model(mesh, model):
+    """mapERTModel (dcfemmodelling.cpp:1211-1218) + ModellingBase::
+    createMappedModel (modellingbase.cpp:401-470) for meshes whose cells all
+    carry a parameter marker >= 0."""
+    model = np.asarray(model, np.float64)
+    if len(model) == mesh.cellCount():
+        return model.copy()
+    cm = mesh.cellMarkers()
+    if np.any(cm < 0):
+        raise NotImplementedError("background (marker -1) prolongation is not part of this path")
+    if cm.max() >= len(model):
+        raise ValueError("marker >= model.size()")
+    return model[cm]
+
+
+# --------------------------------------------------------------------------- a7/a9/a10
+class Forward:
+    """DCSRMultiElectrodeModelling restated: response() and createJacobian()."""
+
+    def __init__(self, mesh, data, kvalues=None, weights=None):
+        self.mesh = mesh
+        self.data = data
+        self.setup = Setup(mesh, data.sensorPositions())
+        if self.setup.topography:
+            raise NotImplementedError("topography / Neumann-only domains are not part of this path")
+        if kvalues is None:
+            kvalues, weights = kwave_list(mesh.dim(), self.setup.elecPos)
+        self.k = np.asarray(kvalues, np.float64)
+        self.w = np.asarray(weights, np.float64)
+        self.geo = element_matrices(mesh)
+        self.nE = len(self.setup.elecNode)
+        self.subpots = None
+        self.prim = None
+
+    def primaries(self):
+        """checkPrimpotentials_ (dcfemmodelling.cpp:2086-2110): rows
+        source + nE * kIdx."""
+        if self.prim is None:
+            P = np.empty((len(self.k) * self.nE, self.mesh.nodeCount()))
+            for ki, k in enumerate(self.k):
+                for i in range(self.nE):
+                    P[i + ki * self.nE] = self.setup.primary(i, k)
+            self.prim = P
+        return self.prim
+
+    def solve_subpotentials(self, rho):
+        """DCSRMultiElectrodeModelling::calculateK (dcfemmodelling.cpp:2152-2296)
+        for every wavenumber."""
+        rho = np.asarray(rho, np.float64)
+        P = self.primaries()
+        ones = np.ones_like(rho)
+        U = np.empty_like(P)
+        src = self.setup.sourceCenter
+        for ki, k in enumerate(self.k):
+            S = assemble(self.mesh, rho, k, src, self.geo)
+            S1 = assemble(self.mesh, ones, k, src, self.geo)
+            lu = spla.splu(S.tocsc())
+            for i in range(self.nE):
+                rs = self.setup.geom_mean_rho(i, rho)
+                prim = P[i + ki * self.nE] * rs
+                rhs = S1 @ prim / rs - S @ prim
+                U[i + ki * self.nE] = lu.solve(rhs) + prim
+        return U
+
+    def potentials(self, U=None):
+        """Wavenumber sum (dcfemmodelling.cpp:1707-1729)."""
+        U = self.subpots if U is None else U
+        nK = len(self.k)
+        return sum(self.w[ki] * U[ki * self.nE:(ki + 1) * self.nE] for ki in range(nK))
+
+    def electrode_map(self, sol):
+        """DataMap::collect (datamap.cpp:57-98): map[source][electrode]."""
+        return sol[:, self.setup.elecNode]
+
+    def geometric_factors(self):
+        return geometric_factors(self.data.sensorPositions(), self.data.abmn(), self.mesh.dim())
+
+    def response(self, model):
+        """DCMultiElectrodeModelling::response (dcfemmodelling.cpp:1085-1197)."""
+        model = np.asarray(model, np.float64)
+        if model.min() < TOLERANCE:
+            raise ValueError("response for model with negative or zero resistivity is not defined")
+        kf = self.data["k"]
+        if not self.data.haveData("k") or np.min(np.abs(kf)) < TOLERANCE:
+            kf = self.geometric_factors()
+            self.data.set("k", kf)
+        rho = map_model(self.mesh, model)
+        self.subpots = self.solve_subpotentials(rho)
+        emap = self.electrode_map(self.potentials())
+        u = collect_data(emap, self.data.abmn(), False)
+        ur = collect_data(emap, self.data.abmn(), True)
+        resp = np.rint(u / 1e-10) * 1e-10 * kf
+        respRez = np.rint(ur / 1e-10) * 1e-10 * kf
+        return np.sqrt(np.abs(resp * respRez))
+
+    def jacobian(self, model):
+        """createJacobian_ (dcfemmodelling.cpp:1318-1408) with
+        createSensitivityCol / calc2 (bertJacobian.cpp:170-219, :367-521)."""
+        model = np.asarray(model, np.float64)
+        if self.subpots is None:
+            self.response(model)
+        J = sensitivity_calc2(self.mesh, self.geo, self.data.abmn(), self.subpots,
+                              self.k, self.w, self.data.sensorCount())
+        if len(model) == J.shape[1]:
+            J *= self.data["k"][:, None] / (model * model)[None, :]
+        return J
+
+
+def collect_data(emap, abmn, reciprocity):
+    """DataMap::data (datamap.cpp:159-231)."""
+    a, b, m, n = [abmn[:, i].copy() for i in range(4)]
+    if reciprocity:
+        a, m = m, a
+        b, n = n, b
+
+    def g(i, j):
+        ok = (i > -1) & (j > -1)
+        return np.where(ok, emap[np.maximum(i, 0), np.maximum(j, 0)], 0.0)
+
+    return (g(a, m) - g(a, n)) - (g(b, m) - g(b, n))
+
+
+def sensitivity_calc2(mesh, geo, abmn, U, kvals, weights, nE):
+    """CreateSensitivityColMT::calc2 (bertJacobian.cpp:170-219), vectorised
+    over cells: J[i][marker_c] += w_k (m-n)^T (K_c + k^2 M_c) (a-b) over the
+    cell's nodes, potentials row = electrode + nE * kIdx."""
+    K, M, _, _ = geo
+    cm = mesh.cellMarkers()
+    use = cm >= 0
+    nModel = int(cm.max()) + 1
+    cells = mesh.cells[use]
+    mk = cm[use]
+    K = K[use]
+    M = M[use]
+    D = len(abmn)
+    J = np.zeros((D, nModel))
+    nn = U.shape[1]
+    zero = np.zeros(nn)
+    for ki, k in enumerate(kvals):
+        S = M * (k * k) + K
+        Uk = U[ki * nE:(ki + 1) * nE]
+        for i in range(D):
+            a, b, m, n = abmn[i]
+            va = Uk[a] if a > -1 else zero
+            vb = Uk[b] if b > -1 else zero
+            vm = Uk[m] if m > -1 else zero
+            vn = Uk[n] if n > -1 else zero
+            dab = (va - vb)[cells]
+            dmn = (vm - vn)[cells]
+            val = np.einsum("ci,cij,cj->c", dmn, S, dab)
+            J[i] += weights[ki] * np.bincount(mk, weights=val, minlength=nModel)
+    return J

@@ -1,0 +1,168 @@
+"""Unstructured P1 mesh container (triangles in 2D, tetrahedra in 3D).
+
+Holds exactly the read-only arrays the ERT forward/Jacobian path consumes from
+a pyGIMLi ``Mesh`` (core/src/mesh.h): node positions and markers, cell
+connectivity and markers, boundary connectivity and markers, and each
+boundary's "left" cell (core/src/meshentities.h Boundary::leftCell, set by
+Mesh::createNeighborInfos).  Marker conventions follow the reference
+(core/src/gimli.h): boundary -1 homogeneous Neumann (surface), -2 mixed
+(Robin), -3 homogeneous Dirichlet; node -99 electrode, -1000 calibration.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+MARKER_BOUND_HOMOGEN_NEUMANN = -1
+MARKER_BOUND_MIXED = -2
+MARKER_BOUND_HOMOGEN_DIRICHLET = -3
+MARKER_BOUND_DIRICHLET = -4
+MARKER_NODE_ELECTRODE = -99
+MARKER_NODE_REFERENCEELECTRODE = -999
+MARKER_NODE_CALIBRATION = -1000
+
+
+class Mesh:
+    """Minimal pyGIMLi-compatible mesh for the ERT hot path."""
+
+    def __init__(self, dim, nodes, cells, cellMarkers=None, boundaries=None,
+                 boundaryMarkers=None, nodeMarkers=None):
+        self._dim = int(dim)
+        if self._dim not in (2, 3):
+            raise ValueError("only 2D triangle and 3D tetrahedron meshes are supported")
+        nodes = np.asarray(nodes, dtype=np.float64)
+        if nodes.ndim != 2 or nodes.shape[1] not in (2, 3):
+            raise ValueError("nodes must be (n, 2) or (n, 3)")
+        if nodes.shape[1] == 2:
+            nodes = np.hstack([nodes, np.zeros((nodes.shape[0], 1))])
+        self.nodes = np.ascontiguousarray(nodes)
+        self.cells = np.ascontiguousarray(np.asarray(cells, dtype=np.int32))
+        if self.cells.ndim != 2 or self.cells.shape[1] != self._dim + 1:
+            raise ValueError("cells must be (C, dim+1)")
+        if self.cells.size and (self.cells.min() < 0 or self.cells.max() >= len(self.nodes)):
+            raise ValueError("cell node index out of range")
+        nc = self.cells.shape[0]
+        self.cellMarkers_ = (np.zeros(nc, np.int32) if cellMarkers is None
+                             else np.ascontiguousarray(np.asarray(cellMarkers, np.int32)))
+        self.nodeMarkers_ = (np.zeros(len(self.nodes), np.int32) if nodeMarkers is None
+                             else np.ascontiguousarray(np.asarray(nodeMarkers, np.int32)))
+        if boundaries is None:
+            boundaries = np.zeros((0, self._dim), np.int32)
+            boundaryMarkers = np.zeros(0, np.int32)
+        self.bounds = np.ascontiguousarray(np.asarray(boundaries, np.int32)).reshape(-1, self._dim)
+        self.boundMarkers_ = np.ascontiguousarray(np.asarray(boundaryMarkers, np.int32))
+        self._leftCell = None
+        self._cellAttributes = np.ones(nc)
+
+    # -- pyGIMLi Mesh API subset --------------------------------------------
+    def dim(self):
+        return self._dim
+
+    def dimension(self):
+        return self._dim
+
+    def nodeCount(self):
+        return self.nodes.shape[0]
+
+    def cellCount(self):
+        return self.cells.shape[0]
+
+    def boundaryCount(self):
+        return self.bounds.shape[0]
+
+    def positions(self):
+        return self.nodes
+
+    def cellMarkers(self):
+        return self.cellMarkers_
+
+    def setCellMarkers(self, m):
+        m = np.asarray(m, np.int32)
+        if m.shape != (self.cellCount(),):
+            raise ValueError("marker size mismatch")
+        self.cellMarkers_ = np.ascontiguousarray(m)
+
+    def nodeMarkers(self):
+        return self.nodeMarkers_
+
+    def boundaryMarkers(self):
+        return self.boundMarkers_
+
+    def findNodesIdxByMarker(self, marker):
+        return np.nonzero(self.nodeMarkers_ == marker)[0]
+
+    def cellAttributes(self):
+        return self._cellAttributes
+
+    def setCellAttributes(self, a):
+        a = np.asarray(a, np.float64)
+        if a.ndim == 0:
+            a = np.full(self.cellCount(), float(a))
+        if a.shape != (self.cellCount(),):
+            raise ValueError("attribute size mismatch")
+        self._cellAttributes = a.copy()
+
+    def cellCenters(self):
+        return self.nodes[self.cells].mean(axis=1)
+
+    def boundaryCenters(self):
+        return self.nodes[self.bounds].mean(axis=1)
+
+    def cellSizes(self):
+        p = self.nodes[self.cells]
+        if self._dim == 2:
+            a = p[:, 1, :2] - p[:, 0, :2]
+            b = p[:, 2, :2] - p[:, 0, :2]
+            return 0.5 * np.abs(a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0])
+        a = p[:, 1] - p[:, 0]
+        b = p[:, 2] - p[:, 0]
+        c = p[:, 3] - p[:, 0]
+        return np.abs(np.einsum("ij,ij->i", a, np.cross(b, c))) / 6.0
+
+    def boundaryLeftCell(self):
+        """Cell adjacent to each boundary (Mesh::createNeighborInfos).
+
+        For an outer boundary the only adjacent cell; for an inner boundary
+        the lower-indexed of the two.  -1 where no cell shares the face.
+        """
+        if self._leftCell is None:
+            self._leftCell = _face_owner(self.cells, self.bounds, self.nodeCount())
+        return self._leftCell
+
+    def __repr__(self):
+        return (f"Mesh: Nodes: {self.nodeCount()} Cells: {self.cellCount()} "
+                f"Boundaries: {self.boundaryCount()} (dim={self._dim})")
+
+
+def _face_keys(faces, n):
+    f = np.sort(np.asarray(faces, np.int64), axis=1)
+    key = np.zeros(len(f), np.int64)
+    for j in range(f.shape[1]):
+        key = key * np.int64(n) + f[:, j]
+    return key
+
+
+def _face_owner(cells, bounds, n):
+    nv = cells.shape[1]
+    d = nv - 1
+    if len(bounds) == 0:
+        return np.zeros(0, np.int32)
+    # every cell face, i.e. every (d)-subset of the cell's nodes
+    faces = []
+    owners = []
+    cid = np.arange(len(cells), dtype=np.int64)
+    for skip in range(nv):
+        cols = [j for j in range(nv) if j != skip]
+        faces.append(cells[:, cols])
+        owners.append(cid)
+    faces = np.concatenate(faces)
+    owners = np.concatenate(owners)
+    fk = _face_keys(faces, n)
+    order = np.lexsort((owners, fk))
+    fk = fk[order]
+    owners = owners[order]
+    bk = _face_keys(bounds.reshape(-1, d), n)
+    pos = np.searchsorted(fk, bk)
+    pos = np.minimum(pos, len(fk) - 1)
+    hit = fk[pos] == bk
+    out = np.where(hit, owners[pos], -1).astype(np.int32)
+    return out
