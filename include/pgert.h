@@ -1,0 +1,150 @@
+/*
+ * pgert.h -- C-ABI of the MI355X-native ERT forward + Jacobian library
+ * (libpgert.so, HIP kernels for gfx950).
+ *
+ * It replaces, behind pyGIMLi's own Python API, the native core that
+ * pygimli.physics.ert.ERTModelling drives through Boost.Python:
+ *   DCSRMultiElectrodeModelling  (core/src/bert/dcfemmodelling.h:344-415)
+ *   createSensitivityCol         (core/src/bert/bertJacobian.h:43)
+ *   RSparseMatrix fill*Matrix    (core/src/sparsematrix.h:1039-1065)
+ *   CHOLMODWrapper / LinSolver   (core/src/cholmodWrapper.cpp, linSolver.cpp)
+ *
+ * Conventions
+ *  - Every call returns 0 on success and a negative code on failure; the
+ *    message is available from pgert_last_error(ctx).  This mirrors the
+ *    reference's throwError / log(Critical) exceptions (core/src/gimli.cpp:439).
+ *  - Set-up calls (pgert_set_*) take HOST pointers and copy.
+ *  - Compute calls take DEVICE pointers (memory of the context's device) for
+ *    their per-evaluation inputs and outputs; all work is queued on the
+ *    context's stream (pgert_set_stream, default: the null stream) and the
+ *    call returns after the stream is synchronised unless stated otherwise.
+ *  - A context is not thread-safe; use one context per host thread.
+ */
+#ifndef PGERT_H
+#define PGERT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pgert_ctx pgert_ctx;
+
+/* Solver selection for pgert_forward. */
+enum pgert_method {
+    /* independent-column Jacobi-PCG in FP64 (the plain method) */
+    PGERT_PCG_FP64 = 0,
+    /* block-PCG (all electrodes of one wavenumber share one Krylov block), FP64 */
+    PGERT_BLOCK_FP64 = 1,
+    /* block-PCG with FP32 SpMM/inner iterations and FP64 residual refinement */
+    PGERT_BLOCK_MIXED = 2,
+};
+
+typedef struct pgert_solve_opts {
+    int method;          /* enum pgert_method */
+    double rtol;         /* relative residual ||b - A x|| / ||b|| per column */
+    int max_iter;        /* cap on (inner) iterations per solve */
+    int check_every;     /* convergence is read back every check_every iterations */
+    double inner_rtol;   /* PGERT_BLOCK_MIXED: inner FP32 reduction per refinement step */
+    const double * warm_start; /* optional DEVICE pointer [nK][n][nE] secondary-field guess, NULL = zero */
+} pgert_solve_opts;
+
+typedef struct pgert_solve_stats {
+    int iterations;        /* total (inner) iterations over all wavenumbers */
+    int max_iterations;    /* largest per-wavenumber iteration count */
+    int refinements;       /* outer FP64 refinement steps (mixed) */
+    int converged;         /* 1 if every column met rtol */
+    double max_rel_residual; /* final FP64 ||b - A x|| / ||b||, max over columns */
+} pgert_solve_stats;
+
+/* Context ----------------------------------------------------------------- */
+int  pgert_create(pgert_ctx ** ctx, int device);
+void pgert_destroy(pgert_ctx * ctx);
+const char * pgert_last_error(const pgert_ctx * ctx);
+/* stream: a hipStream_t (e.g. torch.cuda.current_stream().cuda_stream) */
+int  pgert_set_stream(pgert_ctx * ctx, void * stream);
+
+/* Mesh (ModellingBase::setMesh, core/src/modellingbase.cpp:179-211).
+ * dim 2 (triangles) or 3 (tetrahedra).  xyz: n x 3; cells: C x (dim+1);
+ * cell_marker: C (Jacobian column = marker, bertJacobian.cpp:191,380);
+ * bounds: F x dim node ids, bound_marker: F (-1 Neumann, -2 mixed);
+ * bound_cell: F, the cell adjacent to each boundary (Boundary::leftCell).
+ * Builds the CSR pattern once (SparseMatrix::buildSparsityPattern,
+ * sparsematrix.h:966-1032) and the element geometry on the device. */
+int pgert_set_mesh(pgert_ctx * ctx, int dim, int n_nodes, const double * xyz,
+                   int n_cells, const int * cells, const int * cell_marker,
+                   int n_bounds, const int * bounds, const int * bound_marker,
+                   const int * bound_cell);
+
+/* Electrodes after DCMultiElectrodeModelling::searchElectrodes_
+ * (dcfemmodelling.cpp:790-1073): node id of every current electrode (pole
+ * sources, createCurrentPattern :1482-1525), the mean electrode position used
+ * by the mixed boundary condition (sourceCenterPos_, :1025-1030) and the
+ * Neumann surface height surfaceZ_ (:711-773; -DBL_MAX = full space). */
+int pgert_set_electrodes(pgert_ctx * ctx, int n_elec, const int * elec_node,
+                         const double * source_center, double surface_z);
+
+/* Wavenumbers and weights (setkValues / setWeights, dcfemmodelling.h:239-243;
+ * 3D: k = {0}, w = {1}). */
+int pgert_set_wavenumbers(pgert_ctx * ctx, int nk, const double * k, const double * w);
+
+/* Data: electrode indices a, b, m, n (-1 = none) and geometric factors k
+ * (DataContainerERT tokens). */
+int pgert_set_data(pgert_ctx * ctx, int n_data, const int * a, const int * b,
+                   const int * m, const int * n, const double * kfac);
+
+/* Query sizes: nnz of the CSR pattern, number of parameters (max marker + 1). */
+int pgert_info(const pgert_ctx * ctx, int * nnz, int * n_param);
+
+/* Copy the CSR pattern (row pointer n+1, column index nnz) to HOST memory
+ * (RSparseMatrix vecColPtr / vecRowIdx, sparsematrix.h:1072-1076). */
+int pgert_pattern(const pgert_ctx * ctx, int * rowptr, int * colidx);
+
+/* Cell resistivity from a model vector (mapERTModel, dcfemmodelling.cpp:1211;
+ * createMappedModel, modellingbase.cpp:401-470): len == n_cells -> copy,
+ * else rho_c = model[marker_c].  DEVICE pointers. */
+int pgert_map_model(pgert_ctx * ctx, const double * model, int n_model, double * rho);
+
+/* System matrix values of one wavenumber on the CSR pattern, exactly as
+ * DCSRMultiElectrodeModelling::calculateK assembles them
+ * (dcfemDomainAssembleStiffnessMatrix + dcfemBoundaryAssembleStiffnessMatrix,
+ * dcfemmodelling.cpp:163-292).  rho: DEVICE [n_cells]; vals: DEVICE [nnz]. */
+int pgert_assemble(pgert_ctx * ctx, const double * rho, double k, double * vals);
+
+/* RSparseMatrix::fillStiffnessMatrix(mesh, a) (mass = 0) or
+ * fillMassMatrix(mesh, a) (mass = 1) (sparsematrix.h:1039-1065), the back end
+ * of pg.solver.createStiffnessMatrix / createMassMatrix.  DEVICE pointers. */
+int pgert_fill_matrix(pgert_ctx * ctx, int mass, const double * a, double * vals);
+
+/* Forward response (DCMultiElectrodeModelling::response,
+ * dcfemmodelling.cpp:1085-1197, SR variant :2152-2296).
+ * rho: DEVICE [n_cells] cell resistivities; rhoa: DEVICE [n_data] output.
+ * The per-wavenumber total potentials are kept in the context for
+ * pgert_potentials / pgert_jacobian. */
+int pgert_forward(pgert_ctx * ctx, const double * rho, const pgert_solve_opts * opts,
+                  double * rhoa, pgert_solve_stats * stats);
+
+/* Per-wavenumber unweighted total potentials of the last forward in the
+ * reference's subSolutions_ layout: row = electrode + nE * kIdx, col = node
+ * (dcfemmodelling.cpp:1681-1682).  DEVICE [nK * nE * n]. */
+int pgert_potentials(pgert_ctx * ctx, double * out);
+
+/* Same potentials, node-major as held in HBM: [nK][n][nE].  DEVICE. */
+int pgert_potentials_node_major(pgert_ctx * ctx, double * out);
+
+/* Jacobian of the last forward (createJacobian_, dcfemmodelling.cpp:1318-1408;
+ * createSensitivityCol calc2, bertJacobian.cpp:170-219, :367-521).
+ * model: DEVICE [n_model]; J: DEVICE [n_data x n_param] row-major, FP64
+ * (dtype 0) or FP32 (dtype 1).  Scaled by k_i / m_j^2 when n_model == n_param. */
+int pgert_jacobian(pgert_ctx * ctx, const double * model, int n_model, void * J, int dtype);
+
+/* Timing of the last pgert_forward / pgert_jacobian phases in milliseconds
+ * (HIP events on the context stream): [0] assembly, [1] primary + SR rhs,
+ * [2] solve, [3] response, [4] jacobian. */
+int pgert_last_timings(const pgert_ctx * ctx, double * ms5);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PGERT_H */

@@ -1,1 +1,19 @@
-"""MI355X-native ERT forward + Jacobian path with pyGIMLi's API."""
+"""MI355X-native ERT forward + Jacobian path with pyGIMLi's API.
+
+    from pygimli_amd import ERTModelling, DataContainerERT, Mesh
+    fop = ERTModelling(); fop.setMesh(mesh); fop.setData(data)
+    rhoa = fop.response(rho); fop.createJacobian(rho); J = fop.jacobian()
+"""
+from .data import DataContainerERT  # noqa: F401
+from .mesh import Mesh  # noqa: F401
+
+
+def __getattr__(name):
+    # torch-dependent parts load lazily
+    if name == "ERTModelling":
+        from .ertModelling import ERTModelling
+        return ERTModelling
+    if name in ("createStiffnessMatrix", "createMassMatrix"):
+        from . import solver
+        return getattr(solver, name)
+    raise AttributeError(name)

@@ -1,0 +1,113 @@
+"""ctypes binding of libpgert.so (C-ABI in include/pgert.h).
+
+The product path has no CPU fallback: if the HIP library is missing or no GPU
+is visible, every compute call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libpgert.so")
+
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+_c_double_p = ctypes.POINTER(ctypes.c_double)
+
+PGERT_PCG_FP64 = 0
+PGERT_BLOCK_FP64 = 1
+PGERT_BLOCK_MIXED = 2
+
+METHODS = {"pcg64": PGERT_PCG_FP64, "block64": PGERT_BLOCK_FP64, "mixed": PGERT_BLOCK_MIXED}
+
+# every symbol include/pgert.h declares
+EXPORTS = (
+    "pgert_create", "pgert_destroy", "pgert_last_error", "pgert_set_stream", "pgert_set_mesh",
+    "pgert_set_electrodes", "pgert_set_wavenumbers", "pgert_set_data", "pgert_info",
+    "pgert_pattern", "pgert_map_model", "pgert_assemble", "pgert_fill_matrix", "pgert_forward",
+    "pgert_potentials", "pgert_potentials_node_major", "pgert_jacobian", "pgert_last_timings",
+)
+
+
+class SolveOpts(ctypes.Structure):
+    _fields_ = [("method", ctypes.c_int), ("rtol", ctypes.c_double), ("max_iter", ctypes.c_int),
+                ("check_every", ctypes.c_int), ("inner_rtol", ctypes.c_double),
+                ("warm_start", ctypes.c_void_p)]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("max_iterations", ctypes.c_int),
+                ("refinements", ctypes.c_int), ("converged", ctypes.c_int),
+                ("max_rel_residual", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_lib = None
+
+
+def load():
+    """Load libpgert.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libpgert.so not built ({LIB_PATH}); run python -m pygimli_amd.build")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    L.pgert_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int]
+    L.pgert_destroy.argtypes = [vp]
+    L.pgert_destroy.restype = None
+    L.pgert_last_error.argtypes = [vp]
+    L.pgert_last_error.restype = ctypes.c_char_p
+    L.pgert_set_stream.argtypes = [vp, vp]
+    L.pgert_set_mesh.argtypes = [vp, ctypes.c_int, ctypes.c_int, _c_double_p, ctypes.c_int, _c_int_p,
+                                 _c_int_p, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p]
+    L.pgert_set_electrodes.argtypes = [vp, ctypes.c_int, _c_int_p, _c_double_p, ctypes.c_double]
+    L.pgert_set_wavenumbers.argtypes = [vp, ctypes.c_int, _c_double_p, _c_double_p]
+    L.pgert_set_data.argtypes = [vp, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_int_p, _c_double_p]
+    L.pgert_info.argtypes = [vp, _c_int_p, _c_int_p]
+    L.pgert_pattern.argtypes = [vp, _c_int_p, _c_int_p]
+    L.pgert_map_model.argtypes = [vp, vp, ctypes.c_int, vp]
+    L.pgert_assemble.argtypes = [vp, vp, ctypes.c_double, vp]
+    L.pgert_fill_matrix.argtypes = [vp, ctypes.c_int, vp, vp]
+    L.pgert_forward.argtypes = [vp, vp, ctypes.POINTER(SolveOpts), vp, ctypes.POINTER(SolveStats)]
+    L.pgert_potentials.argtypes = [vp, vp]
+    L.pgert_potentials_node_major.argtypes = [vp, vp]
+    L.pgert_jacobian.argtypes = [vp, vp, ctypes.c_int, vp, ctypes.c_int]
+    L.pgert_last_timings.argtypes = [vp, _c_double_p]
+    for name in EXPORTS:
+        if name not in ("pgert_destroy", "pgert_last_error"):
+            getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+class Context:
+    """Owns one pgert_ctx on one device."""
+
+    def __init__(self, device=0):
+        self.L = load()
+        h = ctypes.c_void_p()
+        rc = self.L.pgert_create(ctypes.byref(h), int(device))
+        if rc != 0:
+            raise RuntimeError(f"pgert_create failed ({rc}): no usable HIP device {device}")
+        self.h = h
+        self.device = device
+
+    def check(self, rc, what=""):
+        if rc != 0:
+            msg = self.L.pgert_last_error(self.h).decode(errors="replace")
+            raise RuntimeError(f"{what}: {msg}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.pgert_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

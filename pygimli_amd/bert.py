@@ -1,0 +1,177 @@
+"""Host-side set-up of the ERT operator: wavenumbers, geometric factors and
+electrode search (set-up cost O(nE^2 + D), done once per mesh/data set).
+
+Restates core/src/bert/bertMisc.cpp and the electrode/mesh analysis of
+DCMultiElectrodeModelling (core/src/bert/dcfemmodelling.cpp:711-1073).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+MAX_DOUBLE = float(np.finfo(np.float64).max)
+TOLERANCE = 1e-12
+
+
+def _gauss_legendre_nr(n):
+    """Gauss-Legendre nodes/weights on [0, 1] by Newton iteration on P_n with
+    the reference's stopping rule |dz| <= 3e-6 (core/src/numericbase.cpp:102-156)."""
+    x = np.zeros(n)
+    w = np.zeros(n)
+    m = (n + 1.0) / 2.0
+    i = 1
+    while i <= m:
+        z = math.cos(math.pi * (i - 0.25) / (n + 0.5))
+        z1 = z + 6e-6
+        pp = 1.0
+        while abs(z - z1) > 3.0e-6:
+            p_prev, p = 0.0, 1.0
+            for j in range(1, n + 1):
+                p_prev, p = p, ((2.0 * j - 1.0) * z * p - (j - 1.0) * p_prev) / j
+            pp = n * (z * p - p_prev) / (z * z - 1.0)
+            z1, z = z, z - p / pp
+        x[i - 1] = 0.5 - 0.5 * z
+        x[n - i] = 0.5 + 0.5 * z
+        w[i - 1] = w[n - i] = 1.0 / ((1.0 - z * z) * pp * pp)
+        i += 1
+    return x, w
+
+
+def _gauss_laguerre_nr(n):
+    """Gauss-Laguerre (alpha = 0) by Newton iteration, |dz| <= 3e-11, at most
+    20 steps (core/src/numericbase.cpp:51-100)."""
+    x = np.zeros(n)
+    w = np.zeros(n)
+    z = 0.0
+    for i in range(1, n + 1):
+        if i == 1:
+            z = 3.0 / (1.0 + 2.4 * n)
+        elif i == 2:
+            z += 15.0 / (1.0 + 2.5 * n)
+        else:
+            ai = i - 2
+            z += (1.0 + 2.55 * ai) / (1.9 * ai) * (z - x[ai - 1])
+        p = p_prev = pp = 0.0
+        for _ in range(20):
+            p_prev, p = 0.0, 1.0
+            for j in range(1, n + 1):
+                p_prev, p = p, ((2.0 * j - 1 - z) * p - (j - 1) * p_prev) / j
+            pp = n * (p - p_prev) / z
+            z1 = z
+            z = z1 - p / pp
+            if abs(z - z1) <= 3.0e-11:
+                break
+        x[i - 1] = z
+        w[i - 1] = -1.0 / (pp * n * p_prev)
+    return x, w
+
+
+def kwave_from_range(rmin, rmax, n_leg, n_lag=4):
+    """Wavenumbers/weights of the 2.5D inverse Fourier transform
+    (initKWaveList, core/src/bert/bertMisc.cpp:89-106)."""
+    k0 = 1.0 / (2.0 * rmin)
+    x, w = _gauss_legendre_nr(n_leg)
+    kl = k0 * x * x
+    wl = 2.0 * k0 * x * w / math.pi
+    x, w = _gauss_laguerre_nr(n_lag)
+    kg = k0 * (x + 1.0)
+    wg = k0 * np.exp(x) * w / math.pi
+    return np.concatenate([kl, kg]), np.concatenate([wl, wg])
+
+
+def kwave_list(dim, sensors, n_leg=None, n_lag=4):
+    """initKWaveList (bertMisc.cpp:36-86): 3D -> k = {0}, w = {1}; 2.5D ->
+    rMin = min pair distance / 2, rMax = max * 2, nLeg = max(floor(6 log10(rMax/rMin)), 4)."""
+    if dim == 3:
+        return np.array([0.0]), np.array([1.0])
+    s = np.asarray(sensors, np.float64)
+    diff = s[:, None, :] - s[None, :, :]
+    d = np.sqrt((diff ** 2).sum(-1))[np.triu_indices(len(s), 1)]
+    rmin = d.min() / 2.0
+    rmax = d.max() * 2.0
+    if n_leg is None:
+        n_leg = max(int(math.floor(6.0 * math.log10(rmax / rmin))), 4)
+    return kwave_from_range(rmin, rmax, n_leg, n_lag)
+
+
+def halfspace_potential(points, src):
+    """3D half-space unit potential with mirror plane z = 0:
+    (1/r + 1/r') / (4 pi), 1 where r = 0 (bertMisc.cpp:178-181, :209-214)."""
+    p = np.asarray(points, np.float64)
+    s = np.asarray(src, np.float64)
+    r = np.sqrt(((p - s) ** 2).sum(-1))
+    sm = s * np.array([1.0, 1.0, -1.0])
+    rm = np.sqrt(((p - sm) ** 2).sum(-1))
+    with np.errstate(divide="ignore"):
+        out = (1.0 / r + 1.0 / rm) / (4.0 * math.pi)
+    return np.where(r < TOLERANCE, 1.0, out)
+
+
+def geometric_factors(sensors, abmn, dim=3):
+    """geometricFactors (bertMisc.cpp:131-176): k = 1 / (G_am - G_bm - G_an + G_bn).
+    2D sensors with y != 0 are read as (x, z)."""
+    s = np.array(sensors, np.float64)
+    if dim == 2:
+        sw = s[:, 1] != 0.0
+        s[sw, 2] = s[sw, 1]
+        s[sw, 1] = 0.0
+    abmn = np.asarray(abmn)
+    a, b, m, n = (abmn[:, i] for i in range(4))
+
+    def G(i, j):
+        ok = (i > -1) & (j > -1)
+        out = np.zeros(len(i))
+        if ok.any():
+            out[ok] = np.array([halfspace_potential(s[jj][None], s[ii])[0]
+                                for ii, jj in zip(i[ok], j[ok])])
+        return out
+
+    with np.errstate(divide="ignore"):
+        return 1.0 / (G(a, m) - G(b, m) - G(a, n) + G(b, n))
+
+
+class MeshAnalysis:
+    """Boundary/topography analysis and electrode search
+    (updateMeshDependency_ dcfemmodelling.cpp:711-773, searchElectrodes_ :790-1073)
+    for node electrodes (marker -99)."""
+
+    def __init__(self, mesh, sensors):
+        dim = mesh.dim()
+        bm = mesh.boundaryMarkers()
+        bc = mesh.boundaryCenters() if mesh.boundaryCount() else np.zeros((0, 3))
+        self.neumannDomain = not bool(np.isin(bm, (-2, -3, -4)).any())
+        self.topography = False
+        self.surfaceZ = -MAX_DOUBLE
+        neu = np.nonzero(bm == -1)[0]
+        if len(neu):
+            self.surfaceZ = float(bc[neu[0], dim - 1])
+            if np.any(bc[neu, dim - 1] != self.surfaceZ):
+                self.topography = True
+        if self.neumannDomain:
+            self.topography = True
+            if dim == 2:
+                self.neumannDomain = False
+        pos = np.array(sensors, np.float64)
+        if dim == 2:
+            zvar = np.ptp(pos[:, 2]) > 0 or np.abs(pos[:, 2]).max() > 0
+            ystill = not (np.ptp(pos[:, 1]) > 0) and np.abs(pos[:, 1]).max() < 1e-8
+            if zvar and ystill:
+                pos[:, [1, 2]] = pos[:, [2, 1]]
+        cand = list(mesh.findNodesIdxByMarker(-99))
+        nodes = mesh.positions()
+        self.elecNode = np.empty(len(pos), np.int32)
+        for i, p in enumerate(pos):
+            hit = None
+            if cand:
+                d = np.sqrt(((nodes[cand] - p) ** 2).sum(-1))
+                close = np.nonzero(d < 0.01)[0]
+                if len(close):
+                    hit = int(close[0])
+            if hit is None:
+                raise NotImplementedError(
+                    f"sensor {i} at {p} matches no electrode node (marker -99); "
+                    "free (in-cell) electrodes are not supported")
+            self.elecNode[i] = cand.pop(hit)
+        self.elecPos = nodes[self.elecNode]
+        self.sourceCenter = self.elecPos.sum(0) / len(self.elecPos)

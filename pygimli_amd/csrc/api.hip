@@ -1,0 +1,519 @@
+// C-ABI entry points of libpgert (declared in include/pgert.h) and the host
+// side set-up they need: CSR pattern, atomics-free assembly map, boundary
+// and electrode tables.
+#include "internal.h"
+
+#include <algorithm>
+#include <array>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+
+using namespace pgert;
+
+struct pgert_ctx : public Ctx {};
+
+namespace {
+
+template <class F> int guarded(pgert_ctx * c, F f) {
+    if (!c) return -2;
+    try {
+        PG_CHECK(hipSetDevice(c->device));
+        f();
+        return 0;
+    } catch (std::exception & e) {
+        c->err = e.what();
+        return -1;
+    } catch (...) {
+        c->err = "unknown error";
+        return -1;
+    }
+}
+
+// Sorted neighbour lists of every node (SparseMatrix::buildSparsityPattern,
+// sparsematrix.h:966-1032: all node pairs sharing a cell, sorted per row).
+void build_pattern(Ctx & c, const int * cells) {
+    const int n = c.n, C = c.C, nv = c.nv;
+    std::vector<int> cnt(n + 1, 0);
+    for (long e = 0; e < (long)C * nv; ++e) cnt[cells[e] + 1] += nv;
+    for (int i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+    std::vector<int> cand(cnt[n]);
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (int cc = 0; cc < C; ++cc) {
+        const int * v = cells + (long)cc * nv;
+        for (int i = 0; i < nv; ++i)
+            for (int j = 0; j < nv; ++j) cand[fill[v[i]]++] = v[j];
+    }
+    c.hRowptr.assign(n + 1, 0);
+    c.hColidx.clear();
+    c.hColidx.reserve(cnt[n] / 2);
+    for (int i = 0; i < n; ++i) {
+        auto b = cand.begin() + cnt[i], e = cand.begin() + cnt[i + 1];
+        std::sort(b, e);
+        auto u = std::unique(b, e);
+        c.hColidx.insert(c.hColidx.end(), b, u);
+        c.hRowptr[i + 1] = (int)c.hColidx.size();
+    }
+    c.nnz = (int)c.hColidx.size();
+}
+
+inline int find_slot(const Ctx & c, int row, int col) {
+    const int * b = c.hColidx.data() + c.hRowptr[row];
+    const int * e = c.hColidx.data() + c.hRowptr[row + 1];
+    const int * p = std::lower_bound(b, e, col);
+    if (p == e || *p != col) return -1;
+    return (int)(p - c.hColidx.data());
+}
+
+void sync(Ctx & c) { PG_CHECK(hipStreamSynchronize(c.stream)); }
+
+}  // namespace
+
+extern "C" {
+
+int pgert_create(pgert_ctx ** out, int device) {
+    if (!out) return -2;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return -3;
+    if (device < 0 || device >= ndev) return -4;
+    pgert_ctx * c = new pgert_ctx();
+    c->device = device;
+    int rc = guarded(c, [&] {
+        for (auto & e : c->ev) PG_CHECK(hipEventCreate(&e));
+    });
+    if (rc) {
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return 0;
+}
+
+void pgert_destroy(pgert_ctx * c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipDeviceSynchronize();
+    for (auto & e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    delete c;
+}
+
+const char * pgert_last_error(const pgert_ctx * c) { return c ? c->err.c_str() : "null context"; }
+
+int pgert_set_stream(pgert_ctx * c, void * stream) {
+    return guarded(c, [&] { c->stream = (hipStream_t)stream; });
+}
+
+int pgert_set_mesh(pgert_ctx * c, int dim, int n_nodes, const double * xyz, int n_cells,
+                   const int * cells, const int * cell_marker, int n_bounds, const int * bounds,
+                   const int * bound_marker, const int * bound_cell) {
+    return guarded(c, [&] {
+        PG_REQUIRE(dim == 2 || dim == 3, "pgert_set_mesh: dim must be 2 or 3");
+        PG_REQUIRE(n_nodes > 0 && n_cells > 0, "pgert_set_mesh: empty mesh");
+        PG_REQUIRE(xyz && cells && cell_marker, "pgert_set_mesh: null array");
+        PG_REQUIRE(n_bounds == 0 || (bounds && bound_marker && bound_cell),
+                   "pgert_set_mesh: null boundary array");
+        c->dim = dim;
+        c->nv = dim + 1;
+        c->n = n_nodes;
+        c->C = n_cells;
+        for (long e = 0; e < (long)n_cells * c->nv; ++e)
+            PG_REQUIRE(cells[e] >= 0 && cells[e] < n_nodes, "pgert_set_mesh: cell node index out of range");
+        // parameters: cells with marker >= 0, sorted by marker (bertJacobian.cpp:398-399)
+        int maxMarker = -1;
+        std::vector<int> order;
+        order.reserve(n_cells);
+        for (int i = 0; i < n_cells; ++i) {
+            maxMarker = std::max(maxMarker, cell_marker[i]);
+            if (cell_marker[i] >= 0) order.push_back(i);
+        }
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return cell_marker[a] < cell_marker[b]; });
+        c->nParam = maxMarker + 1;
+        c->nSorted = (int)order.size();
+        bool uniq = true;
+        for (size_t i = 1; i < order.size(); ++i)
+            if (cell_marker[order[i]] == cell_marker[order[i - 1]]) uniq = false;
+        c->uniqueMarkers = uniq;
+
+        hipStream_t s = c->stream;
+        c->xyz.upload(xyz, (size_t)n_nodes * 3, s);
+        c->cells.upload(cells, (size_t)n_cells * c->nv, s);
+        c->cellMarker.upload(cell_marker, n_cells, s);
+        c->sortedCells.upload(order, s);
+        c->grads.alloc((size_t)n_cells * c->nv * 3);
+        c->vol.alloc(n_cells);
+        launch_cell_geometry(*c);
+
+        // pattern + contributions sorted by slot (counting sort, cell order kept)
+        build_pattern(*c, cells);
+        const int nv = c->nv;
+        std::vector<int> slotOf((size_t)n_cells * nv * nv);
+        std::vector<int> cptr(c->nnz + 1, 0);
+        for (int cc = 0; cc < n_cells; ++cc) {
+            const int * v = cells + (long)cc * nv;
+            for (int i = 0; i < nv; ++i)
+                for (int j = 0; j < nv; ++j) {
+                    int sl = find_slot(*c, v[i], v[j]);
+                    slotOf[((size_t)cc * nv + i) * nv + j] = sl;
+                    cptr[sl + 1]++;
+                }
+        }
+        for (int i = 0; i < c->nnz; ++i) cptr[i + 1] += cptr[i];
+        std::vector<int> pos(cptr.begin(), cptr.end() - 1);
+        std::vector<int> ccell(cptr[c->nnz]), ccode(cptr[c->nnz]);
+        for (int cc = 0; cc < n_cells; ++cc)
+            for (int ij = 0; ij < nv * nv; ++ij) {
+                int sl = slotOf[(size_t)cc * nv * nv + ij];
+                int p = pos[sl]++;
+                ccell[p] = cc;
+                ccode[p] = ij;
+            }
+        std::vector<int> diag(n_nodes);
+        for (int i = 0; i < n_nodes; ++i) {
+            diag[i] = find_slot(*c, i, i);
+            PG_REQUIRE(diag[i] >= 0, "pgert_set_mesh: node " + std::to_string(i) + " belongs to no cell");
+        }
+        c->rowptr.upload(c->hRowptr, s);
+        c->colidx.upload(c->hColidx, s);
+        c->diagSlot.upload(diag, s);
+        c->cptr.upload(cptr, s);
+        c->ccell.upload(ccell, s);
+        DBuf<int> code;
+        code.upload(ccode, s);
+        c->cK.alloc(ccell.size());
+        c->cM.alloc(ccell.size());
+        launch_contrib_geometry(*c, code.p);
+        sync(*c);  // the code table is released at the end of this scope
+
+        // mixed boundaries (marker -2); Dirichlet markers are rejected
+        std::vector<int> mixCell;
+        std::vector<double> mixCtr, mixNrm;
+        std::vector<std::array<int, 3>> bcon;  // slot, face, code (i*dim+j)
+        std::vector<double> bcoef;
+        for (int f = 0; f < n_bounds; ++f) {
+            int mk = bound_marker[f];
+            PG_REQUIRE(mk != -3 && mk != -4, "pgert_set_mesh: Dirichlet boundaries are not supported");
+            if (mk != -2) continue;
+            const int * bn = bounds + (long)f * dim;
+            PG_REQUIRE(bound_cell[f] >= 0 && bound_cell[f] < n_cells,
+                       "pgert_set_mesh: no cell found for a mixed boundary");
+            double p[3][3];
+            for (int a = 0; a < dim; ++a) {
+                PG_REQUIRE(bn[a] >= 0 && bn[a] < n_nodes, "pgert_set_mesh: boundary node out of range");
+                for (int q = 0; q < 3; ++q) p[a][q] = xyz[3 * (long)bn[a] + q];
+            }
+            double ctr[3] = {0, 0, 0}, nrm[3] = {0, 0, 0}, size;
+            for (int a = 0; a < dim; ++a)
+                for (int q = 0; q < 3; ++q) ctr[q] += p[a][q] / dim;
+            if (dim == 2) {
+                double tx = p[1][0] - p[0][0], ty = p[1][1] - p[0][1];
+                size = std::sqrt(tx * tx + ty * ty);
+                nrm[0] = ty / size;
+                nrm[1] = -tx / size;
+            } else {
+                double u[3], v[3];
+                for (int q = 0; q < 3; ++q) { u[q] = p[1][q] - p[0][q]; v[q] = p[2][q] - p[0][q]; }
+                double cr[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2],
+                                u[0] * v[1] - u[1] * v[0]};
+                double a2 = std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+                size = a2 / 2.0;
+                for (int q = 0; q < 3; ++q) nrm[q] = cr[q] / a2;
+            }
+            int fi = (int)mixCell.size();
+            mixCell.push_back(bound_cell[f]);
+            for (int q = 0; q < 3; ++q) { mixCtr.push_back(ctr[q]); mixNrm.push_back(nrm[q]); }
+            // boundary mass: edge len/6 (1 + d_ij), triangle area/12 (1 + d_ij)
+            double mb = size / (dim * (dim + 1));
+            for (int i = 0; i < dim; ++i)
+                for (int j = 0; j < dim; ++j) {
+                    int sl = find_slot(*c, bn[i], bn[j]);
+                    PG_REQUIRE(sl >= 0, "pgert_set_mesh: boundary face is not a cell face");
+                    bcon.push_back({sl, fi, i * dim + j});
+                    bcoef.push_back(mb * (i == j ? 2.0 : 1.0));
+                }
+        }
+        c->nMixed = (int)mixCell.size();
+        std::vector<int> idx(bcon.size());
+        for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
+        std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return bcon[a][0] < bcon[b][0]; });
+        std::vector<int> bslot, bptr(1, 0), bface;
+        std::vector<double> bc;
+        for (size_t q = 0; q < idx.size(); ++q) {
+            int sl = bcon[idx[q]][0];
+            if (bslot.empty() || bslot.back() != sl) {
+                if (!bslot.empty()) bptr.push_back((int)bface.size());
+                bslot.push_back(sl);
+            }
+            bface.push_back(bcon[idx[q]][1]);
+            bc.push_back(bcoef[idx[q]]);
+        }
+        bptr.push_back((int)bface.size());
+        if (bslot.empty()) bptr.assign(1, 0);
+        c->nBSlot = (int)bslot.size();
+        c->mixCell.upload(mixCell, s);
+        c->mixCtr.upload(mixCtr, s);
+        c->mixNrm.upload(mixNrm, s);
+        c->bSlot.upload(bslot, s);
+        c->bPtr.upload(bptr, s);
+        c->bFace.upload(bface, s);
+        c->bCoef.upload(bc, s);
+
+        c->hElecNode.clear();
+        c->nE = 0;
+        c->s1Valid = c->primValid = c->haveU = false;
+        sync(*c);
+    });
+}
+
+int pgert_set_electrodes(pgert_ctx * c, int n_elec, const int * elec_node,
+                         const double * source_center, double surface_z) {
+    return guarded(c, [&] {
+        PG_REQUIRE(c->n > 0, "pgert_set_electrodes: no mesh");
+        PG_REQUIRE(n_elec > 0 && elec_node && source_center, "pgert_set_electrodes: bad arguments");
+        PG_REQUIRE(n_elec <= 256, "pgert_set_electrodes: at most 256 current electrodes per context");
+        std::vector<double> hx((size_t)c->n * 3);
+        PG_CHECK(hipMemcpy(hx.data(), c->xyz.p, hx.size() * 8, hipMemcpyDeviceToHost));
+        std::vector<int> hc((size_t)c->C * c->nv);
+        PG_CHECK(hipMemcpy(hc.data(), c->cells.p, hc.size() * 4, hipMemcpyDeviceToHost));
+        c->nE = n_elec;
+        c->hElecNode.assign(elec_node, elec_node + n_elec);
+        // per-node linked list of the electrodes sitting on it
+        std::vector<int> headE(c->n, -1), nextE(n_elec, -1);
+        for (int e = n_elec - 1; e >= 0; --e) {
+            PG_REQUIRE(elec_node[e] >= 0 && elec_node[e] < c->n, "pgert_set_electrodes: node out of range");
+            nextE[e] = headE[elec_node[e]];
+            headE[elec_node[e]] = e;
+        }
+        std::vector<std::vector<int>> cl(n_elec);
+        for (int cc = 0; cc < c->C; ++cc)
+            for (int i = 0; i < c->nv; ++i)
+                for (int e = headE[hc[(size_t)cc * c->nv + i]]; e >= 0; e = nextE[e])
+                    cl[e].push_back(cc);
+        std::vector<int> ptr(1, 0), lst;
+        std::vector<double> pos, rmin;
+        for (int e = 0; e < n_elec; ++e) {
+            PG_REQUIRE(!cl[e].empty(), "pgert_set_electrodes: electrode node has no cell");
+            lst.insert(lst.end(), cl[e].begin(), cl[e].end());
+            ptr.push_back((int)lst.size());
+            int v = elec_node[e];
+            for (int q = 0; q < 3; ++q) pos.push_back(hx[3 * (size_t)v + q]);
+            // nearest neighbour distance (ElectrodeShapeNode::setSingValue, electrode.cpp:157-171)
+            double r = DBL_MAX;
+            for (int cc : cl[e])
+                for (int i = 0; i < c->nv; ++i) {
+                    int u = hc[(size_t)cc * c->nv + i];
+                    if (u == v) continue;
+                    double d2 = 0;
+                    for (int q = 0; q < 3; ++q) {
+                        double d = hx[3 * (size_t)u + q] - hx[3 * (size_t)v + q];
+                        d2 += d * d;
+                    }
+                    r = std::min(r, std::sqrt(d2));
+                }
+            rmin.push_back(r);
+        }
+        hipStream_t s = c->stream;
+        c->elecNode.upload(c->hElecNode, s);
+        c->elecPos.upload(pos, s);
+        c->minRadius.upload(rmin, s);
+        c->ecPtr.upload(ptr, s);
+        c->ecCell.upload(lst, s);
+        for (int q = 0; q < 3; ++q) c->srcCenter[q] = source_center[q];
+        c->surfaceZ = surface_z;
+        c->s1Valid = c->primValid = c->haveU = false;
+        sync(*c);
+    });
+}
+
+int pgert_set_wavenumbers(pgert_ctx * c, int nk, const double * k, const double * w) {
+    return guarded(c, [&] {
+        PG_REQUIRE(nk > 0 && k && w, "pgert_set_wavenumbers: bad arguments");
+        for (int i = 0; i < nk; ++i) PG_REQUIRE(k[i] >= 0.0, "pgert_set_wavenumbers: k < 0");
+        if (c->dim == 3) PG_REQUIRE(nk == 1 && k[0] == 0.0, "pgert_set_wavenumbers: 3D uses k = {0}");
+        c->nK = nk;
+        c->kv.assign(k, k + nk);
+        c->wv.assign(w, w + nk);
+        c->dk.upload(c->kv, c->stream);
+        c->dw.upload(c->wv, c->stream);
+        c->s1Valid = c->primValid = c->haveU = false;
+        sync(*c);
+    });
+}
+
+int pgert_set_data(pgert_ctx * c, int n_data, const int * a, const int * b, const int * m,
+                   const int * n, const double * kfac) {
+    return guarded(c, [&] {
+        PG_REQUIRE(c->nE > 0, "pgert_set_data: set electrodes first");
+        PG_REQUIRE(n_data > 0 && a && b && m && n && kfac, "pgert_set_data: bad arguments");
+        std::vector<int> h((size_t)n_data * 4);
+        for (int i = 0; i < n_data; ++i) {
+            int q[4] = {a[i], b[i], m[i], n[i]};
+            for (int j = 0; j < 4; ++j) {
+                // DataMap::data (datamap.cpp:185-194)
+                PG_REQUIRE(q[j] >= -1 && q[j] < c->nE, "pgert_set_data: electrode index out of range");
+                h[4 * (size_t)i + j] = q[j];
+            }
+        }
+        c->D = n_data;
+        c->abmn.upload(h, c->stream);
+        c->kfac.upload(kfac, n_data, c->stream);
+        sync(*c);
+    });
+}
+
+int pgert_info(const pgert_ctx * c, int * nnz, int * n_param) {
+    if (!c) return -2;
+    if (nnz) *nnz = c->nnz;
+    if (n_param) *n_param = c->nParam;
+    return 0;
+}
+
+int pgert_pattern(const pgert_ctx * c, int * rowptr, int * colidx) {
+    if (!c || c->n == 0) return -2;
+    std::memcpy(rowptr, c->hRowptr.data(), c->hRowptr.size() * sizeof(int));
+    std::memcpy(colidx, c->hColidx.data(), c->hColidx.size() * sizeof(int));
+    return 0;
+}
+
+int pgert_map_model(pgert_ctx * c, const double * model, int n_model, double * rho) {
+    return guarded(c, [&] {
+        PG_REQUIRE(c->C > 0, "pgert_map_model: no mesh");
+        PG_REQUIRE(n_model == c->C || n_model >= c->nParam, "pgert_map_model: marker >= model size");
+        launch_map_model(*c, model, n_model, rho);
+        sync(*c);
+    });
+}
+
+int pgert_assemble(pgert_ctx * c, const double * rho, double k, double * vals) {
+    return guarded(c, [&] {
+        PG_REQUIRE(c->C > 0, "pgert_assemble: no mesh");
+        PG_REQUIRE(k >= 0.0, "pgert_assemble: k < 0");
+        launch_assemble(*c, rho, k, vals, false);
+        sync(*c);
+    });
+}
+
+int pgert_fill_matrix(pgert_ctx * c, int mass, const double * a, double * vals) {
+    return guarded(c, [&] {
+        PG_REQUIRE(c->C > 0, "pgert_fill_matrix: no mesh");
+        launch_fill_matrix(*c, mass, a, vals);
+        sync(*c);
+    });
+}
+
+int pgert_forward(pgert_ctx * c, const double * rho, const pgert_solve_opts * opts, double * rhoa,
+                  pgert_solve_stats * stats) {
+    return guarded(c, [&] {
+        PG_REQUIRE(c->C > 0 && c->nE > 0 && c->nK > 0 && c->D > 0,
+                   "pgert_forward: mesh, electrodes, wavenumbers and data must be set");
+        PG_REQUIRE(rho && rhoa, "pgert_forward: null pointer");
+        pgert_solve_opts o;
+        if (opts) o = *opts;
+        else {
+            o.method = PGERT_BLOCK_MIXED;
+            o.rtol = 1e-8;
+            o.max_iter = 20000;
+            o.check_every = 8;
+            o.inner_rtol = 1e-3;
+            o.warm_start = nullptr;
+        }
+        PG_REQUIRE(o.rtol > 0 && o.max_iter > 0, "pgert_forward: bad solver options");
+        if (o.check_every <= 0) o.check_every = 8;
+        hipStream_t s = c->stream;
+        const size_t nk = c->nK, nn = c->n, ne = c->nE;
+        c->rho.alloc(c->C);
+        PG_CHECK(hipMemcpyAsync(c->rho.p, rho, (size_t)c->C * 8, hipMemcpyDeviceToDevice, s));
+        PG_CHECK(hipEventRecord(c->ev[0], s));
+        // assembly (A_k for this model, S1_k with rho == 1 once per mesh)
+        c->valsA.alloc(nk * c->nnz);
+        for (size_t kk = 0; kk < nk; ++kk)
+            launch_assemble(*c, c->rho.p, c->kv[kk], c->valsA.p + kk * c->nnz, false);
+        if (!c->s1Valid) {
+            c->valsS1.alloc(nk * c->nnz);
+            for (size_t kk = 0; kk < nk; ++kk)
+                launch_assemble(*c, nullptr, c->kv[kk], c->valsS1.p + kk * c->nnz, true);
+            c->s1Valid = true;
+        }
+        launch_diag_inv(*c);
+        PG_CHECK(hipEventRecord(c->ev[1], s));
+        // primary potentials (once per electrode/wavenumber set) and SR right-hand sides
+        if (!c->primValid) {
+            c->prim.alloc(nk * nn * ne);
+            launch_primary(*c);
+            c->primValid = true;
+        }
+        c->rhoS.alloc(ne);
+        launch_rho_source(*c, c->rho.p);
+        c->B.alloc(nk * nn * ne);
+        launch_sr_rhs(*c);
+        c->X.alloc(nk * nn * ne);
+        if (o.warm_start)
+            PG_CHECK(hipMemcpyAsync(c->X.p, o.warm_start, c->X.bytes(), hipMemcpyDeviceToDevice, s));
+        else
+            PG_CHECK(hipMemsetAsync(c->X.p, 0, c->X.bytes(), s));
+        PG_CHECK(hipEventRecord(c->ev[2], s));
+        pgert_solve_stats st{};
+        if (o.method == PGERT_PCG_FP64)
+            solve_pcg_fp64(*c, o, st);
+        else if (o.method == PGERT_BLOCK_FP64)
+            solve_block(*c, o, st, false);
+        else if (o.method == PGERT_BLOCK_MIXED)
+            solve_block(*c, o, st, true);
+        else
+            throw std::runtime_error("pgert_forward: unknown method");
+        PG_CHECK(hipEventRecord(c->ev[3], s));
+        c->U.alloc(nk * nn * ne);
+        launch_total_potential(*c);
+        c->haveU = true;
+        launch_response(*c, rhoa);
+        PG_CHECK(hipEventRecord(c->ev[4], s));
+        sync(*c);
+        float t;
+        for (int i = 0; i < 4; ++i) {
+            PG_CHECK(hipEventElapsedTime(&t, c->ev[i], c->ev[i + 1]));
+            c->timings[i] = t;
+        }
+        if (stats) *stats = st;
+    });
+}
+
+int pgert_potentials(pgert_ctx * c, double * out) {
+    return guarded(c, [&] {
+        PG_REQUIRE(c->haveU, "pgert_potentials: no forward result");
+        launch_transpose_pots(*c, out);
+        sync(*c);
+    });
+}
+
+int pgert_potentials_node_major(pgert_ctx * c, double * out) {
+    return guarded(c, [&] {
+        PG_REQUIRE(c->haveU, "pgert_potentials_node_major: no forward result");
+        PG_CHECK(hipMemcpyAsync(out, c->U.p, c->U.bytes(), hipMemcpyDeviceToDevice, c->stream));
+        sync(*c);
+    });
+}
+
+int pgert_jacobian(pgert_ctx * c, const double * model, int n_model, void * J, int dtype) {
+    return guarded(c, [&] {
+        PG_REQUIRE(c->haveU, "pgert_jacobian: call pgert_forward first");
+        PG_REQUIRE(J && (dtype == 0 || dtype == 1), "pgert_jacobian: bad output");
+        PG_REQUIRE(n_model >= 0 && (n_model == 0 || model), "pgert_jacobian: bad model");
+        PG_CHECK(hipEventRecord(c->ev[4], c->stream));
+        launch_jacobian(*c, model, n_model, J, dtype);
+        PG_CHECK(hipEventRecord(c->ev[5], c->stream));
+        sync(*c);
+        float t;
+        PG_CHECK(hipEventElapsedTime(&t, c->ev[4], c->ev[5]));
+        c->timings[4] = t;
+    });
+}
+
+int pgert_last_timings(const pgert_ctx * c, double * ms5) {
+    if (!c || !ms5) return -2;
+    for (int i = 0; i < 5; ++i) ms5[i] = c->timings[i];
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,156 @@
+// Internal definitions shared by the libpgert HIP sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/pgert.h"
+
+#define PG_CHECK(x)                                                                  \
+    do {                                                                             \
+        hipError_t e_ = (x);                                                         \
+        if (e_ != hipSuccess)                                                        \
+            throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define PG_REQUIRE(cond, msg)                                 \
+    do {                                                      \
+        if (!(cond)) throw std::runtime_error(msg);           \
+    } while (0)
+
+namespace pgert {
+
+// Owned device allocation.
+template <class T> struct DBuf {
+    T * p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf & operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        if (count == n && p) return;
+        release();
+        if (count) PG_CHECK(hipMalloc((void **)&p, count * sizeof(T)));
+        n = count;
+    }
+    void upload(const T * h, size_t count, hipStream_t s) {
+        alloc(count);
+        if (count) PG_CHECK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T> & v, hipStream_t s) { upload(v.data(), v.size(), s); }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // ---- mesh
+    int dim = 0, nv = 0, n = 0, C = 0;
+    int nParam = 0;
+    bool uniqueMarkers = false;  // marker -> cell is a bijection over cells with marker >= 0
+    DBuf<double> xyz;            // n * 3
+    DBuf<int> cells;             // C * nv
+    DBuf<int> cellMarker;        // C
+    DBuf<double> grads;          // C * nv * 3 (components >= dim are 0)
+    DBuf<double> vol;            // C
+    DBuf<int> sortedCells;       // cells with marker >= 0, sorted by marker
+    int nSorted = 0;
+
+    // ---- CSR pattern and atomics-free assembly map
+    int nnz = 0;
+    std::vector<int> hRowptr, hColidx;
+    DBuf<int> rowptr, colidx, diagSlot;
+    DBuf<int> cptr;      // nnz + 1: contributions of each slot
+    DBuf<int> ccell;     // contribution -> cell
+    DBuf<double> cK;     // contribution -> stiffness value (geometry only)
+    DBuf<double> cM;     // contribution -> mass value (geometry only)
+
+    // ---- mixed boundary faces
+    int nMixed = 0;
+    DBuf<int> mixCell;     // adjacent cell per mixed face
+    DBuf<double> mixCtr;   // face centre (3)
+    DBuf<double> mixNrm;   // unit normal (3)
+    int nBSlot = 0;
+    DBuf<int> bSlot;       // slots touched by mixed faces (unique)
+    DBuf<int> bPtr;        // nBSlot + 1
+    DBuf<int> bFace;       // contribution -> mixed face index
+    DBuf<double> bCoef;    // contribution -> face mass value (geometry only)
+
+    // ---- electrodes
+    int nE = 0;
+    std::vector<int> hElecNode;
+    DBuf<int> elecNode;
+    DBuf<double> elecPos;    // nE * 3
+    DBuf<double> minRadius;  // nE
+    DBuf<int> ecPtr, ecCell; // cells around each electrode node
+    double srcCenter[3] = {0, 0, 0};
+    double surfaceZ = 0;
+
+    // ---- wavenumbers
+    int nK = 0;
+    std::vector<double> kv, wv;
+    DBuf<double> dk, dw;
+
+    // ---- data
+    int D = 0;
+    DBuf<int> abmn;     // D * 4
+    DBuf<double> kfac;  // D
+
+    // ---- per-evaluation state
+    DBuf<double> rho;       // C
+    DBuf<double> rhoS;      // nE   geometric-mean resistivity at each source
+    DBuf<double> valsA;     // nK * nnz
+    DBuf<double> valsS1;    // nK * nnz (rho == 1), mesh-only
+    bool s1Valid = false;
+    DBuf<double> dinv;      // nK * n
+    DBuf<double> prim;      // nK * n * nE   unit-resistivity primary potentials
+    bool primValid = false;
+    DBuf<double> U;         // nK * n * nE   total potentials (secondary + rho_s * primary)
+    bool haveU = false;
+    DBuf<double> emap;      // nE * nE
+    // solver work
+    DBuf<double> B, X, R, P, Q, Z;
+    DBuf<float> fA, fX, fR, fP, fQ, fZ, fDinv;
+    DBuf<double> part;      // reduction partials
+    DBuf<double> scal;      // per-column scalars
+    DBuf<double> small;     // block-CG small matrices
+
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    double timings[5] = {0, 0, 0, 0, 0};
+};
+
+// ---- kernels / launchers (defined in the .hip files)
+void launch_cell_geometry(Ctx & c);
+void launch_contrib_geometry(Ctx & c, const int * codes);
+void launch_assemble(Ctx & c, const double * rho, double k, double * vals, bool unitRho);
+void launch_fill_matrix(Ctx & c, int mass, const double * a, double * vals);
+void launch_map_model(Ctx & c, const double * model, int nModel, double * rho);
+void launch_rho_source(Ctx & c, const double * rho);
+void launch_primary(Ctx & c);
+void launch_sr_rhs(Ctx & c);
+void launch_diag_inv(Ctx & c);
+void launch_total_potential(Ctx & c);
+void launch_response(Ctx & c, double * rhoa);
+void launch_transpose_pots(Ctx & c, double * out);
+void launch_jacobian(Ctx & c, const double * model, int nModel, void * J, int dtype);
+
+// solvers (solver.hip); return per-solve statistics
+void solve_pcg_fp64(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st);
+void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bool mixed);
+double residual_check(Ctx & c);
+
+inline int div_up(long a, long b) { return (int)((a + b - 1) / b); }
+
+}  // namespace pgert

@@ -1,0 +1,322 @@
+"""ERT forward operator with pyGIMLi's ERTModelling API on MI355X.
+
+Mirrors pygimli/physics/ert/ertModelling.py:73-246 (ERTModelling): the
+reference wraps the C++ DCSRMultiElectrodeModelling core
+(core/src/bert/dcfemmodelling.cpp); here the core is libpgert.so (HIP kernels
+behind the C-ABI of include/pgert.h).  Device memory is held in torch
+tensors; all arithmetic runs in the library's kernels.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .bert import MeshAnalysis, geometric_factors, kwave_list
+from .data import DataContainerERT
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+
+
+def _dptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _host_d(a):
+    a = np.ascontiguousarray(a, np.float64)
+    return a, a.ctypes.data_as(_D)
+
+
+def _host_i(a):
+    a = np.ascontiguousarray(a, np.int32)
+    return a, a.ctypes.data_as(_I)
+
+
+class ERTModelling:
+    """Forward operator for Electrical Resistivity Tomography.
+
+    Parameters
+    ----------
+    sr : bool
+        Singularity removal (the reference default).  Only the SR operator is
+        provided.
+    verbose : bool
+        Print solver statistics.
+    device : int
+        HIP device index.
+    """
+
+    def __init__(self, sr=True, verbose=False, device=0):
+        if not sr:
+            raise NotImplementedError("only the singularity-removal operator (sr=True) is provided")
+        if not torch.cuda.is_available():
+            raise RuntimeError("ERTModelling needs a HIP device (no GPU visible)")
+        self.verbose = verbose
+        self.device = torch.device("cuda", device)
+        self._ctx = _lib.Context(device)
+        self._mesh = None
+        self._data = None
+        self._kv = None
+        self._wv = None
+        self._kUser = False
+        self._analysis = None
+        self._J = None
+        self._model = None
+        self.jacobianDtype = torch.float64
+        self.solverOptions = dict(method="mixed", rtol=1e-6, max_iter=20000, check_every=8,
+                                  inner_rtol=1e-3)
+        self.lastStats = None
+        self.warmStart = None
+        self._dirty = True
+
+    # ------------------------------------------------------------------ set-up
+    def setVerbose(self, v):
+        self.verbose = bool(v)
+
+    def setMesh(self, mesh, ignoreRegionManager=False):
+        """ModellingBase::setMesh (core/src/modellingbase.cpp:179-211)."""
+        self._mesh = mesh
+        ctx = self._ctx
+        xyz, pxyz = _host_d(mesh.positions())
+        cells, pc = _host_i(mesh.cells)
+        cm, pcm = _host_i(mesh.cellMarkers())
+        b, pb = _host_i(mesh.bounds)
+        bm, pbm = _host_i(mesh.boundaryMarkers())
+        lc, plc = _host_i(mesh.boundaryLeftCell())
+        self._set_stream()
+        ctx.check(ctx.L.pgert_set_mesh(ctx.h, mesh.dim(), mesh.nodeCount(), pxyz, mesh.cellCount(), pc, pcm,
+                                       mesh.boundaryCount(), pb, pbm, plc), "setMesh")
+        nnz = ctypes.c_int()
+        npar = ctypes.c_int()
+        ctx.L.pgert_info(ctx.h, ctypes.byref(nnz), ctypes.byref(npar))
+        self._nnz = nnz.value
+        self._nParam = npar.value
+        self._dirty = True
+        if self._data is not None:
+            self._setup_electrodes()
+
+    def mesh(self):
+        return self._mesh
+
+    def setData(self, data):
+        self._data = data
+        self._dirty = True
+        if self._mesh is not None:
+            self._setup_electrodes()
+
+    def data(self):
+        return self._data
+
+    def setkValues(self, k):
+        self._kv = np.asarray(k, np.float64)
+        self._kUser = True
+        self._dirty = True
+
+    def setWeights(self, w):
+        self._wv = np.asarray(w, np.float64)
+        self._kUser = True
+        self._dirty = True
+
+    def kValues(self):
+        return self._kv
+
+    def weights(self):
+        return self._wv
+
+    def _set_stream(self):
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        self._ctx.check(self._ctx.L.pgert_set_stream(self._ctx.h, ctypes.c_void_p(s)), "set_stream")
+
+    def _setup_electrodes(self):
+        mesh, data = self._mesh, self._data
+        an = MeshAnalysis(mesh, data.sensorPositions())
+        if an.topography:
+            raise NotImplementedError("topography / Neumann-only domains need numerical primary "
+                                      "potentials, which this path does not provide")
+        self._analysis = an
+        en, pen = _host_i(an.elecNode)
+        sc, psc = _host_d(an.sourceCenter)
+        ctx = self._ctx
+        ctx.check(ctx.L.pgert_set_electrodes(ctx.h, len(en), pen, psc, an.surfaceZ), "setElectrodes")
+        if not self._kUser or self._kv is None:
+            self._kv, self._wv = kwave_list(mesh.dim(), an.elecPos)
+        self._dirty = True
+
+    def _sync_setup(self):
+        if self._mesh is None or self._data is None:
+            raise RuntimeError("mesh and data must be set")
+        if not self._dirty:
+            return
+        if self._analysis is None:
+            self._setup_electrodes()
+        ctx = self._ctx
+        k, pk = _host_d(self._kv)
+        w, pw = _host_d(self._wv)
+        if len(k) != len(w):
+            raise ValueError("kValues and weights differ in length")
+        ctx.check(ctx.L.pgert_set_wavenumbers(ctx.h, len(k), pk, pw), "setWavenumbers")
+        self._ensure_k()
+        self._upload_data()
+        self._dirty = False
+
+    def _ensure_k(self):
+        """response() fills missing geometric factors analytically
+        (dcfemmodelling.cpp:1088-1098)."""
+        d = self._data
+        kf = d["k"]
+        if not d.haveData("k") or np.min(np.abs(kf)) < 1e-12:
+            d.set("k", self.calcGeometricFactor(d))
+
+    def _upload_data(self):
+        d = self._data
+        a, pa = _host_i(d["a"])
+        b, pb = _host_i(d["b"])
+        m, pm = _host_i(d["m"])
+        n, pn = _host_i(d["n"])
+        k, pk = _host_d(d["k"])
+        ctx = self._ctx
+        ctx.check(ctx.L.pgert_set_data(ctx.h, d.size(), pa, pb, pm, pn, pk), "setData")
+
+    def calcGeometricFactor(self, data=None, nModel=0):
+        """Analytic geometric factors (geometricFactors, bertMisc.cpp:131-176)."""
+        data = self._data if data is None else data
+        return geometric_factors(data.sensorPositions(), data.abmn(), self._mesh.dim())
+
+    # ------------------------------------------------------------------ model
+    def _model_tensor(self, model):
+        if isinstance(model, torch.Tensor):
+            t = model.to(device=self.device, dtype=torch.float64).contiguous()
+        else:
+            t = torch.as_tensor(np.ascontiguousarray(model, np.float64), device=self.device)
+        return t
+
+    def mapERTModel(self, model, background=-9e99):
+        """Cell resistivities from a model (dcfemmodelling.cpp:1211-1218,
+        modellingbase.cpp:401-470).  Returns a device tensor."""
+        t = self._model_tensor(model)
+        nm = t.numel()
+        C = self._mesh.cellCount()
+        if nm != C:
+            if np.any(self._mesh.cellMarkers() < 0):
+                raise NotImplementedError("background (marker -1) prolongation is not supported")
+            if self._mesh.cellMarkers().max() >= nm:
+                raise IndexError("marker >= model.size()")
+        rho = torch.empty(C, dtype=torch.float64, device=self.device)
+        self._set_stream()
+        self._ctx.check(self._ctx.L.pgert_map_model(self._ctx.h, _dptr(t), nm, _dptr(rho)), "mapERTModel")
+        return rho
+
+    # ------------------------------------------------------------------ forward
+    def _opts(self):
+        o = self.solverOptions
+        opts = _lib.SolveOpts()
+        opts.method = _lib.METHODS[o["method"]] if isinstance(o["method"], str) else int(o["method"])
+        opts.rtol = float(o["rtol"])
+        opts.max_iter = int(o["max_iter"])
+        opts.check_every = int(o["check_every"])
+        opts.inner_rtol = float(o["inner_rtol"])
+        opts.warm_start = None
+        if self.warmStart is not None:
+            ws = self.warmStart
+            if ws.dtype != torch.float64 or not ws.is_contiguous():
+                raise ValueError("warm start must be a contiguous float64 device tensor [nK][n][nE]")
+            opts.warm_start = ctypes.c_void_p(ws.data_ptr())
+        return opts
+
+    def response_device(self, model):
+        """Apparent resistivities as a float64 device tensor."""
+        self._sync_setup()
+        t = self._model_tensor(model)
+        if float(t.min()) < 1e-12:
+            raise ValueError("response for model with negative or zero resistivity is not defined")
+        rho = self.mapERTModel(t)
+        rhoa = torch.empty(self._data.size(), dtype=torch.float64, device=self.device)
+        stats = _lib.SolveStats()
+        opts = self._opts()
+        self._set_stream()
+        self._ctx.check(self._ctx.L.pgert_forward(self._ctx.h, _dptr(rho), ctypes.byref(opts), _dptr(rhoa),
+                                                  ctypes.byref(stats)), "response")
+        self.lastStats = stats.as_dict()
+        self._model = t
+        if self.verbose:
+            print("ERTModelling.response:", self.lastStats, self.timings())
+        return rhoa
+
+    def response(self, mod):
+        """Forward response (apparent resistivity), numpy float64 [nData]
+        (ERTModelling.response, ertModelling.py:204-219)."""
+        return self.response_device(mod).cpu().numpy()
+
+    def timings(self):
+        ms = (ctypes.c_double * 5)()
+        self._ctx.L.pgert_last_timings(self._ctx.h, ms)
+        return dict(zip(("assembly", "primary_rhs", "solve", "response", "jacobian"), list(ms)))
+
+    def subPotentials(self, node_major=False):
+        """Per-wavenumber unweighted total potentials of the last response:
+        rows electrode + nE * kIdx, columns nodes (subSolutions_,
+        dcfemmodelling.cpp:1681-1682); node_major=True gives [nK, n, nE]."""
+        nK, nE, n = len(self._kv), len(self._analysis.elecNode), self._mesh.nodeCount()
+        if node_major:
+            out = torch.empty((nK, n, nE), dtype=torch.float64, device=self.device)
+            self._ctx.check(self._ctx.L.pgert_potentials_node_major(self._ctx.h, _dptr(out)), "potentials")
+        else:
+            out = torch.empty((nK * nE, n), dtype=torch.float64, device=self.device)
+            self._ctx.check(self._ctx.L.pgert_potentials(self._ctx.h, _dptr(out)), "potentials")
+        return out
+
+    def solution(self):
+        """k-weighted potentials of every current electrode, [nE, n]
+        (ModellingBase::solution)."""
+        P = self.subPotentials(node_major=True)
+        w = torch.as_tensor(self._wv, device=self.device)
+        return torch.einsum("k,kne->en", w, P).contiguous()
+
+    # ------------------------------------------------------------------ Jacobian
+    def createJacobian(self, mod):
+        """Compute and store the Jacobian (ERTModelling.createJacobian,
+        ertModelling.py:221-238 -> DCMultiElectrodeModelling::createJacobian,
+        dcfemmodelling.cpp:1446-1476).  Reuses the potentials of the last
+        response() when it was computed for the same model."""
+        t = self._model_tensor(mod)
+        if self._model is None or self._model.shape != t.shape or not torch.equal(self._model, t):
+            self.response_device(t)
+        dtype = self.jacobianDtype
+        J = torch.empty((self._data.size(), self._nParam), dtype=dtype, device=self.device)
+        code = 0 if dtype == torch.float64 else 1
+        self._set_stream()
+        self._ctx.check(self._ctx.L.pgert_jacobian(self._ctx.h, _dptr(t), t.numel(), _dptr(J), code),
+                        "createJacobian")
+        self._J = J
+        return None
+
+    def jacobian(self):
+        """The Jacobian of the last createJacobian, [nData, nParam] device tensor."""
+        return self._J
+
+    @property
+    def parameterCount(self):
+        return self._nParam
+
+    def close(self):
+        self._ctx.close()
+
+
+def system_matrix(fop, model, k):
+    """System matrix of one wavenumber for a model, as scipy CSR (the matrix
+    DCSRMultiElectrodeModelling::calculateK factorises, dcfemmodelling.cpp:2179-2182)."""
+    import scipy.sparse as sp
+    fop._sync_setup()
+    rho = fop.mapERTModel(model)
+    ctx = fop._ctx
+    n = fop._mesh.nodeCount()
+    rowptr = np.empty(n + 1, np.int32)
+    colidx = np.empty(fop._nnz, np.int32)
+    ctx.check(ctx.L.pgert_pattern(ctx.h, rowptr.ctypes.data_as(_I), colidx.ctypes.data_as(_I)), "pattern")
+    vals = torch.empty(fop._nnz, dtype=torch.float64, device=fop.device)
+    fop._set_stream()
+    ctx.check(ctx.L.pgert_assemble(ctx.h, _dptr(rho), float(k), _dptr(vals)), "assemble")
+    return sp.csr_matrix((vals.cpu().numpy(), colidx, rowptr), shape=(n, n))
