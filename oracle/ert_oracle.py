@@ -507,6 +507,22 @@ class Forward:
                 U[i + ki * self.nE] = lu.solve(rhs) + prim
         return U
 
+    def solve_point_source(self, rho):
+        """Non-SR total-field potentials: unit point source at the electrode
+        node (DCMultiElectrodeModelling::calculateK_, dcfemmodelling.cpp:1755-1928
+        with ElectrodeShapeNode::assembleRHS, electrode.cpp:133-151)."""
+        rho = np.asarray(rho, np.float64)
+        n = self.mesh.nodeCount()
+        U = np.empty((len(self.k) * self.nE, n))
+        for ki, k in enumerate(self.k):
+            S = assemble(self.mesh, rho, k, self.setup.sourceCenter, self.geo)
+            lu = spla.splu(S.tocsc())
+            for i in range(self.nE):
+                rhs = np.zeros(n)
+                rhs[self.setup.elecNode[i]] = 1.0
+                U[i + ki * self.nE] = lu.solve(rhs)
+        return U
+
     def potentials(self, U=None):
         """Wavenumber sum (dcfemmodelling.cpp:1707-1729)."""
         U = self.subpots if U is None else U

@@ -1,0 +1,312 @@
+"""CPU tests: pin the FP64 oracle (oracle/ert_oracle.py) against
+  * the reference's own C++ core (golden fixtures written from oracle/_ref by
+    tests/golden/make_golden.py, and the live build when present),
+  * closed forms (A&S vs scipy Bessel, Gauss rules vs numpy, the hand
+    geometric factor of pygimli/testing/test_geometricFactors.py),
+  * FEM invariants (core/tests/unittest/testFEM.h:380-436),
+  * physics invariants (homogeneous half-space, SR with homogeneous rho,
+    reciprocity, finite differences of the forward for the Jacobian).
+"""
+import copy
+import os
+
+import numpy as np
+import pytest
+import scipy.special as ss
+
+from oracle import ert_oracle as eo
+from pygimli_amd import workloads
+from pygimli_amd.meshgen import grid_mesh_2d, grid_mesh_3d
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _gold(name):
+    return np.load(os.path.join(GOLD, name))
+
+
+# ---------------------------------------------------------------- a2 Bessel
+@pytest.mark.parametrize("f", ["i0", "i1", "k0", "k1"])
+def test_bessel_matches_reference_fixture(f):
+    g = _gold("ref_numeric.npz")
+    mine = getattr(eo, "bessel_" + f)(g["x"])
+    np.testing.assert_allclose(mine, g[f], rtol=1e-15, atol=0)
+
+
+@pytest.mark.parametrize("f,lib", [("i0", ss.i0), ("i1", ss.i1), ("k0", ss.k0), ("k1", ss.k1)])
+def test_bessel_abramowitz_stegun_accuracy(f, lib):
+    """A&S 9.8.1-9.8.8 error bounds: |eps| < ~2e-7 relative (2.2e-7 for the
+    large-argument K forms)."""
+    x = np.concatenate([np.geomspace(1e-3, 2.0, 200), np.linspace(2.0, 30.0, 200)])
+    mine = getattr(eo, "bessel_" + f)(x)
+    ref = lib(x)
+    assert np.max(np.abs(mine / ref - 1.0)) < 2.5e-7
+
+
+def test_bessel_plants_detected():
+    """A dropped term in the K0 series moves it far outside the A&S bound."""
+    x = np.geomspace(1e-3, 2.0, 50)
+    bad = (-np.log(x / 2.0) * eo.bessel_i0(x)) + (-0.57721566)
+    assert np.max(np.abs(bad / ss.k0(x) - 1.0)) > 1e-3
+
+
+# ---------------------------------------------------------------- a1 quadrature
+@pytest.mark.parametrize("n", [4, 5, 7, 8, 12, 13])
+def test_gauss_rules_match_reference_fixture(n):
+    g = _gold("ref_numeric.npz")
+    x, w = eo.gauss_legendre(0.0, 1.0, n)
+    np.testing.assert_array_equal(x, g[f"gl_x_{n}"])
+    np.testing.assert_array_equal(w, g[f"gl_w_{n}"])
+    x, w = eo.gauss_laguerre(n)
+    np.testing.assert_array_equal(x, g[f"glag_x_{n}"])
+    np.testing.assert_array_equal(w, g[f"glag_w_{n}"])
+
+
+@pytest.mark.parametrize("n", [4, 7, 13])
+def test_gauss_rules_vs_numpy(n):
+    """Newton stopped at |dz| <= 3e-6 (Legendre) / 3e-11 (Laguerre): the rules
+    agree with numpy's to that level."""
+    x, w = eo.gauss_legendre(0.0, 1.0, n)
+    xn, wn = np.polynomial.legendre.leggauss(n)
+    np.testing.assert_allclose(np.sort(x), np.sort((xn + 1) / 2), atol=1e-9)
+    # weights use P_n' at the previous Newton iterate: O(n * 3e-6) relative
+    np.testing.assert_allclose(np.sort(w), np.sort(wn / 2), rtol=1e-4)
+    x, w = eo.gauss_laguerre(n)
+    xn, wn = np.polynomial.laguerre.laggauss(n)
+    np.testing.assert_allclose(x, xn, rtol=1e-10)
+    np.testing.assert_allclose(w, wn, rtol=1e-8)
+
+
+def test_kwave_table_matches_reference_fixture():
+    g = _gold("ref_numeric.npz")
+    k, w = eo.kwave_from_range(0.5, 80.0, 13, 4)
+    np.testing.assert_allclose(k, g["kw_k"], rtol=1e-15)
+    np.testing.assert_allclose(w, g["kw_w"], rtol=1e-15)
+
+
+def test_kwave_inverts_fourier_of_k0():
+    """int_0^inf K0(k r) dk = pi / (2 r) and the weights carry 1/pi, so
+    sum_k w_k K0(k r) = 1 / (2 r): the 2.5D surface potentials K0(k r)/pi
+    sum to the 3D half-space potential 1/(2 pi r) over the electrode range."""
+    k, w = eo.kwave_from_range(0.5, 80.0, 13, 4)
+    for r in (1.0, 3.0, 10.0, 30.0):
+        approx = np.sum(w * ss.k0(k * r))
+        assert abs(approx * 2.0 * r - 1.0) < 2e-3
+
+
+# ---------------------------------------------------------------- a14 geometric factors
+def _geom_fact_hand(x, y, z):
+    """Hand formula of pygimli/testing/test_geometricFactors.py:6-24."""
+    dd1 = np.zeros(4)
+    dd2 = np.zeros(4)
+    for i, (c, p) in enumerate([[0, 2], [0, 3], [1, 2], [1, 3]]):
+        dd1[i] = np.sqrt((x[c] - x[p]) ** 2 + (y[c] - y[p]) ** 2 + (z[c] - z[p]) ** 2)
+        dd2[i] = np.sqrt((x[c] - x[p]) ** 2 + (y[c] - y[p]) ** 2 + (-z[c] - z[p]) ** 2)
+    dI = 1 / dd1 + 1 / dd2
+    return 4 * np.pi / (dI[0] - dI[1] - dI[2] + dI[3])
+
+
+@pytest.mark.parametrize("layout", ["surface2d", "surface3d", "crosshole2d", "crosshole3d"])
+def test_geometric_factors_closed_form(layout):
+    """The four sensor layouts of test_geometricFactors.py:37-63."""
+    from pygimli_amd import bert
+    if layout == "surface2d":
+        s = np.array([[i, 0, 0] for i in range(4)], float)
+    elif layout == "surface3d":
+        s = np.array([[i * 2, i, 0] for i in range(4)], float)
+    elif layout == "crosshole2d":
+        s = np.array([[1, -2, 0], [1, -3, 0], [3, -2, 0], [3, -3, 0]], float)
+    else:
+        s = np.array([[1, 2, -2], [1, 3, -3], [3, 4, -2], [3, 5, -3]], float)
+    abmn = np.array([[0, 1, 2, 3]])
+    # the reference test treats 2D (x, y) sensors as (x, z) (bertMisc.cpp:133-146)
+    s3 = s.copy()
+    if layout.endswith("2d"):
+        s3[:, 2], s3[:, 1] = s[:, 1], 0.0
+    hand = _geom_fact_hand(s3[:, 0], s3[:, 1], s3[:, 2])
+    dim = 2 if layout.endswith("2d") else 3
+    np.testing.assert_allclose(eo.geometric_factors(s, abmn, dim)[0], hand, rtol=1e-12)
+    np.testing.assert_allclose(bert.geometric_factors(s, abmn, dim)[0], hand, rtol=1e-12)
+
+
+# ---------------------------------------------------------------- a3/a4 FEM invariants
+def _tiny_meshes():
+    m2 = grid_mesh_2d(np.array([0.0, 0.7, 1.5, 3.0]), np.array([-2.0, -0.6, 0.0]))
+    m3 = grid_mesh_3d(np.array([0.0, 0.5, 1.4]), np.array([0.0, 0.8, 2.0]), np.array([-1.5, -0.4, 0.0]))
+    return [m2, m3]
+
+
+@pytest.mark.parametrize("mi", [0, 1])
+def test_element_matrix_invariants(mi):
+    """testFEM.h:380-436: stiffness rows sum to 0, mass sums to the domain size;
+    also under rotation + translation + scaling of the mesh."""
+    mesh = _tiny_meshes()[mi]
+    for transform in (False, True):
+        if transform:
+            th = 0.7
+            R = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1]])
+            if mesh.dim() == 3:
+                R = R @ np.array([[1, 0, 0], [0, np.cos(0.3), -np.sin(0.3)], [0, np.sin(0.3), np.cos(0.3)]])
+            mesh.nodes = 2.5 * mesh.nodes @ R.T + np.array([3.0, -1.0, 0.5 if mesh.dim() == 3 else 0.0])
+        K, M, vol, grads = eo.element_matrices(mesh)
+        np.testing.assert_allclose(K.sum(axis=2), 0.0, atol=1e-12 * np.abs(K).max())
+        np.testing.assert_allclose(M.sum(axis=(1, 2)), vol, rtol=1e-14)
+        S = eo.stiffness_matrix(mesh, np.ones(mesh.cellCount()))
+        Mm = eo.mass_matrix(mesh, np.ones(mesh.cellCount()))
+        np.testing.assert_allclose(np.asarray(S.sum(axis=1)).ravel(), 0.0, atol=1e-12 * abs(S).max())
+        assert abs(Mm.sum() - vol.sum()) < 1e-12 * vol.sum()
+        # a linear function has zero stiffness energy against constants and
+        # exact energy sum_c vol |grad f|^2 = |grad f|^2 * volume
+        g = np.array([0.3, -1.1, 0.7])[:mesh.dim()]
+        f = mesh.nodes[:, :mesh.dim()] @ g
+        np.testing.assert_allclose(f @ (S @ f), (g @ g) * vol.sum(), rtol=1e-12)
+
+
+def test_reference_triangle_closed_form():
+    """P1 stiffness of the unit right triangle: 1/2 [[2,-1,-1],[-1,1,0],[-1,0,1]]."""
+    from pygimli_amd.mesh import Mesh
+    m = Mesh(2, [[0, 0], [1, 0], [0, 1]], [[0, 1, 2]])
+    K, M, vol, _ = eo.element_matrices(m)
+    np.testing.assert_allclose(K[0], 0.5 * np.array([[2, -1, -1], [-1, 1, 0], [-1, 0, 1]]), atol=1e-15)
+    np.testing.assert_allclose(M[0], (0.5 / 12) * np.array([[2, 1, 1], [1, 2, 1], [1, 1, 2]]), atol=1e-16)
+
+
+def test_sparsity_pattern_matches_scipy_coo():
+    mesh, _, _ = workloads.small_2d()
+    ptr, idx = eo.sparsity_pattern(mesh)
+    import scipy.sparse as sp
+    c = mesh.cells.astype(np.int64)
+    r = np.repeat(c, 3, axis=1).ravel()
+    cc = np.tile(c, (1, 3)).ravel()
+    A = sp.coo_matrix((np.ones(len(r)), (r, cc)), shape=(mesh.nodeCount(),) * 2).tocsr()
+    A.sort_indices()
+    np.testing.assert_array_equal(ptr, A.indptr)
+    np.testing.assert_array_equal(idx, A.indices)
+    for i in range(mesh.nodeCount()):
+        assert np.all(np.diff(idx[ptr[i]:ptr[i + 1]]) > 0)
+
+
+# ---------------------------------------------------------------- oracle vs reference fixtures
+def _case_from_fixture(name):
+    from pygimli_amd.data import DataContainerERT
+    from pygimli_amd.mesh import Mesh
+    g = _gold(f"ref_{name}.npz")
+    mesh = Mesh(int(g["dim"]), g["nodes"], g["cells"], g["cellMarkers"], g["bounds"], g["boundMarkers"],
+                g["nodeMarkers"])
+    ab = g["abmn"]
+    data = DataContainerERT(g["sensors"], ab[:, 0], ab[:, 1], ab[:, 2], ab[:, 3])
+    return mesh, data, g["rho"], g
+
+
+@pytest.mark.parametrize("name", ["small_2d", "small_3d"])
+def test_oracle_matches_reference_fixture(name):
+    mesh, data, rho, g = _case_from_fixture(name)
+    fo = eo.Forward(mesh, data)
+    np.testing.assert_allclose(fo.k, g["kvalues"], rtol=1e-15)
+    np.testing.assert_allclose(fo.w, g["weights"], rtol=1e-15)
+    rhoa = fo.response(rho)
+    np.testing.assert_allclose(data["k"], g["k"], rtol=1e-13)
+    assert np.abs(rhoa - g["rhoa"]).max() / np.abs(g["rhoa"]).max() < 1e-11
+    assert np.abs(fo.subpots - g["U"]).max() / np.abs(g["U"]).max() < 1e-10
+    J = fo.jacobian(rho)
+    assert np.linalg.norm(J - g["J"]) / np.linalg.norm(g["J"]) < 1e-11
+    A = eo.assemble(mesh, rho, float(g["A_k"]), fo.setup.sourceCenter)
+    np.testing.assert_array_equal(A.indptr, g["A_indptr"])
+    np.testing.assert_array_equal(A.indices, g["A_indices"])
+    assert np.abs(A.data - g["A_data"]).max() / np.abs(g["A_data"]).max() < 1e-14
+
+
+def test_oracle_matches_live_reference_build():
+    from oracle import ref
+    if not ref.available():
+        pytest.skip("oracle/_ref not built")
+    mesh, data, rho = workloads.small_2d(n_elec=6, seed=7)
+    s = ref.Session(mesh, copy.deepcopy(data))
+    fo = eo.Forward(mesh, copy.deepcopy(data))
+    r1, r2 = s.response(rho), fo.response(rho)
+    assert np.abs(r1 - r2).max() / np.abs(r1).max() < 1e-11
+    J1, J2 = s.jacobian(rho), fo.jacobian(rho)
+    assert np.linalg.norm(J1 - J2) / np.linalg.norm(J1) < 1e-11
+
+
+# ---------------------------------------------------------------- physics invariants
+def test_homogeneous_halfspace_3d_exact():
+    """SR with homogeneous rho: the secondary RHS vanishes, u = rho * primary,
+    so rhoa == rho (dcfemmodelling.cpp:2253; bertMisc.cpp:209-214)."""
+    mesh, data, _ = workloads.small_3d()
+    fo = eo.Forward(mesh, data)
+    rho = np.full(mesh.cellCount(), 42.0)
+    rhoa = fo.response(rho)
+    np.testing.assert_allclose(rhoa, 42.0, rtol=1e-9)
+    P = fo.primaries()
+    np.testing.assert_allclose(fo.subpots, 42.0 * P, rtol=1e-9, atol=1e-12 * np.abs(P).max())
+
+
+def test_homogeneous_halfspace_25d_quadrature():
+    """2.5D: rhoa == rho up to the wavenumber quadrature error."""
+    mesh, data, _ = workloads.small_2d()
+    fo = eo.Forward(mesh, data)
+    rhoa = fo.response(np.full(mesh.cellCount(), 10.0))
+    assert np.max(np.abs(rhoa / 10.0 - 1.0)) < 0.02
+
+
+def test_response_homogeneous_of_degree_one():
+    """A(c rho) = A(rho) / c and the SR source S1 p does not depend on rho, so
+    the potentials, hence rhoa, scale exactly with c (up to the 1e-10 rounding
+    of the response).  (Normal and reciprocal potentials differ under SR --
+    S1 p is not a point source -- which is why the response is their geometric
+    mean, dcfemmodelling.cpp:1196.)"""
+    mesh, data, rho = workloads.small_3d()
+    fo = eo.Forward(mesh, copy.deepcopy(data))
+    r1 = fo.response(rho)
+    U1 = fo.subpots.copy()
+    r2 = fo.response(7.5 * rho)
+    np.testing.assert_allclose(fo.subpots, 7.5 * U1, rtol=1e-9, atol=1e-12 * np.abs(U1).max())
+    np.testing.assert_allclose(r2, 7.5 * r1, rtol=1e-8)
+
+
+@pytest.mark.parametrize("name", ["small_3d", "small_2d"])
+def test_jacobian_calc2_finite_differences(name):
+    """calc2 (bertJacobian.cpp:170-219) fed with point-source potentials is the
+    exact derivative of the point-source transfer resistances (S symmetric =>
+    adjoint = reciprocal field): J_ij = k_i d U_i / d rho_j, checked by central
+    differences on cells off the mixed boundary (the reference J omits the
+    boundary-term derivative).  Under SR the fed potentials are S^-1 S1 p, not
+    Green's functions, so there J is pinned by the reference fixtures instead."""
+    if name == "small_3d":
+        mesh, data, rho = workloads.small_3d(n_side=2)
+    else:
+        mesh, data, rho = workloads.small_2d(n_elec=5)
+    fo = eo.Forward(mesh, copy.deepcopy(data))
+    fo.data.set("k", fo.geometric_factors())
+    kf = fo.data["k"]
+    abmn = data.abmn()
+    U = fo.solve_point_source(rho)
+    J = eo.sensitivity_calc2(mesh, fo.geo, abmn, U, fo.k, fo.w, data.sensorCount())
+    J *= kf[:, None] / (rho * rho)[None, :]
+
+    def resp(r):
+        Ur = fo.solve_point_source(r)
+        return eo.collect_data(fo.electrode_map(fo.potentials(Ur)), abmn, False) * kf
+
+    lc = mesh.boundaryLeftCell()[mesh.boundaryMarkers() == -2]
+    interior = np.setdiff1d(np.arange(mesh.cellCount()), lc)
+    cols = interior[np.argsort(-np.abs(J[:, interior]).sum(0))[:5]]
+    for j in cols:
+        h = 1e-4 * rho[j]
+        rp = rho.copy(); rp[j] += h
+        rm = rho.copy(); rm[j] -= h
+        fd = (resp(rp) - resp(rm)) / (2 * h)
+        np.testing.assert_allclose(J[:, j], fd, rtol=1e-5, atol=1e-7 * np.abs(fd).max())
+
+
+def test_jacobian_plant_detected():
+    """A transposed electrode pairing (a<->m only) changes J beyond the test tolerance."""
+    mesh, data, rho = workloads.small_3d(n_side=2)
+    fo = eo.Forward(mesh, copy.deepcopy(data))
+    J = fo.jacobian(rho)
+    ab = data.abmn().copy()
+    ab[:, [0, 2]] = ab[:, [2, 0]]
+    Jbad = eo.sensitivity_calc2(mesh, fo.geo, ab, fo.subpots, fo.k, fo.w, data.sensorCount())
+    Jbad *= fo.data["k"][:, None] / (rho * rho)[None, :]
+    assert np.linalg.norm(Jbad - J) / np.linalg.norm(J) > 1e-2
