@@ -144,6 +144,16 @@ int pgert_jacobian(pgert_ctx * ctx, const double * model, int n_model, void * J,
  * [2] solve, [3] response, [4] jacobian. */
 int pgert_last_timings(const pgert_ctx * ctx, double * ms5);
 
+/* Per-launch timing of the solver's dominant kernel (the multi-RHS SpMM:
+ * k_bspmm_gram for the block methods, k_spmm_dot for PGERT_PCG_FP64) with HIP
+ * events on the context stream.  enable: time subsequent solves; reset: zero
+ * the accumulators.  Returns the accumulated milliseconds, number of timed
+ * launches and the compulsory HBM bytes of one launch (matrix values and
+ * indices once, every block vector read or written once).  Diagnostic entry
+ * point with no counterpart in the reference. */
+int pgert_kernel_timing(pgert_ctx * ctx, int enable, int reset, double * total_ms, long * launches,
+                        double * bytes_per_launch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,32 @@
+"""Test infrastructure only: ctypes access to oracle/_build/liboracle.so (the
+C calc2 port, CPU baseline of bench.py)."""
+import ctypes
+import os
+
+import numpy as np
+
+LIB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_build", "liboracle.so")
+
+
+def calc2(mesh, U, kvals, weights, abmn, nParam, nthreads=1, cell_range=None, J=None):
+    L = ctypes.CDLL(LIB)
+    D = ctypes.POINTER(ctypes.c_double)
+    I = ctypes.POINTER(ctypes.c_int)
+    xyz = np.ascontiguousarray(mesh.nodes, np.float64)
+    cells = np.ascontiguousarray(mesh.cells, np.int32)
+    mk = np.ascontiguousarray(mesh.cellMarkers(), np.int32)
+    U = np.ascontiguousarray(U, np.float64)
+    kv = np.ascontiguousarray(kvals, np.float64)
+    wv = np.ascontiguousarray(weights, np.float64)
+    ab = np.ascontiguousarray(abmn, np.int32)
+    if J is None:
+        J = np.zeros((len(ab), nParam))
+    c0, c1 = (0, mesh.cellCount()) if cell_range is None else cell_range
+    nE = U.shape[0] // len(kv)
+    L.oracle_calc2.argtypes = [ctypes.c_int, ctypes.c_int, D, ctypes.c_long, I, I, ctypes.c_int, ctypes.c_int, D, D,
+                               D, ctypes.c_int, I, ctypes.c_int, D, ctypes.c_int, ctypes.c_long, ctypes.c_long]
+    L.oracle_calc2(mesh.dim(), mesh.nodeCount(), xyz.ctypes.data_as(D), mesh.cellCount(), cells.ctypes.data_as(I),
+                   mk.ctypes.data_as(I), nE, len(kv), kv.ctypes.data_as(D), wv.ctypes.data_as(D),
+                   U.ctypes.data_as(D), len(ab), ab.ctypes.data_as(I), nParam, J.ctypes.data_as(D), int(nthreads),
+                   int(c0), int(c1))
+    return J

@@ -26,6 +26,7 @@ EXPORTS = (
     "pgert_set_electrodes", "pgert_set_wavenumbers", "pgert_set_data", "pgert_info",
     "pgert_pattern", "pgert_map_model", "pgert_assemble", "pgert_fill_matrix", "pgert_forward",
     "pgert_potentials", "pgert_potentials_node_major", "pgert_jacobian", "pgert_last_timings",
+    "pgert_kernel_timing",
 )
 
 
@@ -77,6 +78,8 @@ def load():
     L.pgert_potentials_node_major.argtypes = [vp, vp]
     L.pgert_jacobian.argtypes = [vp, vp, ctypes.c_int, vp, ctypes.c_int]
     L.pgert_last_timings.argtypes = [vp, _c_double_p]
+    L.pgert_kernel_timing.argtypes = [vp, ctypes.c_int, ctypes.c_int, _c_double_p,
+                                      ctypes.POINTER(ctypes.c_long), _c_double_p]
     for name in EXPORTS:
         if name not in ("pgert_destroy", "pgert_last_error"):
             getattr(L, name).restype = ctypes.c_int

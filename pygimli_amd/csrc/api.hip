@@ -510,6 +510,20 @@ int pgert_jacobian(pgert_ctx * c, const double * model, int n_model, void * J, i
     });
 }
 
+int pgert_kernel_timing(pgert_ctx * c, int enable, int reset, double * total_ms, long * launches,
+                        double * bytes_per_launch) {
+    return guarded(c, [&] {
+        if (reset) {
+            c->kTotalMs = 0.0;
+            c->kLaunches = 0;
+        }
+        c->kTiming = enable != 0;
+        if (total_ms) *total_ms = c->kTotalMs;
+        if (launches) *launches = c->kLaunches;
+        if (bytes_per_launch) *bytes_per_launch = c->kBytesPerLaunch;
+    });
+}
+
 int pgert_last_timings(const pgert_ctx * c, double * ms5) {
     if (!c || !ms5) return -2;
     for (int i = 0; i < 5; ++i) ms5[i] = c->timings[i];

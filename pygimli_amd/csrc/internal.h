@@ -129,6 +129,43 @@ struct Ctx {
 
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     double timings[5] = {0, 0, 0, 0, 0};
+
+    // optional per-launch timing of the solver's SpMM kernel (pgert_kernel_timing)
+    bool kTiming = false;
+    std::vector<hipEvent_t> kEvents;
+    int kUsed = 0;              // events used in the current solve
+    double kTotalMs = 0.0;      // accumulated SpMM time since the last reset
+    long kLaunches = 0;
+    double kBytesPerLaunch = 0; // algorithmic bytes of one SpMM launch
+    hipEvent_t kev_begin() {
+        if (!kTiming) return nullptr;
+        if (kUsed + 2 > (int)kEvents.size()) {
+            for (int i = 0; i < 256; ++i) {
+                hipEvent_t e;
+                PG_CHECK(hipEventCreate(&e));
+                kEvents.push_back(e);
+            }
+        }
+        hipEvent_t e = kEvents[kUsed++];
+        PG_CHECK(hipEventRecord(e, stream));
+        return e;
+    }
+    void kev_end() {
+        if (!kTiming) return;
+        hipEvent_t e = kEvents[kUsed++];
+        PG_CHECK(hipEventRecord(e, stream));
+    }
+    // called after the stream is synchronised
+    void kev_collect() {
+        if (!kTiming) return;
+        for (int i = 0; i + 1 < kUsed; i += 2) {
+            float t = 0;
+            PG_CHECK(hipEventElapsedTime(&t, kEvents[i], kEvents[i + 1]));
+            kTotalMs += t;
+            ++kLaunches;
+        }
+        kUsed = 0;
+    }
 };
 
 // ---- kernels / launchers (defined in the .hip files)

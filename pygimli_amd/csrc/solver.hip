@@ -267,6 +267,9 @@ void solve_pcg_fp64(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st)
     const dim3 gridR(NGROUPS, nK), blk(256);
     const dim3 gridS(div_up((long)nK * nE, 64)), blkS(64);
     const double tol2 = o.rtol * o.rtol;
+    // compulsory bytes of one k_spmm_dot launch: A values + column indices +
+    // row pointers, P read once, Q written
+    c.kBytesPerLaunch = (double)nK * ((double)c.nnz * 12.0 + 4.0 * (n + 1)) + 2.0 * (double)tot * 8.0;
     // r0 = b - A x0
     hipLaunchKernelGGL(k_spmm_dot, gridR, blk, 0, s, n, nE, c.nnz, c.rowptr.p, c.colidx.p, c.valsA.p, c.X.p,
                        c.Q.p, (double *)nullptr);
@@ -279,8 +282,10 @@ void solve_pcg_fp64(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st)
     while (!done && it < o.max_iter) {
         int chunk = std::min(o.check_every, o.max_iter - it);
         for (int j = 0; j < chunk; ++j) {
+            c.kev_begin();
             hipLaunchKernelGGL(k_spmm_dot, gridR, blk, 0, s, n, nE, c.nnz, c.rowptr.p, c.colidx.p, c.valsA.p,
                                c.P.p, c.Q.p, c.part.p);
+            c.kev_end();
             hipLaunchKernelGGL(k_pcg_reduce, gridS, blkS, 0, s, nK, nE, NGROUPS, (int)RED_ALPHA, tol2, c.part.p,
                                c.scal.p);
             hipLaunchKernelGGL(k_pcg_update, gridR, blk, 0, s, n, nE, c.scal.p, c.P.p, c.Q.p, c.dinv.p, c.X.p,
@@ -294,6 +299,7 @@ void solve_pcg_fp64(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st)
         it += chunk;
         PG_CHECK(hipMemcpyAsync(h.data(), c.scal.p, h.size() * 8, hipMemcpyDeviceToHost, s));
         PG_CHECK(hipStreamSynchronize(s));
+        c.kev_collect();
         done = true;
         for (size_t t = 0; t < (size_t)nK * nE; ++t)
             if (h[t * NS + S_ACT] != 0.0) done = false;

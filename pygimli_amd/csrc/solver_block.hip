@@ -526,6 +526,10 @@ void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bo
         using VT = decltype(vtag);
         using AT = decltype(atag);
         const bool fp32 = std::is_same<VT, float>::value;
+        // compulsory bytes of one k_bspmm_gram launch: A values + column
+        // indices + row pointers, P read once, R read, Q written
+        c.kBytesPerLaunch = (double)c.nK * ((double)c.nnz * (sizeof(AT) + 4) + 4.0 * (c.n + 1)) +
+                            3.0 * (double)tot * sizeof(VT);
         // working vectors: X (correction since last replacement), R, P, Q
         DBuf<VT> Xw, Rw, Pw, Qw;
         DBuf<double> dinv;
@@ -569,8 +573,10 @@ void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bo
         while (!done && it < o.max_iter) {
             int chunk = std::min(o.check_every, o.max_iter - it);
             for (int j = 0; j < chunk; ++j) {
+                c.kev_begin();
                 hipLaunchKernelGGL((k_bspmm_gram<VT, AT>), gG, blk, 0, s, c.n, c.nE, c.nnz, pl.d.p, c.rowptr.p,
                                    c.colidx.p, Aw, Pw.p, Rw.p, Qw.p, c.part.p);
+                c.kev_end();
                 hipLaunchKernelGGL(k_gram_reduce, dim3(div_up(2 * S * S, 256), nb), blk, 0, s, G, 2 * S * S,
                                    c.part.p, c.small.p, SL_G1 * S * S);
                 hipLaunchKernelGGL(k_small_alpha, dim3(nb), blk, 0, s, pl.d.p, c.small.p);
@@ -590,6 +596,7 @@ void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bo
                 PG_CHECK(hipMemcpyAsync(hs.data() + (size_t)b * S, c.small.p + ((size_t)b * NSL + SL_RR) * S * S,
                                         S * 8, hipMemcpyDeviceToHost, s));
             PG_CHECK(hipStreamSynchronize(s));
+            c.kev_collect();
             double worst = 0;
             for (int b = 0; b < nb; ++b)
                 for (int j = 0; j < pl.h[b].s; ++j) {

@@ -1,0 +1,118 @@
+"""CPU tests of the host side: the C-ABI library loads and exports every
+entry point include/pgert.h declares (no compute call without a GPU), the
+host set-up logic (electrode search, wavenumbers, geometric factors) agrees
+with the oracle, and the synthetic meshes are conforming."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import ert_oracle as eo
+from pygimli_amd import bert, workloads
+from pygimli_amd.data import DataContainerERT
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "pgert.h")).read()
+    return sorted(set(re.findall(r"\b(pgert_[a-z_0-9]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    import ctypes
+    from pygimli_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libpgert.so not built")
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    names = _declared()
+    assert len(names) >= 18
+    for name in names:
+        assert hasattr(L, name), name
+    assert set(names) == set(_lib.EXPORTS)
+
+
+def test_create_without_gpu_fails_cleanly():
+    import torch
+    from pygimli_amd import _lib
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libpgert.so not built")
+    with pytest.raises(RuntimeError):
+        _lib.Context(0)
+
+
+def test_ertmodelling_refuses_without_gpu():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    from pygimli_amd.ertModelling import ERTModelling
+    with pytest.raises(RuntimeError):
+        ERTModelling()
+
+
+@pytest.mark.parametrize("name", ["small_2d", "small_3d"])
+def test_electrode_search_and_wavenumbers_match_oracle(name):
+    mesh, data, rho = getattr(workloads, name)()
+    an = bert.MeshAnalysis(mesh, data.sensorPositions())
+    st = eo.Setup(mesh, data.sensorPositions())
+    np.testing.assert_array_equal(an.elecNode, st.elecNode)
+    np.testing.assert_allclose(an.sourceCenter, st.sourceCenter, rtol=1e-15)
+    assert an.surfaceZ == st.surfaceZ and an.topography == st.topography
+    k1, w1 = bert.kwave_list(mesh.dim(), an.elecPos)
+    k2, w2 = eo.kwave_list(mesh.dim(), st.elecPos)
+    np.testing.assert_array_equal(k1, k2)
+    np.testing.assert_array_equal(w1, w2)
+    g1 = bert.geometric_factors(data.sensorPositions(), data.abmn(), mesh.dim())
+    g2 = eo.geometric_factors(data.sensorPositions(), data.abmn(), mesh.dim())
+    np.testing.assert_allclose(g1, g2, rtol=1e-14)
+
+
+def test_electrode_search_2d_swap_and_errors():
+    """2D sensors given as (x, 0, z) are read as (x, z) (dcfemmodelling.cpp:859-867);
+    a sensor off every electrode node raises."""
+    mesh, data, _ = workloads.small_2d()
+    s = data.sensorPositions().copy()
+    s2 = np.stack([s[:, 0], np.zeros(len(s)), s[:, 1]], 1)
+    an = bert.MeshAnalysis(mesh, s2)
+    an0 = bert.MeshAnalysis(mesh, s)
+    np.testing.assert_array_equal(an.elecNode, an0.elecNode)
+    bad = s.copy()
+    bad[0, 0] += 0.3
+    with pytest.raises(NotImplementedError):
+        bert.MeshAnalysis(mesh, bad)
+
+
+@pytest.mark.parametrize("name", ["small_2d", "small_3d"])
+def test_meshes_are_conforming(name):
+    mesh, _, _ = getattr(workloads, name)()
+    nv = mesh.cells.shape[1]
+    faces = np.concatenate([np.sort(np.delete(mesh.cells, j, axis=1), axis=1) for j in range(nv)])
+    _, cnt = np.unique(faces, axis=0, return_counts=True)
+    assert cnt.max() == 2
+    outer = (cnt == 1).sum()
+    assert outer == mesh.boundaryCount()
+    assert (mesh.boundaryLeftCell() >= 0).all()
+    assert np.all(mesh.cellSizes() > 0)
+
+
+def test_config_sizes():
+    """The benchmarked workloads have the stated sizes."""
+    m1, d1, r1 = workloads.ert2d_layers_block()
+    assert 4500 < m1.nodeCount() < 5600 and 9000 < m1.cellCount() < 10500
+    assert len(m1.findNodesIdxByMarker(-99)) == 41
+    assert set(np.unique(r1)) == {10.0, 20.0, 100.0, 500.0}
+
+
+def test_datacontainer_api():
+    d = DataContainerERT()
+    for i in range(4):
+        d.createSensor([i, 0, 0])
+    assert d.createSensor([1.0004, 0, 0]) == 1  # within 1e-3: same sensor
+    d.resize(2)
+    d.createFourPointData(0, 0, 1, 2, 3)
+    d.createFourPointData(1, 1, -1, 2, -1)
+    assert d.size() == 2 and d.sensorCount() == 4
+    np.testing.assert_array_equal(d.abmn()[1], [1, -1, 2, -1])
