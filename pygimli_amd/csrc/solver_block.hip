@@ -252,19 +252,17 @@ __global__ void k_gram_reduce(int G, int count, const double * __restrict__ part
 
 // ---------------------------------------------------------------- small dense kernels
 // Cholesky with dropping in LDS: L lower, keep[j] = 0 for dropped pivots.
+// A pivot is dropped when the remaining diagonal falls below 1e-12 of the
+// column's own diagonal (the column is dependent on the previous ones), a
+// criterion independent of how the columns are scaled against each other.
 __device__ void chol_drop(double (*M)[S + 1], int s, int * keep) {
-    __shared__ double scale;
-    if (threadIdx.x == 0) {
-        double mx = 0;
-        for (int j = 0; j < s; ++j) mx = fmax(mx, fabs(M[j][j]));
-        scale = mx > 0 ? mx : 1e-300;
-    }
+    __shared__ double d0[S];
+    if (threadIdx.x < s) d0[threadIdx.x] = M[threadIdx.x][threadIdx.x];
     __syncthreads();
     for (int j = 0; j < s; ++j) {
-        // pivot
         if (threadIdx.x == 0) {
             double d = M[j][j];
-            if (d > 1e-13 * scale) {
+            if (d0[j] > 0.0 && d > 1e-12 * d0[j]) {
                 keep[j] = 1;
                 M[j][j] = sqrt(d);
             } else {
@@ -275,12 +273,11 @@ __device__ void chol_drop(double (*M)[S + 1], int s, int * keep) {
         __syncthreads();
         const double piv = M[j][j];
         const int kj = keep[j];
-        // column below the pivot
         for (int i = j + 1 + threadIdx.x; i < s; i += blockDim.x) M[i][j] = kj ? M[i][j] / piv : 0.0;
         __syncthreads();
-        // trailing update (lower triangle)
-        for (int t = threadIdx.x; t < (s - j - 1) * (s - j - 1); t += blockDim.x) {
-            int i = j + 1 + t / (s - j - 1), c = j + 1 + t % (s - j - 1);
+        const int m = s - j - 1;
+        for (int t = threadIdx.x; t < m * m; t += blockDim.x) {
+            int i = j + 1 + t / m, c = j + 1 + t % m;
             if (c <= i) M[i][c] -= M[i][j] * M[c][j];
         }
         __syncthreads();
@@ -360,27 +357,24 @@ __global__ void __launch_bounds__(256) k_small_beta(const BlockDesc * __restrict
     }
     chol_drop(W, s, keep2);
     // T = L^-T.  Thread c forms column c of L^-1 by forward substitution
-    // (x_i = 0 for i < c and for dropped pivots); that column is row c of T.
+    // (x_i = 0 for i < c and for dropped pivots) into Pz (free after the W
+    // update); that column is row c of T.
     {
         int c = threadIdx.x;
         if (c < s) {
-            double x[S];
             for (int i = 0; i < s; ++i) {
-                if (i < c || !keep2[i]) { x[i] = 0.0; continue; }
+                if (i < c || !keep2[i]) { Pz[c][i] = 0.0; continue; }
                 double v = (i == c) ? 1.0 : 0.0;
-                for (int q = c; q < i; ++q) v -= W[i][q] * x[q];
-                x[i] = v / W[i][i];
+                for (int q = c; q < i; ++q) v -= W[i][q] * Pz[c][q];
+                Pz[c][i] = v / W[i][i];
             }
-            for (int i = 0; i < s; ++i) L[c][i] = x[i];
         }
     }
     __syncthreads();
-    // L now holds T
     for (int t = threadIdx.x; t < S * S; t += blockDim.x) {
         int i = t / S, j = t % S;
         bool in = i < s && j < s;
-        double tv = in ? L[i][j] : 0.0;
-        sm[SL_T * S * S + t] = tv;
+        sm[SL_T * S * S + t] = in ? Pz[i][j] : 0.0;
     }
     __syncthreads();
     // M2 = beta T = -Bt T
@@ -388,7 +382,7 @@ __global__ void __launch_bounds__(256) k_small_beta(const BlockDesc * __restrict
         int i = t / S, j = t % S;
         double v = 0.0;
         if (i < s && j < s && !init)
-            for (int q = 0; q < s; ++q) v -= Bt[i][q] * L[q][j];
+            for (int q = 0; q < s; ++q) v -= Bt[i][q] * Pz[q][j];
         sm[SL_M2 * S * S + t] = v;
     }
 }
@@ -553,7 +547,7 @@ void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bo
             for (int j = 0; j < S; ++j) bnorm2[b * S + j] = hn[(size_t)b * 2 * S + S + j];
         double rel = max_rel(hn, nullptr);
         double lastRel = rel;
-        int it = 0, refine = 0;
+        int it = 0, refine = 0, lastReplace = 0;
         const dim3 gG(G, nb), blk(256);
         const int cQZ = 3 * S * S + S;
         auto start_block = [&]() {
@@ -603,11 +597,14 @@ void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bo
                     double bb = bnorm2[b * S + j];
                     if (bb > 0) worst = std::max(worst, std::sqrt(hs[(size_t)b * S + j] / bb));
                 }
-            bool replace = fp32 && (worst < 0.1 * lastRel || worst <= o.rtol);
+            // replace when the FP32 recursion has gained a decade, reached the
+            // target, or has run 256 iterations since the last replacement
+            bool replace = fp32 && (worst < 0.1 * lastRel || worst <= o.rtol || it - lastReplace >= 256);
             if (!fp32 && worst <= o.rtol) {
                 done = true;
             }
             if (replace) {
+                lastReplace = it;
                 // residual replacement: fold the FP32 correction into the FP64 solution
                 hipLaunchKernelGGL(k_accumulate, dim3(div_up((long)c.n * S, 256), nb), blk, 0, s, c.n, c.nE, pl.d.p,
                                    (float *)Xp, c.X.p);

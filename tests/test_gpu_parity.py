@@ -77,3 +77,63 @@ def test_forward_and_jacobian(name, method, oracle_results):
     jerr = np.linalg.norm(J - J_o) / np.linalg.norm(J_o)
     print("J rel fro", jerr)
     assert jerr < 1e-5
+
+
+def test_config1_2d_vs_oracle():
+    """2.5D workload at full size (5203 nodes, 10080 triangles, 41 electrodes,
+    888 dipole-dipole data, 11 wavenumbers set explicitly) against the oracle."""
+    from pygimli_amd import bert
+    from pygimli_amd.ertModelling import ERTModelling
+    mesh, data, rho = workloads.ert2d_layers_block()
+    an = bert.MeshAnalysis(mesh, data.sensorPositions())
+    d = np.sqrt(((an.elecPos[:, None] - an.elecPos[None]) ** 2).sum(-1))
+    k, w = bert.kwave_from_range(d[d > 0].min() / 2, d.max() * 2, 7, 4)
+    fo = eo.Forward(mesh, copy.deepcopy(data), k, w)
+    rhoa_o = fo.response(rho)
+    J_o = fo.jacobian(rho)
+    fop = ERTModelling()
+    fop.setMesh(mesh)
+    fop.setData(copy.deepcopy(data))
+    fop.setkValues(k)
+    fop.setWeights(w)
+    fop.solverOptions.update(method="mixed", rtol=1e-8)
+    rhoa = fop.response(rho)
+    print(fop.lastStats, fop.timings())
+    assert fop.lastStats["converged"] == 1
+    assert np.abs(rhoa - rhoa_o).max() / np.abs(rhoa_o).max() < 1e-6
+    fop.createJacobian(rho)
+    J = fop.jacobian().cpu().numpy()
+    assert np.linalg.norm(J - J_o) / np.linalg.norm(J_o) < 1e-5
+
+
+def test_config2_3d_invariants():
+    """3D workload at full size (201898 nodes, 1150488 tets, 64 sources,
+    2000 data): homogeneous model gives rhoa == rho; the GRF model converges
+    to the FP64 residual target with every method, the mixed solution agrees
+    with the FP64 block solution, and rhoa is homogeneous of degree one."""
+    import torch
+    from pygimli_amd.ertModelling import ERTModelling
+    mesh, data, rho = workloads.ert3d_grid_grf()
+    fop = ERTModelling()
+    fop.setMesh(mesh)
+    fop.setData(copy.deepcopy(data))
+    r = fop.response(np.full(mesh.cellCount(), 30.0))
+    np.testing.assert_allclose(r, 30.0, rtol=1e-9)
+    fop.solverOptions.update(method="block64", rtol=1e-10)
+    r64 = fop.response(rho)
+    U64 = fop.subPotentials(node_major=True).clone()
+    print("block64", fop.lastStats, fop.timings())
+    assert fop.lastStats["converged"] == 1
+    fop.solverOptions.update(method="mixed", rtol=1e-6)
+    r32 = fop.response(rho)
+    U32 = fop.subPotentials(node_major=True)
+    print("mixed", fop.lastStats, fop.timings())
+    assert fop.lastStats["converged"] == 1 and fop.lastStats["max_rel_residual"] <= 1e-6
+    err = float(torch.linalg.norm(U32 - U64) / torch.linalg.norm(U64))
+    print("mixed vs fp64 potentials", err)
+    assert err < 1e-6
+    assert np.abs(r32 - r64).max() / np.abs(r64).max() < 1e-6
+    fop.solverOptions.update(method="mixed", rtol=1e-8)
+    r2 = fop.response(2.0 * rho)
+    fop.response(rho)
+    np.testing.assert_allclose(r2, 2.0 * fop.response(rho), rtol=1e-6)
