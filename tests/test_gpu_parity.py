@@ -118,7 +118,8 @@ def test_config2_3d_invariants():
     fop.setMesh(mesh)
     fop.setData(copy.deepcopy(data))
     r = fop.response(np.full(mesh.cellCount(), 30.0))
-    np.testing.assert_allclose(r, 30.0, rtol=1e-9)
+    # potentials are rounded to 1e-10 before the ratio (vector.h:992-1000)
+    np.testing.assert_allclose(r, 30.0, rtol=1e-7)
     fop.solverOptions.update(method="block64", rtol=1e-10)
     r64 = fop.response(rho)
     U64 = fop.subPotentials(node_major=True).clone()
