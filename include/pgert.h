@@ -48,6 +48,7 @@ typedef struct pgert_solve_opts {
     int check_every;     /* convergence is read back every check_every iterations */
     double inner_rtol;   /* PGERT_BLOCK_MIXED: inner FP32 reduction per refinement step */
     const double * warm_start; /* optional DEVICE pointer [nK][n][nE] secondary-field guess, NULL = zero */
+    int learned_warm_start;    /* 1: start from the learned network's prediction (pgert_mlp_*) */
 } pgert_solve_opts;
 
 typedef struct pgert_solve_stats {
@@ -143,6 +144,27 @@ int pgert_jacobian(pgert_ctx * ctx, const double * model, int n_model, void * J,
  * (HIP events on the context stream): [0] assembly, [1] primary + SR rhs,
  * [2] solve, [3] response, [4] jacobian. */
 int pgert_last_timings(const pgert_ctx * ctx, double * ms5);
+
+/* Learned warm start (no counterpart in the reference, which always starts
+ * its direct solves from scratch): a per-(node, source) MLP 32-128-128-128-1,
+ * GELU, BF16 weights on the matrix cores with FP32 accumulation, predicting
+ * the relative secondary potential u_s / (rho_s p).  Parameters are a packed
+ * FP32 vector of pgert_mlp_param_count() floats: W1[32][128], b1[128],
+ * W2[128][128], b2, W3[128][128], b3, w4[128], b4 (row-major [in][out]). */
+int pgert_mlp_param_count(void);
+int pgert_mlp_set_params(pgert_ctx * ctx, const float * params, int count);   /* HOST */
+int pgert_mlp_get_params(pgert_ctx * ctx, float * params, int count);         /* HOST */
+/* Features [count][32] and targets [count] (NULL: none) of samples
+ * idx = node * nE + e of wavenumber k_index, from the last forward.  DEVICE. */
+int pgert_mlp_collect(pgert_ctx * ctx, int k_index, const long * idx, long count, float * feat,
+                      float * target);
+/* Network output for features [count][32].  DEVICE. */
+int pgert_mlp_predict(pgert_ctx * ctx, const float * feat, int count, float * out);
+/* Loss 0.5 mean((y - t)^2) and its gradient on a batch (count a multiple of
+ * 16, DEVICE features/targets); apply = 1 takes one Adam step with rate lr.
+ * grad: optional HOST array of pgert_mlp_param_count() floats. */
+int pgert_mlp_train_step(pgert_ctx * ctx, const float * feat, const float * target, int count, float lr,
+                         int apply, float * loss, float * grad);
 
 /* Per-launch timing of the solver's dominant kernel (the multi-RHS SpMM:
  * k_bspmm_gram for the block methods, k_spmm_dot for PGERT_PCG_FP64) with HIP

@@ -60,7 +60,10 @@ def forward(X, W, b, round_bf16=True):
 
 
 def backward(X, y_target, W, b, round_bf16=True):
-    """Gradients of 0.5 * mean((y - t)^2) w.r.t. W, b (FP64 accumulation)."""
+    """Gradients of 0.5 * mean((y - t)^2) w.r.t. W, b.  The gradient flowing
+    into a hidden layer through W3 / W2 is rounded to bf16 first (it is the
+    matrix-core operand), the one through the 128 -> 1 head is not; weight
+    gradients use the bf16 layer inputs and FP32/FP64 gradients."""
     y, pre, acts = forward(X, W, b, round_bf16)
     N = len(y)
     q = bf16 if round_bf16 else (lambda a: np.asarray(a, np.float32))
@@ -71,7 +74,25 @@ def backward(X, y_target, W, b, round_bf16=True):
         gW[li] = acts[li].astype(np.float64).T @ g
         gb[li] = g.sum(0)
         if li > 0:
-            gh = g @ q(W[li]).astype(np.float64).T
-            g = gh * gelu_grad(pre[li - 1].astype(np.float64))
+            gin = g if li == len(W) - 1 else q(g.astype(np.float32)).astype(np.float64)
+            gh = gin @ q(W[li]).astype(np.float64).T
+            g = (gh * gelu_grad(pre[li - 1].astype(np.float64))).astype(np.float32).astype(np.float64)
     loss = 0.5 * np.mean((y - y_target) ** 2)
     return loss, gW, gb
+
+
+def pack(W, b):
+    """Packed FP32 layout of pgert_mlp_set_params: W1 b1 W2 b2 W3 b3 w4 b4."""
+    parts = []
+    for Wi, bi in zip(W, b):
+        parts += [np.asarray(Wi, np.float32).ravel(), np.asarray(bi, np.float32).ravel()]
+    return np.concatenate(parts)
+
+
+def unpack(p):
+    W, b = [], []
+    o = 0
+    for fi, fo in zip(SIZES[:-1], SIZES[1:]):
+        W.append(p[o:o + fi * fo].reshape(fi, fo)); o += fi * fo
+        b.append(p[o:o + fo]); o += fo
+    return W, b

@@ -26,14 +26,15 @@ EXPORTS = (
     "pgert_set_electrodes", "pgert_set_wavenumbers", "pgert_set_data", "pgert_info",
     "pgert_pattern", "pgert_map_model", "pgert_assemble", "pgert_fill_matrix", "pgert_forward",
     "pgert_potentials", "pgert_potentials_node_major", "pgert_jacobian", "pgert_last_timings",
-    "pgert_kernel_timing",
+    "pgert_kernel_timing", "pgert_mlp_param_count", "pgert_mlp_set_params", "pgert_mlp_get_params",
+    "pgert_mlp_collect", "pgert_mlp_predict", "pgert_mlp_train_step",
 )
 
 
 class SolveOpts(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("rtol", ctypes.c_double), ("max_iter", ctypes.c_int),
                 ("check_every", ctypes.c_int), ("inner_rtol", ctypes.c_double),
-                ("warm_start", ctypes.c_void_p)]
+                ("warm_start", ctypes.c_void_p), ("learned_warm_start", ctypes.c_int)]
 
 
 class SolveStats(ctypes.Structure):
@@ -80,6 +81,13 @@ def load():
     L.pgert_last_timings.argtypes = [vp, _c_double_p]
     L.pgert_kernel_timing.argtypes = [vp, ctypes.c_int, ctypes.c_int, _c_double_p,
                                       ctypes.POINTER(ctypes.c_long), _c_double_p]
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.pgert_mlp_param_count.argtypes = []
+    L.pgert_mlp_set_params.argtypes = [vp, fp, ctypes.c_int]
+    L.pgert_mlp_get_params.argtypes = [vp, fp, ctypes.c_int]
+    L.pgert_mlp_collect.argtypes = [vp, ctypes.c_int, vp, ctypes.c_long, vp, vp]
+    L.pgert_mlp_predict.argtypes = [vp, vp, ctypes.c_int, vp]
+    L.pgert_mlp_train_step.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_float, ctypes.c_int, fp, fp]
     for name in EXPORTS:
         if name not in ("pgert_destroy", "pgert_last_error"):
             getattr(L, name).restype = ctypes.c_int

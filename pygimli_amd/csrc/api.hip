@@ -322,6 +322,25 @@ int pgert_set_electrodes(pgert_ctx * c, int n_elec, const int * elec_node,
         c->ecCell.upload(lst, s);
         for (int q = 0; q < 3; ++q) c->srcCenter[q] = source_center[q];
         c->surfaceZ = surface_z;
+        // feature length scale: 4 x median nearest-neighbour electrode distance
+        std::vector<double> nn;
+        for (int e = 0; e < n_elec; ++e) {
+            double best = DBL_MAX;
+            for (int f = 0; f < n_elec; ++f) {
+                if (f == e) continue;
+                double d2 = 0;
+                for (int q = 0; q < 3; ++q) {
+                    double d = pos[3 * e + q] - pos[3 * f + q];
+                    d2 += d * d;
+                }
+                if (d2 > 0) best = std::min(best, std::sqrt(d2));
+            }
+            if (best < DBL_MAX) nn.push_back(best);
+        }
+        if (!nn.empty()) {
+            std::nth_element(nn.begin(), nn.begin() + nn.size() / 2, nn.end());
+            c->featureScale = (float)(4.0 * nn[nn.size() / 2]);
+        }
         c->s1Valid = c->primValid = c->haveU = false;
         sync(*c);
     });
@@ -418,6 +437,7 @@ int pgert_forward(pgert_ctx * c, const double * rho, const pgert_solve_opts * op
             o.check_every = 8;
             o.inner_rtol = 1e-3;
             o.warm_start = nullptr;
+            o.learned_warm_start = 0;
         }
         PG_REQUIRE(o.rtol > 0 && o.max_iter > 0, "pgert_forward: bad solver options");
         if (o.check_every <= 0) o.check_every = 8;
@@ -451,6 +471,8 @@ int pgert_forward(pgert_ctx * c, const double * rho, const pgert_solve_opts * op
         c->X.alloc(nk * nn * ne);
         if (o.warm_start)
             PG_CHECK(hipMemcpyAsync(c->X.p, o.warm_start, c->X.bytes(), hipMemcpyDeviceToDevice, s));
+        else if (o.learned_warm_start)
+            mlp_warm_start(*c);
         else
             PG_CHECK(hipMemsetAsync(c->X.p, 0, c->X.bytes(), s));
         PG_CHECK(hipEventRecord(c->ev[2], s));
@@ -507,6 +529,48 @@ int pgert_jacobian(pgert_ctx * c, const double * model, int n_model, void * J, i
         float t;
         PG_CHECK(hipEventElapsedTime(&t, c->ev[4], c->ev[5]));
         c->timings[4] = t;
+    });
+}
+
+int pgert_mlp_param_count(void) { return mlp_param_count(); }
+
+int pgert_mlp_set_params(pgert_ctx * c, const float * params, int count) {
+    return guarded(c, [&] {
+        PG_REQUIRE(params && count == mlp_param_count(), "pgert_mlp_set_params: wrong parameter count");
+        mlp_set_params(*c, params);
+        sync(*c);
+    });
+}
+
+int pgert_mlp_get_params(pgert_ctx * c, float * params, int count) {
+    return guarded(c, [&] {
+        PG_REQUIRE(params && count == mlp_param_count(), "pgert_mlp_get_params: wrong parameter count");
+        mlp_get_params(*c, params);
+    });
+}
+
+int pgert_mlp_collect(pgert_ctx * c, int k_index, const long * idx, long count, float * feat, float * target) {
+    return guarded(c, [&] {
+        PG_REQUIRE(k_index >= 0 && k_index < c->nK, "pgert_mlp_collect: bad wavenumber index");
+        PG_REQUIRE(count > 0 && idx && feat, "pgert_mlp_collect: bad arguments");
+        mlp_collect(*c, k_index, idx, count, feat, target);
+        sync(*c);
+    });
+}
+
+int pgert_mlp_predict(pgert_ctx * c, const float * feat, int count, float * out) {
+    return guarded(c, [&] {
+        PG_REQUIRE(count > 0 && feat && out, "pgert_mlp_predict: bad arguments");
+        mlp_predict(*c, feat, count, out);
+        sync(*c);
+    });
+}
+
+int pgert_mlp_train_step(pgert_ctx * c, const float * feat, const float * target, int count, float lr, int apply,
+                         float * loss, float * grad) {
+    return guarded(c, [&] {
+        PG_REQUIRE(feat && target && loss, "pgert_mlp_train_step: bad arguments");
+        *loss = mlp_train_step(*c, feat, target, count, lr, apply != 0, grad);
     });
 }
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -23,6 +24,8 @@
     } while (0)
 
 namespace pgert {
+
+struct MlpState;
 
 // Owned device allocation.
 template <class T> struct DBuf {
@@ -97,6 +100,7 @@ struct Ctx {
     DBuf<int> ecPtr, ecCell; // cells around each electrode node
     double srcCenter[3] = {0, 0, 0};
     double surfaceZ = 0;
+    float featureScale = 1.0f;  // length scale of the warm-start features
 
     // ---- wavenumbers
     int nK = 0;
@@ -126,6 +130,8 @@ struct Ctx {
     DBuf<double> part;      // reduction partials
     DBuf<double> scal;      // per-column scalars
     DBuf<double> small;     // block-CG small matrices
+
+    std::shared_ptr<MlpState> mlp;  // learned warm start (mlp.hip)
 
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     double timings[5] = {0, 0, 0, 0, 0};
@@ -187,6 +193,16 @@ void launch_jacobian(Ctx & c, const double * model, int nModel, void * J, int dt
 void solve_pcg_fp64(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st);
 void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bool mixed);
 double residual_check(Ctx & c);
+
+// learned warm start (mlp.hip)
+int mlp_param_count();
+void mlp_set_params(Ctx & c, const float * host);
+void mlp_get_params(Ctx & c, float * host);
+void mlp_warm_start(Ctx & c);
+void mlp_collect(Ctx & c, int kk, const long * idx, long count, float * feat, float * target);
+void mlp_predict(Ctx & c, const float * feat, int nsamp, float * out);
+float mlp_train_step(Ctx & c, const float * feat, const float * target, int nsamp, float lr, bool apply,
+                     float * grad_host);
 
 inline int div_up(long a, long b) { return (int)((a + b - 1) / b); }
 

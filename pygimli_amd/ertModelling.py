@@ -67,7 +67,7 @@ class ERTModelling:
         self._model = None
         self.jacobianDtype = torch.float64
         self.solverOptions = dict(method="mixed", rtol=1e-6, max_iter=20000, check_every=8,
-                                  inner_rtol=1e-3)
+                                  inner_rtol=1e-3, learned=False)
         self.lastStats = None
         self.warmStart = None
         self._dirty = True
@@ -219,6 +219,7 @@ class ERTModelling:
         opts.check_every = int(o["check_every"])
         opts.inner_rtol = float(o["inner_rtol"])
         opts.warm_start = None
+        opts.learned_warm_start = 1 if o.get("learned", False) else 0
         if self.warmStart is not None:
             ws = self.warmStart
             if ws.dtype != torch.float64 or not ws.is_contiguous():
