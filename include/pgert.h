@@ -49,6 +49,7 @@ typedef struct pgert_solve_opts {
     double inner_rtol;   /* PGERT_BLOCK_MIXED: inner FP32 reduction per refinement step */
     const double * warm_start; /* optional DEVICE pointer [nK][n][nE] secondary-field guess, NULL = zero */
     int learned_warm_start;    /* 1: start from the learned network's prediction (pgert_mlp_*) */
+    int coarse;                /* 1: two-level preconditioner (block methods, see pgert_set_coarse) */
 } pgert_solve_opts;
 
 typedef struct pgert_solve_stats {
@@ -94,6 +95,12 @@ int pgert_set_wavenumbers(pgert_ctx * ctx, int nk, const double * k, const doubl
  * (DataContainerERT tokens). */
 int pgert_set_data(pgert_ctx * ctx, int n_data, const int * a, const int * b,
                    const int * m, const int * n, const double * kfac);
+
+/* Two-level preconditioner for the block solvers (no counterpart in the
+ * reference, whose direct solver needs none): n_agg aggregates (rounded down
+ * to a power of two, at most n/4) by recursive coordinate bisection of the
+ * nodes; 0 switches it off.  Used by pgert_forward when opts.coarse != 0. */
+int pgert_set_coarse(pgert_ctx * ctx, int n_agg);
 
 /* Query sizes: nnz of the CSR pattern, number of parameters (max marker + 1). */
 int pgert_info(const pgert_ctx * ctx, int * nnz, int * n_param);

@@ -27,14 +27,14 @@ EXPORTS = (
     "pgert_pattern", "pgert_map_model", "pgert_assemble", "pgert_fill_matrix", "pgert_forward",
     "pgert_potentials", "pgert_potentials_node_major", "pgert_jacobian", "pgert_last_timings",
     "pgert_kernel_timing", "pgert_mlp_param_count", "pgert_mlp_set_params", "pgert_mlp_get_params",
-    "pgert_mlp_collect", "pgert_mlp_predict", "pgert_mlp_train_step",
+    "pgert_mlp_collect", "pgert_mlp_predict", "pgert_mlp_train_step", "pgert_set_coarse",
 )
 
 
 class SolveOpts(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("rtol", ctypes.c_double), ("max_iter", ctypes.c_int),
                 ("check_every", ctypes.c_int), ("inner_rtol", ctypes.c_double),
-                ("warm_start", ctypes.c_void_p), ("learned_warm_start", ctypes.c_int)]
+                ("warm_start", ctypes.c_void_p), ("learned_warm_start", ctypes.c_int), ("coarse", ctypes.c_int)]
 
 
 class SolveStats(ctypes.Structure):
@@ -82,6 +82,7 @@ def load():
     L.pgert_kernel_timing.argtypes = [vp, ctypes.c_int, ctypes.c_int, _c_double_p,
                                       ctypes.POINTER(ctypes.c_long), _c_double_p]
     fp = ctypes.POINTER(ctypes.c_float)
+    L.pgert_set_coarse.argtypes = [vp, ctypes.c_int]
     L.pgert_mlp_param_count.argtypes = []
     L.pgert_mlp_set_params.argtypes = [vp, fp, ctypes.c_int]
     L.pgert_mlp_get_params.argtypes = [vp, fp, ctypes.c_int]

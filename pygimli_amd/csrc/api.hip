@@ -138,6 +138,8 @@ int pgert_set_mesh(pgert_ctx * c, int dim, int n_nodes, const double * xyz, int 
         c->uniqueMarkers = uniq;
 
         hipStream_t s = c->stream;
+        c->hXyz.assign(xyz, xyz + (size_t)n_nodes * 3);
+        c->nAgg = 0;
         c->xyz.upload(xyz, (size_t)n_nodes * 3, s);
         c->cells.upload(cells, (size_t)n_cells * c->nv, s);
         c->cellMarker.upload(cell_marker, n_cells, s);
@@ -382,6 +384,81 @@ int pgert_set_data(pgert_ctx * c, int n_data, const int * a, const int * b, cons
     });
 }
 
+int pgert_set_coarse(pgert_ctx * c, int n_agg) {
+    return guarded(c, [&] {
+        PG_REQUIRE(c->n > 0, "pgert_set_coarse: no mesh");
+        if (n_agg <= 0) {
+            c->nAgg = 0;
+            return;
+        }
+        // power of two, at most n / 4 aggregates
+        int m = 1;
+        while (m * 2 <= n_agg && m * 2 <= c->n / 4) m *= 2;
+        PG_REQUIRE(m >= 2, "pgert_set_coarse: mesh too small for a coarse space");
+        const int n = c->n;
+        // recursive coordinate bisection: split the longest extent at the median
+        std::vector<int> idx(n), agg(n);
+        for (int i = 0; i < n; ++i) idx[i] = i;
+        struct Job { int b, e, lo, cnt; };
+        std::vector<Job> stack{{0, n, 0, m}};
+        while (!stack.empty()) {
+            Job j = stack.back();
+            stack.pop_back();
+            if (j.cnt == 1) {
+                for (int q = j.b; q < j.e; ++q) agg[idx[q]] = j.lo;
+                continue;
+            }
+            double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+            for (int q = j.b; q < j.e; ++q)
+                for (int d = 0; d < 3; ++d) {
+                    double v = c->hXyz[3 * (size_t)idx[q] + d];
+                    lo[d] = std::min(lo[d], v);
+                    hi[d] = std::max(hi[d], v);
+                }
+            int ax = 0;
+            for (int d = 1; d < 3; ++d)
+                if (hi[d] - lo[d] > hi[ax] - lo[ax]) ax = d;
+            int mid = j.b + (j.e - j.b) / 2;
+            std::nth_element(idx.begin() + j.b, idx.begin() + mid, idx.begin() + j.e, [&](int a, int b2) {
+                double va = c->hXyz[3 * (size_t)a + ax], vb = c->hXyz[3 * (size_t)b2 + ax];
+                return va < vb || (va == vb && a < b2);
+            });
+            stack.push_back({j.b, mid, j.lo, j.cnt / 2});
+            stack.push_back({mid, j.e, j.lo + j.cnt / 2, j.cnt / 2});
+        }
+        std::vector<int> aptr(m + 1, 0), anodes(n);
+        for (int i = 0; i < n; ++i) aptr[agg[i] + 1]++;
+        for (int a = 0; a < m; ++a) aptr[a + 1] += aptr[a];
+        std::vector<int> fill(aptr.begin(), aptr.end() - 1);
+        for (int i = 0; i < n; ++i) anodes[fill[agg[i]]++] = i;
+        // coarse entries: slots grouped by (agg(row), agg(col))
+        std::vector<std::pair<long, int>> key((size_t)c->nnz);
+        for (int r = 0; r < n; ++r)
+            for (int q = c->hRowptr[r]; q < c->hRowptr[r + 1]; ++q)
+                key[q] = {(long)agg[r] * m + agg[c->hColidx[q]], q};
+        std::sort(key.begin(), key.end());
+        std::vector<int> cptr(1, 0), cslot(c->nnz), cij;
+        for (size_t q = 0; q < key.size(); ++q) {
+            cslot[q] = key[q].second;
+            if (q + 1 == key.size() || key[q + 1].first != key[q].first) {
+                cptr.push_back((int)q + 1);
+                cij.push_back((int)(key[q].first / m));
+                cij.push_back((int)(key[q].first % m));
+            }
+        }
+        hipStream_t s = c->stream;
+        c->agg.upload(agg, s);
+        c->aggPtr.upload(aptr, s);
+        c->aggNodes.upload(anodes, s);
+        c->cePtr.upload(cptr, s);
+        c->ceSlot.upload(cslot, s);
+        c->ceIJ.upload(cij, s);
+        c->nCE = (int)cij.size() / 2;
+        c->nAgg = m;
+        sync(*c);
+    });
+}
+
 int pgert_info(const pgert_ctx * c, int * nnz, int * n_param) {
     if (!c) return -2;
     if (nnz) *nnz = c->nnz;
@@ -438,6 +515,7 @@ int pgert_forward(pgert_ctx * c, const double * rho, const pgert_solve_opts * op
             o.inner_rtol = 1e-3;
             o.warm_start = nullptr;
             o.learned_warm_start = 0;
+            o.coarse = 0;
         }
         PG_REQUIRE(o.rtol > 0 && o.max_iter > 0, "pgert_forward: bad solver options");
         if (o.check_every <= 0) o.check_every = 8;

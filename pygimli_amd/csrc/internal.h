@@ -102,6 +102,16 @@ struct Ctx {
     double surfaceZ = 0;
     float featureScale = 1.0f;  // length scale of the warm-start features
 
+    // ---- two-level preconditioner (coarse.hip): aggregates and E^-1 per wavenumber
+    std::vector<double> hXyz;
+    int nAgg = 0;           // 0: off
+    DBuf<int> agg;          // node -> aggregate
+    DBuf<int> aggPtr, aggNodes;
+    int nCE = 0;            // coarse nonzeros
+    DBuf<int> cePtr, ceSlot, ceIJ;
+    DBuf<double> Einv;      // nK * m * m
+    DBuf<double> ycoarse, ccoarse;  // per block m * 64
+
     // ---- wavenumbers
     int nK = 0;
     std::vector<double> kv, wv;
@@ -131,7 +141,10 @@ struct Ctx {
     DBuf<double> scal;      // per-column scalars
     DBuf<double> small;     // block-CG small matrices
 
-    std::shared_ptr<MlpState> mlp;  // learned warm start (mlp.hip)
+    std::shared_ptr<MlpState> mlp;  // two-level preconditioner (coarse.hip)
+void coarse_prepare(Ctx & c);
+
+// learned warm start (mlp.hip)
 
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     double timings[5] = {0, 0, 0, 0, 0};
@@ -193,6 +206,9 @@ void launch_jacobian(Ctx & c, const double * model, int nModel, void * J, int dt
 void solve_pcg_fp64(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st);
 void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bool mixed);
 double residual_check(Ctx & c);
+
+// two-level preconditioner (coarse.hip)
+void coarse_prepare(Ctx & c);
 
 // learned warm start (mlp.hip)
 int mlp_param_count();

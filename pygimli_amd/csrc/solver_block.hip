@@ -44,9 +44,10 @@ constexpr int SL_KEEP = 8; // kept pivots of P'Q (first S entries)
 constexpr int NSL = 9;
 
 struct BlockDesc {
-    int k;   // wavenumber index
-    int c0;  // first electrode column
-    int s;   // width
+    int k;       // wavenumber index
+    int c0;      // first electrode column
+    int s;       // width
+    int active;  // 0 once the block has converged: its iteration kernels return at once
 };
 
 // ---------------------------------------------------------------- Gram helpers
@@ -81,6 +82,7 @@ __global__ void __launch_bounds__(256) k_bspmm_gram(int n, int nE, int nnz, cons
                                                     double * __restrict__ part) {
     __shared__ double Ps[CH][S], Qs[CH][S], Rs[CH][S];
     const BlockDesc bd = blocks[blockIdx.y];
+    if (!bd.active) return;
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const size_t voff = (size_t)bd.k * n * nE + bd.c0;
     const AT * A = vals + (size_t)bd.k * nnz;
@@ -124,10 +126,11 @@ __global__ void __launch_bounds__(256) k_bupdate(int n, int nE, const BlockDesc 
                                                  const double * __restrict__ small, const double * __restrict__ dinv,
                                                  const VT * __restrict__ P, const VT * __restrict__ Q,
                                                  VT * __restrict__ X, VT * __restrict__ R,
-                                                 double * __restrict__ part) {
+                                                 double * __restrict__ part, int grams) {
     __shared__ double Ps[CH][S], Qs[CH][S], Zs[CH][S];
     __shared__ double al[S][S];
     const BlockDesc bd = blocks[blockIdx.y];
+    if (!bd.active) return;
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const size_t voff = (size_t)bd.k * n * nE + bd.c0;
     const double * dk = dinv + (size_t)bd.k * n;
@@ -175,13 +178,15 @@ __global__ void __launch_bounds__(256) k_bupdate(int n, int nE, const BlockDesc 
             Zs[rr][lane] = z;
         }
         __syncthreads();
-        for (int rr = 0; rr < CH; ++rr) {
-            gram_acc(gqz, Qs[rr], Zs[rr], a0, b0);
-            gram_acc(gzz, Zs[rr], Zs[rr], a0, b0);
-            gram_acc(gpz, Ps[rr], Zs[rr], a0, b0);
-        }
+        if (grams)
+            for (int rr = 0; rr < CH; ++rr) {
+                gram_acc(gqz, Qs[rr], Zs[rr], a0, b0);
+                gram_acc(gzz, Zs[rr], Zs[rr], a0, b0);
+                gram_acc(gpz, Ps[rr], Zs[rr], a0, b0);
+            }
         __syncthreads();
     }
+    if (!grams) return;
     double * out = part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * (3 * S * S + S);
     gram_store(gqz, out, a0, b0);
     gram_store(gzz, out + S * S, a0, b0);
@@ -197,10 +202,12 @@ __global__ void __launch_bounds__(256) k_bupdate(int n, int nE, const BlockDesc 
 template <typename VT>
 __global__ void __launch_bounds__(256) k_bpupdate(int n, int nE, const BlockDesc * __restrict__ blocks,
                                                   const double * __restrict__ small, const double * __restrict__ dinv,
-                                                  const VT * __restrict__ R, VT * __restrict__ P) {
+                                                  const VT * __restrict__ R, VT * __restrict__ P,
+                                                  const double * __restrict__ cc, const int * __restrict__ agg, int m) {
     __shared__ double Tm[S][S], Mm[S][S];
     __shared__ double Zs[CH][S], Ps[CH][S];
     const BlockDesc bd = blocks[blockIdx.y];
+    if (!bd.active) return;
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const size_t voff = (size_t)bd.k * n * nE + bd.c0;
     const double * dk = dinv + (size_t)bd.k * n;
@@ -219,6 +226,7 @@ __global__ void __launch_bounds__(256) k_bpupdate(int n, int nE, const BlockDesc
             if (row < n && lane < bd.s) {
                 const size_t o = voff + (size_t)row * nE + lane;
                 z = dk[row] * (double)R[o];
+                if (cc) z += cc[((size_t)blockIdx.y * m + agg[row]) * S + lane];
                 p = (double)P[o];
             }
             Zs[rr][lane] = z;
@@ -237,11 +245,110 @@ __global__ void __launch_bounds__(256) k_bpupdate(int n, int nE, const BlockDesc
     }
 }
 
+// ---------------------------------------------------------------- coarse correction
+// y[b][I][c] = sum over the nodes of aggregate I of R[node][c0 + c]
+template <typename VT>
+__global__ void __launch_bounds__(256) k_restrict(int n, int nE, int m, const BlockDesc * __restrict__ blocks,
+                                                  const int * __restrict__ aggPtr, const int * __restrict__ aggNodes,
+                                                  const VT * __restrict__ R, double * __restrict__ y) {
+    const BlockDesc bd = blocks[blockIdx.y];
+    if (!bd.active) return;
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int I = blockIdx.x * 4 + wave;
+    if (I >= m) return;
+    const size_t voff = (size_t)bd.k * n * nE + bd.c0;
+    double s = 0.0;
+    if (lane < bd.s)
+        for (int q = aggPtr[I]; q < aggPtr[I + 1]; ++q) s += (double)R[voff + (size_t)aggNodes[q] * nE + lane];
+    y[((size_t)blockIdx.y * m + I) * S + lane] = s;
+}
+
+// cc[b] = Einv_k y[b]: 4 coarse rows x 64 columns per workgroup, y staged in LDS
+__global__ void __launch_bounds__(256) k_coarse_apply(int m, const BlockDesc * __restrict__ blocks,
+                                                      const double * __restrict__ Einv, const double * __restrict__ y,
+                                                      double * __restrict__ cc) {
+    __shared__ double ys[64][S];
+    const BlockDesc bd = blocks[blockIdx.y];
+    if (!bd.active) return;
+    const int r = blockIdx.x * 4 + threadIdx.x / 64, col = threadIdx.x % 64;
+    const double * E = Einv + (size_t)bd.k * m * m;
+    const double * yb = y + (size_t)blockIdx.y * m * S;
+    double acc = 0.0;
+    for (int k0 = 0; k0 < m; k0 += 64) {
+        for (int t = threadIdx.x; t < 64 * S; t += 256) {
+            int kk = k0 + t / S;
+            ys[t / S][t % S] = kk < m ? yb[(size_t)kk * S + t % S] : 0.0;
+        }
+        __syncthreads();
+        if (r < m) {
+            const double * Er = E + (size_t)r * m + k0;
+            const int kn = min(64, m - k0);
+            for (int q = 0; q < kn; ++q) acc += Er[q] * ys[q][col];
+        }
+        __syncthreads();
+    }
+    if (r < m) cc[((size_t)blockIdx.y * m + r) * S + col] = acc;
+}
+
+// Z = D^-1 R + prolong(cc); Grams Q'Z, Z'Z, P'Z and |R|^2 (k_bupdate's partial layout)
+template <typename VT>
+__global__ void __launch_bounds__(256) k_zgram(int n, int nE, int m, const BlockDesc * __restrict__ blocks,
+                                               const double * __restrict__ dinv, const VT * __restrict__ P,
+                                               const VT * __restrict__ Q, const VT * __restrict__ R,
+                                               const double * __restrict__ cc, const int * __restrict__ agg,
+                                               double * __restrict__ part) {
+    __shared__ double Ps[CH][S], Qs[CH][S], Zs[CH][S];
+    __shared__ double cn[4][S];
+    const BlockDesc bd = blocks[blockIdx.y];
+    if (!bd.active) return;
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const size_t voff = (size_t)bd.k * n * nE + bd.c0;
+    const double * dk = dinv + (size_t)bd.k * n;
+    int a0, b0;
+    tile_of(a0, b0);
+    double gqz[4][4] = {}, gzz[4][4] = {}, gpz[4][4] = {};
+    double rr2 = 0.0;
+    const int nchunk = (n + CH - 1) / CH;
+    for (int ch = blockIdx.x; ch < nchunk; ch += gridDim.x) {
+        const int base = ch * CH;
+        for (int rr = wave; rr < CH; rr += 4) {
+            const int row = base + rr;
+            double p = 0.0, q = 0.0, z = 0.0;
+            if (row < n && lane < bd.s) {
+                const size_t o = voff + (size_t)row * nE + lane;
+                p = (double)P[o];
+                q = (double)Q[o];
+                double r = (double)R[o];
+                rr2 += r * r;
+                z = dk[row] * r + cc[((size_t)blockIdx.y * m + agg[row]) * S + lane];
+            }
+            Ps[rr][lane] = p;
+            Qs[rr][lane] = q;
+            Zs[rr][lane] = z;
+        }
+        __syncthreads();
+        for (int rr = 0; rr < CH; ++rr) {
+            gram_acc(gqz, Qs[rr], Zs[rr], a0, b0);
+            gram_acc(gzz, Zs[rr], Zs[rr], a0, b0);
+            gram_acc(gpz, Ps[rr], Zs[rr], a0, b0);
+        }
+        __syncthreads();
+    }
+    double * out = part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * (3 * S * S + S);
+    gram_store(gqz, out, a0, b0);
+    gram_store(gzz, out + S * S, a0, b0);
+    gram_store(gpz, out + 2 * S * S, a0, b0);
+    cn[wave][lane] = rr2;
+    __syncthreads();
+    if (wave == 0) out[3 * S * S + lane] = cn[0][lane] + cn[1][lane] + cn[2][lane] + cn[3][lane];
+}
+
 // ---------------------------------------------------------------- reductions
 // sum over the groups of `count` doubles per (block, group) into small[block][dst ...]
-__global__ void k_gram_reduce(int G, int count, const double * __restrict__ part, double * __restrict__ small,
-                              int dstOff) {
+__global__ void k_gram_reduce(const BlockDesc * __restrict__ blocks, int G, int count,
+                              const double * __restrict__ part, double * __restrict__ small, int dstOff) {
     const int b = blockIdx.y;
+    if (!blocks[b].active) return;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count) return;
     double s = 0.0;
@@ -307,6 +414,7 @@ __device__ void chol_solve(double (*L)[S + 1], const int * keep, int s, double (
 __global__ void __launch_bounds__(256) k_small_alpha(const BlockDesc * __restrict__ blocks, double * __restrict__ small) {
     __shared__ double L[S][S + 1], Bm[S][S + 1];
     __shared__ int keep[S];
+    if (!blocks[blockIdx.x].active) return;
     const int s = blocks[blockIdx.x].s;
     double * sm = small + (size_t)blockIdx.x * NSL * S * S;
     for (int t = threadIdx.x; t < S * S; t += blockDim.x) {
@@ -332,6 +440,7 @@ __global__ void __launch_bounds__(256) k_small_beta(const BlockDesc * __restrict
                                                     int init) {
     __shared__ double L[S][S + 1], Bt[S][S + 1], W[S][S + 1], Pz[S][S + 1];
     __shared__ int keep[S], keep2[S];
+    if (!blocks[blockIdx.x].active) return;
     const int s = blocks[blockIdx.x].s;
     double * sm = small + (size_t)blockIdx.x * NSL * S * S;
     for (int t = threadIdx.x; t < S * S; t += blockDim.x) {
@@ -459,7 +568,7 @@ struct Plan {
 void make_plan(Ctx & c, Plan & pl) {
     pl.h.clear();
     for (int k = 0; k < c.nK; ++k)
-        for (int c0 = 0; c0 < c.nE; c0 += S) pl.h.push_back({k, c0, std::min(S, c.nE - c0)});
+        for (int c0 = 0; c0 < c.nE; c0 += S) pl.h.push_back({k, c0, std::min(S, c.nE - c0), 1});
     pl.d.upload(pl.h, c.stream);
     pl.G = std::max(1, std::min(MAXG, div_up(c.n, CH)));
 }
@@ -489,6 +598,13 @@ void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bo
     c.small.alloc((size_t)nb * NSL * S * S);
     PG_CHECK(hipMemsetAsync(c.small.p, 0, c.small.bytes(), s));
     c.part.alloc((size_t)nb * G * (3 * S * S + S));
+    const bool useCoarse = o.coarse && c.nAgg > 0;
+    if (useCoarse) {
+        coarse_prepare(c);
+        c.ycoarse.alloc((size_t)nb * c.nAgg * S);
+        c.ccoarse.alloc((size_t)nb * c.nAgg * S);
+        PG_CHECK(hipMemsetAsync(c.ccoarse.p, 0, c.ccoarse.bytes(), s));
+    }
     DBuf<double> norms;
     norms.alloc((size_t)nb * 2 * S);
     DBuf<double> npart;
@@ -550,16 +666,36 @@ void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bo
         int it = 0, refine = 0, lastReplace = 0;
         const dim3 gG(G, nb), blk(256);
         const int cQZ = 3 * S * S + S;
+        const int m = useCoarse ? c.nAgg : 0;
+        const double * ccp = useCoarse ? c.ccoarse.p : nullptr;
+        const int * aggp = useCoarse ? c.agg.p : nullptr;
+        // X += P alpha, R -= Q alpha, then Z = M^-1 R and the Grams Q'Z, Z'Z, P'Z, |R|^2
+        auto update_and_z = [&](bool updateXR) {
+            if (!useCoarse) {
+                hipLaunchKernelGGL(k_bupdate<VT>, gG, blk, 0, s, c.n, c.nE, pl.d.p, c.small.p, c.dinv.p, Pw.p, Qw.p,
+                                   Xp, Rw.p, c.part.p, 1);
+            } else {
+                if (updateXR)
+                    hipLaunchKernelGGL(k_bupdate<VT>, gG, blk, 0, s, c.n, c.nE, pl.d.p, c.small.p, c.dinv.p, Pw.p,
+                                       Qw.p, Xp, Rw.p, c.part.p, 0);
+                hipLaunchKernelGGL(k_restrict<VT>, dim3(div_up(m, 4), nb), blk, 0, s, c.n, c.nE, m, pl.d.p,
+                                   c.aggPtr.p, c.aggNodes.p, Rw.p, c.ycoarse.p);
+                hipLaunchKernelGGL(k_coarse_apply, dim3(div_up(m, 4), nb), blk, 0, s, m, pl.d.p, c.Einv.p,
+                                   c.ycoarse.p, c.ccoarse.p);
+                hipLaunchKernelGGL(k_zgram<VT>, gG, blk, 0, s, c.n, c.nE, m, pl.d.p, c.dinv.p, Pw.p, Qw.p, Rw.p,
+                                   c.ccoarse.p, c.agg.p, c.part.p);
+            }
+            hipLaunchKernelGGL(k_gram_reduce, dim3(div_up(cQZ, 256), nb), blk, 0, s, pl.d.p, G, cQZ, c.part.p,
+                               c.small.p, SL_QZ * S * S);
+        };
         auto start_block = [&]() {
-            // alpha = 0 -> k_bupdate only forms Z = D^-1 R and Z'Z; then P = Z chol(Z'Z)^-T
+            // alpha = 0: only Z = M^-1 R and Z'Z are formed; then P = Z chol(Z'Z)^-T
             PG_CHECK(hipMemsetAsync(c.small.p, 0, c.small.bytes(), s));
             PG_CHECK(hipMemsetAsync(Pw.p, 0, Pw.bytes(), s));
-            hipLaunchKernelGGL(k_bupdate<VT>, gG, blk, 0, s, c.n, c.nE, pl.d.p, c.small.p, c.dinv.p, Pw.p, Qw.p, Xp,
-                               Rw.p, c.part.p);
-            hipLaunchKernelGGL(k_gram_reduce, dim3(div_up(cQZ, 256), nb), blk, 0, s, G, cQZ, c.part.p, c.small.p,
-                               SL_QZ * S * S);
+            update_and_z(false);
             hipLaunchKernelGGL(k_small_beta, dim3(nb), blk, 0, s, pl.d.p, c.small.p, 1);
-            hipLaunchKernelGGL(k_bpupdate<VT>, gG, blk, 0, s, c.n, c.nE, pl.d.p, c.small.p, c.dinv.p, Rw.p, Pw.p);
+            hipLaunchKernelGGL(k_bpupdate<VT>, gG, blk, 0, s, c.n, c.nE, pl.d.p, c.small.p, c.dinv.p, Rw.p, Pw.p,
+                               ccp, aggp, m);
             PG_CHECK(hipGetLastError());
         };
         if (rel > o.rtol) start_block();
@@ -571,16 +707,13 @@ void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bo
                 hipLaunchKernelGGL((k_bspmm_gram<VT, AT>), gG, blk, 0, s, c.n, c.nE, c.nnz, pl.d.p, c.rowptr.p,
                                    c.colidx.p, Aw, Pw.p, Rw.p, Qw.p, c.part.p);
                 c.kev_end();
-                hipLaunchKernelGGL(k_gram_reduce, dim3(div_up(2 * S * S, 256), nb), blk, 0, s, G, 2 * S * S,
+                hipLaunchKernelGGL(k_gram_reduce, dim3(div_up(2 * S * S, 256), nb), blk, 0, s, pl.d.p, G, 2 * S * S,
                                    c.part.p, c.small.p, SL_G1 * S * S);
                 hipLaunchKernelGGL(k_small_alpha, dim3(nb), blk, 0, s, pl.d.p, c.small.p);
-                hipLaunchKernelGGL(k_bupdate<VT>, gG, blk, 0, s, c.n, c.nE, pl.d.p, c.small.p, c.dinv.p, Pw.p,
-                                   Qw.p, Xp, Rw.p, c.part.p);
-                hipLaunchKernelGGL(k_gram_reduce, dim3(div_up(cQZ, 256), nb), blk, 0, s, G, cQZ, c.part.p,
-                                   c.small.p, SL_QZ * S * S);
+                update_and_z(true);
                 hipLaunchKernelGGL(k_small_beta, dim3(nb), blk, 0, s, pl.d.p, c.small.p, 0);
                 hipLaunchKernelGGL(k_bpupdate<VT>, gG, blk, 0, s, c.n, c.nE, pl.d.p, c.small.p, c.dinv.p, Rw.p,
-                                   Pw.p);
+                                   Pw.p, ccp, aggp, m);
             }
             PG_CHECK(hipGetLastError());
             it += chunk;
@@ -591,29 +724,57 @@ void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bo
                                         S * 8, hipMemcpyDeviceToHost, s));
             PG_CHECK(hipStreamSynchronize(s));
             c.kev_collect();
+            // per-block worst recursive residual of the still-active blocks
+            std::vector<double> wb(nb, 0.0);
             double worst = 0;
-            for (int b = 0; b < nb; ++b)
+            bool anyAtTarget = false;
+            for (int b = 0; b < nb; ++b) {
+                if (!pl.h[b].active) continue;
                 for (int j = 0; j < pl.h[b].s; ++j) {
                     double bb = bnorm2[b * S + j];
-                    if (bb > 0) worst = std::max(worst, std::sqrt(hs[(size_t)b * S + j] / bb));
+                    if (bb > 0) wb[b] = std::max(wb[b], std::sqrt(hs[(size_t)b * S + j] / bb));
                 }
-            // replace when the FP32 recursion has gained a decade, reached the
-            // target, or has run 256 iterations since the last replacement
-            bool replace = fp32 && (worst < 0.1 * lastRel || worst <= o.rtol || it - lastReplace >= 256);
-            if (!fp32 && worst <= o.rtol) {
-                done = true;
+                worst = std::max(worst, wb[b]);
+                if (wb[b] <= o.rtol) anyAtTarget = true;
             }
-            if (replace) {
+            bool changed = false;
+            if (!fp32) {
+                // FP64 recursion: a block is done when its recursive residual meets rtol
+                for (int b = 0; b < nb; ++b)
+                    if (pl.h[b].active && wb[b] <= o.rtol) {
+                        pl.h[b].active = 0;
+                        changed = true;
+                    }
+            } else if (worst < 0.1 * lastRel || anyAtTarget || it - lastReplace >= 256) {
+                // residual replacement: fold the FP32 correction into the FP64
+                // solution; the true FP64 residual decides which blocks are done
                 lastReplace = it;
-                // residual replacement: fold the FP32 correction into the FP64 solution
                 hipLaunchKernelGGL(k_accumulate, dim3(div_up((long)c.n * S, 256), nb), blk, 0, s, c.n, c.nE, pl.d.p,
                                    (float *)Xp, c.X.p);
                 true_resid(Rw.p);
                 read_norms(c, pl, norms, hn);
-                lastRel = max_rel(hn, nullptr);
                 ++refine;
-                if (lastRel <= o.rtol) done = true;
+                lastRel = 0.0;
+                for (int b = 0; b < nb; ++b) {
+                    if (!pl.h[b].active) continue;
+                    double tb = 0.0;
+                    for (int j = 0; j < pl.h[b].s; ++j) {
+                        double rr = hn[(size_t)b * 2 * S + j], bb = hn[(size_t)b * 2 * S + S + j];
+                        if (bb > 0) tb = std::max(tb, std::sqrt(rr / bb));
+                    }
+                    if (tb <= o.rtol) {
+                        pl.h[b].active = 0;
+                        changed = true;
+                    } else {
+                        lastRel = std::max(lastRel, tb);
+                    }
+                }
             }
+            done = true;
+            for (int b = 0; b < nb; ++b)
+                if (pl.h[b].active) done = false;
+            if (changed && !done)
+                PG_CHECK(hipMemcpyAsync(pl.d.p, pl.h.data(), pl.h.size() * sizeof(BlockDesc), hipMemcpyHostToDevice, s));
         }
         if (fp32) {
             hipLaunchKernelGGL(k_accumulate, dim3(div_up((long)c.n * S, 256), nb), dim3(256), 0, s, c.n, c.nE,

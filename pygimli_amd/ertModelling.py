@@ -67,7 +67,8 @@ class ERTModelling:
         self._model = None
         self.jacobianDtype = torch.float64
         self.solverOptions = dict(method="mixed", rtol=1e-6, max_iter=20000, check_every=8,
-                                  inner_rtol=1e-3, learned=False)
+                                  inner_rtol=1e-3, learned=False, coarse=0)
+        self._coarseSet = None
         self.lastStats = None
         self.warmStart = None
         self._dirty = True
@@ -94,6 +95,7 @@ class ERTModelling:
         ctx.L.pgert_info(ctx.h, ctypes.byref(nnz), ctypes.byref(npar))
         self._nnz = nnz.value
         self._nParam = npar.value
+        self._coarseSet = None
         self._dirty = True
         if self._data is not None:
             self._setup_electrodes()
@@ -220,6 +222,11 @@ class ERTModelling:
         opts.inner_rtol = float(o["inner_rtol"])
         opts.warm_start = None
         opts.learned_warm_start = 1 if o.get("learned", False) else 0
+        nagg = int(o.get("coarse", 0) or 0)
+        if nagg != self._coarseSet:
+            self._ctx.check(self._ctx.L.pgert_set_coarse(self._ctx.h, nagg), "set_coarse")
+            self._coarseSet = nagg
+        opts.coarse = 1 if nagg > 0 else 0
         if self.warmStart is not None:
             ws = self.warmStart
             if ws.dtype != torch.float64 or not ws.is_contiguous():

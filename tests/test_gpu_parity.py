@@ -39,7 +39,10 @@ def _fop(name, method, rtol=1e-10):
     fop = ERTModelling()
     fop.setMesh(mesh)
     fop.setData(copy.deepcopy(data))
-    fop.solverOptions.update(method=method, rtol=rtol)
+    coarse = 0
+    if method.endswith("+coarse"):
+        method, coarse = method.split("+")[0], 64
+    fop.solverOptions.update(method=method, rtol=rtol, coarse=coarse)
     return fop, mesh, data, rho
 
 
@@ -58,7 +61,7 @@ def test_system_matrix(name, oracle_results):
 
 
 @pytest.mark.parametrize("name", CASES)
-@pytest.mark.parametrize("method", ["pcg64", "block64", "mixed"])
+@pytest.mark.parametrize("method", ["pcg64", "block64", "mixed", "block64+coarse", "mixed+coarse"])
 def test_forward_and_jacobian(name, method, oracle_results):
     fo, rhoa_o, J_o = oracle_results[name]
     fop, mesh, data, rho = _fop(name, method)
@@ -96,7 +99,7 @@ def test_config1_2d_vs_oracle():
     fop.setData(copy.deepcopy(data))
     fop.setkValues(k)
     fop.setWeights(w)
-    fop.solverOptions.update(method="mixed", rtol=1e-8)
+    fop.solverOptions.update(method="mixed", rtol=1e-8, coarse=1024)
     rhoa = fop.response(rho)
     print(fop.lastStats, fop.timings())
     assert fop.lastStats["converged"] == 1
