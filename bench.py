@@ -106,6 +106,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--method", default="mixed")
     ap.add_argument("--cpu-timeout", type=float, default=420.0)
+    ap.add_argument("--coarse", type=int, default=1024, help="aggregates of the two-level preconditioner (0: off)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported "
@@ -139,13 +140,13 @@ def main():
     log(f"[bench] set-up done at {time.time() - t0:.1f}s")
 
     # accurate reference potentials: FP64 block-PCG to 1e-11
-    fop.solverOptions.update(method="block64", rtol=1e-11)
+    fop.solverOptions.update(method="block64", rtol=1e-11, coarse=args.coarse)
     fop.response_device(model)
     U_ref = fop.subPotentials(node_major=True).clone()
     log(f"[bench] reference solve: {fop.lastStats} {fop.timings()}")
 
-    def run(method, steps, warmup):
-        fop.solverOptions.update(method=method, rtol=args.rtol)
+    def run(method, steps, warmup, coarse=0):
+        fop.solverOptions.update(method=method, rtol=args.rtol, coarse=coarse)
         for _ in range(warmup):
             fop.response_device(model)
             fop.createJacobian(model)
@@ -176,7 +177,7 @@ def main():
         log(f"[bench] {method}: {res}")
         return res
 
-    best = run(args.method, args.steps, args.warmup)
+    best = run(args.method, args.steps, args.warmup, args.coarse if args.method != "pcg64" else 0)
     plain = None
     if not args.no_plain:
         plain = run("pcg64", 1, 0)
@@ -198,7 +199,7 @@ def main():
         "config": {
             "workload": "3D ERT: 201898-node / 1150488-tet graded mesh, 64 electrodes (8x8 @ 2 m), "
                         "2000 random quadrupoles, log10-rho GRF (2.0, 0.5, 5 m); J 2000 x 1150488 FP32",
-            "method": args.method, "rtol": args.rtol, "nodes": mesh.nodeCount(), "cells": mesh.cellCount(),
+            "method": args.method, "rtol": args.rtol, "coarse_aggregates": args.coarse, "nodes": mesh.nodeCount(), "cells": mesh.cellCount(),
             "electrodes": 64, "data": data.size(), "rhs": 64,
         },
         "accuracy": {"potential_rel_err_fro": best["pot_err"], "potential_rel_err_max_col": best["pot_err_max_col"],

@@ -738,19 +738,17 @@ void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bo
                 if (wb[b] <= o.rtol) anyAtTarget = true;
             }
             bool changed = false;
-            if (!fp32) {
-                // FP64 recursion: a block is done when its recursive residual meets rtol
-                for (int b = 0; b < nb; ++b)
-                    if (pl.h[b].active && wb[b] <= o.rtol) {
-                        pl.h[b].active = 0;
-                        changed = true;
-                    }
-            } else if (worst < 0.1 * lastRel || anyAtTarget || it - lastReplace >= 256) {
-                // residual replacement: fold the FP32 correction into the FP64
-                // solution; the true FP64 residual decides which blocks are done
+            // Residual replacement: the true FP64 residual b - A x decides which
+            // blocks are done.  FP32 recursions are replaced when they have
+            // gained a decade, reached the target or run 256 iterations; FP64
+            // ones when a block's recursive residual reaches the target.
+            const bool replaceNow = fp32 ? (worst < 0.1 * lastRel || anyAtTarget || it - lastReplace >= 256)
+                                         : anyAtTarget;
+            if (replaceNow) {
                 lastReplace = it;
-                hipLaunchKernelGGL(k_accumulate, dim3(div_up((long)c.n * S, 256), nb), blk, 0, s, c.n, c.nE, pl.d.p,
-                                   (float *)Xp, c.X.p);
+                if (fp32)
+                    hipLaunchKernelGGL(k_accumulate, dim3(div_up((long)c.n * S, 256), nb), blk, 0, s, c.n, c.nE,
+                                       pl.d.p, (float *)Xp, c.X.p);
                 true_resid(Rw.p);
                 read_norms(c, pl, norms, hn);
                 ++refine;
