@@ -173,13 +173,22 @@ int pgert_mlp_predict(pgert_ctx * ctx, const float * feat, int count, float * ou
 int pgert_mlp_train_step(pgert_ctx * ctx, const float * feat, const float * target, int count, float lr,
                          int apply, float * loss, float * grad);
 
-/* Per-launch timing of the solver's dominant kernel (the multi-RHS SpMM:
- * k_bspmm_gram for the block methods, k_spmm_dot for PGERT_PCG_FP64) with HIP
- * events on the context stream.  enable: time subsequent solves; reset: zero
- * the accumulators.  Returns the accumulated milliseconds, number of timed
- * launches and the compulsory HBM bytes of one launch (matrix values and
- * indices once, every block vector read or written once).  Diagnostic entry
- * point with no counterpart in the reference. */
+/* Per-launch timing of one solver kernel with HIP events on the context
+ * stream.  enable selects the kernel (0: off):
+ *   PGERT_TIME_SPMM    1  sparse product Q = A P (k_spmm; k_spmm_dot for PGERT_PCG_FP64)
+ *   PGERT_TIME_UPDATE  2  X += P alpha, R -= Q alpha (k_update, block methods)
+ *   PGERT_TIME_GRAM_Z  3  Z = M^-1 R and the Grams Q'Z, Z'Z, P'Z (block methods)
+ *   PGERT_TIME_PUPDATE 4  P = Z T + P beta T (block methods)
+ *   PGERT_TIME_GRAM_PQ 5  the Grams P'Q, P'R (block methods)
+ * reset: zero the accumulators.  Returns the accumulated milliseconds, number
+ * of timed launches and the compulsory HBM bytes of one launch (matrix values
+ * and indices once, every vector the kernel reads or writes once, all blocks
+ * active).  Diagnostic entry point with no counterpart in the reference. */
+#define PGERT_TIME_SPMM 1
+#define PGERT_TIME_UPDATE 2
+#define PGERT_TIME_GRAM_Z 3
+#define PGERT_TIME_PUPDATE 4
+#define PGERT_TIME_GRAM_PQ 5
 int pgert_kernel_timing(pgert_ctx * ctx, int enable, int reset, double * total_ms, long * launches,
                         double * bytes_per_launch);
 

@@ -136,21 +136,19 @@ struct Ctx {
     DBuf<double> emap;      // nE * nE
     // solver work
     DBuf<double> B, X, R, P, Q, Z;
-    DBuf<float> fA, fX, fR, fP, fQ, fZ, fDinv;
+    DBuf<float> fA;         // FP32 copy of valsA (mixed-precision solver)
     DBuf<double> part;      // reduction partials
     DBuf<double> scal;      // per-column scalars
     DBuf<double> small;     // block-CG small matrices
 
-    std::shared_ptr<MlpState> mlp;  // two-level preconditioner (coarse.hip)
-void coarse_prepare(Ctx & c);
-
-// learned warm start (mlp.hip)
+    std::shared_ptr<MlpState> mlp;  // learned warm start (mlp.hip)
 
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     double timings[5] = {0, 0, 0, 0, 0};
 
     // optional per-launch timing of the solver's SpMM kernel (pgert_kernel_timing)
     bool kTiming = false;
+    int kSelect = 0;            // PGERT_TIME_* kernel being timed
     std::vector<hipEvent_t> kEvents;
     int kUsed = 0;              // events used in the current solve
     double kTotalMs = 0.0;      // accumulated SpMM time since the last reset

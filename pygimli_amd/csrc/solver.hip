@@ -269,7 +269,7 @@ void solve_pcg_fp64(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st)
     const double tol2 = o.rtol * o.rtol;
     // compulsory bytes of one k_spmm_dot launch: A values + column indices +
     // row pointers, P read once, Q written
-    c.kBytesPerLaunch = (double)nK * ((double)c.nnz * 12.0 + 4.0 * (n + 1)) + 2.0 * (double)tot * 8.0;
+    if (c.kSelect == PGERT_TIME_SPMM) c.kBytesPerLaunch = (double)nK * ((double)c.nnz * 12.0 + 4.0 * (n + 1)) + 2.0 * (double)tot * 8.0;
     // r0 = b - A x0
     hipLaunchKernelGGL(k_spmm_dot, gridR, blk, 0, s, n, nE, c.nnz, c.rowptr.p, c.colidx.p, c.valsA.p, c.X.p,
                        c.Q.p, (double *)nullptr);
@@ -282,10 +282,10 @@ void solve_pcg_fp64(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st)
     while (!done && it < o.max_iter) {
         int chunk = std::min(o.check_every, o.max_iter - it);
         for (int j = 0; j < chunk; ++j) {
-            c.kev_begin();
+            if (c.kSelect == PGERT_TIME_SPMM) c.kev_begin();
             hipLaunchKernelGGL(k_spmm_dot, gridR, blk, 0, s, n, nE, c.nnz, c.rowptr.p, c.colidx.p, c.valsA.p,
                                c.P.p, c.Q.p, c.part.p);
-            c.kev_end();
+            if (c.kSelect == PGERT_TIME_SPMM) c.kev_end();
             hipLaunchKernelGGL(k_pcg_reduce, gridS, blkS, 0, s, nK, nE, NGROUPS, (int)RED_ALPHA, tol2, c.part.p,
                                c.scal.p);
             hipLaunchKernelGGL(k_pcg_update, gridR, blk, 0, s, n, nE, c.scal.p, c.P.p, c.Q.p, c.dinv.p, c.X.p,

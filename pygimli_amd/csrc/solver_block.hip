@@ -4,19 +4,37 @@
 // All sources of one wavenumber share A_k (the mixed boundary condition uses
 // the mean electrode position, dcfemmodelling.cpp:1025-1030), so their
 // Krylov spaces are merged: each iteration applies A_k to an n x s block P
-// (s <= 64 columns) and takes an s x s step.  Per iteration:
-//   1. Q = A P            fused with the Grams P'Q, P'R          (k_bspmm_gram)
-//   2. alpha = (P'Q)^-1 P'R                                      (k_small_alpha)
-//   3. X += P alpha; R -= Q alpha; Z = D^-1 R, fused with the
-//      Grams Q'Z, Z'Z, P'Z and the column norms |R|^2            (k_bupdate)
-//   4. beta = -(P'Q)^-1 Q'Z;  W = Z + P beta;  W'W from the Grams
-//      (P'P = I);  L = chol(W'W);  T = L^-T                      (k_small_beta)
-//   5. P = Z T + P (beta T)                                      (k_bpupdate)
-// Dependent directions are dropped in both Choleskys (zero pivot -> zero
-// column), which keeps the iteration breakdown-free when columns converge.
-// The vector type VT is double (plain) or float (mixed precision: FP32 HBM
-// traffic, FP64 accumulation, Grams and small matrices; the solution is
-// accumulated in FP64 with periodic FP64 residual replacement).
+// (s <= 64 columns) and takes an s x s step:
+//   1. Q = A P                                              (k_spmm)
+//   2. G1 = P'Q, G2 = P'R;  alpha = G1^-1 G2                (k_gram<PQR>, k_small_alpha)
+//   3. X += P alpha;  R -= Q alpha;  |R|^2 per column       (k_update)
+//   4. Z = M^-1 R;  Q'Z, Z'Z, P'Z                           (k_gram<Z>; M = Jacobi [+ coarse])
+//   5. beta = -G1^-1 Q'Z;  W = Z + P beta;  W'W from the Grams (P'P = I);
+//      T = chol(W'W)^-T;  P = Z T + P (beta T)              (k_small_beta, k_pupdate)
+// Dependent directions are dropped in both factorisations (a pivot below
+// 1e-12 of its own diagonal gives a zero row and column), which keeps the
+// iteration breakdown-free when columns converge.
+//
+// Layout: every work vector is held block-major, V[block][row][64] with the
+// columns beyond the block width zero, so a row is one aligned 256 B (FP32)
+// or 512 B (FP64) segment.  The tall-skinny products run on the matrix cores
+// (v_mfma 16x16x4, FP32 or FP64):
+//   * Grams L'R over the rows: a wavefront takes 4 rows per step, lane l
+//     holding columns 4(l%16)..+3 of row l/16 as one vector load; component
+//     t of that vector is directly the MFMA operand of column tile t, so
+//     the 64x64 Gram is 16 MFMAs per 4 rows with no data movement.
+//   * Updates C = L M (M 64 x 64): 16-row strips, lane l holding row l%16;
+//     computed transposed (C' = M' L') so that operands and results share the
+//     layout of a plain vector load/store; M' comes from LDS, pre-swizzled
+//     to one conflict-free read per MFMA.
+// Partial Grams are written per workgroup in FP64 and reduced in a fixed
+// order (deterministic).  The 64 x 64 algebra runs in one workgroup per
+// block in LDS (FP64).
+//
+// The vector type VT is double (plain block) or float (mixed precision: FP32
+// HBM traffic and matrix-core products; Grams reduced and small matrices kept
+// in FP64; the solution accumulated in FP64 with residual replacement by the
+// true FP64 residual).
 #include "internal.h"
 
 #include <algorithm>
@@ -27,21 +45,19 @@ namespace pgert {
 
 namespace {
 
-constexpr int S = 64;      // max block width
-constexpr int CH = 16;     // rows per chunk in the Gram passes
-constexpr int MAXG = 512;  // workgroups per block for the Gram passes
+constexpr int S = 64;  // max block width
+constexpr int SS = S * S;
 
 // small-matrix slots per block (each S*S doubles)
-constexpr int SL_G1 = 0;   // P'Q, replaced by its Cholesky factor
-constexpr int SL_G2 = 1;   // P'R, replaced by alpha
-constexpr int SL_QZ = 2;   // Q'Z
-constexpr int SL_ZZ = 3;   // Z'Z
-constexpr int SL_PZ = 4;   // P'Z
-constexpr int SL_RR = 5;   // |R|^2 per column (first S entries)
-constexpr int SL_T = 6;    // T = chol(W'W)^-T
-constexpr int SL_M2 = 7;   // beta T
-constexpr int SL_KEEP = 8; // kept pivots of P'Q (first S entries)
-constexpr int NSL = 9;
+constexpr int SL_G1 = 0;  // P'Q, replaced by its inverse
+constexpr int SL_G2 = 1;  // P'R, replaced by alpha
+constexpr int SL_QZ = 2;  // Q'Z
+constexpr int SL_ZZ = 3;  // Z'Z
+constexpr int SL_PZ = 4;  // P'Z
+constexpr int SL_RR = 5;  // |R|^2 per column (first S entries)
+constexpr int SL_T = 6;   // T = chol(W'W)^-T
+constexpr int SL_M2 = 7;  // beta T
+constexpr int NSL = 8;
 
 struct BlockDesc {
     int k;       // wavenumber index
@@ -50,483 +66,555 @@ struct BlockDesc {
     int active;  // 0 once the block has converged: its iteration kernels return at once
 };
 
-// ---------------------------------------------------------------- Gram helpers
-// Each of the 256 threads owns a 4x4 tile (a0..a0+3, b0..b0+3) of a 64x64 Gram.
-__device__ inline void tile_of(int & a0, int & b0) {
-    a0 = (threadIdx.x / 16) * 4;
-    b0 = (threadIdx.x % 16) * 4;
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+template <typename T> struct Vec4Of;
+template <> struct Vec4Of<float> { using type = f32x4; };
+template <> struct Vec4Of<double> { using type = f64x4; };
+template <typename T> using V4 = typename Vec4Of<T>::type;
+
+__device__ inline f32x4 mfma4(float a, float b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+__device__ inline f64x4 mfma4(double a, double b, f64x4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+
+// Row (within the 16 x 16 tile) of result register r of lane l; the column
+// is l % 16 and the contraction index of both operands is l / 16.  FP32 and
+// FP64 differ: registers run along consecutive rows in FP32, across lane
+// groups in FP64.
+template <typename VT> __device__ inline int drow(int lane, int r) {
+    return sizeof(VT) == 4 ? 4 * (lane / 16) + r : 4 * r + lane / 16;
 }
 
-__device__ inline void gram_acc(double (&acc)[4][4], const double * Ls, const double * Rs, int a0, int b0) {
-    double l[4], r[4];
-    for (int i = 0; i < 4; ++i) { l[i] = Ls[a0 + i]; r[i] = Rs[b0 + i]; }
-    for (int i = 0; i < 4; ++i)
-        for (int j = 0; j < 4; ++j) acc[i][j] += l[i] * r[j];
-}
+template <typename VT> __device__ inline V4<VT> ld4(const VT * p) { return *reinterpret_cast<const V4<VT> *>(p); }
+template <typename VT> __device__ inline void st4(VT * p, V4<VT> v) { *reinterpret_cast<V4<VT> *>(p) = v; }
 
-__device__ inline void gram_store(const double (&acc)[4][4], double * out, int a0, int b0) {
-    for (int i = 0; i < 4; ++i)
-        for (int j = 0; j < 4; ++j) out[(a0 + i) * S + (b0 + j)] = acc[i][j];
-}
-
-template <typename VT>
-__device__ inline VT ld(const VT * p) { return *p; }
-
-// ---------------------------------------------------------------- 1. Q = A P + Grams
-// part layout per (block, group): [G1 (S*S)][G2 (S*S)]
+// ---------------------------------------------------------------- 1. Q = A P
+// one wavefront per row (row index wavefront-uniform: the CSR row is read
+// through the scalar unit), lanes over the 64 columns, 4 nonzeros in flight
 template <typename VT, typename AT>
-__global__ void __launch_bounds__(256) k_bspmm_gram(int n, int nE, int nnz, const BlockDesc * __restrict__ blocks,
-                                                    const int * __restrict__ rowptr, const int * __restrict__ colidx,
-                                                    const AT * __restrict__ vals, const VT * __restrict__ P,
-                                                    const VT * __restrict__ R, VT * __restrict__ Q,
-                                                    double * __restrict__ part) {
-    __shared__ double Ps[CH][S], Qs[CH][S], Rs[CH][S];
-    const BlockDesc bd = blocks[blockIdx.y];
+__global__ void __launch_bounds__(256) k_spmm(int n, int nnz, const BlockDesc * __restrict__ blocks,
+                                              const int * __restrict__ rowptr, const int * __restrict__ colidx,
+                                              const AT * __restrict__ vals, const VT * __restrict__ P,
+                                              VT * __restrict__ Q) {
+    const int b = blockIdx.y;
+    const BlockDesc bd = blocks[b];
     if (!bd.active) return;
-    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    const size_t voff = (size_t)bd.k * n * nE + bd.c0;
+    const int lane = threadIdx.x % 64;
+    const VT * Pb = P + (size_t)b * n * S + lane;
+    VT * Qb = Q + (size_t)b * n * S + lane;
     const AT * A = vals + (size_t)bd.k * nnz;
-    int a0, b0;
-    tile_of(a0, b0);
-    double g1[4][4] = {}, g2[4][4] = {};
-    const int nchunk = (n + CH - 1) / CH;
-    for (int ch = blockIdx.x; ch < nchunk; ch += gridDim.x) {
-        const int base = ch * CH;
-        for (int rr = wave; rr < CH; rr += 4) {
-            const int row = base + rr;
-            double q = 0.0, p = 0.0, r = 0.0;
-            if (row < n && lane < bd.s) {
-                for (int t = rowptr[row]; t < rowptr[row + 1]; ++t)
-                    q += (double)A[t] * (double)P[voff + (size_t)colidx[t] * nE + lane];
-                const size_t o = voff + (size_t)row * nE + lane;
-                Q[o] = (VT)q;
-                p = (double)P[o];
-                r = (double)R[o];
-            }
-            Ps[rr][lane] = p;
-            Qs[rr][lane] = q;
-            Rs[rr][lane] = r;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+    for (int row = blockIdx.x * 4 + w; row < n; row += gridDim.x * 4) {
+        const int q0 = rowptr[row], q1 = rowptr[row + 1];
+        VT acc0 = 0, acc1 = 0;
+        int q = q0;
+        for (; q + 4 <= q1; q += 4) {
+            const int c0 = colidx[q], c1 = colidx[q + 1], c2 = colidx[q + 2], c3 = colidx[q + 3];
+            const VT x0 = Pb[(size_t)c0 * S], x1 = Pb[(size_t)c1 * S];
+            const VT x2 = Pb[(size_t)c2 * S], x3 = Pb[(size_t)c3 * S];
+            acc0 += (VT)A[q] * x0 + (VT)A[q + 1] * x1;
+            acc1 += (VT)A[q + 2] * x2 + (VT)A[q + 3] * x3;
         }
-        __syncthreads();
-        for (int rr = 0; rr < CH; ++rr) {
-            gram_acc(g1, Ps[rr], Qs[rr], a0, b0);
-            gram_acc(g2, Ps[rr], Rs[rr], a0, b0);
-        }
-        __syncthreads();
+        for (; q < q1; ++q) acc0 += (VT)A[q] * Pb[(size_t)colidx[q] * S];
+        Qb[(size_t)row * S] = acc0 + acc1;
     }
-    double * out = part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 2 * S * S;
-    gram_store(g1, out, a0, b0);
-    gram_store(g2, out + S * S, a0, b0);
 }
 
-// ---------------------------------------------------------------- 3. update + Grams
-// part layout per (block, group): [Q'Z][Z'Z][P'Z][|R|^2 (S)]
-template <typename VT>
-__global__ void __launch_bounds__(256) k_bupdate(int n, int nE, const BlockDesc * __restrict__ blocks,
-                                                 const double * __restrict__ small, const double * __restrict__ dinv,
-                                                 const VT * __restrict__ P, const VT * __restrict__ Q,
-                                                 VT * __restrict__ X, VT * __restrict__ R,
-                                                 double * __restrict__ part, int grams) {
-    __shared__ double Ps[CH][S], Qs[CH][S], Zs[CH][S];
-    __shared__ double al[S][S];
-    const BlockDesc bd = blocks[blockIdx.y];
+// ---------------------------------------------------------------- Gram passes
+// MODE_PQR: G1 = P'Q, G2 = P'R.  MODE_Z: Z = D^-1 R (+ coarse prolongation
+// cc[agg[row]]) and Q'Z, Z'Z, P'Z.
+// Wavefront w of a workgroup: row group w / WS, column tiles tb = w % WS +
+// WS x (WS = 2 halves the accumulators of the FP64 variant).  Tile (ta, tb)
+// of a Gram holds entries (4i + ta, 4j + tb), i, j < 16; its MFMA output
+// register jj of lane l is entry (4 drow(l, jj) + ta, 4 (l%16) + tb).
+// Workgroup partials: part[((blk * G + g) * NG + gram) * S*S].
+constexpr int MODE_PQR = 0;
+constexpr int MODE_Z = 1;
+
+template <typename VT, int MODE, int WS>
+__global__ void __launch_bounds__(256) k_gram(int n, int m, const BlockDesc * __restrict__ blocks,
+                                              const double * __restrict__ dinv, const VT * __restrict__ P,
+                                              const VT * __restrict__ Q, const VT * __restrict__ R,
+                                              const double * __restrict__ cc, const int * __restrict__ agg,
+                                              double * __restrict__ part) {
+    constexpr int NG = MODE == MODE_PQR ? 2 : 3;
+    constexpr int RG = 4 / WS;   // row groups per workgroup
+    constexpr int NTB = 4 / WS;  // column tiles per wavefront
+    using V = V4<VT>;
+    __shared__ double red[S][S + 1];
+    const int b = blockIdx.y;
+    const BlockDesc bd = blocks[b];
     if (!bd.active) return;
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    const size_t voff = (size_t)bd.k * n * nE + bd.c0;
+    const int rg = wave / WS, ws = wave % WS;
+    const size_t vb = (size_t)b * n * S;
+    const int col = 4 * (lane % 16);
     const double * dk = dinv + (size_t)bd.k * n;
-    const double * alpha = small + ((size_t)blockIdx.y * NSL + SL_G2) * S * S;
-    for (int t = threadIdx.x; t < S * S; t += 256) al[t / S][t % S] = alpha[t];
-    int a0, b0;
-    tile_of(a0, b0);
-    double gqz[4][4] = {}, gzz[4][4] = {}, gpz[4][4] = {};
-    double rr2 = 0.0;
-    const int nchunk = (n + CH - 1) / CH;
-    __syncthreads();
-    for (int ch = blockIdx.x; ch < nchunk; ch += gridDim.x) {
-        const int base = ch * CH;
-        for (int rr = wave; rr < CH; rr += 4) {
-            const int row = base + rr;
-            double p = 0.0, q = 0.0;
-            if (row < n && lane < bd.s) {
-                const size_t o = voff + (size_t)row * nE + lane;
-                p = (double)P[o];
-                q = (double)Q[o];
-            }
-            Ps[rr][lane] = p;
-            Qs[rr][lane] = q;
+    V acc[NG][4][NTB];
+#pragma unroll
+    for (int g = 0; g < NG; ++g)
+#pragma unroll
+        for (int ta = 0; ta < 4; ++ta)
+#pragma unroll
+            for (int x = 0; x < NTB; ++x) acc[g][ta][x] = V{0, 0, 0, 0};
+    const int nst = (n + 3) / 4;
+    for (int st = blockIdx.x * RG + rg; st < nst; st += gridDim.x * RG) {
+        const int row = 4 * st + lane / 16;
+        V p = V{0, 0, 0, 0}, q = p, r = p;
+        if (row < n) {
+            const size_t o = vb + (size_t)row * S + col;
+            p = ld4(P + o);
+            q = ld4(Q + o);
+            r = ld4(R + o);
         }
-        __syncthreads();
-        for (int rr = wave; rr < CH; rr += 4) {
-            const int row = base + rr;
-            double z = 0.0;
-            if (row < n && lane < bd.s) {
-                double dx = 0.0, dr = 0.0;
-                for (int a = 0; a < bd.s; ++a) {
-                    double w = al[a][lane];
-                    dx += Ps[rr][a] * w;
-                    dr += Qs[rr][a] * w;
+        V a[NG], bo[NG];
+        if constexpr (MODE == MODE_PQR) {
+            a[0] = p;
+            a[1] = p;
+            bo[0] = q;
+            bo[1] = r;
+        } else {
+            V z = V{0, 0, 0, 0};
+            if (row < n) {
+                const double d = dk[row];
+                if (cc) {
+                    const double * cr = cc + ((size_t)b * m + agg[row]) * S + col;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) z[e] = (VT)(d * (double)r[e] + cr[e]);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) z[e] = (VT)(d * (double)r[e]);
                 }
-                const size_t o = voff + (size_t)row * nE + lane;
-                X[o] = (VT)((double)X[o] + dx);
-                double r = (double)R[o] - dr;
-                VT rs = (VT)r;
-                R[o] = rs;
-                r = (double)rs;
-                rr2 += r * r;
-                z = dk[row] * r;
             }
-            Zs[rr][lane] = z;
+            a[0] = q;
+            a[1] = z;
+            a[2] = p;
+            bo[0] = z;
+            bo[1] = z;
+            bo[2] = z;
         }
-        __syncthreads();
-        if (grams)
-            for (int rr = 0; rr < CH; ++rr) {
-                gram_acc(gqz, Qs[rr], Zs[rr], a0, b0);
-                gram_acc(gzz, Zs[rr], Zs[rr], a0, b0);
-                gram_acc(gpz, Ps[rr], Zs[rr], a0, b0);
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+#pragma unroll
+            for (int ta = 0; ta < 4; ++ta)
+#pragma unroll
+                for (int x = 0; x < NTB; ++x) acc[g][ta][x] = mfma4(a[g][ta], bo[g][ws + WS * x], acc[g][ta][x]);
+    }
+    double * out = part + ((size_t)b * gridDim.x + blockIdx.x) * NG * SS;
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+        for (int rr = 0; rr < RG; ++rr) {
+            if (rg == rr) {
+#pragma unroll
+                for (int ta = 0; ta < 4; ++ta)
+#pragma unroll
+                    for (int x = 0; x < NTB; ++x)
+#pragma unroll
+                        for (int jj = 0; jj < 4; ++jj) {
+                            const int ai = 4 * drow<VT>(lane, jj) + ta, bj = 4 * (lane % 16) + ws + WS * x;
+                            const double v = (double)acc[g][ta][x][jj];
+                            red[ai][bj] = rr == 0 ? v : red[ai][bj] + v;
+                        }
             }
+            __syncthreads();
+        }
+        for (int t = threadIdx.x; t < SS; t += blockDim.x) out[(size_t)g * SS + t] = red[t / S][t % S];
         __syncthreads();
     }
-    if (!grams) return;
-    double * out = part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * (3 * S * S + S);
-    gram_store(gqz, out, a0, b0);
-    gram_store(gzz, out + S * S, a0, b0);
-    gram_store(gpz, out + 2 * S * S, a0, b0);
-    // column norms: the four wavefronts hold partial sums of the same columns
-    __shared__ double cn[4][S];
-    cn[wave][lane] = rr2;
-    __syncthreads();
-    if (wave == 0) out[3 * S * S + lane] = cn[0][lane] + cn[1][lane] + cn[2][lane] + cn[3][lane];
 }
 
-// ---------------------------------------------------------------- 5. P = Z T + P M2
+// ---------------------------------------------------------------- updates over 16-row strips
+// Lane l of a wavefront holds row 16 s + l%16, columns 16 u + 4 (l/16) + e
+// (u, e < 4) as four vector loads.  C = L M is computed as C' = M' L': MFMA
+// step (u, e) contracts over the four columns 16 u + 4 k + e (k = l/16), the
+// A operand of column tile tb is M[16 u + 4 (l/16) + e][16 tb + l%16] and the
+// result register jj of tile tb is C[row][16 tb + drow(l, jj)] -- in FP32
+// the same layout as the loads (one vector store), in FP64 columns strided
+// by 4 (four scalar accesses).
+// Pre-swizzled LDS operand: op[((4 u + e) * 4 + tb) * 64 + lane].
 template <typename VT>
-__global__ void __launch_bounds__(256) k_bpupdate(int n, int nE, const BlockDesc * __restrict__ blocks,
-                                                  const double * __restrict__ small, const double * __restrict__ dinv,
-                                                  const VT * __restrict__ R, VT * __restrict__ P,
-                                                  const double * __restrict__ cc, const int * __restrict__ agg, int m) {
-    __shared__ double Tm[S][S], Mm[S][S];
-    __shared__ double Zs[CH][S], Ps[CH][S];
-    const BlockDesc bd = blocks[blockIdx.y];
-    if (!bd.active) return;
-    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    const size_t voff = (size_t)bd.k * n * nE + bd.c0;
-    const double * dk = dinv + (size_t)bd.k * n;
-    const double * sm = small + (size_t)blockIdx.y * NSL * S * S;
-    for (int t = threadIdx.x; t < S * S; t += 256) {
-        Tm[t / S][t % S] = sm[SL_T * S * S + t];
-        Mm[t / S][t % S] = sm[SL_M2 * S * S + t];
+__device__ inline void stage_op(VT * op, const double * M) {
+    for (int t = threadIdx.x; t < 16 * 4 * 64; t += blockDim.x) {
+        const int step = t / 256, tb = (t / 64) % 4, ln = t % 64;
+        const int u = step / 4, e = step % 4;
+        op[t] = (VT)M[(16 * u + 4 * (ln / 16) + e) * S + 16 * tb + ln % 16];
     }
+}
+
+template <typename VT>
+__device__ inline void strip_mm(const VT * __restrict__ op, const V4<VT> (&x)[4], V4<VT> (&c)[4], int lane) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+            for (int tb = 0; tb < 4; ++tb) c[tb] = mfma4(op[((4 * u + e) * 4 + tb) * 64 + lane], x[u][e], c[tb]);
+}
+
+// 3. X += P alpha, R -= Q alpha; partial |R|^2 per column -> part[(blk * G + g) * S]
+template <typename VT>
+__global__ void __launch_bounds__(256) k_update(int n, const BlockDesc * __restrict__ blocks,
+                                                const double * __restrict__ small, const VT * __restrict__ P,
+                                                const VT * __restrict__ Q, VT * __restrict__ X, VT * __restrict__ R,
+                                                double * __restrict__ part) {
+    using V = V4<VT>;
+    __shared__ VT op[16 * 4 * 64];
+    __shared__ double cn[4][S];
+    const int b = blockIdx.y;
+    const BlockDesc bd = blocks[b];
+    if (!bd.active) return;
+    stage_op(op, small + ((size_t)b * NSL + SL_G2) * SS);
     __syncthreads();
-    const int nchunk = (n + CH - 1) / CH;
-    for (int ch = blockIdx.x; ch < nchunk; ch += gridDim.x) {
-        const int base = ch * CH;
-        for (int rr = wave; rr < CH; rr += 4) {
-            const int row = base + rr;
-            double z = 0.0, p = 0.0;
-            if (row < n && lane < bd.s) {
-                const size_t o = voff + (size_t)row * nE + lane;
-                z = dk[row] * (double)R[o];
-                if (cc) z += cc[((size_t)blockIdx.y * m + agg[row]) * S + lane];
-                p = (double)P[o];
-            }
-            Zs[rr][lane] = z;
-            Ps[rr][lane] = p;
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const size_t vb = (size_t)b * n * S;
+    const int cq = 4 * (lane / 16);
+    double rr[4][4] = {};
+    const int nstrip = (n + 15) / 16;
+    for (int sidx = blockIdx.x * 4 + wave; sidx < nstrip; sidx += gridDim.x * 4) {
+        const int row = 16 * sidx + lane % 16;
+        const bool ok = row < n;
+        const size_t o = vb + (size_t)row * S + cq;
+        V p[4], q[4], cx[4], cr[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            p[u] = ok ? ld4(P + o + 16 * u) : V{0, 0, 0, 0};
+            q[u] = ok ? ld4(Q + o + 16 * u) : V{0, 0, 0, 0};
+            cx[u] = cr[u] = V{0, 0, 0, 0};
         }
-        __syncthreads();
-        for (int rr = wave; rr < CH; rr += 4) {
-            const int row = base + rr;
-            if (row < n && lane < bd.s) {
-                double acc = 0.0;
-                for (int a = 0; a < bd.s; ++a) acc += Zs[rr][a] * Tm[a][lane] + Ps[rr][a] * Mm[a][lane];
-                P[voff + (size_t)row * nE + lane] = (VT)acc;
+        strip_mm(op, p, cx, lane);
+        strip_mm(op, q, cr, lane);
+        if (ok) {
+            if constexpr (sizeof(VT) == 4) {
+#pragma unroll
+                for (int tb = 0; tb < 4; ++tb) {
+                    st4(X + o + 16 * tb, ld4(X + o + 16 * tb) + cx[tb]);
+                    const V rn = ld4(R + o + 16 * tb) - cr[tb];
+                    st4(R + o + 16 * tb, rn);
+#pragma unroll
+                    for (int jj = 0; jj < 4; ++jj) rr[tb][jj] += (double)rn[jj] * (double)rn[jj];
+                }
+            } else {
+                const size_t ob = vb + (size_t)row * S;
+#pragma unroll
+                for (int tb = 0; tb < 4; ++tb)
+#pragma unroll
+                    for (int jj = 0; jj < 4; ++jj) {
+                        const size_t oc = ob + 16 * tb + drow<VT>(lane, jj);
+                        X[oc] += cx[tb][jj];
+                        const VT rn = R[oc] - cr[tb][jj];
+                        R[oc] = rn;
+                        rr[tb][jj] += (double)rn * (double)rn;
+                    }
             }
         }
-        __syncthreads();
+    }
+    // sum over the 16 rows of a lane group, then over the wavefronts
+#pragma unroll
+    for (int tb = 0; tb < 4; ++tb)
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+            double v = rr[tb][jj];
+            v += __shfl_xor(v, 8);
+            v += __shfl_xor(v, 4);
+            v += __shfl_xor(v, 2);
+            v += __shfl_xor(v, 1);
+            if (lane % 16 == 0) cn[wave][16 * tb + drow<VT>(lane, jj)] = v;
+        }
+    __syncthreads();
+    if (wave == 0)
+        part[((size_t)b * gridDim.x + blockIdx.x) * S + lane] = cn[0][lane] + cn[1][lane] + cn[2][lane] + cn[3][lane];
+}
+
+// 5. P = Z T + P M2, Z = D^-1 R (+ coarse prolongation)
+template <typename VT>
+__global__ void __launch_bounds__(256) k_pupdate(int n, int m, const BlockDesc * __restrict__ blocks,
+                                                 const double * __restrict__ small, const double * __restrict__ dinv,
+                                                 const VT * __restrict__ R, VT * __restrict__ P,
+                                                 const double * __restrict__ cc, const int * __restrict__ agg) {
+    using V = V4<VT>;
+    __shared__ VT opT[16 * 4 * 64], opM[16 * 4 * 64];
+    const int b = blockIdx.y;
+    const BlockDesc bd = blocks[b];
+    if (!bd.active) return;
+    stage_op(opT, small + ((size_t)b * NSL + SL_T) * SS);
+    stage_op(opM, small + ((size_t)b * NSL + SL_M2) * SS);
+    __syncthreads();
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const size_t vb = (size_t)b * n * S;
+    const int cq = 4 * (lane / 16);
+    const double * dk = dinv + (size_t)bd.k * n;
+    const int nstrip = (n + 15) / 16;
+    for (int sidx = blockIdx.x * 4 + wave; sidx < nstrip; sidx += gridDim.x * 4) {
+        const int row = 16 * sidx + lane % 16;
+        const bool ok = row < n;
+        const size_t o = vb + (size_t)row * S + cq;
+        V z[4], p[4], c[4];
+        const double d = ok ? dk[row] : 0.0;
+        const double * crow = (cc && ok) ? cc + ((size_t)b * m + agg[row]) * S + cq : nullptr;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            c[u] = V{0, 0, 0, 0};
+            p[u] = ok ? ld4(P + o + 16 * u) : V{0, 0, 0, 0};
+            const V r = ok ? ld4(R + o + 16 * u) : V{0, 0, 0, 0};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) z[u][e] = (VT)(d * (double)r[e] + (crow ? crow[16 * u + e] : 0.0));
+        }
+        strip_mm(opT, z, c, lane);
+        strip_mm(opM, p, c, lane);
+        if (ok) {
+            if constexpr (sizeof(VT) == 4) {
+#pragma unroll
+                for (int tb = 0; tb < 4; ++tb) st4(P + o + 16 * tb, c[tb]);
+            } else {
+                const size_t ob = vb + (size_t)row * S;
+#pragma unroll
+                for (int tb = 0; tb < 4; ++tb)
+#pragma unroll
+                    for (int jj = 0; jj < 4; ++jj) P[ob + 16 * tb + drow<VT>(lane, jj)] = c[tb][jj];
+            }
+        }
     }
 }
 
 // ---------------------------------------------------------------- coarse correction
-// y[b][I][c] = sum over the nodes of aggregate I of R[node][c0 + c]
+// y[b][I][c] = sum over the nodes of aggregate I of R[b][node][c]: one
+// workgroup per aggregate, 4 wavefronts over its node list
 template <typename VT>
-__global__ void __launch_bounds__(256) k_restrict(int n, int nE, int m, const BlockDesc * __restrict__ blocks,
+__global__ void __launch_bounds__(256) k_restrict(int n, int m, const BlockDesc * __restrict__ blocks,
                                                   const int * __restrict__ aggPtr, const int * __restrict__ aggNodes,
                                                   const VT * __restrict__ R, double * __restrict__ y) {
-    const BlockDesc bd = blocks[blockIdx.y];
-    if (!bd.active) return;
+    __shared__ double cn[4][S];
+    const int b = blockIdx.y;
+    if (!blocks[b].active) return;
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    const int I = blockIdx.x * 4 + wave;
-    if (I >= m) return;
-    const size_t voff = (size_t)bd.k * n * nE + bd.c0;
+    const int I = blockIdx.x;
+    const VT * Rb = R + (size_t)b * n * S + lane;
     double s = 0.0;
-    if (lane < bd.s)
-        for (int q = aggPtr[I]; q < aggPtr[I + 1]; ++q) s += (double)R[voff + (size_t)aggNodes[q] * nE + lane];
-    y[((size_t)blockIdx.y * m + I) * S + lane] = s;
+    for (int q = aggPtr[I] + wave; q < aggPtr[I + 1]; q += 4) s += (double)Rb[(size_t)aggNodes[q] * S];
+    cn[wave][lane] = s;
+    __syncthreads();
+    if (wave == 0) y[((size_t)b * m + I) * S + lane] = cn[0][lane] + cn[1][lane] + cn[2][lane] + cn[3][lane];
 }
 
-// cc[b] = Einv_k y[b]: 4 coarse rows x 64 columns per workgroup, y staged in LDS
+// cc[b] = Einv_k y[b] (FP64 MFMA): a workgroup computes one 16 x 16 tile
+// (rows 16 blockIdx.x, columns 16 * tile), its 4 wavefronts split the m-long
+// contraction and are summed through LDS
 __global__ void __launch_bounds__(256) k_coarse_apply(int m, const BlockDesc * __restrict__ blocks,
                                                       const double * __restrict__ Einv, const double * __restrict__ y,
                                                       double * __restrict__ cc) {
-    __shared__ double ys[64][S];
-    const BlockDesc bd = blocks[blockIdx.y];
-    if (!bd.active) return;
-    const int r = blockIdx.x * 4 + threadIdx.x / 64, col = threadIdx.x % 64;
-    const double * E = Einv + (size_t)bd.k * m * m;
-    const double * yb = y + (size_t)blockIdx.y * m * S;
-    double acc = 0.0;
-    for (int k0 = 0; k0 < m; k0 += 64) {
-        for (int t = threadIdx.x; t < 64 * S; t += 256) {
-            int kk = k0 + t / S;
-            ys[t / S][t % S] = kk < m ? yb[(size_t)kk * S + t % S] : 0.0;
-        }
-        __syncthreads();
-        if (r < m) {
-            const double * Er = E + (size_t)r * m + k0;
-            const int kn = min(64, m - k0);
-            for (int q = 0; q < kn; ++q) acc += Er[q] * ys[q][col];
-        }
-        __syncthreads();
-    }
-    if (r < m) cc[((size_t)blockIdx.y * m + r) * S + col] = acc;
-}
-
-// Z = D^-1 R + prolong(cc); Grams Q'Z, Z'Z, P'Z and |R|^2 (k_bupdate's partial layout)
-template <typename VT>
-__global__ void __launch_bounds__(256) k_zgram(int n, int nE, int m, const BlockDesc * __restrict__ blocks,
-                                               const double * __restrict__ dinv, const VT * __restrict__ P,
-                                               const VT * __restrict__ Q, const VT * __restrict__ R,
-                                               const double * __restrict__ cc, const int * __restrict__ agg,
-                                               double * __restrict__ part) {
-    __shared__ double Ps[CH][S], Qs[CH][S], Zs[CH][S];
-    __shared__ double cn[4][S];
-    const BlockDesc bd = blocks[blockIdx.y];
+    __shared__ double red[4][64][4];
+    const int b = blockIdx.z;
+    const BlockDesc bd = blocks[b];
     if (!bd.active) return;
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    const size_t voff = (size_t)bd.k * n * nE + bd.c0;
-    const double * dk = dinv + (size_t)bd.k * n;
-    int a0, b0;
-    tile_of(a0, b0);
-    double gqz[4][4] = {}, gzz[4][4] = {}, gpz[4][4] = {};
-    double rr2 = 0.0;
-    const int nchunk = (n + CH - 1) / CH;
-    for (int ch = blockIdx.x; ch < nchunk; ch += gridDim.x) {
-        const int base = ch * CH;
-        for (int rr = wave; rr < CH; rr += 4) {
-            const int row = base + rr;
-            double p = 0.0, q = 0.0, z = 0.0;
-            if (row < n && lane < bd.s) {
-                const size_t o = voff + (size_t)row * nE + lane;
-                p = (double)P[o];
-                q = (double)Q[o];
-                double r = (double)R[o];
-                rr2 += r * r;
-                z = dk[row] * r + cc[((size_t)blockIdx.y * m + agg[row]) * S + lane];
-            }
-            Ps[rr][lane] = p;
-            Qs[rr][lane] = q;
-            Zs[rr][lane] = z;
-        }
-        __syncthreads();
-        for (int rr = 0; rr < CH; ++rr) {
-            gram_acc(gqz, Qs[rr], Zs[rr], a0, b0);
-            gram_acc(gzz, Zs[rr], Zs[rr], a0, b0);
-            gram_acc(gpz, Ps[rr], Zs[rr], a0, b0);
-        }
-        __syncthreads();
+    const int I0 = 16 * blockIdx.x, c0 = 16 * blockIdx.y;
+    const double * E = Einv + (size_t)bd.k * m * m;
+    const double * yb = y + (size_t)b * m * S;
+    const int i = I0 + lane % 16;
+    f64x4 acc = {0, 0, 0, 0};
+    const int nk = (m + 3) / 4;
+    const int k0 = nk * wave / 4, k1 = nk * (wave + 1) / 4;
+    for (int ks = k0; ks < k1; ++ks) {
+        const int J = 4 * ks + lane / 16;
+        const double a = (i < m && J < m) ? E[(size_t)i * m + J] : 0.0;
+        const double bv = J < m ? yb[(size_t)J * S + c0 + lane % 16] : 0.0;
+        acc = mfma4(a, bv, acc);
     }
-    double * out = part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * (3 * S * S + S);
-    gram_store(gqz, out, a0, b0);
-    gram_store(gzz, out + S * S, a0, b0);
-    gram_store(gpz, out + 2 * S * S, a0, b0);
-    cn[wave][lane] = rr2;
+    for (int jj = 0; jj < 4; ++jj) red[wave][lane][jj] = acc[jj];
     __syncthreads();
-    if (wave == 0) out[3 * S * S + lane] = cn[0][lane] + cn[1][lane] + cn[2][lane] + cn[3][lane];
+    if (wave == 0)
+        for (int jj = 0; jj < 4; ++jj) {
+            const int row = I0 + drow<double>(lane, jj);
+            if (row < m)
+                cc[((size_t)b * m + row) * S + c0 + lane % 16] =
+                    red[0][lane][jj] + red[1][lane][jj] + red[2][lane][jj] + red[3][lane][jj];
+        }
 }
 
-// ---------------------------------------------------------------- reductions
-// sum over the groups of `count` doubles per (block, group) into small[block][dst ...]
-__global__ void k_gram_reduce(const BlockDesc * __restrict__ blocks, int G, int count,
-                              const double * __restrict__ part, double * __restrict__ small, int dstOff) {
+// ---------------------------------------------------------------- partial reduction
+// small[b][dstOff + t] = sum_g part[(b * G + g) * count + t]; 32 outputs x 8
+// group ranges per workgroup, fixed summation order
+__global__ void __launch_bounds__(256) k_part_reduce(const BlockDesc * __restrict__ blocks, int G, int count,
+                                                     const double * __restrict__ part, double * __restrict__ small,
+                                                     int dstOff) {
+    __shared__ double acc[8][32];
     const int b = blockIdx.y;
     if (!blocks[b].active) return;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count) return;
+    const int o = threadIdx.x % 32, gc = threadIdx.x / 32;
+    const int t = blockIdx.x * 32 + o;
     double s = 0.0;
-    const double * p = part + (size_t)b * G * count + t;
-    for (int g = 0; g < G; ++g) s += p[(size_t)g * count];
-    small[(size_t)b * NSL * S * S + dstOff + t] = s;
+    if (t < count) {
+        const int g0 = G * gc / 8, g1 = G * (gc + 1) / 8;
+        const double * p = part + (size_t)b * G * count + t;
+        for (int g = g0; g < g1; ++g) s += p[(size_t)g * count];
+    }
+    acc[gc][o] = s;
+    __syncthreads();
+    if (gc == 0 && t < count) {
+        double v = 0.0;
+        for (int q = 0; q < 8; ++q) v += acc[q][o];
+        small[(size_t)b * NSL * SS + dstOff + t] = v;
+    }
 }
 
-// ---------------------------------------------------------------- small dense kernels
-// Cholesky with dropping in LDS: L lower, keep[j] = 0 for dropped pivots.
-// A pivot is dropped when the remaining diagonal falls below 1e-12 of the
-// column's own diagonal (the column is dependent on the previous ones), a
-// criterion independent of how the columns are scaled against each other.
-__device__ void chol_drop(double (*M)[S + 1], int s, int * keep) {
-    __shared__ double d0[S];
-    if (threadIdx.x < s) d0[threadIdx.x] = M[threadIdx.x][threadIdx.x];
+// ---------------------------------------------------------------- small dense kernels (LDS, FP64)
+constexpr int LD = S + 1;  // padded leading dimension against bank conflicts
+constexpr int SMAT = S * LD;
+constexpr size_t SMALL_LDS = 4 * SMAT * sizeof(double);
+
+// C = op(A) B (64 x 64; each thread a 4 x 4 tile); transA: A'.  C must not alias A or B.
+__device__ void mm64(const double * A, const double * B, double * C, bool transA, double scale) {
+    const int i0 = (threadIdx.x / 16) * 4, j0 = (threadIdx.x % 16) * 4;
+    double acc[4][4] = {};
+    for (int k = 0; k < S; ++k) {
+        double a[4], bb[4];
+        for (int i = 0; i < 4; ++i) a[i] = transA ? A[k * LD + i0 + i] : A[(i0 + i) * LD + k];
+        for (int j = 0; j < 4; ++j) bb[j] = B[k * LD + j0 + j];
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * bb[j];
+    }
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) C[(i0 + i) * LD + j0 + j] = scale * acc[i][j];
+    __syncthreads();
+}
+
+// Li = L^-1 for G = L L' (lower Cholesky factor) with dependent directions
+// dropped: forward elimination of [G | I] (one barrier per pivot; G is
+// destroyed), the right half becomes the inverse unit-lower factor, rows
+// scaled by the pivots^-1/2.  A pivot at or below 1e-12 of the column's own
+// diagonal is dropped: no elimination with it, zero row and column in Li.
+__device__ void inv_chol(double * G, double * Li, int s) {
+    __shared__ double d0[S], piv[S];
+    for (int t = threadIdx.x; t < SS; t += blockDim.x) {
+        const int i = t / S, j = t % S;
+        Li[i * LD + j] = (i == j && i < s) ? 1.0 : 0.0;
+    }
+    if (threadIdx.x < S) d0[threadIdx.x] = threadIdx.x < s ? G[threadIdx.x * LD + threadIdx.x] : 0.0;
     __syncthreads();
     for (int j = 0; j < s; ++j) {
-        if (threadIdx.x == 0) {
-            double d = M[j][j];
-            if (d0[j] > 0.0 && d > 1e-12 * d0[j]) {
-                keep[j] = 1;
-                M[j][j] = sqrt(d);
-            } else {
-                keep[j] = 0;
-                M[j][j] = 0.0;
+        const double d = G[j * LD + j];
+        const bool keep = d0[j] > 0.0 && d > 1e-12 * d0[j];
+        if (threadIdx.x == 0) piv[j] = keep ? d : 0.0;
+        if (keep) {
+            const double inv = 1.0 / d;
+            const int nl = s - j - 1;  // left columns j+1 .. s-1, then right columns 0 .. j
+            const int cols = s;
+            for (int t = threadIdx.x; t < nl * cols; t += blockDim.x) {
+                const int i = j + 1 + t / cols, cc = t % cols;
+                const double f = G[i * LD + j] * inv;
+                if (cc < nl) {
+                    const int c = j + 1 + cc;
+                    G[i * LD + c] -= f * G[j * LD + c];
+                } else {
+                    const int c = cc - nl;
+                    Li[i * LD + c] -= f * Li[j * LD + c];
+                }
             }
         }
         __syncthreads();
-        const double piv = M[j][j];
-        const int kj = keep[j];
-        for (int i = j + 1 + threadIdx.x; i < s; i += blockDim.x) M[i][j] = kj ? M[i][j] / piv : 0.0;
-        __syncthreads();
-        const int m = s - j - 1;
-        for (int t = threadIdx.x; t < m * m; t += blockDim.x) {
-            int i = j + 1 + t / m, c = j + 1 + t % m;
-            if (c <= i) M[i][c] -= M[i][j] * M[c][j];
-        }
-        __syncthreads();
     }
-}
-
-// Solve L L^T X = B in place (B: s x s columns), dropped rows -> 0.  One thread per column.
-__device__ void chol_solve(double (*L)[S + 1], const int * keep, int s, double (*B)[S + 1]) {
-    int c = threadIdx.x;
-    if (c < s) {
-        for (int i = 0; i < s; ++i) {
-            if (!keep[i]) { B[i][c] = 0.0; continue; }
-            double v = B[i][c];
-            for (int q = 0; q < i; ++q) v -= L[i][q] * B[q][c];
-            B[i][c] = v / L[i][i];
-        }
-        for (int i = s - 1; i >= 0; --i) {
-            if (!keep[i]) { B[i][c] = 0.0; continue; }
-            double v = B[i][c];
-            for (int q = i + 1; q < s; ++q) v -= L[q][i] * B[q][c];
-            B[i][c] = v / L[i][i];
-        }
+    for (int t = threadIdx.x; t < SS; t += blockDim.x) {
+        const int i = t / S, c = t % S;
+        Li[i * LD + c] = (i < s && piv[i] > 0.0) ? Li[i * LD + c] / sqrt(piv[i]) : 0.0;
     }
     __syncthreads();
 }
 
+__device__ void load_small(double * M, const double * src, int s) {
+    for (int t = threadIdx.x; t < SS; t += blockDim.x) {
+        const int i = t / S, j = t % S;
+        M[i * LD + j] = (i < s && j < s) ? src[t] : 0.0;
+    }
+    __syncthreads();
+}
+
+__device__ void store_small(const double * M, double * dst) {
+    for (int t = threadIdx.x; t < SS; t += blockDim.x) dst[t] = M[(t / S) * LD + t % S];
+    __syncthreads();
+}
+
+// alpha = G1^-1 G2; G1 is replaced by its inverse (reused for beta)
 __global__ void __launch_bounds__(256) k_small_alpha(const BlockDesc * __restrict__ blocks, double * __restrict__ small) {
-    __shared__ double L[S][S + 1], Bm[S][S + 1];
-    __shared__ int keep[S];
+    extern __shared__ double lds[];
     if (!blocks[blockIdx.x].active) return;
     const int s = blocks[blockIdx.x].s;
-    double * sm = small + (size_t)blockIdx.x * NSL * S * S;
-    for (int t = threadIdx.x; t < S * S; t += blockDim.x) {
-        int i = t / S, j = t % S;
-        L[i][j] = sm[SL_G1 * S * S + t];
-        Bm[i][j] = sm[SL_G2 * S * S + t];
-    }
-    __syncthreads();
-    chol_drop(L, s, keep);
-    chol_solve(L, keep, s, Bm);
-    for (int t = threadIdx.x; t < S * S; t += blockDim.x) {
-        int i = t / S, j = t % S;
-        bool in = i < s && j < s;
-        sm[SL_G2 * S * S + t] = in ? Bm[i][j] : 0.0;
-        // keep the factor of P'Q for the beta solve (lower triangle, stored in SL_G1)
-        sm[SL_G1 * S * S + t] = (in && j <= i) ? L[i][j] : 0.0;
-    }
-    if (threadIdx.x < S) sm[SL_KEEP * S * S + threadIdx.x] = (threadIdx.x < s) ? (double)keep[threadIdx.x] : 0.0;
+    double * sm = small + (size_t)blockIdx.x * NSL * SS;
+    double *M0 = lds, *M1 = lds + SMAT, *M2 = lds + 2 * SMAT;
+    load_small(M0, sm + SL_G1 * SS, s);
+    inv_chol(M0, M1, s);
+    mm64(M1, M1, M2, true, 1.0);  // G1^-1 = Li' Li
+    load_small(M0, sm + SL_G2 * SS, s);
+    mm64(M2, M0, M1, false, 1.0);  // alpha
+    store_small(M2, sm + SL_G1 * SS);
+    store_small(M1, sm + SL_G2 * SS);
 }
 
-// init = 1: first iteration (no previous block): beta = 0, W = Z.
+// init = 1: first step of a block (no previous direction): beta = 0, W = Z.
 __global__ void __launch_bounds__(256) k_small_beta(const BlockDesc * __restrict__ blocks, double * __restrict__ small,
                                                     int init) {
-    __shared__ double L[S][S + 1], Bt[S][S + 1], W[S][S + 1], Pz[S][S + 1];
-    __shared__ int keep[S], keep2[S];
+    extern __shared__ double lds[];
     if (!blocks[blockIdx.x].active) return;
     const int s = blocks[blockIdx.x].s;
-    double * sm = small + (size_t)blockIdx.x * NSL * S * S;
-    for (int t = threadIdx.x; t < S * S; t += blockDim.x) {
-        int i = t / S, j = t % S;
-        L[i][j] = sm[SL_G1 * S * S + t];
-        Bt[i][j] = init ? 0.0 : sm[SL_QZ * S * S + t];  // Q'Z
-        W[i][j] = sm[SL_ZZ * S * S + t];
-        Pz[i][j] = init ? 0.0 : sm[SL_PZ * S * S + t];
-    }
-    if (threadIdx.x < S) keep[threadIdx.x] = init ? 0 : (int)sm[SL_KEEP * S * S + threadIdx.x];
-    __syncthreads();
+    double * sm = small + (size_t)blockIdx.x * NSL * SS;
+    double *M0 = lds, *M1 = lds + SMAT, *Bt = lds + 2 * SMAT, *W = lds + 3 * SMAT;
+    load_small(W, sm + SL_ZZ * SS, s);
     if (!init) {
-        chol_solve(L, keep, s, Bt);  // Bt = (P'Q)^-1 Q'Z ; beta = -Bt
-        // W'W = Z'Z + Pz' beta + beta' Pz + beta' beta   (P'P = I)
-        for (int t = threadIdx.x; t < s * s; t += blockDim.x) {
-            int i = t / s, j = t % s;
-            double v = 0.0;
-            for (int q = 0; q < s; ++q)
-                v += -Pz[q][i] * Bt[q][j] - Bt[q][i] * Pz[q][j] + Bt[q][i] * Bt[q][j];
-            W[i][j] += v;
+        load_small(M0, sm + SL_G1 * SS, s);  // G1^-1
+        load_small(M1, sm + SL_QZ * SS, s);  // Q'Z
+        mm64(M0, M1, Bt, false, -1.0);       // beta = -G1^-1 Q'Z
+        load_small(M0, sm + SL_PZ * SS, s);  // P'Z
+        mm64(M0, Bt, M1, true, 1.0);         // (P'Z)' beta
+        for (int t = threadIdx.x; t < SS; t += blockDim.x) {
+            const int i = t / S, j = t % S;
+            W[i * LD + j] += M1[i * LD + j] + M1[j * LD + i];
         }
         __syncthreads();
+        mm64(Bt, Bt, M1, true, 1.0);         // beta' beta  (P'P = I)
+        for (int t = threadIdx.x; t < SS; t += blockDim.x) W[(t / S) * LD + t % S] += M1[(t / S) * LD + t % S];
+        __syncthreads();
+    } else {
+        for (int t = threadIdx.x; t < SS; t += blockDim.x) Bt[(t / S) * LD + t % S] = 0.0;
+        __syncthreads();
     }
-    chol_drop(W, s, keep2);
-    // T = L^-T.  Thread c forms column c of L^-1 by forward substitution
-    // (x_i = 0 for i < c and for dropped pivots) into Pz (free after the W
-    // update); that column is row c of T.
-    {
-        int c = threadIdx.x;
-        if (c < s) {
-            for (int i = 0; i < s; ++i) {
-                if (i < c || !keep2[i]) { Pz[c][i] = 0.0; continue; }
-                double v = (i == c) ? 1.0 : 0.0;
-                for (int q = c; q < i; ++q) v -= W[i][q] * Pz[c][q];
-                Pz[c][i] = v / W[i][i];
-            }
-        }
-    }
-    __syncthreads();
-    for (int t = threadIdx.x; t < S * S; t += blockDim.x) {
-        int i = t / S, j = t % S;
-        bool in = i < s && j < s;
-        sm[SL_T * S * S + t] = in ? Pz[i][j] : 0.0;
-    }
-    __syncthreads();
-    // M2 = beta T = -Bt T
-    for (int t = threadIdx.x; t < S * S; t += blockDim.x) {
-        int i = t / S, j = t % S;
-        double v = 0.0;
-        if (i < s && j < s && !init)
-            for (int q = 0; q < s; ++q) v -= Bt[i][q] * Pz[q][j];
-        sm[SL_M2 * S * S + t] = v;
-    }
+    inv_chol(W, M0, s);  // M0 = L^-1
+    for (int t = threadIdx.x; t < SS; t += blockDim.x) M1[(t / S) * LD + t % S] = M0[(t % S) * LD + t / S];
+    __syncthreads();     // M1 = T = L^-T
+    store_small(M1, sm + SL_T * SS);
+    mm64(Bt, M1, M0, false, 1.0);  // beta T
+    store_small(M0, sm + SL_M2 * SS);
 }
 
-// ---------------------------------------------------------------- residual for the block solver
-// R = B - A X (X in FP64, R in VT); partial |R|^2, |B|^2 per column -> part[(b*G+g)*2S]
+// ---------------------------------------------------------------- residual and layout kernels
+// R = B - A X (FP64, block layout), R stored as VT; partial |R|^2, |B|^2 per
+// column -> part[(b * G + g) * 2S]
 template <typename VT>
-__global__ void __launch_bounds__(256) k_bresid(int n, int nE, int nnz, const BlockDesc * __restrict__ blocks,
+__global__ void __launch_bounds__(256) k_bresid(int n, int nnz, const BlockDesc * __restrict__ blocks,
                                                 const int * __restrict__ rowptr, const int * __restrict__ colidx,
                                                 const double * __restrict__ vals, const double * __restrict__ X,
                                                 const double * __restrict__ B, VT * __restrict__ R,
                                                 double * __restrict__ part) {
     __shared__ double cn[4][2][S];
-    const BlockDesc bd = blocks[blockIdx.y];
+    const int b = blockIdx.y;
+    const BlockDesc bd = blocks[b];
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    const size_t voff = (size_t)bd.k * n * nE + bd.c0;
+    const size_t vb = (size_t)b * n * S + lane;
     const double * A = vals + (size_t)bd.k * nnz;
+    const int w = __builtin_amdgcn_readfirstlane(wave);
     double rr = 0.0, bb = 0.0;
-    for (int row = blockIdx.x * 4 + wave; row < n; row += gridDim.x * 4) {
-        if (lane < bd.s) {
-            double y = 0.0;
-            for (int t = rowptr[row]; t < rowptr[row + 1]; ++t) y += A[t] * X[voff + (size_t)colidx[t] * nE + lane];
-            const size_t o = voff + (size_t)row * nE + lane;
-            double b = B[o];
-            double r = b - y;
-            if (R) R[o] = (VT)r;
-            rr += r * r;
-            bb += b * b;
-        }
+    for (int row = blockIdx.x * 4 + w; row < n; row += gridDim.x * 4) {
+        double y = 0.0;
+        for (int t = rowptr[row]; t < rowptr[row + 1]; ++t) y += A[t] * X[vb + (size_t)colidx[t] * S];
+        const size_t o = vb + (size_t)row * S;
+        const double bv = B[o];
+        const double r = bv - y;
+        R[o] = (VT)r;
+        rr += r * r;
+        bb += bv * bv;
     }
     cn[wave][0][lane] = rr;
     cn[wave][1][lane] = bb;
     __syncthreads();
     if (wave == 0) {
-        double * out = part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 2 * S;
+        double * out = part + ((size_t)b * gridDim.x + blockIdx.x) * 2 * S;
         out[lane] = cn[0][0][lane] + cn[1][0][lane] + cn[2][0][lane] + cn[3][0][lane];
         out[S + lane] = cn[0][1][lane] + cn[1][1][lane] + cn[2][1][lane] + cn[3][1][lane];
     }
@@ -540,21 +628,38 @@ __global__ void k_norm_reduce(int G, const double * __restrict__ part, double * 
     out[(size_t)b * 2 * S + t] = s;
 }
 
-// X64 += (double) Xw ; Xw = 0   over the block's columns
-__global__ void k_accumulate(int n, int nE, const BlockDesc * __restrict__ blocks, float * __restrict__ Xw,
-                             double * __restrict__ X64) {
+// block layout <-> electrode-major solver layout [k][row][nE]
+__global__ void k_to_blocks(int n, int nE, const BlockDesc * __restrict__ blocks, const double * __restrict__ src,
+                            double * __restrict__ dst) {
     const BlockDesc bd = blocks[blockIdx.y];
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)n * S) return;
-    int row = (int)(t / S), c = (int)(t % S);
-    if (c >= bd.s) return;
-    size_t o = (size_t)bd.k * n * nE + bd.c0 + (size_t)row * nE + c;
+    const int row = (int)(t / S), c = (int)(t % S);
+    dst[(size_t)blockIdx.y * n * S + t] = c < bd.s ? src[((size_t)bd.k * n + row) * nE + bd.c0 + c] : 0.0;
+}
+
+__global__ void k_from_blocks(int n, int nE, const BlockDesc * __restrict__ blocks, const double * __restrict__ src,
+                              double * __restrict__ dst) {
+    const BlockDesc bd = blocks[blockIdx.y];
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)n * S) return;
+    const int row = (int)(t / S), c = (int)(t % S);
+    if (c < bd.s) dst[((size_t)bd.k * n + row) * nE + bd.c0 + c] = src[(size_t)blockIdx.y * n * S + t];
+}
+
+// X64 += (double) Xw ; Xw = 0   over the active blocks
+__global__ void k_accumulate(int n, const BlockDesc * __restrict__ blocks, float * __restrict__ Xw,
+                             double * __restrict__ X64) {
+    if (!blocks[blockIdx.y].active) return;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)n * S) return;
+    const size_t o = (size_t)blockIdx.y * n * S + t;
     X64[o] += (double)Xw[o];
     Xw[o] = 0.0f;
 }
 
 __global__ void k_to_float(long nn, const double * __restrict__ a, float * __restrict__ b) {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < nn) b[t] = (float)a[t];
 }
 
@@ -562,27 +667,6 @@ __global__ void k_to_float(long nn, const double * __restrict__ a, float * __res
 struct Plan {
     std::vector<BlockDesc> h;
     DBuf<BlockDesc> d;
-    int G = 0;
-};
-
-void make_plan(Ctx & c, Plan & pl) {
-    pl.h.clear();
-    for (int k = 0; k < c.nK; ++k)
-        for (int c0 = 0; c0 < c.nE; c0 += S) pl.h.push_back({k, c0, std::min(S, c.nE - c0), 1});
-    pl.d.upload(pl.h, c.stream);
-    pl.G = std::max(1, std::min(MAXG, div_up(c.n, CH)));
-}
-
-// column norms |R|^2 and |B|^2 of the block solver's current state, host copy
-void read_norms(Ctx & c, Plan & pl, DBuf<double> & nbuf, std::vector<double> & h) {
-    h.resize(pl.h.size() * 2 * S);
-    PG_CHECK(hipMemcpyAsync(h.data(), nbuf.p, h.size() * 8, hipMemcpyDeviceToHost, c.stream));
-    PG_CHECK(hipStreamSynchronize(c.stream));
-}
-
-template <typename VT, typename AT>
-struct BlockState {
-    DBuf<VT> X, R, P, Q;
 };
 
 }  // namespace
@@ -591,175 +675,213 @@ void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bo
     PG_REQUIRE(c.nE >= 1, "block solver: no electrodes");
     hipStream_t s = c.stream;
     Plan pl;
-    make_plan(c, pl);
+    for (int k = 0; k < c.nK; ++k)
+        for (int c0 = 0; c0 < c.nE; c0 += S) pl.h.push_back({k, c0, std::min(S, c.nE - c0), 1});
+    pl.d.upload(pl.h, s);
     const int nb = (int)pl.h.size();
-    const int G = pl.G;
-    const size_t tot = (size_t)c.nK * c.n * c.nE;
-    c.small.alloc((size_t)nb * NSL * S * S);
+    const size_t tot = (size_t)nb * c.n * S;
+    // workgroups per block: Gram passes (partials of 2-3 64 x 64 FP64
+    // matrices each), strip updates, SpMM and residual rows
+    const int G = std::max(1, std::min({256, div_up(1024, nb), div_up(c.n, 16)}));
+    const int GU = std::max(1, std::min({1024, div_up(4096, nb), div_up(c.n, 64)}));
+    const int GS = std::max(1, std::min(4096, div_up(c.n, 4)));
+    const int GR = std::max(1, std::min(1024, div_up(c.n, 4)));
+    c.small.alloc((size_t)nb * NSL * SS);
     PG_CHECK(hipMemsetAsync(c.small.p, 0, c.small.bytes(), s));
-    c.part.alloc((size_t)nb * G * (3 * S * S + S));
+    c.part.alloc((size_t)nb * std::max((size_t)G * 3 * SS, (size_t)GU * S));
+    DBuf<double> npart, norms;
+    npart.alloc((size_t)nb * GR * 2 * S);
+    norms.alloc((size_t)nb * 2 * S);
     const bool useCoarse = o.coarse && c.nAgg > 0;
+    const int m = useCoarse ? c.nAgg : 0;
     if (useCoarse) {
         coarse_prepare(c);
-        c.ycoarse.alloc((size_t)nb * c.nAgg * S);
-        c.ccoarse.alloc((size_t)nb * c.nAgg * S);
+        c.ycoarse.alloc((size_t)nb * m * S);
+        c.ccoarse.alloc((size_t)nb * m * S);
         PG_CHECK(hipMemsetAsync(c.ccoarse.p, 0, c.ccoarse.bytes(), s));
     }
-    DBuf<double> norms;
-    norms.alloc((size_t)nb * 2 * S);
-    DBuf<double> npart;
-    const int GR = std::max(1, std::min(1024, div_up(c.n, 4)));
-    npart.alloc((size_t)nb * GR * 2 * S);
-    std::vector<double> hn;
+    const double * ccp = useCoarse ? c.ccoarse.p : nullptr;
+    const int * aggp = useCoarse ? c.agg.p : nullptr;
+    PG_CHECK(hipFuncSetAttribute((const void *)k_small_alpha, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)SMALL_LDS));
+    PG_CHECK(hipFuncSetAttribute((const void *)k_small_beta, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)SMALL_LDS));
 
-    // FP64 true residual R = B - A X64 into Rw (VT), norms -> norms buffer
+    // right-hand sides and the FP64 solution in block layout
+    DBuf<double> Bb, Xb;
+    Bb.alloc(tot);
+    Xb.alloc(tot);
+    const dim3 gT(div_up((long)c.n * S, 256), nb), blk(256);
+    hipLaunchKernelGGL(k_to_blocks, gT, blk, 0, s, c.n, c.nE, pl.d.p, c.B.p, Bb.p);
+    hipLaunchKernelGGL(k_to_blocks, gT, blk, 0, s, c.n, c.nE, pl.d.p, c.X.p, Xb.p);
+    PG_CHECK(hipGetLastError());
+
+    std::vector<double> hn(nb * 2 * S);
+    // FP64 true residual R = B - A X into Rw (VT); norms to the host
     auto true_resid = [&](auto * Rw) {
         using VT = std::remove_pointer_t<decltype(Rw)>;
-        hipLaunchKernelGGL(k_bresid<VT>, dim3(GR, nb), dim3(256), 0, s, c.n, c.nE, c.nnz, pl.d.p, c.rowptr.p,
-                           c.colidx.p, c.valsA.p, c.X.p, c.B.p, Rw, npart.p);
+        hipLaunchKernelGGL(k_bresid<VT>, dim3(GR, nb), blk, 0, s, c.n, c.nnz, pl.d.p, c.rowptr.p, c.colidx.p,
+                           c.valsA.p, Xb.p, Bb.p, Rw, npart.p);
         hipLaunchKernelGGL(k_norm_reduce, dim3(nb), dim3(2 * S), 0, s, GR, npart.p, norms.p);
         PG_CHECK(hipGetLastError());
+        PG_CHECK(hipMemcpyAsync(hn.data(), norms.p, hn.size() * 8, hipMemcpyDeviceToHost, s));
+        PG_CHECK(hipStreamSynchronize(s));
     };
-    auto max_rel = [&](const std::vector<double> & h, std::vector<double> * per) {
-        double worst = 0;
-        for (int b = 0; b < nb; ++b)
-            for (int j = 0; j < pl.h[b].s; ++j) {
-                double rr = h[(size_t)b * 2 * S + j], bb = h[(size_t)b * 2 * S + S + j];
-                double r = bb > 0 ? std::sqrt(rr / bb) : 0.0;
-                worst = std::max(worst, r);
-            }
-        (void)per;
-        return worst;
+    // worst relative residual of block b from the norms just read
+    auto block_rel = [&](int b) {
+        double w = 0.0;
+        for (int j = 0; j < pl.h[b].s; ++j) {
+            const double rr = hn[(size_t)b * 2 * S + j], bb = hn[(size_t)b * 2 * S + S + j];
+            if (bb > 0) w = std::max(w, std::sqrt(rr / bb));
+        }
+        return w;
     };
 
     auto run = [&](auto vtag, auto atag, const auto * Aw) {
         using VT = decltype(vtag);
         using AT = decltype(atag);
-        const bool fp32 = std::is_same<VT, float>::value;
-        // compulsory bytes of one k_bspmm_gram launch: A values + column
-        // indices + row pointers, P read once, R read, Q written
-        c.kBytesPerLaunch = (double)c.nK * ((double)c.nnz * (sizeof(AT) + 4) + 4.0 * (c.n + 1)) +
-                            3.0 * (double)tot * sizeof(VT);
-        // working vectors: X (correction since last replacement), R, P, Q
+        constexpr bool fp32 = std::is_same<VT, float>::value;
+        constexpr int WS = fp32 ? 1 : 2;
+        // compulsory HBM bytes of one launch of the timed kernel (all blocks
+        // active): matrix values and indices once, each vector once
+        const double vec = (double)tot * sizeof(VT);
+        const double pre = (double)c.nK * c.n * 8.0 + (useCoarse ? c.n * 4.0 + (double)nb * m * S * 8.0 : 0.0);
+        switch (c.kSelect) {
+        case PGERT_TIME_SPMM:
+            c.kBytesPerLaunch = (double)c.nK * ((double)c.nnz * (sizeof(AT) + 4) + 4.0 * (c.n + 1)) + 2.0 * vec;
+            break;
+        case PGERT_TIME_UPDATE: c.kBytesPerLaunch = 6.0 * vec; break;
+        case PGERT_TIME_GRAM_Z: c.kBytesPerLaunch = 3.0 * vec + pre; break;
+        case PGERT_TIME_PUPDATE: c.kBytesPerLaunch = 3.0 * vec + pre; break;
+        case PGERT_TIME_GRAM_PQ: c.kBytesPerLaunch = 3.0 * vec; break;
+        default: break;
+        }
+        auto tb = [&](int id) {
+            if (c.kSelect == id) c.kev_begin();
+        };
+        auto te = [&](int id) {
+            if (c.kSelect == id) c.kev_end();
+        };
         DBuf<VT> Xw, Rw, Pw, Qw;
-        DBuf<double> dinv;
         Rw.alloc(tot);
         Pw.alloc(tot);
         Qw.alloc(tot);
         VT * Xp;
-        if (fp32) {
+        if constexpr (fp32) {
             Xw.alloc(tot);
             PG_CHECK(hipMemsetAsync(Xw.p, 0, Xw.bytes(), s));
             Xp = Xw.p;
         } else {
-            Xp = (VT *)c.X.p;
+            Xp = Xb.p;
         }
         PG_CHECK(hipMemsetAsync(Pw.p, 0, Pw.bytes(), s));
         PG_CHECK(hipMemsetAsync(Qw.p, 0, Qw.bytes(), s));
         true_resid(Rw.p);
-        read_norms(c, pl, norms, hn);
         std::vector<double> bnorm2(nb * S);
         for (int b = 0; b < nb; ++b)
             for (int j = 0; j < S; ++j) bnorm2[b * S + j] = hn[(size_t)b * 2 * S + S + j];
-        double rel = max_rel(hn, nullptr);
-        double lastRel = rel;
+        double lastRel = 0.0;
+        for (int b = 0; b < nb; ++b) {
+            const double r = block_rel(b);
+            if (r <= o.rtol) pl.h[b].active = 0;
+            lastRel = std::max(lastRel, r);
+        }
+        PG_CHECK(hipMemcpyAsync(pl.d.p, pl.h.data(), pl.h.size() * sizeof(BlockDesc), hipMemcpyHostToDevice, s));
         int it = 0, refine = 0, lastReplace = 0;
-        const dim3 gG(G, nb), blk(256);
-        const int cQZ = 3 * S * S + S;
-        const int m = useCoarse ? c.nAgg : 0;
-        const double * ccp = useCoarse ? c.ccoarse.p : nullptr;
-        const int * aggp = useCoarse ? c.agg.p : nullptr;
-        // X += P alpha, R -= Q alpha, then Z = M^-1 R and the Grams Q'Z, Z'Z, P'Z, |R|^2
-        auto update_and_z = [&](bool updateXR) {
-            if (!useCoarse) {
-                hipLaunchKernelGGL(k_bupdate<VT>, gG, blk, 0, s, c.n, c.nE, pl.d.p, c.small.p, c.dinv.p, Pw.p, Qw.p,
-                                   Xp, Rw.p, c.part.p, 1);
-            } else {
-                if (updateXR)
-                    hipLaunchKernelGGL(k_bupdate<VT>, gG, blk, 0, s, c.n, c.nE, pl.d.p, c.small.p, c.dinv.p, Pw.p,
-                                       Qw.p, Xp, Rw.p, c.part.p, 0);
-                hipLaunchKernelGGL(k_restrict<VT>, dim3(div_up(m, 4), nb), blk, 0, s, c.n, c.nE, m, pl.d.p,
-                                   c.aggPtr.p, c.aggNodes.p, Rw.p, c.ycoarse.p);
-                hipLaunchKernelGGL(k_coarse_apply, dim3(div_up(m, 4), nb), blk, 0, s, m, pl.d.p, c.Einv.p,
+        const dim3 gG(G, nb), gU(GU, nb), gS(GS, nb);
+        auto z_grams = [&]() {
+            if (useCoarse) {
+                hipLaunchKernelGGL(k_restrict<VT>, dim3(m, nb), blk, 0, s, c.n, m, pl.d.p, c.aggPtr.p, c.aggNodes.p,
+                                   Rw.p, c.ycoarse.p);
+                hipLaunchKernelGGL(k_coarse_apply, dim3(div_up(m, 16), 4, nb), blk, 0, s, m, pl.d.p, c.Einv.p,
                                    c.ycoarse.p, c.ccoarse.p);
-                hipLaunchKernelGGL(k_zgram<VT>, gG, blk, 0, s, c.n, c.nE, m, pl.d.p, c.dinv.p, Pw.p, Qw.p, Rw.p,
-                                   c.ccoarse.p, c.agg.p, c.part.p);
             }
-            hipLaunchKernelGGL(k_gram_reduce, dim3(div_up(cQZ, 256), nb), blk, 0, s, pl.d.p, G, cQZ, c.part.p,
-                               c.small.p, SL_QZ * S * S);
+            tb(PGERT_TIME_GRAM_Z);
+            hipLaunchKernelGGL((k_gram<VT, MODE_Z, WS>), gG, blk, 0, s, c.n, m, pl.d.p, c.dinv.p, Pw.p, Qw.p, Rw.p,
+                               ccp, aggp, c.part.p);
+            te(PGERT_TIME_GRAM_Z);
+            hipLaunchKernelGGL(k_part_reduce, dim3(3 * SS / 32, nb), blk, 0, s, pl.d.p, G, 3 * SS, c.part.p,
+                               c.small.p, SL_QZ * SS);
         };
         auto start_block = [&]() {
-            // alpha = 0: only Z = M^-1 R and Z'Z are formed; then P = Z chol(Z'Z)^-T
+            // P = 0, Q = 0: Z = M^-1 R and Z'Z only; then P = Z chol(Z'Z)^-T
             PG_CHECK(hipMemsetAsync(c.small.p, 0, c.small.bytes(), s));
-            PG_CHECK(hipMemsetAsync(Pw.p, 0, Pw.bytes(), s));
-            update_and_z(false);
-            hipLaunchKernelGGL(k_small_beta, dim3(nb), blk, 0, s, pl.d.p, c.small.p, 1);
-            hipLaunchKernelGGL(k_bpupdate<VT>, gG, blk, 0, s, c.n, c.nE, pl.d.p, c.small.p, c.dinv.p, Rw.p, Pw.p,
-                               ccp, aggp, m);
+            z_grams();
+            hipLaunchKernelGGL(k_small_beta, dim3(nb), blk, SMALL_LDS, s, pl.d.p, c.small.p, 1);
+            hipLaunchKernelGGL(k_pupdate<VT>, gU, blk, 0, s, c.n, m, pl.d.p, c.small.p, c.dinv.p, Rw.p, Pw.p, ccp,
+                               aggp);
             PG_CHECK(hipGetLastError());
         };
-        if (rel > o.rtol) start_block();
-        bool done = rel <= o.rtol;
+        bool done = true;
+        for (int b = 0; b < nb; ++b)
+            if (pl.h[b].active) done = false;
+        if (!done) start_block();
+        std::vector<double> hs((size_t)nb * S);
         while (!done && it < o.max_iter) {
-            int chunk = std::min(o.check_every, o.max_iter - it);
+            const int chunk = std::min(o.check_every, o.max_iter - it);
             for (int j = 0; j < chunk; ++j) {
-                c.kev_begin();
-                hipLaunchKernelGGL((k_bspmm_gram<VT, AT>), gG, blk, 0, s, c.n, c.nE, c.nnz, pl.d.p, c.rowptr.p,
-                                   c.colidx.p, Aw, Pw.p, Rw.p, Qw.p, c.part.p);
-                c.kev_end();
-                hipLaunchKernelGGL(k_gram_reduce, dim3(div_up(2 * S * S, 256), nb), blk, 0, s, pl.d.p, G, 2 * S * S,
-                                   c.part.p, c.small.p, SL_G1 * S * S);
-                hipLaunchKernelGGL(k_small_alpha, dim3(nb), blk, 0, s, pl.d.p, c.small.p);
-                update_and_z(true);
-                hipLaunchKernelGGL(k_small_beta, dim3(nb), blk, 0, s, pl.d.p, c.small.p, 0);
-                hipLaunchKernelGGL(k_bpupdate<VT>, gG, blk, 0, s, c.n, c.nE, pl.d.p, c.small.p, c.dinv.p, Rw.p,
-                                   Pw.p, ccp, aggp, m);
+                tb(PGERT_TIME_SPMM);
+                hipLaunchKernelGGL((k_spmm<VT, AT>), gS, blk, 0, s, c.n, c.nnz, pl.d.p, c.rowptr.p, c.colidx.p, Aw,
+                                   Pw.p, Qw.p);
+                te(PGERT_TIME_SPMM);
+                tb(PGERT_TIME_GRAM_PQ);
+                hipLaunchKernelGGL((k_gram<VT, MODE_PQR, WS>), gG, blk, 0, s, c.n, m, pl.d.p, c.dinv.p, Pw.p, Qw.p,
+                                   Rw.p, ccp, aggp, c.part.p);
+                te(PGERT_TIME_GRAM_PQ);
+                hipLaunchKernelGGL(k_part_reduce, dim3(2 * SS / 32, nb), blk, 0, s, pl.d.p, G, 2 * SS, c.part.p,
+                                   c.small.p, SL_G1 * SS);
+                hipLaunchKernelGGL(k_small_alpha, dim3(nb), blk, SMALL_LDS, s, pl.d.p, c.small.p);
+                tb(PGERT_TIME_UPDATE);
+                hipLaunchKernelGGL(k_update<VT>, gU, blk, 0, s, c.n, pl.d.p, c.small.p, Pw.p, Qw.p, Xp, Rw.p,
+                                   c.part.p);
+                te(PGERT_TIME_UPDATE);
+                if (j == chunk - 1)
+                    hipLaunchKernelGGL(k_part_reduce, dim3(S / 32, nb), blk, 0, s, pl.d.p, GU, S, c.part.p,
+                                       c.small.p, SL_RR * SS);
+                z_grams();
+                hipLaunchKernelGGL(k_small_beta, dim3(nb), blk, SMALL_LDS, s, pl.d.p, c.small.p, 0);
+                tb(PGERT_TIME_PUPDATE);
+                hipLaunchKernelGGL(k_pupdate<VT>, gU, blk, 0, s, c.n, m, pl.d.p, c.small.p, c.dinv.p, Rw.p, Pw.p,
+                                   ccp, aggp);
+                te(PGERT_TIME_PUPDATE);
             }
             PG_CHECK(hipGetLastError());
             it += chunk;
-            // recursive residual norms of the last k_bupdate
-            std::vector<double> hs((size_t)nb * S);
+            // recursive residual norms of the last k_update
             for (int b = 0; b < nb; ++b)
-                PG_CHECK(hipMemcpyAsync(hs.data() + (size_t)b * S, c.small.p + ((size_t)b * NSL + SL_RR) * S * S,
-                                        S * 8, hipMemcpyDeviceToHost, s));
+                PG_CHECK(hipMemcpyAsync(hs.data() + (size_t)b * S, c.small.p + ((size_t)b * NSL + SL_RR) * SS, S * 8,
+                                        hipMemcpyDeviceToHost, s));
             PG_CHECK(hipStreamSynchronize(s));
             c.kev_collect();
-            // per-block worst recursive residual of the still-active blocks
-            std::vector<double> wb(nb, 0.0);
             double worst = 0;
             bool anyAtTarget = false;
             for (int b = 0; b < nb; ++b) {
                 if (!pl.h[b].active) continue;
+                double wb = 0.0;
                 for (int j = 0; j < pl.h[b].s; ++j) {
-                    double bb = bnorm2[b * S + j];
-                    if (bb > 0) wb[b] = std::max(wb[b], std::sqrt(hs[(size_t)b * S + j] / bb));
+                    const double bb = bnorm2[b * S + j];
+                    if (bb > 0) wb = std::max(wb, std::sqrt(hs[(size_t)b * S + j] / bb));
                 }
-                worst = std::max(worst, wb[b]);
-                if (wb[b] <= o.rtol) anyAtTarget = true;
+                worst = std::max(worst, wb);
+                if (wb <= o.rtol) anyAtTarget = true;
             }
-            bool changed = false;
             // Residual replacement: the true FP64 residual b - A x decides which
             // blocks are done.  FP32 recursions are replaced when they have
             // gained a decade, reached the target or run 256 iterations; FP64
             // ones when a block's recursive residual reaches the target.
-            const bool replaceNow = fp32 ? (worst < 0.1 * lastRel || anyAtTarget || it - lastReplace >= 256)
-                                         : anyAtTarget;
+            const bool replaceNow =
+                fp32 ? (worst < 0.1 * lastRel || anyAtTarget || it - lastReplace >= 256) : anyAtTarget;
+            bool changed = false;
             if (replaceNow) {
                 lastReplace = it;
-                if (fp32)
-                    hipLaunchKernelGGL(k_accumulate, dim3(div_up((long)c.n * S, 256), nb), blk, 0, s, c.n, c.nE,
-                                       pl.d.p, (float *)Xp, c.X.p);
+                if constexpr (fp32)
+                    hipLaunchKernelGGL(k_accumulate, gT, blk, 0, s, c.n, pl.d.p, Xw.p, Xb.p);
                 true_resid(Rw.p);
-                read_norms(c, pl, norms, hn);
                 ++refine;
                 lastRel = 0.0;
                 for (int b = 0; b < nb; ++b) {
                     if (!pl.h[b].active) continue;
-                    double tb = 0.0;
-                    for (int j = 0; j < pl.h[b].s; ++j) {
-                        double rr = hn[(size_t)b * 2 * S + j], bb = hn[(size_t)b * 2 * S + S + j];
-                        if (bb > 0) tb = std::max(tb, std::sqrt(rr / bb));
-                    }
+                    const double tb = block_rel(b);
                     if (tb <= o.rtol) {
                         pl.h[b].active = 0;
                         changed = true;
@@ -774,24 +896,26 @@ void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bo
             if (changed && !done)
                 PG_CHECK(hipMemcpyAsync(pl.d.p, pl.h.data(), pl.h.size() * sizeof(BlockDesc), hipMemcpyHostToDevice, s));
         }
-        if (fp32) {
-            hipLaunchKernelGGL(k_accumulate, dim3(div_up((long)c.n * S, 256), nb), dim3(256), 0, s, c.n, c.nE,
-                               pl.d.p, (float *)Xp, c.X.p);
+        if constexpr (fp32) {
+            hipLaunchKernelGGL(k_accumulate, gT, blk, 0, s, c.n, pl.d.p, Xw.p, Xb.p);
             PG_CHECK(hipGetLastError());
         }
         st.iterations = it;
         st.max_iterations = it;
         st.refinements = refine;
+        PG_CHECK(hipStreamSynchronize(s));  // work buffers are released on return
     };
 
     if (mixed) {
         c.fA.alloc((size_t)c.nK * c.nnz);
-        long nn = (long)c.fA.n;
-        hipLaunchKernelGGL(k_to_float, dim3(div_up(nn, 256)), dim3(256), 0, s, nn, c.valsA.p, c.fA.p);
+        const long nn = (long)c.fA.n;
+        hipLaunchKernelGGL(k_to_float, dim3(div_up(nn, 256)), blk, 0, s, nn, c.valsA.p, c.fA.p);
         run(float(0), float(0), (const float *)c.fA.p);
     } else {
         run(double(0), double(0), (const double *)c.valsA.p);
     }
+    hipLaunchKernelGGL(k_from_blocks, gT, blk, 0, s, c.n, c.nE, pl.d.p, Xb.p, c.X.p);
+    PG_CHECK(hipGetLastError());
     st.max_rel_residual = residual_check(c);
     st.converged = st.max_rel_residual <= o.rtol * 1.0001 ? 1 : 0;
 }
