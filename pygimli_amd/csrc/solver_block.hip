@@ -88,57 +88,82 @@ template <typename VT> __device__ inline V4<VT> ld4(const VT * p) { return *rein
 template <typename VT> __device__ inline void st4(VT * p, V4<VT> v) { *reinterpret_cast<V4<VT> *>(p) = v; }
 
 // ---------------------------------------------------------------- 1. Q = A P
-// one wavefront per row (row index wavefront-uniform: the CSR row is read
-// through the scalar unit), lanes over the 64 columns, 4 nonzeros in flight
+// a wavefront takes 4 rows at once (16 lanes per row, 4 columns per lane as
+// one vector load) with 4 nonzeros of each row in flight.
+// XCD-aware rows: workgroups are dispatched round-robin over the 8 XCDs
+// (workgroup id % 8), so XCD x is given the contiguous rows [x n/8, (x+1) n/8)
+// and the block rows it gathers (neighbours within a few mesh planes) stay in
+// its own L2.  gridDim.x is a multiple of 8.
+constexpr int NXCD = 8;
+
+__device__ inline void xcd_rows(int n, int & lo, int & hi, int & first, int & stride) {
+    const int x = blockIdx.x % NXCD, wi = blockIdx.x / NXCD, nw = gridDim.x / NXCD;
+    lo = (int)((long)n * x / NXCD);
+    hi = (int)((long)n * (x + 1) / NXCD);
+    first = lo + wi * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+    stride = nw * 4;
+}
 template <typename VT, typename AT>
 __global__ void __launch_bounds__(256) k_spmm(int n, int nnz, const BlockDesc * __restrict__ blocks,
                                               const int * __restrict__ rowptr, const int * __restrict__ colidx,
                                               const AT * __restrict__ vals, const VT * __restrict__ P,
                                               VT * __restrict__ Q) {
+    using V = V4<VT>;
     const int b = blockIdx.y;
     const BlockDesc bd = blocks[b];
     if (!bd.active) return;
     const int lane = threadIdx.x % 64;
-    const VT * Pb = P + (size_t)b * n * S + lane;
-    VT * Qb = Q + (size_t)b * n * S + lane;
+    const VT * Pb = P + (size_t)b * n * S + 4 * (lane % 16);
+    VT * Qb = Q + (size_t)b * n * S + 4 * (lane % 16);
     const AT * A = vals + (size_t)bd.k * nnz;
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
-    for (int row = blockIdx.x * 4 + w; row < n; row += gridDim.x * 4) {
-        const int q0 = rowptr[row], q1 = rowptr[row + 1];
-        VT acc0 = 0, acc1 = 0;
-        int q = q0;
+    const int x = blockIdx.x % NXCD, nw = gridDim.x / NXCD;
+    const int lo = (int)((long)n * x / NXCD), hi = (int)((long)n * (x + 1) / NXCD);
+    const int g0 = (blockIdx.x / NXCD) * 4 + threadIdx.x / 64;
+    for (int r0 = lo + 4 * g0; r0 < hi; r0 += 16 * nw) {
+        const int row = r0 + lane / 16;
+        if (row >= hi) continue;
+        V acc0 = V{0, 0, 0, 0}, acc1 = acc0;
+        int q = rowptr[row];
+        const int q1 = rowptr[row + 1];
         for (; q + 4 <= q1; q += 4) {
             const int c0 = colidx[q], c1 = colidx[q + 1], c2 = colidx[q + 2], c3 = colidx[q + 3];
-            const VT x0 = Pb[(size_t)c0 * S], x1 = Pb[(size_t)c1 * S];
-            const VT x2 = Pb[(size_t)c2 * S], x3 = Pb[(size_t)c3 * S];
-            acc0 += (VT)A[q] * x0 + (VT)A[q + 1] * x1;
-            acc1 += (VT)A[q + 2] * x2 + (VT)A[q + 3] * x3;
+            const VT a0 = (VT)A[q], a1 = (VT)A[q + 1], a2 = (VT)A[q + 2], a3 = (VT)A[q + 3];
+            const V x0 = ld4(Pb + (size_t)c0 * S), x1 = ld4(Pb + (size_t)c1 * S);
+            const V x2 = ld4(Pb + (size_t)c2 * S), x3 = ld4(Pb + (size_t)c3 * S);
+            acc0 += a0 * x0 + a1 * x1;
+            acc1 += a2 * x2 + a3 * x3;
         }
-        for (; q < q1; ++q) acc0 += (VT)A[q] * Pb[(size_t)colidx[q] * S];
-        Qb[(size_t)row * S] = acc0 + acc1;
+        for (; q < q1; ++q) acc0 += (VT)A[q] * ld4(Pb + (size_t)colidx[q] * S);
+        st4(Qb + (size_t)row * S, acc0 + acc1);
     }
 }
 
 // ---------------------------------------------------------------- Gram passes
 // MODE_PQR: G1 = P'Q, G2 = P'R.  MODE_Z: Z = D^-1 R (+ coarse prolongation
 // cc[agg[row]]) and Q'Z, Z'Z, P'Z.
-// Wavefront w of a workgroup: row group w / WS, column tiles tb = w % WS +
-// WS x (WS = 2 halves the accumulators of the FP64 variant).  Tile (ta, tb)
-// of a Gram holds entries (4i + ta, 4j + tb), i, j < 16; its MFMA output
-// register jj of lane l is entry (4 drow(l, jj) + ta, 4 (l%16) + tb).
+// Wavefront w of a workgroup: row group w / 2, column tiles tb = w % 2 + 2 x.
+// Tile (ta, tb) of a Gram holds entries (4i + ta, 4j + tb), i, j < 16; its
+// MFMA output register jj of lane l is entry (4 drow(l, jj) + ta, 4 (l%16) + tb).
+// FP32 inputs: the matrix cores accumulate FP32 over FLUSH steps (16 rows),
+// then the tile is added into FP64 registers -- the Grams keep FP64
+// accumulation (their rounding steers the block recurrence) at the FP32
+// matrix-core rate.  FP64 inputs accumulate in FP64 directly.
 // Workgroup partials: part[((blk * G + g) * NG + gram) * S*S].
 constexpr int MODE_PQR = 0;
 constexpr int MODE_Z = 1;
+constexpr int FLUSH = 4;
 
-template <typename VT, int MODE, int WS>
+template <typename VT, int MODE>
 __global__ void __launch_bounds__(256) k_gram(int n, int m, const BlockDesc * __restrict__ blocks,
                                               const double * __restrict__ dinv, const VT * __restrict__ P,
                                               const VT * __restrict__ Q, const VT * __restrict__ R,
                                               const double * __restrict__ cc, const int * __restrict__ agg,
                                               double * __restrict__ part) {
     constexpr int NG = MODE == MODE_PQR ? 2 : 3;
+    constexpr int WS = 2;        // wavefronts sharing a row group
     constexpr int RG = 4 / WS;   // row groups per workgroup
     constexpr int NTB = 4 / WS;  // column tiles per wavefront
+    constexpr bool F32 = sizeof(VT) == 4;
     using V = V4<VT>;
     __shared__ double red[S][S + 1];
     const int b = blockIdx.y;
@@ -150,22 +175,70 @@ __global__ void __launch_bounds__(256) k_gram(int n, int m, const BlockDesc * __
     const int col = 4 * (lane % 16);
     const double * dk = dinv + (size_t)bd.k * n;
     V acc[NG][4][NTB];
+    f64x4 acc64[F32 ? NG : 1][4][NTB];
 #pragma unroll
     for (int g = 0; g < NG; ++g)
 #pragma unroll
         for (int ta = 0; ta < 4; ++ta)
 #pragma unroll
-            for (int x = 0; x < NTB; ++x) acc[g][ta][x] = V{0, 0, 0, 0};
-    const int nst = (n + 3) / 4;
-    for (int st = blockIdx.x * RG + rg; st < nst; st += gridDim.x * RG) {
-        const int row = 4 * st + lane / 16;
-        V p = V{0, 0, 0, 0}, q = p, r = p;
-        if (row < n) {
-            const size_t o = vb + (size_t)row * S + col;
-            p = ld4(P + o);
-            q = ld4(Q + o);
-            r = ld4(R + o);
+            for (int x = 0; x < NTB; ++x) {
+                acc[g][ta][x] = V{0, 0, 0, 0};
+                if constexpr (F32) acc64[g][ta][x] = f64x4{0, 0, 0, 0};
+            }
+    auto flush = [&]() {
+        if constexpr (F32) {
+#pragma unroll
+            for (int g = 0; g < NG; ++g)
+#pragma unroll
+                for (int ta = 0; ta < 4; ++ta)
+#pragma unroll
+                    for (int x = 0; x < NTB; ++x) {
+#pragma unroll
+                        for (int jj = 0; jj < 4; ++jj) acc64[g][ta][x][jj] += (double)acc[g][ta][x][jj];
+                        acc[g][ta][x] = V{0, 0, 0, 0};
+                    }
         }
+    };
+    const int nst = (n + 3) / 4;
+    const int stride = gridDim.x * RG;
+    // software pipeline: the rows of the next step are in flight while the
+    // matrix cores work on the current one
+    auto fetch = [&](int stp, V & pp, V & qq, V & rr, double & dd, int & ag) {
+        const int row = 4 * stp + lane / 16;
+        pp = qq = rr = V{0, 0, 0, 0};
+        dd = 0.0;
+        ag = 0;
+        if (stp < nst && row < n) {
+            if constexpr (MODE == MODE_Z) {
+                if (cc) ag = agg[row];
+                dd = dk[row];
+            }
+            const size_t o = vb + (size_t)row * S + col;
+            pp = ld4(P + o);
+            qq = ld4(Q + o);
+            rr = ld4(R + o);
+        }
+    };
+    auto fetch_cc = [&](int stp, int ag) {
+        f64x4 v = {0, 0, 0, 0};
+        if constexpr (MODE == MODE_Z) {
+            const int row = 4 * stp + lane / 16;
+            if (cc && stp < nst && row < n) v = ld4(cc + ((size_t)b * m + ag) * S + col);
+        }
+        return v;
+    };
+    int st = blockIdx.x * RG + rg;
+    V p, q, r;
+    double d;
+    int ag;
+    fetch(st, p, q, r, d, ag);
+    f64x4 c4 = fetch_cc(st, ag);
+    int cnt = 0;
+    for (; st < nst; st += stride) {
+        V pn, qn, rn;
+        double dn;
+        int agn;
+        fetch(st + stride, pn, qn, rn, dn, agn);
         V a[NG], bo[NG];
         if constexpr (MODE == MODE_PQR) {
             a[0] = p;
@@ -173,18 +246,9 @@ __global__ void __launch_bounds__(256) k_gram(int n, int m, const BlockDesc * __
             bo[0] = q;
             bo[1] = r;
         } else {
-            V z = V{0, 0, 0, 0};
-            if (row < n) {
-                const double d = dk[row];
-                if (cc) {
-                    const double * cr = cc + ((size_t)b * m + agg[row]) * S + col;
+            V z;
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) z[e] = (VT)(d * (double)r[e] + cr[e]);
-                } else {
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) z[e] = (VT)(d * (double)r[e]);
-                }
-            }
+            for (int e = 0; e < 4; ++e) z[e] = (VT)(d * (double)r[e] + c4[e]);
             a[0] = q;
             a[1] = z;
             a[2] = p;
@@ -198,7 +262,17 @@ __global__ void __launch_bounds__(256) k_gram(int n, int m, const BlockDesc * __
             for (int ta = 0; ta < 4; ++ta)
 #pragma unroll
                 for (int x = 0; x < NTB; ++x) acc[g][ta][x] = mfma4(a[g][ta], bo[g][ws + WS * x], acc[g][ta][x]);
+        if (++cnt == FLUSH) {
+            cnt = 0;
+            flush();
+        }
+        c4 = fetch_cc(st + stride, agn);
+        p = pn;
+        q = qn;
+        r = rn;
+        d = dn;
     }
+    flush();
     double * out = part + ((size_t)b * gridDim.x + blockIdx.x) * NG * SS;
 #pragma unroll
     for (int g = 0; g < NG; ++g) {
@@ -211,7 +285,11 @@ __global__ void __launch_bounds__(256) k_gram(int n, int m, const BlockDesc * __
 #pragma unroll
                         for (int jj = 0; jj < 4; ++jj) {
                             const int ai = 4 * drow<VT>(lane, jj) + ta, bj = 4 * (lane % 16) + ws + WS * x;
-                            const double v = (double)acc[g][ta][x][jj];
+                            double v;
+                            if constexpr (F32)
+                                v = acc64[g][ta][x][jj];
+                            else
+                                v = acc[g][ta][x][jj];
                             red[ai][bj] = rr == 0 ? v : red[ai][bj] + v;
                         }
             }
@@ -269,42 +347,71 @@ __global__ void __launch_bounds__(256) k_update(int n, const BlockDesc * __restr
     const int cq = 4 * (lane / 16);
     double rr[4][4] = {};
     const int nstrip = (n + 15) / 16;
-    for (int sidx = blockIdx.x * 4 + wave; sidx < nstrip; sidx += gridDim.x * 4) {
-        const int row = 16 * sidx + lane % 16;
-        const bool ok = row < n;
+    const int stride = gridDim.x * 4;
+    // software pipeline: the next strip's P and Q are in flight while the
+    // matrix cores work on the current one
+    auto fetch = [&](int si, V (&pp)[4], V (&qq)[4]) {
+        const int row = 16 * si + lane % 16;
+        const bool ok = si < nstrip && row < n;
         const size_t o = vb + (size_t)row * S + cq;
-        V p[4], q[4], cx[4], cr[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            p[u] = ok ? ld4(P + o + 16 * u) : V{0, 0, 0, 0};
-            q[u] = ok ? ld4(Q + o + 16 * u) : V{0, 0, 0, 0};
-            cx[u] = cr[u] = V{0, 0, 0, 0};
+            pp[u] = ok ? ld4(P + o + 16 * u) : V{0, 0, 0, 0};
+            qq[u] = ok ? ld4(Q + o + 16 * u) : V{0, 0, 0, 0};
         }
+    };
+    int sidx = blockIdx.x * 4 + wave;
+    V p[4], q[4];
+    fetch(sidx, p, q);
+    for (; sidx < nstrip; sidx += stride) {
+        const int row = 16 * sidx + lane % 16;
+        const bool ok = row < n;
+        const size_t o = vb + (size_t)row * S + cq, ob = vb + (size_t)row * S;
+        // this strip's X and R in the result layout (independent of the products)
+        V xv[4], rv[4];
+#pragma unroll
+        for (int tb = 0; tb < 4; ++tb) {
+            if constexpr (sizeof(VT) == 4) {
+                xv[tb] = ok ? ld4(X + o + 16 * tb) : V{0, 0, 0, 0};
+                rv[tb] = ok ? ld4(R + o + 16 * tb) : V{0, 0, 0, 0};
+            } else {
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj) {
+                    const size_t oc = ob + 16 * tb + drow<VT>(lane, jj);
+                    xv[tb][jj] = ok ? X[oc] : 0.0;
+                    rv[tb][jj] = ok ? R[oc] : 0.0;
+                }
+            }
+        }
+        V pn[4], qn[4], cx[4], cr[4];
+        fetch(sidx + stride, pn, qn);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) cx[u] = cr[u] = V{0, 0, 0, 0};
         strip_mm(op, p, cx, lane);
         strip_mm(op, q, cr, lane);
         if (ok) {
-            if constexpr (sizeof(VT) == 4) {
 #pragma unroll
-                for (int tb = 0; tb < 4; ++tb) {
-                    st4(X + o + 16 * tb, ld4(X + o + 16 * tb) + cx[tb]);
-                    const V rn = ld4(R + o + 16 * tb) - cr[tb];
+            for (int tb = 0; tb < 4; ++tb) {
+                const V xn = xv[tb] + cx[tb], rn = rv[tb] - cr[tb];
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj) rr[tb][jj] += (double)rn[jj] * (double)rn[jj];
+                if constexpr (sizeof(VT) == 4) {
+                    st4(X + o + 16 * tb, xn);
                     st4(R + o + 16 * tb, rn);
-#pragma unroll
-                    for (int jj = 0; jj < 4; ++jj) rr[tb][jj] += (double)rn[jj] * (double)rn[jj];
-                }
-            } else {
-                const size_t ob = vb + (size_t)row * S;
-#pragma unroll
-                for (int tb = 0; tb < 4; ++tb)
+                } else {
 #pragma unroll
                     for (int jj = 0; jj < 4; ++jj) {
                         const size_t oc = ob + 16 * tb + drow<VT>(lane, jj);
-                        X[oc] += cx[tb][jj];
-                        const VT rn = R[oc] - cr[tb][jj];
-                        R[oc] = rn;
-                        rr[tb][jj] += (double)rn * (double)rn;
+                        X[oc] = xn[jj];
+                        R[oc] = rn[jj];
                     }
+                }
             }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            p[u] = pn[u];
+            q[u] = qn[u];
         }
     }
     // sum over the 16 rows of a lane group, then over the wavefronts
@@ -343,20 +450,48 @@ __global__ void __launch_bounds__(256) k_pupdate(int n, int m, const BlockDesc *
     const int cq = 4 * (lane / 16);
     const double * dk = dinv + (size_t)bd.k * n;
     const int nstrip = (n + 15) / 16;
-    for (int sidx = blockIdx.x * 4 + wave; sidx < nstrip; sidx += gridDim.x * 4) {
+    const int stride = gridDim.x * 4;
+    // software pipeline over strips (P, R, D^-1 and the aggregate of the next
+    // strip in flight during the products; its coarse values after them)
+    auto fetch = [&](int si, V (&pp)[4], V (&rr)[4], double & dd, int & ag) {
+        const int row = 16 * si + lane % 16;
+        const bool ok = si < nstrip && row < n;
+        const size_t o = vb + (size_t)row * S + cq;
+        ag = (ok && cc) ? agg[row] : 0;
+        dd = ok ? dk[row] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            pp[u] = ok ? ld4(P + o + 16 * u) : V{0, 0, 0, 0};
+            rr[u] = ok ? ld4(R + o + 16 * u) : V{0, 0, 0, 0};
+        }
+    };
+    auto fetch_cc = [&](int si, int ag, f64x4 (&cv)[4]) {
+        const int row = 16 * si + lane % 16;
+        const bool ok = cc && si < nstrip && row < n;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            cv[u] = ok ? ld4(cc + ((size_t)b * m + ag) * S + cq + 16 * u) : f64x4{0, 0, 0, 0};
+    };
+    int sidx = blockIdx.x * 4 + wave;
+    V p[4], r[4];
+    double d;
+    int ag;
+    f64x4 cv[4];
+    fetch(sidx, p, r, d, ag);
+    fetch_cc(sidx, ag, cv);
+    for (; sidx < nstrip; sidx += stride) {
         const int row = 16 * sidx + lane % 16;
         const bool ok = row < n;
         const size_t o = vb + (size_t)row * S + cq;
-        V z[4], p[4], c[4];
-        const double d = ok ? dk[row] : 0.0;
-        const double * crow = (cc && ok) ? cc + ((size_t)b * m + agg[row]) * S + cq : nullptr;
+        V pn[4], rn[4], z[4], c[4];
+        double dn;
+        int agn;
+        fetch(sidx + stride, pn, rn, dn, agn);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             c[u] = V{0, 0, 0, 0};
-            p[u] = ok ? ld4(P + o + 16 * u) : V{0, 0, 0, 0};
-            const V r = ok ? ld4(R + o + 16 * u) : V{0, 0, 0, 0};
 #pragma unroll
-            for (int e = 0; e < 4; ++e) z[u][e] = (VT)(d * (double)r[e] + (crow ? crow[16 * u + e] : 0.0));
+            for (int e = 0; e < 4; ++e) z[u][e] = (VT)(d * (double)r[u][e] + cv[u][e]);
         }
         strip_mm(opT, z, c, lane);
         strip_mm(opM, p, c, lane);
@@ -372,6 +507,13 @@ __global__ void __launch_bounds__(256) k_pupdate(int n, int m, const BlockDesc *
                     for (int jj = 0; jj < 4; ++jj) P[ob + 16 * tb + drow<VT>(lane, jj)] = c[tb][jj];
             }
         }
+        fetch_cc(sidx + stride, agn, cv);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            p[u] = pn[u];
+            r[u] = rn[u];
+        }
+        d = dn;
     }
 }
 
@@ -461,75 +603,107 @@ constexpr int LD = S + 1;  // padded leading dimension against bank conflicts
 constexpr int SMAT = S * LD;
 constexpr size_t SMALL_LDS = 4 * SMAT * sizeof(double);
 
-// C = op(A) B (64 x 64; each thread a 4 x 4 tile); transA: A'.  C must not alias A or B.
+// C = scale op(A) B (64 x 64 LDS matrices, FP64 matrix cores); transA: A'.
+// Wavefront w computes rows 16w .. 16w+15.  C must not alias A or B.
 __device__ void mm64(const double * A, const double * B, double * C, bool transA, double scale) {
-    const int i0 = (threadIdx.x / 16) * 4, j0 = (threadIdx.x % 16) * 4;
-    double acc[4][4] = {};
-    for (int k = 0; k < S; ++k) {
-        double a[4], bb[4];
-        for (int i = 0; i < 4; ++i) a[i] = transA ? A[k * LD + i0 + i] : A[(i0 + i) * LD + k];
-        for (int j = 0; j < 4; ++j) bb[j] = B[k * LD + j0 + j];
-        for (int i = 0; i < 4; ++i)
-            for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * bb[j];
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    const int i = 16 * w + l % 16;
+    f64x4 acc[4] = {};
+#pragma unroll 4
+    for (int ks = 0; ks < 16; ++ks) {
+        const int k = 4 * ks + l / 16;
+        const double a = transA ? A[k * LD + i] : A[i * LD + k];
+#pragma unroll
+        for (int tb = 0; tb < 4; ++tb) acc[tb] = mfma4(a, B[k * LD + 16 * tb + l % 16], acc[tb]);
     }
-    for (int i = 0; i < 4; ++i)
-        for (int j = 0; j < 4; ++j) C[(i0 + i) * LD + j0 + j] = scale * acc[i][j];
+#pragma unroll
+    for (int tb = 0; tb < 4; ++tb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) C[(16 * w + drow<double>(l, r)) * LD + 16 * tb + l % 16] = scale * acc[tb][r];
     __syncthreads();
 }
 
 // Li = L^-1 for G = L L' (lower Cholesky factor) with dependent directions
-// dropped: forward elimination of [G | I] (one barrier per pivot; G is
-// destroyed), the right half becomes the inverse unit-lower factor, rows
-// scaled by the pivots^-1/2.  A pivot at or below 1e-12 of the column's own
-// diagonal is dropped: no elimination with it, zero row and column in Li.
-__device__ void inv_chol(double * G, double * Li, int s) {
-    __shared__ double d0[S], piv[S];
-    for (int t = threadIdx.x; t < SS; t += blockDim.x) {
-        const int i = t / S, j = t % S;
-        Li[i * LD + j] = (i == j && i < s) ? 1.0 : 0.0;
-    }
+// dropped: forward elimination of [G | I] whose right half becomes the
+// inverse unit-lower factor, rows then scaled by the pivots^-1/2.  Each thread
+// keeps a 4 x 4 tile of G and of Li in registers; pivot step j publishes
+// column j and row j through double-buffered LDS vectors (one barrier per
+// step).  A pivot at or below 1e-12 of the column's own diagonal is dropped:
+// no elimination with it, zero row and column in Li.  G (LDS) is only read.
+__device__ void inv_chol(const double * G, double * Li, int s) {
+    __shared__ double colb[2][S], rowb[2][S], lrow[2][S], d0[S];
+    const int i0 = 4 * (threadIdx.x / 16), c0 = 4 * (threadIdx.x % 16);
+    double g[4][4], li[4][4], piv[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            g[i][c] = G[(i0 + i) * LD + c0 + c];
+            li[i][c] = (i0 + i == c0 + c && i0 + i < s) ? 1.0 : 0.0;
+        }
     if (threadIdx.x < S) d0[threadIdx.x] = threadIdx.x < s ? G[threadIdx.x * LD + threadIdx.x] : 0.0;
-    __syncthreads();
     for (int j = 0; j < s; ++j) {
-        const double d = G[j * LD + j];
-        const bool keep = d0[j] > 0.0 && d > 1e-12 * d0[j];
-        if (threadIdx.x == 0) piv[j] = keep ? d : 0.0;
-        if (keep) {
-            const double inv = 1.0 / d;
-            const int nl = s - j - 1;  // left columns j+1 .. s-1, then right columns 0 .. j
-            const int cols = s;
-            for (int t = threadIdx.x; t < nl * cols; t += blockDim.x) {
-                const int i = j + 1 + t / cols, cc = t % cols;
-                const double f = G[i * LD + j] * inv;
-                if (cc < nl) {
-                    const int c = j + 1 + cc;
-                    G[i * LD + c] -= f * G[j * LD + c];
-                } else {
-                    const int c = cc - nl;
-                    Li[i * LD + c] -= f * Li[j * LD + c];
+        const int buf = j & 1;
+        // static register indices only (a lane-dependent index into a
+        // register array becomes a loop over the distinct lane values)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (c0 + q == j)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) colb[buf][i0 + i] = g[i][q];
+            if (i0 + q == j)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    rowb[buf][c0 + c] = g[q][c];
+                    lrow[buf][c0 + c] = li[q][c];
                 }
-            }
         }
         __syncthreads();
+        const double d = rowb[buf][j];
+        const bool keep = d0[j] > 0.0 && d > 1e-12 * d0[j];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (i0 + i == j) piv[i] = keep ? d : 0.0;
+        if (keep) {
+            const double inv = 1.0 / d;
+            double f[4], rb[4], lr[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) f[i] = i0 + i > j ? colb[buf][i0 + i] * inv : 0.0;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                rb[c] = c0 + c > j ? rowb[buf][c0 + c] : 0.0;
+                lr[c] = c0 + c <= j ? lrow[buf][c0 + c] : 0.0;
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    g[i][c] -= f[i] * rb[c];
+                    li[i][c] -= f[i] * lr[c];
+                }
+        }
     }
-    for (int t = threadIdx.x; t < SS; t += blockDim.x) {
-        const int i = t / S, c = t % S;
-        Li[i * LD + c] = (i < s && piv[i] > 0.0) ? Li[i * LD + c] / sqrt(piv[i]) : 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const double sc = piv[i] > 0.0 ? 1.0 / sqrt(piv[i]) : 0.0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) Li[(i0 + i) * LD + c0 + c] = li[i][c] * sc;
     }
     __syncthreads();
 }
 
+// global -> LDS copy of one s x s matrix (zero padded); no barrier: callers
+// issue a group of loads, then synchronise once
 __device__ void load_small(double * M, const double * src, int s) {
     for (int t = threadIdx.x; t < SS; t += blockDim.x) {
         const int i = t / S, j = t % S;
         M[i * LD + j] = (i < s && j < s) ? src[t] : 0.0;
     }
-    __syncthreads();
 }
 
+// LDS -> global; the LDS matrix must not be overwritten before a barrier
 __device__ void store_small(const double * M, double * dst) {
     for (int t = threadIdx.x; t < SS; t += blockDim.x) dst[t] = M[(t / S) * LD + t % S];
-    __syncthreads();
 }
 
 // alpha = G1^-1 G2; G1 is replaced by its inverse (reused for beta)
@@ -538,14 +712,15 @@ __global__ void __launch_bounds__(256) k_small_alpha(const BlockDesc * __restric
     if (!blocks[blockIdx.x].active) return;
     const int s = blocks[blockIdx.x].s;
     double * sm = small + (size_t)blockIdx.x * NSL * SS;
-    double *M0 = lds, *M1 = lds + SMAT, *M2 = lds + 2 * SMAT;
+    double *M0 = lds, *M1 = lds + SMAT, *M2 = lds + 2 * SMAT, *M3 = lds + 3 * SMAT;
     load_small(M0, sm + SL_G1 * SS, s);
+    load_small(M3, sm + SL_G2 * SS, s);
+    __syncthreads();
     inv_chol(M0, M1, s);
-    mm64(M1, M1, M2, true, 1.0);  // G1^-1 = Li' Li
-    load_small(M0, sm + SL_G2 * SS, s);
-    mm64(M2, M0, M1, false, 1.0);  // alpha
+    mm64(M1, M1, M2, true, 1.0);   // G1^-1 = Li' Li
+    mm64(M2, M3, M0, false, 1.0);  // alpha
     store_small(M2, sm + SL_G1 * SS);
-    store_small(M1, sm + SL_G2 * SS);
+    store_small(M0, sm + SL_G2 * SS);
 }
 
 // init = 1: first step of a block (no previous direction): beta = 0, W = Z.
@@ -560,8 +735,10 @@ __global__ void __launch_bounds__(256) k_small_beta(const BlockDesc * __restrict
     if (!init) {
         load_small(M0, sm + SL_G1 * SS, s);  // G1^-1
         load_small(M1, sm + SL_QZ * SS, s);  // Q'Z
+        __syncthreads();
         mm64(M0, M1, Bt, false, -1.0);       // beta = -G1^-1 Q'Z
         load_small(M0, sm + SL_PZ * SS, s);  // P'Z
+        __syncthreads();
         mm64(M0, Bt, M1, true, 1.0);         // (P'Z)' beta
         for (int t = threadIdx.x; t < SS; t += blockDim.x) {
             const int i = t / S, j = t % S;
@@ -598,9 +775,10 @@ __global__ void __launch_bounds__(256) k_bresid(int n, int nnz, const BlockDesc 
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const size_t vb = (size_t)b * n * S + lane;
     const double * A = vals + (size_t)bd.k * nnz;
-    const int w = __builtin_amdgcn_readfirstlane(wave);
+    int lo, hi, first, stride;
+    xcd_rows(n, lo, hi, first, stride);
     double rr = 0.0, bb = 0.0;
-    for (int row = blockIdx.x * 4 + w; row < n; row += gridDim.x * 4) {
+    for (int row = first; row < hi; row += stride) {
         double y = 0.0;
         for (int t = rowptr[row]; t < rowptr[row + 1]; ++t) y += A[t] * X[vb + (size_t)colidx[t] * S];
         const size_t o = vb + (size_t)row * S;
@@ -684,8 +862,8 @@ void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bo
     // matrices each), strip updates, SpMM and residual rows
     const int G = std::max(1, std::min({256, div_up(1024, nb), div_up(c.n, 16)}));
     const int GU = std::max(1, std::min({1024, div_up(4096, nb), div_up(c.n, 64)}));
-    const int GS = std::max(1, std::min(4096, div_up(c.n, 4)));
-    const int GR = std::max(1, std::min(1024, div_up(c.n, 4)));
+    const int GS = NXCD * std::max(1, std::min(256, div_up(c.n, 16 * NXCD)));
+    const int GR = NXCD * std::max(1, std::min(128, div_up(c.n, 4 * NXCD)));
     c.small.alloc((size_t)nb * NSL * SS);
     PG_CHECK(hipMemsetAsync(c.small.p, 0, c.small.bytes(), s));
     c.part.alloc((size_t)nb * std::max((size_t)G * 3 * SS, (size_t)GU * S));
@@ -741,7 +919,6 @@ void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bo
         using VT = decltype(vtag);
         using AT = decltype(atag);
         constexpr bool fp32 = std::is_same<VT, float>::value;
-        constexpr int WS = fp32 ? 1 : 2;
         // compulsory HBM bytes of one launch of the timed kernel (all blocks
         // active): matrix values and indices once, each vector once
         const double vec = (double)tot * sizeof(VT);
@@ -797,7 +974,7 @@ void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bo
                                    c.ycoarse.p, c.ccoarse.p);
             }
             tb(PGERT_TIME_GRAM_Z);
-            hipLaunchKernelGGL((k_gram<VT, MODE_Z, WS>), gG, blk, 0, s, c.n, m, pl.d.p, c.dinv.p, Pw.p, Qw.p, Rw.p,
+            hipLaunchKernelGGL((k_gram<VT, MODE_Z>), gG, blk, 0, s, c.n, m, pl.d.p, c.dinv.p, Pw.p, Qw.p, Rw.p,
                                ccp, aggp, c.part.p);
             te(PGERT_TIME_GRAM_Z);
             hipLaunchKernelGGL(k_part_reduce, dim3(3 * SS / 32, nb), blk, 0, s, pl.d.p, G, 3 * SS, c.part.p,
@@ -825,7 +1002,7 @@ void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bo
                                    Pw.p, Qw.p);
                 te(PGERT_TIME_SPMM);
                 tb(PGERT_TIME_GRAM_PQ);
-                hipLaunchKernelGGL((k_gram<VT, MODE_PQR, WS>), gG, blk, 0, s, c.n, m, pl.d.p, c.dinv.p, Pw.p, Qw.p,
+                hipLaunchKernelGGL((k_gram<VT, MODE_PQR>), gG, blk, 0, s, c.n, m, pl.d.p, c.dinv.p, Pw.p, Qw.p,
                                    Rw.p, ccp, aggp, c.part.p);
                 te(PGERT_TIME_GRAM_PQ);
                 hipLaunchKernelGGL(k_part_reduce, dim3(2 * SS / 32, nb), blk, 0, s, pl.d.p, G, 2 * SS, c.part.p,
