@@ -8,7 +8,7 @@
 //   1. Q = A P                                              (k_spmm)
 //   2. G1 = P'Q, G2 = P'R;  alpha = G1^-1 G2                (k_gram<PQR>, k_small_alpha)
 //   3. X += P alpha;  R -= Q alpha;  |R|^2 per column       (k_update)
-//   4. Z = M^-1 R;  Q'Z, Z'Z, P'Z                           (k_gram<Z>; M = Jacobi [+ coarse])
+//   4. Z = M^-1 R;  Q'Z, Z'Z, P'Z                  (k_zform, k_gram<Z>; M = Jacobi [+ coarse])
 //   5. beta = -G1^-1 Q'Z;  W = Z + P beta;  W'W from the Grams (P'P = I);
 //      T = chol(W'W)^-T;  P = Z T + P (beta T)              (k_small_beta, k_pupdate)
 // Dependent directions are dropped in both factorisations (a pivot below
@@ -84,6 +84,15 @@ template <typename VT> __device__ inline int drow(int lane, int r) {
     return sizeof(VT) == 4 ? 4 * (lane / 16) + r : 4 * r + lane / 16;
 }
 
+// Workgroup barrier ordering LDS only.  __syncthreads() also orders global
+// memory, which makes every wavefront wait for all of its outstanding loads
+// (vmcnt(0)) and would drain a software-pipelined load ring at each step.
+__device__ inline void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 template <typename VT> __device__ inline V4<VT> ld4(const VT * p) { return *reinterpret_cast<const V4<VT> *>(p); }
 template <typename VT> __device__ inline void st4(VT * p, V4<VT> v) { *reinterpret_cast<V4<VT> *>(p) = v; }
 
@@ -139,11 +148,18 @@ __global__ void __launch_bounds__(256) k_spmm(int n, int nnz, const BlockDesc * 
 }
 
 // ---------------------------------------------------------------- Gram passes
-// MODE_PQR: G1 = P'Q, G2 = P'R.  MODE_Z: Z = D^-1 R (+ coarse prolongation
-// cc[agg[row]]) and Q'Z, Z'Z, P'Z.
-// Wavefront w of a workgroup: row group w / 2, column tiles tb = w % 2 + 2 x.
-// Tile (ta, tb) of a Gram holds entries (4i + ta, 4j + tb), i, j < 16; its
-// MFMA output register jj of lane l is entry (4 drow(l, jj) + ta, 4 (l%16) + tb).
+// MODE_PQR: vectors (P, Q, R) -> G1 = P'Q, G2 = P'R.
+// MODE_Z:   vectors (P, Q, Z) -> Q'Z, Z'Z, P'Z.
+// A workgroup takes a contiguous range of 4-row steps; the rows of one step
+// are 1 KB (FP32) per vector and contiguous in the block layout.  Wavefronts
+// 0, 1, 2 each fetch one vector (one 16-byte load per lane and step) and keep
+// DEPTH steps in flight in registers (a ring with static indices: the inner
+// loop over the ring is unrolled); the current step is staged through
+// double-buffered LDS, one LDS-only barrier per step.  Every wavefront then
+// holds the operands of all 4 rows (lane l: row l/16, columns 4(l%16)..+3)
+// and runs the MFMAs of its own column tiles tb = wave: tile (ta, tb) holds
+// Gram entries (4i + ta, 4j + tb), result register jj of lane l entry
+// (4 drow(l, jj) + ta, 4 (l%16) + tb).
 // FP32 inputs: the matrix cores accumulate FP32 over FLUSH steps (16 rows),
 // then the tile is added into FP64 registers -- the Grams keep FP64
 // accumulation (their rounding steers the block recurrence) at the FP32
@@ -154,150 +170,123 @@ constexpr int MODE_Z = 1;
 constexpr int FLUSH = 4;
 
 template <typename VT, int MODE>
-__global__ void __launch_bounds__(256) k_gram(int n, int m, const BlockDesc * __restrict__ blocks,
-                                              const double * __restrict__ dinv, const VT * __restrict__ P,
-                                              const VT * __restrict__ Q, const VT * __restrict__ R,
-                                              const double * __restrict__ cc, const int * __restrict__ agg,
+__global__ void __launch_bounds__(256) k_gram(int n, const BlockDesc * __restrict__ blocks, const VT * __restrict__ V0,
+                                              const VT * __restrict__ V1, const VT * __restrict__ V2,
                                               double * __restrict__ part) {
     constexpr int NG = MODE == MODE_PQR ? 2 : 3;
-    constexpr int WS = 2;        // wavefronts sharing a row group
-    constexpr int RG = 4 / WS;   // row groups per workgroup
-    constexpr int NTB = 4 / WS;  // column tiles per wavefront
     constexpr bool F32 = sizeof(VT) == 4;
+    constexpr int DEPTH = F32 ? (MODE == MODE_Z ? 12 : 16) : 8;  // steps in flight per wavefront
     using V = V4<VT>;
-    __shared__ double red[S][S + 1];
+    __shared__ V sv[2][3][64];
     const int b = blockIdx.y;
-    const BlockDesc bd = blocks[b];
-    if (!bd.active) return;
+    if (!blocks[b].active) return;
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    const int rg = wave / WS, ws = wave % WS;
     const size_t vb = (size_t)b * n * S;
-    const int col = 4 * (lane % 16);
-    const double * dk = dinv + (size_t)bd.k * n;
-    V acc[NG][4][NTB];
-    f64x4 acc64[F32 ? NG : 1][4][NTB];
+    const int nst = (n + 3) / 4;
+    const int st0 = (int)((long)nst * blockIdx.x / gridDim.x), st1 = (int)((long)nst * (blockIdx.x + 1) / gridDim.x);
+    const VT * src = (wave == 0 ? V0 : wave == 1 ? V1 : V2) + vb + 4 * lane;
+    auto fetch = [&](int st) -> V {
+        V v = V{0, 0, 0, 0};
+        if (wave < 3 && st < st1 && 4 * st + lane / 16 < n) v = ld4(src + (size_t)st * 4 * S);
+        return v;
+    };
+
+    V acc[NG][4];
+    f64x4 acc64[F32 ? NG : 1][4];
 #pragma unroll
     for (int g = 0; g < NG; ++g)
 #pragma unroll
-        for (int ta = 0; ta < 4; ++ta)
-#pragma unroll
-            for (int x = 0; x < NTB; ++x) {
-                acc[g][ta][x] = V{0, 0, 0, 0};
-                if constexpr (F32) acc64[g][ta][x] = f64x4{0, 0, 0, 0};
-            }
+        for (int ta = 0; ta < 4; ++ta) {
+            acc[g][ta] = V{0, 0, 0, 0};
+            if constexpr (F32) acc64[g][ta] = f64x4{0, 0, 0, 0};
+        }
     auto flush = [&]() {
         if constexpr (F32) {
 #pragma unroll
             for (int g = 0; g < NG; ++g)
 #pragma unroll
-                for (int ta = 0; ta < 4; ++ta)
+                for (int ta = 0; ta < 4; ++ta) {
 #pragma unroll
-                    for (int x = 0; x < NTB; ++x) {
-#pragma unroll
-                        for (int jj = 0; jj < 4; ++jj) acc64[g][ta][x][jj] += (double)acc[g][ta][x][jj];
-                        acc[g][ta][x] = V{0, 0, 0, 0};
-                    }
+                    for (int jj = 0; jj < 4; ++jj) acc64[g][ta][jj] += (double)acc[g][ta][jj];
+                    acc[g][ta] = V{0, 0, 0, 0};
+                }
         }
     };
-    const int nst = (n + 3) / 4;
-    const int stride = gridDim.x * RG;
-    // software pipeline: the rows of the next step are in flight while the
-    // matrix cores work on the current one
-    auto fetch = [&](int stp, V & pp, V & qq, V & rr, double & dd, int & ag) {
-        const int row = 4 * stp + lane / 16;
-        pp = qq = rr = V{0, 0, 0, 0};
-        dd = 0.0;
-        ag = 0;
-        if (stp < nst && row < n) {
-            if constexpr (MODE == MODE_Z) {
-                if (cc) ag = agg[row];
-                dd = dk[row];
+
+    V ring[DEPTH];
+#pragma unroll
+    for (int i = 0; i < DEPTH; ++i) ring[i] = fetch(st0 + i);
+    for (int s0 = st0; s0 < st1; s0 += DEPTH) {
+#pragma unroll
+        for (int i = 0; i < DEPTH; ++i) {
+            const int buf = i & 1;
+            if (wave < 3) sv[buf][wave][lane] = ring[i];
+            ring[i] = fetch(s0 + i + DEPTH);
+            lds_barrier();
+            const V x0 = sv[buf][0][lane], x1 = sv[buf][1][lane], x2 = sv[buf][2][lane];
+            V a[NG], bo[NG];
+            if constexpr (MODE == MODE_PQR) {
+                a[0] = x0;  // P'Q
+                a[1] = x0;  // P'R
+                bo[0] = x1;
+                bo[1] = x2;
+            } else {
+                a[0] = x1;  // Q'Z
+                a[1] = x2;  // Z'Z
+                a[2] = x0;  // P'Z
+                bo[0] = x2;
+                bo[1] = x2;
+                bo[2] = x2;
             }
-            const size_t o = vb + (size_t)row * S + col;
-            pp = ld4(P + o);
-            qq = ld4(Q + o);
-            rr = ld4(R + o);
-        }
-    };
-    auto fetch_cc = [&](int stp, int ag) {
-        f64x4 v = {0, 0, 0, 0};
-        if constexpr (MODE == MODE_Z) {
-            const int row = 4 * stp + lane / 16;
-            if (cc && stp < nst && row < n) v = ld4(cc + ((size_t)b * m + ag) * S + col);
-        }
-        return v;
-    };
-    int st = blockIdx.x * RG + rg;
-    V p, q, r;
-    double d;
-    int ag;
-    fetch(st, p, q, r, d, ag);
-    f64x4 c4 = fetch_cc(st, ag);
-    int cnt = 0;
-    for (; st < nst; st += stride) {
-        V pn, qn, rn;
-        double dn;
-        int agn;
-        fetch(st + stride, pn, qn, rn, dn, agn);
-        V a[NG], bo[NG];
-        if constexpr (MODE == MODE_PQR) {
-            a[0] = p;
-            a[1] = p;
-            bo[0] = q;
-            bo[1] = r;
-        } else {
-            V z;
 #pragma unroll
-            for (int e = 0; e < 4; ++e) z[e] = (VT)(d * (double)r[e] + c4[e]);
-            a[0] = q;
-            a[1] = z;
-            a[2] = p;
-            bo[0] = z;
-            bo[1] = z;
-            bo[2] = z;
+            for (int g = 0; g < NG; ++g) {
+                const VT bw = wave == 0 ? bo[g][0] : wave == 1 ? bo[g][1] : wave == 2 ? bo[g][2] : bo[g][3];
+#pragma unroll
+                for (int ta = 0; ta < 4; ++ta) acc[g][ta] = mfma4(a[g][ta], bw, acc[g][ta]);
+            }
+            if ((i % FLUSH) == FLUSH - 1) flush();
         }
-#pragma unroll
-        for (int g = 0; g < NG; ++g)
-#pragma unroll
-            for (int ta = 0; ta < 4; ++ta)
-#pragma unroll
-                for (int x = 0; x < NTB; ++x) acc[g][ta][x] = mfma4(a[g][ta], bo[g][ws + WS * x], acc[g][ta][x]);
-        if (++cnt == FLUSH) {
-            cnt = 0;
-            flush();
-        }
-        c4 = fetch_cc(st + stride, agn);
-        p = pn;
-        q = qn;
-        r = rn;
-        d = dn;
     }
     flush();
     double * out = part + ((size_t)b * gridDim.x + blockIdx.x) * NG * SS;
 #pragma unroll
-    for (int g = 0; g < NG; ++g) {
-        for (int rr = 0; rr < RG; ++rr) {
-            if (rg == rr) {
+    for (int g = 0; g < NG; ++g)
 #pragma unroll
-                for (int ta = 0; ta < 4; ++ta)
+        for (int ta = 0; ta < 4; ++ta)
 #pragma unroll
-                    for (int x = 0; x < NTB; ++x)
-#pragma unroll
-                        for (int jj = 0; jj < 4; ++jj) {
-                            const int ai = 4 * drow<VT>(lane, jj) + ta, bj = 4 * (lane % 16) + ws + WS * x;
-                            double v;
-                            if constexpr (F32)
-                                v = acc64[g][ta][x][jj];
-                            else
-                                v = acc[g][ta][x][jj];
-                            red[ai][bj] = rr == 0 ? v : red[ai][bj] + v;
-                        }
+            for (int jj = 0; jj < 4; ++jj) {
+                const int ai = 4 * drow<VT>(lane, jj) + ta, bj = 4 * (lane % 16) + wave;
+                double v;
+                if constexpr (F32)
+                    v = acc64[g][ta][jj];
+                else
+                    v = acc[g][ta][jj];
+                out[(size_t)g * SS + ai * S + bj] = v;
             }
-            __syncthreads();
-        }
-        for (int t = threadIdx.x; t < SS; t += blockDim.x) out[(size_t)g * SS + t] = red[t / S][t % S];
-        __syncthreads();
-    }
+}
+
+// Z = D^-1 R (+ coarse prolongation cc[agg[row]]), block layout; one thread
+// per 4 columns of a row
+template <typename VT>
+__global__ void __launch_bounds__(256) k_zform(int n, int m, const BlockDesc * __restrict__ blocks,
+                                               const double * __restrict__ dinv, const VT * __restrict__ R,
+                                               const double * __restrict__ cc, const int * __restrict__ agg,
+                                               VT * __restrict__ Z) {
+    const int b = blockIdx.y;
+    const BlockDesc bd = blocks[b];
+    if (!bd.active) return;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)n * (S / 4)) return;
+    const int row = (int)(t / (S / 4)), c4 = 4 * (int)(t % (S / 4));
+    const size_t o = (size_t)b * n * S + (size_t)row * S + c4;
+    const double d = dinv[(size_t)bd.k * n + row];
+    const V4<VT> r = ld4(R + o);
+    f64x4 cv = {0, 0, 0, 0};
+    if (cc) cv = ld4(cc + ((size_t)b * m + agg[row]) * S + c4);
+    V4<VT> z;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) z[e] = (VT)(d * (double)r[e] + cv[e]);
+    st4(Z + o, z);
 }
 
 // ---------------------------------------------------------------- updates over 16-row strips
@@ -431,12 +420,11 @@ __global__ void __launch_bounds__(256) k_update(int n, const BlockDesc * __restr
         part[((size_t)b * gridDim.x + blockIdx.x) * S + lane] = cn[0][lane] + cn[1][lane] + cn[2][lane] + cn[3][lane];
 }
 
-// 5. P = Z T + P M2, Z = D^-1 R (+ coarse prolongation)
+// 5. P = Z T + P M2
 template <typename VT>
-__global__ void __launch_bounds__(256) k_pupdate(int n, int m, const BlockDesc * __restrict__ blocks,
-                                                 const double * __restrict__ small, const double * __restrict__ dinv,
-                                                 const VT * __restrict__ R, VT * __restrict__ P,
-                                                 const double * __restrict__ cc, const int * __restrict__ agg) {
+__global__ void __launch_bounds__(256) k_pupdate(int n, const BlockDesc * __restrict__ blocks,
+                                                 const double * __restrict__ small, const VT * __restrict__ Z,
+                                                 VT * __restrict__ P) {
     using V = V4<VT>;
     __shared__ VT opT[16 * 4 * 64], opM[16 * 4 * 64];
     const int b = blockIdx.y;
@@ -448,51 +436,30 @@ __global__ void __launch_bounds__(256) k_pupdate(int n, int m, const BlockDesc *
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const size_t vb = (size_t)b * n * S;
     const int cq = 4 * (lane / 16);
-    const double * dk = dinv + (size_t)bd.k * n;
     const int nstrip = (n + 15) / 16;
     const int stride = gridDim.x * 4;
-    // software pipeline over strips (P, R, D^-1 and the aggregate of the next
-    // strip in flight during the products; its coarse values after them)
-    auto fetch = [&](int si, V (&pp)[4], V (&rr)[4], double & dd, int & ag) {
+    // software pipeline: the next strip's Z and P are in flight during the products
+    auto fetch = [&](int si, V (&pp)[4], V (&zz)[4]) {
         const int row = 16 * si + lane % 16;
         const bool ok = si < nstrip && row < n;
         const size_t o = vb + (size_t)row * S + cq;
-        ag = (ok && cc) ? agg[row] : 0;
-        dd = ok ? dk[row] : 0.0;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             pp[u] = ok ? ld4(P + o + 16 * u) : V{0, 0, 0, 0};
-            rr[u] = ok ? ld4(R + o + 16 * u) : V{0, 0, 0, 0};
+            zz[u] = ok ? ld4(Z + o + 16 * u) : V{0, 0, 0, 0};
         }
     };
-    auto fetch_cc = [&](int si, int ag, f64x4 (&cv)[4]) {
-        const int row = 16 * si + lane % 16;
-        const bool ok = cc && si < nstrip && row < n;
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-            cv[u] = ok ? ld4(cc + ((size_t)b * m + ag) * S + cq + 16 * u) : f64x4{0, 0, 0, 0};
-    };
     int sidx = blockIdx.x * 4 + wave;
-    V p[4], r[4];
-    double d;
-    int ag;
-    f64x4 cv[4];
-    fetch(sidx, p, r, d, ag);
-    fetch_cc(sidx, ag, cv);
+    V p[4], z[4];
+    fetch(sidx, p, z);
     for (; sidx < nstrip; sidx += stride) {
         const int row = 16 * sidx + lane % 16;
         const bool ok = row < n;
         const size_t o = vb + (size_t)row * S + cq;
-        V pn[4], rn[4], z[4], c[4];
-        double dn;
-        int agn;
-        fetch(sidx + stride, pn, rn, dn, agn);
+        V pn[4], zn[4], c[4];
+        fetch(sidx + stride, pn, zn);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            c[u] = V{0, 0, 0, 0};
-#pragma unroll
-            for (int e = 0; e < 4; ++e) z[u][e] = (VT)(d * (double)r[u][e] + cv[u][e]);
-        }
+        for (int u = 0; u < 4; ++u) c[u] = V{0, 0, 0, 0};
         strip_mm(opT, z, c, lane);
         strip_mm(opM, p, c, lane);
         if (ok) {
@@ -507,13 +474,11 @@ __global__ void __launch_bounds__(256) k_pupdate(int n, int m, const BlockDesc *
                     for (int jj = 0; jj < 4; ++jj) P[ob + 16 * tb + drow<VT>(lane, jj)] = c[tb][jj];
             }
         }
-        fetch_cc(sidx + stride, agn, cv);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             p[u] = pn[u];
-            r[u] = rn[u];
+            z[u] = zn[u];
         }
-        d = dn;
     }
 }
 
@@ -922,14 +887,13 @@ void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bo
         // compulsory HBM bytes of one launch of the timed kernel (all blocks
         // active): matrix values and indices once, each vector once
         const double vec = (double)tot * sizeof(VT);
-        const double pre = (double)c.nK * c.n * 8.0 + (useCoarse ? c.n * 4.0 + (double)nb * m * S * 8.0 : 0.0);
         switch (c.kSelect) {
         case PGERT_TIME_SPMM:
             c.kBytesPerLaunch = (double)c.nK * ((double)c.nnz * (sizeof(AT) + 4) + 4.0 * (c.n + 1)) + 2.0 * vec;
             break;
         case PGERT_TIME_UPDATE: c.kBytesPerLaunch = 6.0 * vec; break;
-        case PGERT_TIME_GRAM_Z: c.kBytesPerLaunch = 3.0 * vec + pre; break;
-        case PGERT_TIME_PUPDATE: c.kBytesPerLaunch = 3.0 * vec + pre; break;
+        case PGERT_TIME_GRAM_Z: c.kBytesPerLaunch = 3.0 * vec; break;
+        case PGERT_TIME_PUPDATE: c.kBytesPerLaunch = 3.0 * vec; break;
         case PGERT_TIME_GRAM_PQ: c.kBytesPerLaunch = 3.0 * vec; break;
         default: break;
         }
@@ -939,7 +903,8 @@ void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bo
         auto te = [&](int id) {
             if (c.kSelect == id) c.kev_end();
         };
-        DBuf<VT> Xw, Rw, Pw, Qw;
+        DBuf<VT> Xw, Rw, Pw, Qw, Zw;
+        Zw.alloc(tot);
         Rw.alloc(tot);
         Pw.alloc(tot);
         Qw.alloc(tot);
@@ -973,9 +938,10 @@ void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bo
                 hipLaunchKernelGGL(k_coarse_apply, dim3(div_up(m, 16), 4, nb), blk, 0, s, m, pl.d.p, c.Einv.p,
                                    c.ycoarse.p, c.ccoarse.p);
             }
+            hipLaunchKernelGGL(k_zform<VT>, dim3(div_up((long)c.n * (S / 4), 256), nb), blk, 0, s, c.n, m, pl.d.p,
+                               c.dinv.p, Rw.p, ccp, aggp, Zw.p);
             tb(PGERT_TIME_GRAM_Z);
-            hipLaunchKernelGGL((k_gram<VT, MODE_Z>), gG, blk, 0, s, c.n, m, pl.d.p, c.dinv.p, Pw.p, Qw.p, Rw.p,
-                               ccp, aggp, c.part.p);
+            hipLaunchKernelGGL((k_gram<VT, MODE_Z>), gG, blk, 0, s, c.n, pl.d.p, Pw.p, Qw.p, Zw.p, c.part.p);
             te(PGERT_TIME_GRAM_Z);
             hipLaunchKernelGGL(k_part_reduce, dim3(3 * SS / 32, nb), blk, 0, s, pl.d.p, G, 3 * SS, c.part.p,
                                c.small.p, SL_QZ * SS);
@@ -985,8 +951,7 @@ void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bo
             PG_CHECK(hipMemsetAsync(c.small.p, 0, c.small.bytes(), s));
             z_grams();
             hipLaunchKernelGGL(k_small_beta, dim3(nb), blk, SMALL_LDS, s, pl.d.p, c.small.p, 1);
-            hipLaunchKernelGGL(k_pupdate<VT>, gU, blk, 0, s, c.n, m, pl.d.p, c.small.p, c.dinv.p, Rw.p, Pw.p, ccp,
-                               aggp);
+            hipLaunchKernelGGL(k_pupdate<VT>, gU, blk, 0, s, c.n, pl.d.p, c.small.p, Zw.p, Pw.p);
             PG_CHECK(hipGetLastError());
         };
         bool done = true;
@@ -1002,8 +967,7 @@ void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bo
                                    Pw.p, Qw.p);
                 te(PGERT_TIME_SPMM);
                 tb(PGERT_TIME_GRAM_PQ);
-                hipLaunchKernelGGL((k_gram<VT, MODE_PQR>), gG, blk, 0, s, c.n, m, pl.d.p, c.dinv.p, Pw.p, Qw.p,
-                                   Rw.p, ccp, aggp, c.part.p);
+                hipLaunchKernelGGL((k_gram<VT, MODE_PQR>), gG, blk, 0, s, c.n, pl.d.p, Pw.p, Qw.p, Rw.p, c.part.p);
                 te(PGERT_TIME_GRAM_PQ);
                 hipLaunchKernelGGL(k_part_reduce, dim3(2 * SS / 32, nb), blk, 0, s, pl.d.p, G, 2 * SS, c.part.p,
                                    c.small.p, SL_G1 * SS);
@@ -1018,8 +982,7 @@ void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bo
                 z_grams();
                 hipLaunchKernelGGL(k_small_beta, dim3(nb), blk, SMALL_LDS, s, pl.d.p, c.small.p, 0);
                 tb(PGERT_TIME_PUPDATE);
-                hipLaunchKernelGGL(k_pupdate<VT>, gU, blk, 0, s, c.n, m, pl.d.p, c.small.p, c.dinv.p, Rw.p, Pw.p,
-                                   ccp, aggp);
+                hipLaunchKernelGGL(k_pupdate<VT>, gU, blk, 0, s, c.n, pl.d.p, c.small.p, Zw.p, Pw.p);
                 te(PGERT_TIME_PUPDATE);
             }
             PG_CHECK(hipGetLastError());
