@@ -34,15 +34,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline_worker(out_path, nthreads, sample_frac):
-    """Runs in a child process (started before the GPU work): the oracle on
-    the host cores.  Forward = per-wavenumber FP64 sparse direct factorisation
-    (SuperLU, the oracle's stand-in for the reference's CHOLMOD) of the SR
-    system + one solve per source (4 timed, scaled to all 64) + the two
-    assemblies; Jacobian = the C restatement of calc2 on a contiguous sample of
-    cells, scaled to all cells (thread pool of min(8, nCPU-2), the reference's
-    default, modellingbase.cpp:77)."""
-    import scipy.sparse.linalg as spla
+def cpu_baseline_worker(out_path, nthreads, sample_frac, pcg_iters):
+    """Runs in a child process (started before the GPU work): the oracle's C
+    ports on the host cores, on a bounded sample of the same workload.
+    Forward = FP64 assembly of the system (oracle) + the plain FP64 Jacobi-PCG
+    port over all 64 sources, `pcg_iters` iterations timed (per-iteration cost
+    does not depend on the values), scaled by the caller to the iteration
+    count the same algorithm needs on this workload; Jacobian = the C
+    restatement of calc2 on a contiguous sample of cells, scaled to all cells.
+    Threads: min(8, nCPU-2) (the reference's default thread count,
+    modellingbase.cpp:77), the PCG split over right-hand sides."""
     from oracle import ert_oracle as eo
     from oracle import port
     from pygimli_amd import workloads
@@ -51,49 +52,53 @@ def cpu_baseline_worker(out_path, nthreads, sample_frac):
     kv, wv = fo.k, fo.w
     t = time.time()
     S = eo.assemble(mesh, rho, kv[0], fo.setup.sourceCenter, fo.geo)
-    eo.assemble(mesh, np.ones_like(rho), kv[0], fo.setup.sourceCenter, fo.geo)
     ta = time.time() - t
-    # Jacobian sample on potentials of the right shape (values do not change the cost)
+    B = np.random.default_rng(0).standard_normal((S.shape[0], fo.nE * len(kv)))
+    t = time.time()
+    _, it = port.pcg(S, B, maxit=pcg_iters, rtol=0.0, nthreads=nthreads)
+    t_it = (time.time() - t) / max(it, 1)
     C = mesh.cellCount()
     nP = int(mesh.cellMarkers().max()) + 1
     c1 = max(1, int(C * sample_frac))
-    U = np.tile(fo.primaries(), 1)
+    U = fo.primaries()
     t = time.time()
     port.calc2(mesh, U, kv, wv, data.abmn(), nP, nthreads=nthreads, cell_range=(0, c1),
                J=np.zeros((data.size(), nP)))
     tj = (time.time() - t) * C / c1
-    res = {"ta": ta, "tj": tj, "c1": c1, "C": C}
     with open(out_path, "w") as f:
-        json.dump(res, f)
-    t = time.time()
-    lu = spla.splu(S.tocsc(), permc_spec="MMD_AT_PLUS_A")
-    res["tf"] = time.time() - t
-    b = np.ones(S.shape[0])
-    t = time.time()
-    for _ in range(4):
-        lu.solve(b)
-    res["ts"] = (time.time() - t) / 4 * fo.nE * len(kv)
-    res["nsolve"] = fo.nE * len(kv)
-    with open(out_path, "w") as f:
-        json.dump(res, f)
+        json.dump({"ta": ta, "t_it": t_it, "it_sampled": it, "tj": tj, "c1": c1, "C": C, "nrhs": B.shape[1]}, f)
 
 
-def cpu_baseline_result(path, nthreads, timed_out):
+def cpu_baseline_result(path, nthreads, pcg_iters_needed):
     try:
         with open(path) as f:
             r = json.load(f)
     except Exception as e:
         return {"error": f"cpu baseline produced no result ({e!r})"}
-    if "tf" not in r:
-        return {"error": "cpu baseline did not finish the factorisation in time", "partial": r,
-                "cores": nthreads}
-    tfwd = r["ta"] + r["tf"] + r["ts"]
+    if not pcg_iters_needed:
+        return {"error": "no plain-PCG iteration count to scale the CPU PCG sample with (run without --no-plain)",
+                "partial": r, "cores": nthreads}
+    tfwd = r["ta"] + r["t_it"] * pcg_iters_needed
     return {
         "value": 1.0 / (tfwd + r["tj"]), "unit": "evals/s", "cores": nthreads, "kind": "port",
-        "sample": (f"forward: SuperLU(MMD) factorisation {r['tf']:.1f} s + {r['nsolve']} solves "
-                   f"(4 timed) {r['ts']:.1f} s + 2 assemblies {r['ta']:.1f} s; Jacobian: C calc2 port on "
-                   f"{r['c1']} of {r['C']} cells with {nthreads} threads, scaled = {r['tj']:.1f} s"),
+        "sample": (f"forward: FP64 assembly {r['ta']:.1f} s + plain Jacobi-PCG C port over {r['nrhs']} sources, "
+                   f"{r['it_sampled']} iterations timed ({r['t_it'] * 1e3:.0f} ms each) x {pcg_iters_needed} "
+                   f"iterations (the plain FP64 PCG's count on this workload) = {r['t_it'] * pcg_iters_needed:.0f} s; "
+                   f"Jacobian: C calc2 port on {r['c1']} of {r['C']} cells, scaled = {r['tj']:.1f} s; "
+                   f"{nthreads} threads"),
     }
+
+
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc
+    FETCH_SIZE / WRITE_SIZE and scaled by a copy kernel of known size, as
+    committed under profiles/ (None when not measured)."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
 
 
 def main():
@@ -105,9 +110,25 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--method", default="mixed")
-    ap.add_argument("--cpu-timeout", type=float, default=420.0)
+    ap.add_argument("--cpu-timeout", type=float, default=240.0)
     ap.add_argument("--coarse", type=int, default=1024, help="aggregates of the two-level preconditioner (0: off)")
+    ap.add_argument("--learned-fields", type=int, default=8,
+                    help="random fields the learned warm start is trained on before timing (0: off)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run one 1 GiB device-to-device copy (source evicted from the last-level cache): "
+                         "the known-size kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(2**28, dtype=torch.float32, device="cuda")      # 1 GiB
+        evict = torch.zeros(2**28, dtype=torch.float32, device="cuda")   # 1 GiB written after src
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4,
+                          "checksum": float(dst[::4096].sum() + evict[0])}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported "
                                    "(the path shards by RHS / cells, see DESIGN.md)"}))
@@ -120,7 +141,7 @@ def main():
         import multiprocessing as mp
         os.makedirs(os.path.dirname(cpu_path), exist_ok=True)
         ctx = mp.get_context("spawn")  # before anything touches the GPU
-        cpu_proc = ctx.Process(target=cpu_baseline_worker, args=(cpu_path, nth, 0.02), daemon=True)
+        cpu_proc = ctx.Process(target=cpu_baseline_worker, args=(cpu_path, nth, 0.02, 30), daemon=True)
         cpu_proc.start()
         log(f"[bench] cpu baseline started in pid {cpu_proc.pid} ({nth} threads)")
 
@@ -140,22 +161,22 @@ def main():
     log(f"[bench] set-up done at {time.time() - t0:.1f}s")
 
     # accurate reference potentials: FP64 block-PCG to 1e-11
-    fop.solverOptions.update(method="block64", rtol=1e-11, coarse=args.coarse)
+    fop.solverOptions.update(method="block64", rtol=1e-11, coarse=args.coarse, learned=False)
     fop.response_device(model)
     U_ref = fop.subPotentials(node_major=True).clone()
     log(f"[bench] reference solve: {fop.lastStats} {fop.timings()}")
 
-    def run(method, steps, warmup, coarse=0):
-        fop.solverOptions.update(method=method, rtol=args.rtol, coarse=coarse)
+    import ctypes
+    L, h = fop._ctx.L, fop._ctx.h
+    n, nb = mesh.nodeCount(), 1 * ((64 + 63) // 64)
+
+    def run(method, steps, warmup, coarse=0, learned=False, timed_kernel=1):
+        fop.solverOptions.update(method=method, rtol=args.rtol, coarse=coarse, learned=learned)
         for _ in range(warmup):
             fop.response_device(model)
             fop.createJacobian(model)
-        L, h = fop._ctx.L, fop._ctx.h
-        import ctypes
-        tot = ctypes.c_double()
-        nl = ctypes.c_long()
-        bpl = ctypes.c_double()
-        L.pgert_kernel_timing(h, 1, 1, None, None, None)
+        tot, nl, bpl = ctypes.c_double(), ctypes.c_long(), ctypes.c_double()
+        L.pgert_kernel_timing(h, timed_kernel, 1, None, None, None)
         torch.cuda.synchronize()
         t = time.perf_counter()
         iters = []
@@ -169,20 +190,79 @@ def main():
         U = fop.subPotentials(node_major=True)
         err = float(torch.linalg.norm(U - U_ref) / torch.linalg.norm(U_ref))
         colerr = float((torch.linalg.norm(U - U_ref, dim=1) / torch.linalg.norm(U_ref, dim=1)).max())
-        tm = fop.timings()
         res = dict(ms=dt * 1e3, iters=int(np.mean(iters)), pot_err=err, pot_err_max_col=colerr,
-                   stats=fop.lastStats, timings_ms=tm,
-                   spmm_ms=tot.value / max(nl.value, 1), spmm_launches=nl.value,
-                   spmm_bytes=bpl.value)
-        log(f"[bench] {method}: {res}")
+                   stats=dict(fop.lastStats), timings_ms=fop.timings(),
+                   k_ms=tot.value / max(nl.value, 1), k_launches=nl.value, k_bytes=bpl.value)
+        log(f"[bench] {method} coarse={coarse} learned={learned}: {res}")
         return res
 
-    best = run(args.method, args.steps, args.warmup, args.coarse if args.method != "pcg64" else 0)
-    plain = None
+    plain = plain_block = None
     if not args.no_plain:
         plain = run("pcg64", 1, 0)
+        plain_block = run("block64", 1, 0, coarse=0)
 
-    spmm_gbs = best["spmm_bytes"] / (best["spmm_ms"] * 1e-3) / 1e9 if best["spmm_ms"] > 0 else 0.0
+    # learned warm start: trained on the method's own solves of random
+    # log-normal fields (outside the timed region)
+    learned_info = None
+    use_learned = args.learned_fields > 0
+    if use_learned:
+        from pygimli_amd.learned import LearnedWarmStart
+        cc = mesh.cellCenters()
+        fields = [10.0 ** workloads.gaussian_random_field(cc, 2.0, 0.3 + 0.1 * (i % 6), 2.0 + 3.0 * (i % 7), 100 + i,
+                                                           (-30, -30, -40), (30, 30, 0))
+                  for i in range(args.learned_fields)]
+        fop.solverOptions.update(method=args.method, rtol=args.rtol, coarse=args.coarse)
+        lw = LearnedWarmStart(fop, seed=0)
+        t = time.time()
+        losses = lw.train([torch.as_tensor(f, device=dev) for f in fields], samples_per_model=65536, epochs=20,
+                          batch=4096, lr=1e-3)
+        learned_info = {"fields": args.learned_fields, "samples": 65536 * args.learned_fields, "epochs": 20,
+                        "train_s": time.time() - t, "loss_first": losses[0], "loss_last": losses[-1]}
+        log(f"[bench] learned warm start trained: {learned_info}")
+
+    best = run(args.method, args.steps, args.warmup, args.coarse if args.method != "pcg64" else 0,
+               learned=use_learned, timed_kernel=3 if args.method != "pcg64" else 1)
+    ablation = run(args.method, 1, 0, args.coarse, learned=False) if use_learned else None
+
+    # per-kernel time per launch (one solve each) and the roofline of the
+    # dominant one: algorithmic bytes = each vector read/written once, flops =
+    # the MFMA products' 2 m n k
+    vt = 4 if args.method == "mixed" else 8
+    vec = float(nb) * n * 64 * vt
+    kernels = {1: ("k_spmm", None, None), 2: ("k_update", 6 * vec, 2 * 2 * 64 * 64 * n * nb),
+               3: ("k_gram<Z>", 3 * vec, 2 * 3 * 64 * 64 * n * nb), 4: ("k_pupdate", 3 * vec, 2 * 2 * 64 * 64 * n * nb),
+               5: ("k_gram<PQR>", 3 * vec, 2 * 2 * 64 * 64 * n * nb)}
+    mfma_peak = 157.3e12 if args.method == "mixed" else 78.6e12
+    breakdown = {}
+    if args.method != "pcg64":
+        for kid, (name, _, _) in kernels.items():
+            r = best if kid == 3 else run(args.method, 1, 0, args.coarse, learned=use_learned, timed_kernel=kid)
+            breakdown[name] = {"us_per_launch": r["k_ms"] * 1e3, "launches": r["k_launches"], "bytes": r["k_bytes"],
+                               "ms_per_step": r["k_ms"] * r["k_launches"] / (args.steps if kid == 3 else 1)}
+    dom = max(breakdown, key=lambda k: breakdown[k]["ms_per_step"]) if breakdown else None
+
+    def roof(name, us, nbytes, flops):
+        t_h = nbytes / (HBM_PEAK_GBS * 1e9)
+        t_m = flops / mfma_peak if flops else 0.0
+        hbm = {"achieved": nbytes / (us * 1e-6) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+        hbm["frac"] = hbm["achieved"] / HBM_PEAK_GBS
+        if t_m > t_h:
+            r = {"bound": "mfma", "achieved": flops / (us * 1e-6) / 1e12, "peak": mfma_peak / 1e12, "unit": "TFLOP/s"}
+            r["frac"] = r["achieved"] / r["peak"]
+            r["hbm_view"] = hbm
+        else:
+            r = {"bound": "hbm", **hbm}
+        r.update(kernel=name, avg_launch_us=us, bytes_per_launch=nbytes, flops_per_launch=flops,
+                 traffic=traffic_from_profiles(name))
+        return r
+
+    if dom is not None:
+        kid = [k for k, v in kernels.items() if v[0] == dom][0]
+        nbytes = breakdown[dom]["bytes"] if kernels[kid][1] is None else kernels[kid][1]
+        roofline = roof(dom, breakdown[dom]["us_per_launch"], nbytes, kernels[kid][2])
+    else:
+        roofline = roof("k_spmm_dot", best["k_ms"] * 1e3, best["k_bytes"], None)
+    spmm = breakdown.get("k_spmm")
     out = {
         "metric": METRIC,
         "value": 1e3 / best["ms"],
@@ -199,26 +279,31 @@ def main():
         "config": {
             "workload": "3D ERT: 201898-node / 1150488-tet graded mesh, 64 electrodes (8x8 @ 2 m), "
                         "2000 random quadrupoles, log10-rho GRF (2.0, 0.5, 5 m); J 2000 x 1150488 FP32",
-            "method": args.method, "rtol": args.rtol, "coarse_aggregates": args.coarse, "nodes": mesh.nodeCount(), "cells": mesh.cellCount(),
+            "method": args.method, "rtol": args.rtol, "coarse_aggregates": args.coarse,
+            "learned_warm_start": use_learned, "nodes": n, "cells": mesh.cellCount(),
             "electrodes": 64, "data": data.size(), "rhs": 64,
         },
         "accuracy": {"potential_rel_err_fro": best["pot_err"], "potential_rel_err_max_col": best["pot_err_max_col"],
+                     "max_rel_residual": best["stats"]["max_rel_residual"],
                      "reference": "FP64 block-PCG to rtol 1e-11"},
         "iterations": best["iters"],
         "phase_ms": best["timings_ms"],
-        "roofline": {
-            "bound": "hbm", "achieved": spmm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": spmm_gbs / HBM_PEAK_GBS, "traffic": None,
-            "kernel": "k_bspmm_gram" if args.method != "pcg64" else "k_spmm_dot",
-            "bytes_per_launch": best["spmm_bytes"], "avg_launch_ms": best["spmm_ms"],
-        },
+        "roofline": roofline,
+        "kernels": breakdown,
     }
+    if spmm:
+        out["spmm"] = {"GB/s": spmm["bytes"] / (spmm["us_per_launch"] * 1e-6) / 1e9,
+                       "frac_hbm": spmm["bytes"] / (spmm["us_per_launch"] * 1e-6) / 1e9 / HBM_PEAK_GBS,
+                       "bytes_per_launch": spmm["bytes"], "us_per_launch": spmm["us_per_launch"]}
+    if learned_info:
+        learned_info.update(iterations_without=ablation["iters"], ms_without=ablation["ms"])
+        out["learned"] = learned_info
     if plain is not None:
-        out["plain_fp64_pcg"] = {"value": 1e3 / plain["ms"], "ms_per_step": plain["ms"], "iterations": plain["iters"],
-                                 "potential_rel_err_fro": plain["pot_err"],
-                                 "potential_rel_err_max_col": plain["pot_err_max_col"],
-                                 "spmm_avg_launch_ms": plain["spmm_ms"]}
-        out["speedup_vs_plain_fp64"] = plain["ms"] / best["ms"]
+        for key, r in (("plain_fp64_pcg", plain), ("plain_fp64_block_pcg", plain_block)):
+            out[key] = {"value": 1e3 / r["ms"], "ms_per_step": r["ms"], "iterations": r["iters"],
+                        "potential_rel_err_fro": r["pot_err"], "potential_rel_err_max_col": r["pot_err_max_col"]}
+        out["speedup_vs_plain_fp64_block_pcg"] = plain_block["ms"] / best["ms"]
+        out["speedup_vs_plain_fp64_pcg"] = plain["ms"] / best["ms"]
     if cpu_proc is not None:
         t_wait = time.time()
         while cpu_proc.is_alive() and time.time() - t_wait < args.cpu_timeout:
@@ -227,7 +312,7 @@ def main():
         timed_out = cpu_proc.is_alive()
         if timed_out:
             cpu_proc.terminate()
-        out["cpu_baseline"] = cpu_baseline_result(cpu_path, nth, timed_out)
+        out["cpu_baseline"] = cpu_baseline_result(cpu_path, nth, plain["iters"] if plain else None)
     print(json.dumps(out))
 
 

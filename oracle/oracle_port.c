@@ -121,3 +121,113 @@ int oracle_calc2(int dim, int n, const double * xyz, long C, const int * cells, 
     for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
     return 0;
 }
+
+/* Plain FP64 Jacobi-preconditioned conjugate gradients, every right-hand
+ * side an independent recurrence, used as the forward part of bench.py's CPU
+ * baseline.  (The reference solves the forward systems with a sparse direct
+ * factorisation, cholmodWrapper.cpp:224-459; this per-source CG is the
+ * iterative comparison point, the same algorithm as the GPU's plain FP64
+ * PCG, so the two are compared at equal iteration counts.)  CSR A (n, rowptr, colidx,
+ * vals), B and X row-major n x nrhs, X = 0 start.  Runs until every column
+ * reaches |r| <= rtol |b| or maxit iterations; threads split the columns.
+ * Returns the iterations run (max over threads). */
+typedef struct {
+    int n, nrhs, c0, c1, maxit, iters;
+    double rtol;
+    const int *rowptr, *colidx;
+    const double *vals, *B;
+    double * X;
+} pcg_job_t;
+
+static void * pcg_worker(void * arg) {
+    pcg_job_t * j = (pcg_job_t *)arg;
+    const int n = j->n, w = j->c1 - j->c0, ld = j->nrhs;
+    if (w <= 0) return NULL;
+    double * r = malloc(sizeof(double) * (size_t)n * w);
+    double * z = malloc(sizeof(double) * (size_t)n * w);
+    double * p = malloc(sizeof(double) * (size_t)n * w);
+    double * q = malloc(sizeof(double) * (size_t)n * w);
+    double * dinv = malloc(sizeof(double) * n);
+    double * rz = calloc(w, sizeof(double)), *bb = calloc(w, sizeof(double)), *tmp = calloc(w, sizeof(double));
+    for (int i = 0; i < n; ++i) {
+        double d = 0.0;
+        for (int t = j->rowptr[i]; t < j->rowptr[i + 1]; ++t)
+            if (j->colidx[t] == i) d = j->vals[t];
+        dinv[i] = d != 0.0 ? 1.0 / d : 1.0;
+        for (int c = 0; c < w; ++c) {
+            const double b = j->B[(size_t)i * ld + j->c0 + c];
+            j->X[(size_t)i * ld + j->c0 + c] = 0.0;
+            r[(size_t)i * w + c] = b;
+            z[(size_t)i * w + c] = dinv[i] * b;
+            p[(size_t)i * w + c] = z[(size_t)i * w + c];
+            rz[c] += b * z[(size_t)i * w + c];
+            bb[c] += b * b;
+        }
+    }
+    int it = 0;
+    for (; it < j->maxit; ++it) {
+        /* q = A p, pq = p'q */
+        for (int c = 0; c < w; ++c) tmp[c] = 0.0;
+        for (int i = 0; i < n; ++i) {
+            double * qi = q + (size_t)i * w;
+            for (int c = 0; c < w; ++c) qi[c] = 0.0;
+            for (int t = j->rowptr[i]; t < j->rowptr[i + 1]; ++t) {
+                const double a = j->vals[t];
+                const double * pk = p + (size_t)j->colidx[t] * w;
+                for (int c = 0; c < w; ++c) qi[c] += a * pk[c];
+            }
+            const double * pi = p + (size_t)i * w;
+            for (int c = 0; c < w; ++c) tmp[c] += pi[c] * qi[c];
+        }
+        double worst = 0.0;
+        double * rzn = calloc(w, sizeof(double)), *rr = calloc(w, sizeof(double));
+        for (int i = 0; i < n; ++i)
+            for (int c = 0; c < w; ++c) {
+                const size_t o = (size_t)i * w + c;
+                const double al = tmp[c] != 0.0 ? rz[c] / tmp[c] : 0.0;
+                j->X[(size_t)i * ld + j->c0 + c] += al * p[o];
+                r[o] -= al * q[o];
+                z[o] = dinv[i] * r[o];
+                rzn[c] += r[o] * z[o];
+                rr[c] += r[o] * r[o];
+            }
+        for (int c = 0; c < w; ++c) {
+            const double be = rz[c] != 0.0 ? rzn[c] / rz[c] : 0.0;
+            rz[c] = rzn[c];
+            tmp[c] = be;
+            const double rel = bb[c] > 0.0 ? sqrt(rr[c] / bb[c]) : 0.0;
+            if (rel > worst) worst = rel;
+        }
+        for (int i = 0; i < n; ++i)
+            for (int c = 0; c < w; ++c) p[(size_t)i * w + c] = z[(size_t)i * w + c] + tmp[c] * p[(size_t)i * w + c];
+        free(rzn);
+        free(rr);
+        if (worst <= j->rtol) {
+            ++it;
+            break;
+        }
+    }
+    j->iters = it;
+    free(r), free(z), free(p), free(q), free(dinv), free(rz), free(bb), free(tmp);
+    return NULL;
+}
+
+int oracle_pcg(int n, const int * rowptr, const int * colidx, const double * vals, int nrhs, const double * B,
+               double * X, int maxit, double rtol, int nthreads) {
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > nrhs) nthreads = nrhs;
+    pthread_t th[64];
+    pcg_job_t jobs[64];
+    if (nthreads > 64) nthreads = 64;
+    for (int t = 0; t < nthreads; ++t) {
+        jobs[t] = (pcg_job_t){n, nrhs, (int)((long)nrhs * t / nthreads), (int)((long)nrhs * (t + 1) / nthreads),
+                              maxit, 0, rtol, rowptr, colidx, vals, B, X};
+        pthread_create(&th[t], NULL, pcg_worker, &jobs[t]);
+    }
+    int it = 0;
+    for (int t = 0; t < nthreads; ++t) {
+        pthread_join(th[t], NULL);
+        if (jobs[t].iters > it) it = jobs[t].iters;
+    }
+    return it;
+}

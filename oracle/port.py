@@ -30,3 +30,21 @@ def calc2(mesh, U, kvals, weights, abmn, nParam, nthreads=1, cell_range=None, J=
                    U.ctypes.data_as(D), len(ab), ab.ctypes.data_as(I), nParam, J.ctypes.data_as(D), int(nthreads),
                    int(c0), int(c1))
     return J
+
+
+def pcg(A, B, maxit, rtol=0.0, nthreads=1):
+    """Plain FP64 Jacobi-PCG per column of B (C port, threads over columns).
+    A: scipy CSR; B: [n, nrhs].  Returns (X, iterations run)."""
+    L = ctypes.CDLL(LIB)
+    D = ctypes.POINTER(ctypes.c_double)
+    I = ctypes.POINTER(ctypes.c_int)
+    A = A.tocsr()
+    rp = np.ascontiguousarray(A.indptr, np.int32)
+    ci = np.ascontiguousarray(A.indices, np.int32)
+    va = np.ascontiguousarray(A.data, np.float64)
+    B = np.ascontiguousarray(B, np.float64)
+    X = np.zeros_like(B)
+    L.oracle_pcg.argtypes = [ctypes.c_int, I, I, D, ctypes.c_int, D, D, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+    it = L.oracle_pcg(A.shape[0], rp.ctypes.data_as(I), ci.ctypes.data_as(I), va.ctypes.data_as(D), B.shape[1],
+                      B.ctypes.data_as(D), X.ctypes.data_as(D), int(maxit), float(rtol), int(nthreads))
+    return X, it

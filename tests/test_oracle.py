@@ -310,3 +310,20 @@ def test_jacobian_plant_detected():
     Jbad = eo.sensitivity_calc2(mesh, fo.geo, ab, fo.subpots, fo.k, fo.w, data.sensorCount())
     Jbad *= fo.data["k"][:, None] / (rho * rho)[None, :]
     assert np.linalg.norm(Jbad - J) / np.linalg.norm(J) > 1e-2
+
+
+def test_port_pcg_matches_direct_solve():
+    """The C plain-PCG port (CPU baseline) against a sparse direct solve on a
+    small SPD system from the oracle's assembly."""
+    import scipy.sparse.linalg as spla
+    from oracle import port
+    from pygimli_amd import workloads
+    mesh, data, rho = workloads.small_3d()
+    fo = eo.Forward(mesh, data)
+    A = eo.assemble(mesh, rho, fo.k[0], fo.setup.sourceCenter, fo.geo).tocsr()
+    rng = np.random.default_rng(3)
+    B = rng.standard_normal((A.shape[0], 5))
+    X, it = port.pcg(A, B, maxit=5000, rtol=1e-12, nthreads=2)
+    ref = spla.spsolve(A.tocsc(), B)
+    assert 0 < it < 5000
+    assert np.linalg.norm(X - ref) / np.linalg.norm(ref) < 1e-9
