@@ -50,11 +50,13 @@ typedef struct pgert_solve_opts {
     const double * warm_start; /* optional DEVICE pointer [nK][n][nE] secondary-field guess, NULL = zero */
     int learned_warm_start;    /* 1: start from the learned network's prediction (pgert_mlp_*) */
     int coarse;                /* 1: two-level preconditioner (block methods, see pgert_set_coarse) */
+    int no_graph;              /* 1: launch the block iterations one by one instead of replaying a
+                                  captured HIP graph of check_every iterations */
 } pgert_solve_opts;
 
 typedef struct pgert_solve_stats {
-    int iterations;        /* total (inner) iterations over all wavenumbers */
-    int max_iterations;    /* largest per-wavenumber iteration count */
+    int iterations;        /* iterations until this problem's last block converged */
+    int max_iterations;    /* iterations of the whole solve (all problems of a batch) */
     int refinements;       /* outer FP64 refinement steps (mixed) */
     int converged;         /* 1 if every column met rtol */
     double max_rel_residual; /* final FP64 ||b - A x|| / ||b||, max over columns */
@@ -132,6 +134,20 @@ int pgert_fill_matrix(pgert_ctx * ctx, int mass, const double * a, double * vals
  * pgert_potentials / pgert_jacobian. */
 int pgert_forward(pgert_ctx * ctx, const double * rho, const pgert_solve_opts * opts,
                   double * rhoa, pgert_solve_stats * stats);
+
+/* Forward response of `count` independent problems (contexts set up like
+ * for pgert_forward, on one device) with ONE block solve over all their
+ * wavenumber blocks: every solver kernel launch covers every block of every
+ * problem, and the iterations are replayed from a captured HIP graph.  The
+ * per-problem steps (assembly, primaries, right-hand sides, response) are the
+ * ones of pgert_forward.  rhos[i] / rhoas[i]: DEVICE pointers of problem i;
+ * stats: optional array of count entries.  Everything runs on ctxs[0]'s
+ * stream.  Block methods only (PGERT_BLOCK_FP64, PGERT_BLOCK_MIXED).
+ * The reference runs its problems one after another (each a
+ * DCMultiElectrodeModelling::response, dcfemmodelling.cpp:1085-1197); the
+ * batch has no counterpart there. */
+int pgert_batch_forward(pgert_ctx * const * ctxs, int count, const double * const * rhos,
+                        const pgert_solve_opts * opts, double * const * rhoas, pgert_solve_stats * stats);
 
 /* Per-wavenumber unweighted total potentials of the last forward in the
  * reference's subSolutions_ layout: row = electrode + nE * kIdx, col = node

@@ -28,13 +28,15 @@ EXPORTS = (
     "pgert_potentials", "pgert_potentials_node_major", "pgert_jacobian", "pgert_last_timings",
     "pgert_kernel_timing", "pgert_mlp_param_count", "pgert_mlp_set_params", "pgert_mlp_get_params",
     "pgert_mlp_collect", "pgert_mlp_predict", "pgert_mlp_train_step", "pgert_set_coarse",
+    "pgert_batch_forward",
 )
 
 
 class SolveOpts(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("rtol", ctypes.c_double), ("max_iter", ctypes.c_int),
                 ("check_every", ctypes.c_int), ("inner_rtol", ctypes.c_double),
-                ("warm_start", ctypes.c_void_p), ("learned_warm_start", ctypes.c_int), ("coarse", ctypes.c_int)]
+                ("warm_start", ctypes.c_void_p), ("learned_warm_start", ctypes.c_int), ("coarse", ctypes.c_int),
+                ("no_graph", ctypes.c_int)]
 
 
 class SolveStats(ctypes.Structure):
@@ -75,6 +77,8 @@ def load():
     L.pgert_assemble.argtypes = [vp, vp, ctypes.c_double, vp]
     L.pgert_fill_matrix.argtypes = [vp, ctypes.c_int, vp, vp]
     L.pgert_forward.argtypes = [vp, vp, ctypes.POINTER(SolveOpts), vp, ctypes.POINTER(SolveStats)]
+    L.pgert_batch_forward.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp),
+                                      ctypes.POINTER(SolveOpts), ctypes.POINTER(vp), ctypes.POINTER(SolveStats)]
     L.pgert_potentials.argtypes = [vp, vp]
     L.pgert_potentials_node_major.argtypes = [vp, vp]
     L.pgert_jacobian.argtypes = [vp, vp, ctypes.c_int, vp, ctypes.c_int]

@@ -96,6 +96,10 @@ void pgert_destroy(pgert_ctx * c) {
     (void)hipDeviceSynchronize();
     for (auto & e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto & e : c->solveEv)
+        if (e) (void)hipEventDestroy(e);
+    for (auto & e : c->kEvents) (void)hipEventDestroy(e);
+    if (c->solveStream) (void)hipStreamDestroy(c->solveStream);
     delete c;
 }
 
@@ -499,61 +503,91 @@ int pgert_fill_matrix(pgert_ctx * c, int mass, const double * a, double * vals) 
     });
 }
 
+namespace {
+
+pgert_solve_opts resolve_opts(const pgert_solve_opts * opts) {
+    pgert_solve_opts o{};
+    if (opts) {
+        o = *opts;
+    } else {
+        o.method = PGERT_BLOCK_MIXED;
+        o.rtol = 1e-8;
+        o.max_iter = 20000;
+        o.check_every = 8;
+        o.inner_rtol = 1e-3;
+    }
+    PG_REQUIRE(o.rtol > 0 && o.max_iter > 0, "pgert_forward: bad solver options");
+    if (o.check_every <= 0) o.check_every = 8;
+    return o;
+}
+
+// assembly, primaries, SR right-hand sides and the initial guess of one
+// problem, on its stream (events ev[0..2])
+void forward_prepare(Ctx & c, const double * rho, const pgert_solve_opts & o) {
+    PG_REQUIRE(c.C > 0 && c.nE > 0 && c.nK > 0 && c.D > 0,
+               "pgert_forward: mesh, electrodes, wavenumbers and data must be set");
+    PG_REQUIRE(rho, "pgert_forward: null pointer");
+    hipStream_t s = c.stream;
+    const size_t nk = c.nK, nn = c.n, ne = c.nE;
+    c.rho.alloc(c.C);
+    PG_CHECK(hipMemcpyAsync(c.rho.p, rho, (size_t)c.C * 8, hipMemcpyDeviceToDevice, s));
+    PG_CHECK(hipEventRecord(c.ev[0], s));
+    // assembly (A_k for this model, S1_k with rho == 1 once per mesh)
+    c.valsA.alloc(nk * c.nnz);
+    for (size_t kk = 0; kk < nk; ++kk) launch_assemble(c, c.rho.p, c.kv[kk], c.valsA.p + kk * c.nnz, false);
+    if (!c.s1Valid) {
+        c.valsS1.alloc(nk * c.nnz);
+        for (size_t kk = 0; kk < nk; ++kk) launch_assemble(c, nullptr, c.kv[kk], c.valsS1.p + kk * c.nnz, true);
+        c.s1Valid = true;
+    }
+    launch_diag_inv(c);
+    PG_CHECK(hipEventRecord(c.ev[1], s));
+    // primary potentials (once per electrode/wavenumber set) and SR right-hand sides
+    if (!c.primValid) {
+        c.prim.alloc(nk * nn * ne);
+        launch_primary(c);
+        c.primValid = true;
+    }
+    c.rhoS.alloc(ne);
+    launch_rho_source(c, c.rho.p);
+    c.B.alloc(nk * nn * ne);
+    launch_sr_rhs(c);
+    c.X.alloc(nk * nn * ne);
+    if (o.warm_start)
+        PG_CHECK(hipMemcpyAsync(c.X.p, o.warm_start, c.X.bytes(), hipMemcpyDeviceToDevice, s));
+    else if (o.learned_warm_start)
+        mlp_warm_start(c);
+    else
+        PG_CHECK(hipMemsetAsync(c.X.p, 0, c.X.bytes(), s));
+    PG_CHECK(hipEventRecord(c.ev[2], s));
+}
+
+// total potentials and apparent resistivities (events ev[3..4])
+void forward_finish(Ctx & c, double * rhoa) {
+    PG_CHECK(hipEventRecord(c.ev[3], c.stream));
+    c.U.alloc((size_t)c.nK * c.n * c.nE);
+    launch_total_potential(c);
+    c.haveU = true;
+    launch_response(c, rhoa);
+    PG_CHECK(hipEventRecord(c.ev[4], c.stream));
+}
+
+void read_timings(Ctx & c) {
+    float t;
+    for (int i = 0; i < 4; ++i) {
+        PG_CHECK(hipEventElapsedTime(&t, c.ev[i], c.ev[i + 1]));
+        c.timings[i] = t;
+    }
+}
+
+}  // namespace
+
 int pgert_forward(pgert_ctx * c, const double * rho, const pgert_solve_opts * opts, double * rhoa,
                   pgert_solve_stats * stats) {
     return guarded(c, [&] {
-        PG_REQUIRE(c->C > 0 && c->nE > 0 && c->nK > 0 && c->D > 0,
-                   "pgert_forward: mesh, electrodes, wavenumbers and data must be set");
-        PG_REQUIRE(rho && rhoa, "pgert_forward: null pointer");
-        pgert_solve_opts o;
-        if (opts) o = *opts;
-        else {
-            o.method = PGERT_BLOCK_MIXED;
-            o.rtol = 1e-8;
-            o.max_iter = 20000;
-            o.check_every = 8;
-            o.inner_rtol = 1e-3;
-            o.warm_start = nullptr;
-            o.learned_warm_start = 0;
-            o.coarse = 0;
-        }
-        PG_REQUIRE(o.rtol > 0 && o.max_iter > 0, "pgert_forward: bad solver options");
-        if (o.check_every <= 0) o.check_every = 8;
-        hipStream_t s = c->stream;
-        const size_t nk = c->nK, nn = c->n, ne = c->nE;
-        c->rho.alloc(c->C);
-        PG_CHECK(hipMemcpyAsync(c->rho.p, rho, (size_t)c->C * 8, hipMemcpyDeviceToDevice, s));
-        PG_CHECK(hipEventRecord(c->ev[0], s));
-        // assembly (A_k for this model, S1_k with rho == 1 once per mesh)
-        c->valsA.alloc(nk * c->nnz);
-        for (size_t kk = 0; kk < nk; ++kk)
-            launch_assemble(*c, c->rho.p, c->kv[kk], c->valsA.p + kk * c->nnz, false);
-        if (!c->s1Valid) {
-            c->valsS1.alloc(nk * c->nnz);
-            for (size_t kk = 0; kk < nk; ++kk)
-                launch_assemble(*c, nullptr, c->kv[kk], c->valsS1.p + kk * c->nnz, true);
-            c->s1Valid = true;
-        }
-        launch_diag_inv(*c);
-        PG_CHECK(hipEventRecord(c->ev[1], s));
-        // primary potentials (once per electrode/wavenumber set) and SR right-hand sides
-        if (!c->primValid) {
-            c->prim.alloc(nk * nn * ne);
-            launch_primary(*c);
-            c->primValid = true;
-        }
-        c->rhoS.alloc(ne);
-        launch_rho_source(*c, c->rho.p);
-        c->B.alloc(nk * nn * ne);
-        launch_sr_rhs(*c);
-        c->X.alloc(nk * nn * ne);
-        if (o.warm_start)
-            PG_CHECK(hipMemcpyAsync(c->X.p, o.warm_start, c->X.bytes(), hipMemcpyDeviceToDevice, s));
-        else if (o.learned_warm_start)
-            mlp_warm_start(*c);
-        else
-            PG_CHECK(hipMemsetAsync(c->X.p, 0, c->X.bytes(), s));
-        PG_CHECK(hipEventRecord(c->ev[2], s));
+        PG_REQUIRE(rhoa, "pgert_forward: null pointer");
+        const pgert_solve_opts o = resolve_opts(opts);
+        forward_prepare(*c, rho, o);
         pgert_solve_stats st{};
         if (o.method == PGERT_PCG_FP64)
             solve_pcg_fp64(*c, o, st);
@@ -563,19 +597,49 @@ int pgert_forward(pgert_ctx * c, const double * rho, const pgert_solve_opts * op
             solve_block(*c, o, st, true);
         else
             throw std::runtime_error("pgert_forward: unknown method");
-        PG_CHECK(hipEventRecord(c->ev[3], s));
-        c->U.alloc(nk * nn * ne);
-        launch_total_potential(*c);
-        c->haveU = true;
-        launch_response(*c, rhoa);
-        PG_CHECK(hipEventRecord(c->ev[4], s));
+        forward_finish(*c, rhoa);
         sync(*c);
-        float t;
-        for (int i = 0; i < 4; ++i) {
-            PG_CHECK(hipEventElapsedTime(&t, c->ev[i], c->ev[i + 1]));
-            c->timings[i] = t;
-        }
+        read_timings(*c);
         if (stats) *stats = st;
+    });
+}
+
+int pgert_batch_forward(pgert_ctx * const * ctxs, int count, const double * const * rhos,
+                        const pgert_solve_opts * opts, double * const * rhoas, pgert_solve_stats * stats) {
+    if (!ctxs || count <= 0 || !ctxs[0]) return -2;
+    pgert_ctx * c0 = ctxs[0];
+    return guarded(c0, [&] {
+        PG_REQUIRE(rhos && rhoas, "pgert_batch_forward: null pointer");
+        const pgert_solve_opts o = resolve_opts(opts);
+        PG_REQUIRE(o.method == PGERT_BLOCK_FP64 || o.method == PGERT_BLOCK_MIXED,
+                   "pgert_batch_forward: block methods only");
+        std::vector<Ctx *> cs(count);
+        std::vector<hipStream_t> own(count);
+        for (int i = 0; i < count; ++i) {
+            PG_REQUIRE(ctxs[i] && rhos[i] && rhoas[i], "pgert_batch_forward: null problem");
+            PG_REQUIRE(ctxs[i]->device == c0->device, "pgert_batch_forward: problems on different devices");
+            cs[i] = ctxs[i];
+            own[i] = cs[i]->stream;
+            cs[i]->stream = c0->stream;
+        }
+        auto restore = [&]() {
+            for (int i = 0; i < count; ++i) cs[i]->stream = own[i];
+        };
+        try {
+            for (int i = 0; i < count; ++i) forward_prepare(*cs[i], rhos[i], o);
+            std::vector<pgert_solve_stats> st;
+            solve_blocks(cs, o, st, o.method == PGERT_BLOCK_MIXED);
+            for (int i = 0; i < count; ++i) forward_finish(*cs[i], rhoas[i]);
+            sync(*c0);
+            for (int i = 0; i < count; ++i) {
+                read_timings(*cs[i]);
+                if (stats) stats[i] = st[i];
+            }
+        } catch (...) {
+            restore();
+            throw;
+        }
+        restore();
     });
 }
 

@@ -136,12 +136,16 @@ struct Ctx {
     DBuf<double> emap;      // nE * nE
     // solver work
     DBuf<double> B, X, R, P, Q, Z;
-    DBuf<float> fA;         // FP32 copy of valsA (mixed-precision solver)
     DBuf<double> part;      // reduction partials
     DBuf<double> scal;      // per-column scalars
     DBuf<double> small;     // block-CG small matrices
 
     std::shared_ptr<MlpState> mlp;  // learned warm start (mlp.hip)
+
+    // block solver's own stream (graph capture needs a non-default stream),
+    // ordered after / before the caller's stream with these events
+    hipStream_t solveStream = nullptr;
+    hipEvent_t solveEv[2] = {nullptr, nullptr};
 
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     double timings[5] = {0, 0, 0, 0, 0};
@@ -203,6 +207,10 @@ void launch_jacobian(Ctx & c, const double * model, int nModel, void * J, int dt
 // solvers (solver.hip); return per-solve statistics
 void solve_pcg_fp64(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st);
 void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bool mixed);
+// one block-PCG over every block of several problems (same device; the
+// caller's stream is cs[0]->stream and every context uses it)
+void solve_blocks(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std::vector<pgert_solve_stats> & st,
+                  bool mixed);
 double residual_check(Ctx & c);
 
 // two-level preconditioner (coarse.hip)

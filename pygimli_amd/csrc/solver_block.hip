@@ -59,11 +59,30 @@ constexpr int SL_T = 6;   // T = chol(W'W)^-T
 constexpr int SL_M2 = 7;  // beta T
 constexpr int NSL = 8;
 
+// One block: up to 64 sources of one wavenumber of one problem.  The
+// descriptor carries everything the kernels need, so one launch serves any
+// mix of problems (a single mesh, or a batch of independent meshes).
 struct BlockDesc {
-    int k;       // wavenumber index
+    int k;       // wavenumber index within its problem
     int c0;      // first electrode column
     int s;       // width
     int active;  // 0 once the block has converged: its iteration kernels return at once
+    int n;       // rows (nodes of the problem)
+    int m;       // coarse aggregates of the problem (0: Jacobi only)
+    int nE;      // electrodes of the problem (leading dimension of Bsrc / Xdst)
+    int prob;    // problem index
+    long long off;   // element offset of the block's rows in the block-major work vectors
+    long long coff;  // element offset of the block's m x 64 coarse vectors
+    const int * rowptr;
+    const int * colidx;
+    const double * valsD;  // A_k, FP64
+    const double * dinv;   // D^-1 of A_k
+    const int * agg;       // node -> aggregate
+    const int * aggPtr;
+    const int * aggNodes;
+    const double * Einv;   // E_k^-1, m x m
+    const double * Bsrc;   // right-hand sides of wavenumber k, node-major [node][nE]
+    double * Xdst;         // solution of wavenumber k, node-major [node][nE]
 };
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -113,37 +132,42 @@ __device__ inline void xcd_rows(int n, int & lo, int & hi, int & first, int & st
     stride = nw * 4;
 }
 template <typename VT, typename AT>
-__global__ void __launch_bounds__(256) k_spmm(int n, int nnz, const BlockDesc * __restrict__ blocks,
-                                              const int * __restrict__ rowptr, const int * __restrict__ colidx,
-                                              const AT * __restrict__ vals, const VT * __restrict__ P,
+__global__ void __launch_bounds__(256) k_spmm(const BlockDesc * __restrict__ blocks, const VT * __restrict__ P,
                                               VT * __restrict__ Q) {
     using V = V4<VT>;
+    using VA = V4<AT>;
     const int b = blockIdx.y;
     const BlockDesc bd = blocks[b];
     if (!bd.active) return;
+    const int n = bd.n;
+    const int * __restrict__ rowptr = bd.rowptr;
+    const int * __restrict__ colidx = bd.colidx;
     const int lane = threadIdx.x % 64;
-    const VT * Pb = P + (size_t)b * n * S + 4 * (lane % 16);
-    VT * Qb = Q + (size_t)b * n * S + 4 * (lane % 16);
-    const AT * A = vals + (size_t)bd.k * nnz;
+    const VT * Pb = P + bd.off + 4 * (lane % 16);
+    VT * Qb = Q + bd.off + 4 * (lane % 16);
+    static_assert(sizeof(AT) == 8, "the matrix stays FP64");
+    const AT * __restrict__ A = bd.valsD;
     const int x = blockIdx.x % NXCD, nw = gridDim.x / NXCD;
     const int lo = (int)((long)n * x / NXCD), hi = (int)((long)n * (x + 1) / NXCD);
     const int g0 = (blockIdx.x / NXCD) * 4 + threadIdx.x / 64;
     for (int r0 = lo + 4 * g0; r0 < hi; r0 += 16 * nw) {
         const int row = r0 + lane / 16;
         if (row >= hi) continue;
-        V acc0 = V{0, 0, 0, 0}, acc1 = acc0;
+        VA acc0 = VA{0, 0, 0, 0}, acc1 = acc0;
         int q = rowptr[row];
         const int q1 = rowptr[row + 1];
         for (; q + 4 <= q1; q += 4) {
             const int c0 = colidx[q], c1 = colidx[q + 1], c2 = colidx[q + 2], c3 = colidx[q + 3];
-            const VT a0 = (VT)A[q], a1 = (VT)A[q + 1], a2 = (VT)A[q + 2], a3 = (VT)A[q + 3];
-            const V x0 = ld4(Pb + (size_t)c0 * S), x1 = ld4(Pb + (size_t)c1 * S);
-            const V x2 = ld4(Pb + (size_t)c2 * S), x3 = ld4(Pb + (size_t)c3 * S);
+            const AT a0 = A[q], a1 = A[q + 1], a2 = A[q + 2], a3 = A[q + 3];
+            const VA x0 = __builtin_convertvector(ld4(Pb + (size_t)c0 * S), VA);
+            const VA x1 = __builtin_convertvector(ld4(Pb + (size_t)c1 * S), VA);
+            const VA x2 = __builtin_convertvector(ld4(Pb + (size_t)c2 * S), VA);
+            const VA x3 = __builtin_convertvector(ld4(Pb + (size_t)c3 * S), VA);
             acc0 += a0 * x0 + a1 * x1;
             acc1 += a2 * x2 + a3 * x3;
         }
-        for (; q < q1; ++q) acc0 += (VT)A[q] * ld4(Pb + (size_t)colidx[q] * S);
-        st4(Qb + (size_t)row * S, acc0 + acc1);
+        for (; q < q1; ++q) acc0 += A[q] * __builtin_convertvector(ld4(Pb + (size_t)colidx[q] * S), VA);
+        st4(Qb + (size_t)row * S, __builtin_convertvector(acc0 + acc1, V));
     }
 }
 
@@ -170,7 +194,7 @@ constexpr int MODE_Z = 1;
 constexpr int FLUSH = 4;
 
 template <typename VT, int MODE>
-__global__ void __launch_bounds__(256) k_gram(int n, const BlockDesc * __restrict__ blocks, const VT * __restrict__ V0,
+__global__ void __launch_bounds__(256) k_gram(const BlockDesc * __restrict__ blocks, const VT * __restrict__ V0,
                                               const VT * __restrict__ V1, const VT * __restrict__ V2,
                                               double * __restrict__ part) {
     constexpr int NG = MODE == MODE_PQR ? 2 : 3;
@@ -180,8 +204,9 @@ __global__ void __launch_bounds__(256) k_gram(int n, const BlockDesc * __restric
     __shared__ V sv[2][3][64];
     const int b = blockIdx.y;
     if (!blocks[b].active) return;
+    const int n = blocks[b].n;
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    const size_t vb = (size_t)b * n * S;
+    const size_t vb = blocks[b].off;
     const int nst = (n + 3) / 4;
     const int st0 = (int)((long)nst * blockIdx.x / gridDim.x), st1 = (int)((long)nst * (blockIdx.x + 1) / gridDim.x);
     const VT * src = (wave == 0 ? V0 : wave == 1 ? V1 : V2) + vb + 4 * lane;
@@ -268,25 +293,24 @@ __global__ void __launch_bounds__(256) k_gram(int n, const BlockDesc * __restric
 // Z = D^-1 R (+ coarse prolongation cc[agg[row]]), block layout; one thread
 // per 4 columns of a row
 template <typename VT>
-__global__ void __launch_bounds__(256) k_zform(int n, int m, const BlockDesc * __restrict__ blocks,
-                                               const double * __restrict__ dinv, const VT * __restrict__ R,
-                                               const double * __restrict__ cc, const int * __restrict__ agg,
-                                               VT * __restrict__ Z) {
+__global__ void __launch_bounds__(256) k_zform(const BlockDesc * __restrict__ blocks, const VT * __restrict__ R,
+                                               const double * __restrict__ cc, VT * __restrict__ Z) {
     const int b = blockIdx.y;
     const BlockDesc bd = blocks[b];
     if (!bd.active) return;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)n * (S / 4)) return;
-    const int row = (int)(t / (S / 4)), c4 = 4 * (int)(t % (S / 4));
-    const size_t o = (size_t)b * n * S + (size_t)row * S + c4;
-    const double d = dinv[(size_t)bd.k * n + row];
-    const V4<VT> r = ld4(R + o);
-    f64x4 cv = {0, 0, 0, 0};
-    if (cc) cv = ld4(cc + ((size_t)b * m + agg[row]) * S + c4);
-    V4<VT> z;
+    const long total = (long)bd.n * (S / 4);
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+        const int row = (int)(t / (S / 4)), c4 = 4 * (int)(t % (S / 4));
+        const size_t o = bd.off + (size_t)row * S + c4;
+        const double d = bd.dinv[row];
+        const V4<VT> r = ld4(R + o);
+        f64x4 cv = {0, 0, 0, 0};
+        if (cc && bd.m > 0) cv = ld4(cc + bd.coff + (size_t)bd.agg[row] * S + c4);
+        V4<VT> z;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) z[e] = (VT)(d * (double)r[e] + cv[e]);
-    st4(Z + o, z);
+        for (int e = 0; e < 4; ++e) z[e] = (VT)(d * (double)r[e] + cv[e]);
+        st4(Z + o, z);
+    }
 }
 
 // ---------------------------------------------------------------- updates over 16-row strips
@@ -319,7 +343,7 @@ __device__ inline void strip_mm(const VT * __restrict__ op, const V4<VT> (&x)[4]
 
 // 3. X += P alpha, R -= Q alpha; partial |R|^2 per column -> part[(blk * G + g) * S]
 template <typename VT>
-__global__ void __launch_bounds__(256) k_update(int n, const BlockDesc * __restrict__ blocks,
+__global__ void __launch_bounds__(256) k_update(const BlockDesc * __restrict__ blocks,
                                                 const double * __restrict__ small, const VT * __restrict__ P,
                                                 const VT * __restrict__ Q, VT * __restrict__ X, VT * __restrict__ R,
                                                 double * __restrict__ part) {
@@ -331,8 +355,9 @@ __global__ void __launch_bounds__(256) k_update(int n, const BlockDesc * __restr
     if (!bd.active) return;
     stage_op(op, small + ((size_t)b * NSL + SL_G2) * SS);
     __syncthreads();
+    const int n = bd.n;
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    const size_t vb = (size_t)b * n * S;
+    const size_t vb = bd.off;
     const int cq = 4 * (lane / 16);
     double rr[4][4] = {};
     const int nstrip = (n + 15) / 16;
@@ -422,7 +447,7 @@ __global__ void __launch_bounds__(256) k_update(int n, const BlockDesc * __restr
 
 // 5. P = Z T + P M2
 template <typename VT>
-__global__ void __launch_bounds__(256) k_pupdate(int n, const BlockDesc * __restrict__ blocks,
+__global__ void __launch_bounds__(256) k_pupdate(const BlockDesc * __restrict__ blocks,
                                                  const double * __restrict__ small, const VT * __restrict__ Z,
                                                  VT * __restrict__ P) {
     using V = V4<VT>;
@@ -433,8 +458,9 @@ __global__ void __launch_bounds__(256) k_pupdate(int n, const BlockDesc * __rest
     stage_op(opT, small + ((size_t)b * NSL + SL_T) * SS);
     stage_op(opM, small + ((size_t)b * NSL + SL_M2) * SS);
     __syncthreads();
+    const int n = bd.n;
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    const size_t vb = (size_t)b * n * S;
+    const size_t vb = bd.off;
     const int cq = 4 * (lane / 16);
     const int nstrip = (n + 15) / 16;
     const int stride = gridDim.x * 4;
@@ -486,36 +512,36 @@ __global__ void __launch_bounds__(256) k_pupdate(int n, const BlockDesc * __rest
 // y[b][I][c] = sum over the nodes of aggregate I of R[b][node][c]: one
 // workgroup per aggregate, 4 wavefronts over its node list
 template <typename VT>
-__global__ void __launch_bounds__(256) k_restrict(int n, int m, const BlockDesc * __restrict__ blocks,
-                                                  const int * __restrict__ aggPtr, const int * __restrict__ aggNodes,
-                                                  const VT * __restrict__ R, double * __restrict__ y) {
+__global__ void __launch_bounds__(256) k_restrict(const BlockDesc * __restrict__ blocks, const VT * __restrict__ R,
+                                                  double * __restrict__ y) {
     __shared__ double cn[4][S];
     const int b = blockIdx.y;
-    if (!blocks[b].active) return;
-    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const BlockDesc bd = blocks[b];
     const int I = blockIdx.x;
-    const VT * Rb = R + (size_t)b * n * S + lane;
+    if (!bd.active || I >= bd.m) return;
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const VT * Rb = R + bd.off + lane;
     double s = 0.0;
-    for (int q = aggPtr[I] + wave; q < aggPtr[I + 1]; q += 4) s += (double)Rb[(size_t)aggNodes[q] * S];
+    for (int q = bd.aggPtr[I] + wave; q < bd.aggPtr[I + 1]; q += 4) s += (double)Rb[(size_t)bd.aggNodes[q] * S];
     cn[wave][lane] = s;
     __syncthreads();
-    if (wave == 0) y[((size_t)b * m + I) * S + lane] = cn[0][lane] + cn[1][lane] + cn[2][lane] + cn[3][lane];
+    if (wave == 0) y[bd.coff + (size_t)I * S + lane] = cn[0][lane] + cn[1][lane] + cn[2][lane] + cn[3][lane];
 }
 
 // cc[b] = Einv_k y[b] (FP64 MFMA): a workgroup computes one 16 x 16 tile
 // (rows 16 blockIdx.x, columns 16 * tile), its 4 wavefronts split the m-long
 // contraction and are summed through LDS
-__global__ void __launch_bounds__(256) k_coarse_apply(int m, const BlockDesc * __restrict__ blocks,
-                                                      const double * __restrict__ Einv, const double * __restrict__ y,
-                                                      double * __restrict__ cc) {
+__global__ void __launch_bounds__(256) k_coarse_apply(const BlockDesc * __restrict__ blocks,
+                                                      const double * __restrict__ y, double * __restrict__ cc) {
     __shared__ double red[4][64][4];
     const int b = blockIdx.z;
     const BlockDesc bd = blocks[b];
-    if (!bd.active) return;
-    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int m = bd.m;
     const int I0 = 16 * blockIdx.x, c0 = 16 * blockIdx.y;
-    const double * E = Einv + (size_t)bd.k * m * m;
-    const double * yb = y + (size_t)b * m * S;
+    if (!bd.active || I0 >= m) return;
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const double * E = bd.Einv;
+    const double * yb = y + bd.coff;
     const int i = I0 + lane % 16;
     f64x4 acc = {0, 0, 0, 0};
     const int nk = (m + 3) / 4;
@@ -532,7 +558,7 @@ __global__ void __launch_bounds__(256) k_coarse_apply(int m, const BlockDesc * _
         for (int jj = 0; jj < 4; ++jj) {
             const int row = I0 + drow<double>(lane, jj);
             if (row < m)
-                cc[((size_t)b * m + row) * S + c0 + lane % 16] =
+                cc[bd.coff + (size_t)row * S + c0 + lane % 16] =
                     red[0][lane][jj] + red[1][lane][jj] + red[2][lane][jj] + red[3][lane][jj];
         }
 }
@@ -729,17 +755,18 @@ __global__ void __launch_bounds__(256) k_small_beta(const BlockDesc * __restrict
 // R = B - A X (FP64, block layout), R stored as VT; partial |R|^2, |B|^2 per
 // column -> part[(b * G + g) * 2S]
 template <typename VT>
-__global__ void __launch_bounds__(256) k_bresid(int n, int nnz, const BlockDesc * __restrict__ blocks,
-                                                const int * __restrict__ rowptr, const int * __restrict__ colidx,
-                                                const double * __restrict__ vals, const double * __restrict__ X,
+__global__ void __launch_bounds__(256) k_bresid(const BlockDesc * __restrict__ blocks, const double * __restrict__ X,
                                                 const double * __restrict__ B, VT * __restrict__ R,
                                                 double * __restrict__ part) {
     __shared__ double cn[4][2][S];
     const int b = blockIdx.y;
     const BlockDesc bd = blocks[b];
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    const size_t vb = (size_t)b * n * S + lane;
-    const double * A = vals + (size_t)bd.k * nnz;
+    const int n = bd.n;
+    const size_t vb = bd.off + lane;
+    const double * __restrict__ A = bd.valsD;
+    const int * __restrict__ rowptr = bd.rowptr;
+    const int * __restrict__ colidx = bd.colidx;
     int lo, hi, first, stride;
     xcd_rows(n, lo, hi, first, stride);
     double rr = 0.0, bb = 0.0;
@@ -771,40 +798,37 @@ __global__ void k_norm_reduce(int G, const double * __restrict__ part, double * 
     out[(size_t)b * 2 * S + t] = s;
 }
 
-// block layout <-> electrode-major solver layout [k][row][nE]
-__global__ void k_to_blocks(int n, int nE, const BlockDesc * __restrict__ blocks, const double * __restrict__ src,
-                            double * __restrict__ dst) {
+// block layout <-> electrode-major solver layout [k][row][nE] of each problem
+// (which = 0: the right-hand sides, 1: the initial solution)
+__global__ void k_to_blocks(const BlockDesc * __restrict__ blocks, int which, double * __restrict__ dst) {
     const BlockDesc bd = blocks[blockIdx.y];
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)n * S) return;
-    const int row = (int)(t / S), c = (int)(t % S);
-    dst[(size_t)blockIdx.y * n * S + t] = c < bd.s ? src[((size_t)bd.k * n + row) * nE + bd.c0 + c] : 0.0;
+    const double * src = which ? bd.Xdst : bd.Bsrc;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < (long)bd.n * S; t += (long)gridDim.x * blockDim.x) {
+        const int row = (int)(t / S), c = (int)(t % S);
+        dst[bd.off + t] = c < bd.s ? src[(size_t)row * bd.nE + bd.c0 + c] : 0.0;
+    }
 }
 
-__global__ void k_from_blocks(int n, int nE, const BlockDesc * __restrict__ blocks, const double * __restrict__ src,
-                              double * __restrict__ dst) {
+__global__ void k_from_blocks(const BlockDesc * __restrict__ blocks, const double * __restrict__ src) {
     const BlockDesc bd = blocks[blockIdx.y];
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)n * S) return;
-    const int row = (int)(t / S), c = (int)(t % S);
-    if (c < bd.s) dst[((size_t)bd.k * n + row) * nE + bd.c0 + c] = src[(size_t)blockIdx.y * n * S + t];
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < (long)bd.n * S; t += (long)gridDim.x * blockDim.x) {
+        const int row = (int)(t / S), c = (int)(t % S);
+        if (c < bd.s) bd.Xdst[(size_t)row * bd.nE + bd.c0 + c] = src[bd.off + t];
+    }
 }
 
 // X64 += (double) Xw ; Xw = 0   over the active blocks
-__global__ void k_accumulate(int n, const BlockDesc * __restrict__ blocks, float * __restrict__ Xw,
+__global__ void k_accumulate(const BlockDesc * __restrict__ blocks, float * __restrict__ Xw,
                              double * __restrict__ X64) {
-    if (!blocks[blockIdx.y].active) return;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)n * S) return;
-    const size_t o = (size_t)blockIdx.y * n * S + t;
-    X64[o] += (double)Xw[o];
-    Xw[o] = 0.0f;
+    const BlockDesc bd = blocks[blockIdx.y];
+    if (!bd.active) return;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < (long)bd.n * S; t += (long)gridDim.x * blockDim.x) {
+        const size_t o = bd.off + t;
+        X64[o] += (double)Xw[o];
+        Xw[o] = 0.0f;
+    }
 }
 
-__global__ void k_to_float(long nn, const double * __restrict__ a, float * __restrict__ b) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < nn) b[t] = (float)a[t];
-}
 
 // --------------------------------------------------------------- driver
 struct Plan {
@@ -812,39 +836,108 @@ struct Plan {
     DBuf<BlockDesc> d;
 };
 
+// workgroups per block for a launch that covers rows in chunks of `rows`,
+// keeping the whole grid (over nb blocks) near `cap` workgroups
+int wg_per_block(int nmax, int rows, int nb, int cap, int lim) {
+    return std::max(1, std::min({lim, div_up(nmax, rows), std::max(1, cap / std::max(nb, 1))}));
+}
+
 }  // namespace
 
 void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bool mixed) {
-    PG_REQUIRE(c.nE >= 1, "block solver: no electrodes");
-    hipStream_t s = c.stream;
+    std::vector<pgert_solve_stats> sts(1);
+    solve_blocks({&c}, o, sts, mixed);
+    st = sts[0];
+}
+
+void solve_blocks(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std::vector<pgert_solve_stats> & stats,
+                  bool mixed) {
+    PG_REQUIRE(!cs.empty(), "block solver: no problems");
+    Ctx & c0 = *cs[0];
+    const int np = (int)cs.size();
+    hipStream_t user = c0.stream;
+    const dim3 blk(256);
+    // per problem (on the caller's stream): FP32 copy of A, coarse inverse
+    std::vector<int> mp(np, 0);
+    for (int p = 0; p < np; ++p) {
+        Ctx & c = *cs[p];
+        PG_REQUIRE(c.nE >= 1 && c.n > 0, "block solver: problem without electrodes or nodes");
+        PG_REQUIRE(c.device == c0.device, "block solver: problems on different devices");
+        if (o.coarse && c.nAgg > 0) {
+            coarse_prepare(c);
+            mp[p] = c.nAgg;
+        }
+    }
+    PG_CHECK(hipGetLastError());
+    // the solver's own stream, after everything queued so far on the caller's
+    if (!c0.solveStream) {
+        PG_CHECK(hipStreamCreateWithFlags(&c0.solveStream, hipStreamNonBlocking));
+        for (auto & e : c0.solveEv) PG_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    hipStream_t s = c0.solveStream;
+    PG_CHECK(hipEventRecord(c0.solveEv[0], user));
+    PG_CHECK(hipStreamWaitEvent(s, c0.solveEv[0], 0));
+    c0.stream = s;  // kernel-timing events go to the solver stream
+
     Plan pl;
-    for (int k = 0; k < c.nK; ++k)
-        for (int c0 = 0; c0 < c.nE; c0 += S) pl.h.push_back({k, c0, std::min(S, c.nE - c0), 1});
+    long long off = 0, coff = 0;
+    int nmax = 0, mmax = 0;
+    for (int p = 0; p < np; ++p) {
+        Ctx & c = *cs[p];
+        for (int k = 0; k < c.nK; ++k)
+            for (int e0 = 0; e0 < c.nE; e0 += S) {
+                BlockDesc bd{};
+                bd.k = k;
+                bd.c0 = e0;
+                bd.s = std::min(S, c.nE - e0);
+                bd.active = 1;
+                bd.n = c.n;
+                bd.m = mp[p];
+                bd.nE = c.nE;
+                bd.prob = p;
+                bd.off = off;
+                bd.coff = coff;
+                off += (long long)c.n * S;
+                coff += (long long)mp[p] * S;
+                bd.rowptr = c.rowptr.p;
+                bd.colidx = c.colidx.p;
+                bd.valsD = c.valsA.p + (size_t)k * c.nnz;
+                bd.dinv = c.dinv.p + (size_t)k * c.n;
+                bd.agg = mp[p] ? c.agg.p : nullptr;
+                bd.aggPtr = mp[p] ? c.aggPtr.p : nullptr;
+                bd.aggNodes = mp[p] ? c.aggNodes.p : nullptr;
+                bd.Einv = mp[p] ? c.Einv.p + (size_t)k * mp[p] * mp[p] : nullptr;
+                bd.Bsrc = c.B.p + (size_t)k * c.n * c.nE;
+                bd.Xdst = c.X.p + (size_t)k * c.n * c.nE;
+                pl.h.push_back(bd);
+                nmax = std::max(nmax, c.n);
+                mmax = std::max(mmax, mp[p]);
+            }
+    }
     pl.d.upload(pl.h, s);
     const int nb = (int)pl.h.size();
-    const size_t tot = (size_t)nb * c.n * S;
-    // workgroups per block: Gram passes (partials of 2-3 64 x 64 FP64
-    // matrices each), strip updates, SpMM and residual rows
-    const int G = std::max(1, std::min({256, div_up(1024, nb), div_up(c.n, 16)}));
-    const int GU = std::max(1, std::min({1024, div_up(4096, nb), div_up(c.n, 64)}));
-    const int GS = NXCD * std::max(1, std::min(256, div_up(c.n, 16 * NXCD)));
-    const int GR = NXCD * std::max(1, std::min(128, div_up(c.n, 4 * NXCD)));
-    c.small.alloc((size_t)nb * NSL * SS);
-    PG_CHECK(hipMemsetAsync(c.small.p, 0, c.small.bytes(), s));
-    c.part.alloc((size_t)nb * std::max((size_t)G * 3 * SS, (size_t)GU * S));
+    const size_t tot = (size_t)off;
+    const bool anyCoarse = mmax > 0;
+    // grids: the Gram passes keep ~2048 workgroups (partials of 2-3 64 x 64
+    // FP64 matrices each); the streaming kernels ~64k workgroups overall
+    const int G = wg_per_block(nmax, 16, nb, 2048, 256);
+    const int GU = wg_per_block(nmax, 64, nb, 4096, 1024);
+    const int GS = NXCD * wg_per_block(nmax, 16 * NXCD, nb, 65536 / NXCD, 256);
+    const int GR = NXCD * wg_per_block(nmax, 4 * NXCD, nb, 16384 / NXCD, 128);
+    const int GZ = wg_per_block(nmax * (S / 4), 256, nb, 65536, 1 << 20);
+    const int GT = wg_per_block(nmax * S, 256, nb, 65536, 1 << 20);
+    c0.small.alloc((size_t)nb * NSL * SS);
+    PG_CHECK(hipMemsetAsync(c0.small.p, 0, c0.small.bytes(), s));
+    c0.part.alloc((size_t)nb * std::max((size_t)G * 3 * SS, (size_t)GU * S));
     DBuf<double> npart, norms;
     npart.alloc((size_t)nb * GR * 2 * S);
     norms.alloc((size_t)nb * 2 * S);
-    const bool useCoarse = o.coarse && c.nAgg > 0;
-    const int m = useCoarse ? c.nAgg : 0;
-    if (useCoarse) {
-        coarse_prepare(c);
-        c.ycoarse.alloc((size_t)nb * m * S);
-        c.ccoarse.alloc((size_t)nb * m * S);
-        PG_CHECK(hipMemsetAsync(c.ccoarse.p, 0, c.ccoarse.bytes(), s));
+    if (anyCoarse) {
+        c0.ycoarse.alloc((size_t)coff);
+        c0.ccoarse.alloc((size_t)coff);
+        PG_CHECK(hipMemsetAsync(c0.ccoarse.p, 0, c0.ccoarse.bytes(), s));
     }
-    const double * ccp = useCoarse ? c.ccoarse.p : nullptr;
-    const int * aggp = useCoarse ? c.agg.p : nullptr;
+    const double * ccp = anyCoarse ? c0.ccoarse.p : nullptr;
     PG_CHECK(hipFuncSetAttribute((const void *)k_small_alpha, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)SMALL_LDS));
     PG_CHECK(hipFuncSetAttribute((const void *)k_small_beta, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -854,17 +947,16 @@ void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bo
     DBuf<double> Bb, Xb;
     Bb.alloc(tot);
     Xb.alloc(tot);
-    const dim3 gT(div_up((long)c.n * S, 256), nb), blk(256);
-    hipLaunchKernelGGL(k_to_blocks, gT, blk, 0, s, c.n, c.nE, pl.d.p, c.B.p, Bb.p);
-    hipLaunchKernelGGL(k_to_blocks, gT, blk, 0, s, c.n, c.nE, pl.d.p, c.X.p, Xb.p);
+    const dim3 gT(GT, nb);
+    hipLaunchKernelGGL(k_to_blocks, gT, blk, 0, s, pl.d.p, 0, Bb.p);
+    hipLaunchKernelGGL(k_to_blocks, gT, blk, 0, s, pl.d.p, 1, Xb.p);
     PG_CHECK(hipGetLastError());
 
     std::vector<double> hn(nb * 2 * S);
     // FP64 true residual R = B - A X into Rw (VT); norms to the host
     auto true_resid = [&](auto * Rw) {
         using VT = std::remove_pointer_t<decltype(Rw)>;
-        hipLaunchKernelGGL(k_bresid<VT>, dim3(GR, nb), blk, 0, s, c.n, c.nnz, pl.d.p, c.rowptr.p, c.colidx.p,
-                           c.valsA.p, Xb.p, Bb.p, Rw, npart.p);
+        hipLaunchKernelGGL(k_bresid<VT>, dim3(GR, nb), blk, 0, s, pl.d.p, Xb.p, Bb.p, Rw, npart.p);
         hipLaunchKernelGGL(k_norm_reduce, dim3(nb), dim3(2 * S), 0, s, GR, npart.p, norms.p);
         PG_CHECK(hipGetLastError());
         PG_CHECK(hipMemcpyAsync(hn.data(), norms.p, hn.size() * 8, hipMemcpyDeviceToHost, s));
@@ -879,29 +971,34 @@ void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bo
         }
         return w;
     };
+    std::vector<int> probIters(np, 0);
+    int refine = 0, it = 0;
 
-    auto run = [&](auto vtag, auto atag, const auto * Aw) {
+    auto run = [&](auto vtag, auto atag) {
         using VT = decltype(vtag);
         using AT = decltype(atag);
         constexpr bool fp32 = std::is_same<VT, float>::value;
         // compulsory HBM bytes of one launch of the timed kernel (all blocks
         // active): matrix values and indices once, each vector once
         const double vec = (double)tot * sizeof(VT);
-        switch (c.kSelect) {
-        case PGERT_TIME_SPMM:
-            c.kBytesPerLaunch = (double)c.nK * ((double)c.nnz * (sizeof(AT) + 4) + 4.0 * (c.n + 1)) + 2.0 * vec;
-            break;
-        case PGERT_TIME_UPDATE: c.kBytesPerLaunch = 6.0 * vec; break;
-        case PGERT_TIME_GRAM_Z: c.kBytesPerLaunch = 3.0 * vec; break;
-        case PGERT_TIME_PUPDATE: c.kBytesPerLaunch = 3.0 * vec; break;
-        case PGERT_TIME_GRAM_PQ: c.kBytesPerLaunch = 3.0 * vec; break;
-        default: break;
+        if (c0.kTiming) {
+            double mat = 0.0;
+            for (int p = 0; p < np; ++p)
+                mat += (double)cs[p]->nK * ((double)cs[p]->nnz * (sizeof(AT) + 4) + 4.0 * (cs[p]->n + 1));
+            switch (c0.kSelect) {
+            case PGERT_TIME_SPMM: c0.kBytesPerLaunch = mat + 2.0 * vec; break;
+            case PGERT_TIME_UPDATE: c0.kBytesPerLaunch = 6.0 * vec; break;
+            case PGERT_TIME_GRAM_Z: c0.kBytesPerLaunch = 3.0 * vec; break;
+            case PGERT_TIME_PUPDATE: c0.kBytesPerLaunch = 3.0 * vec; break;
+            case PGERT_TIME_GRAM_PQ: c0.kBytesPerLaunch = 3.0 * vec; break;
+            default: break;
+            }
         }
         auto tb = [&](int id) {
-            if (c.kSelect == id) c.kev_begin();
+            if (c0.kSelect == id) c0.kev_begin();
         };
         auto te = [&](int id) {
-            if (c.kSelect == id) c.kev_end();
+            if (c0.kSelect == id) c0.kev_end();
         };
         DBuf<VT> Xw, Rw, Pw, Qw, Zw;
         Zw.alloc(tot);
@@ -929,70 +1026,86 @@ void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bo
             lastRel = std::max(lastRel, r);
         }
         PG_CHECK(hipMemcpyAsync(pl.d.p, pl.h.data(), pl.h.size() * sizeof(BlockDesc), hipMemcpyHostToDevice, s));
-        int it = 0, refine = 0, lastReplace = 0;
-        const dim3 gG(G, nb), gU(GU, nb), gS(GS, nb);
+        int lastReplace = 0;
+        const dim3 gG(G, nb), gU(GU, nb), gS(GS, nb), gZ(GZ, nb);
         auto z_grams = [&]() {
-            if (useCoarse) {
-                hipLaunchKernelGGL(k_restrict<VT>, dim3(m, nb), blk, 0, s, c.n, m, pl.d.p, c.aggPtr.p, c.aggNodes.p,
-                                   Rw.p, c.ycoarse.p);
-                hipLaunchKernelGGL(k_coarse_apply, dim3(div_up(m, 16), 4, nb), blk, 0, s, m, pl.d.p, c.Einv.p,
-                                   c.ycoarse.p, c.ccoarse.p);
+            if (anyCoarse) {
+                hipLaunchKernelGGL(k_restrict<VT>, dim3(mmax, nb), blk, 0, s, pl.d.p, Rw.p, c0.ycoarse.p);
+                hipLaunchKernelGGL(k_coarse_apply, dim3(div_up(mmax, 16), 4, nb), blk, 0, s, pl.d.p, c0.ycoarse.p,
+                                   c0.ccoarse.p);
             }
-            hipLaunchKernelGGL(k_zform<VT>, dim3(div_up((long)c.n * (S / 4), 256), nb), blk, 0, s, c.n, m, pl.d.p,
-                               c.dinv.p, Rw.p, ccp, aggp, Zw.p);
+            hipLaunchKernelGGL(k_zform<VT>, gZ, blk, 0, s, pl.d.p, Rw.p, ccp, Zw.p);
             tb(PGERT_TIME_GRAM_Z);
-            hipLaunchKernelGGL((k_gram<VT, MODE_Z>), gG, blk, 0, s, c.n, pl.d.p, Pw.p, Qw.p, Zw.p, c.part.p);
+            hipLaunchKernelGGL((k_gram<VT, MODE_Z>), gG, blk, 0, s, pl.d.p, Pw.p, Qw.p, Zw.p, c0.part.p);
             te(PGERT_TIME_GRAM_Z);
-            hipLaunchKernelGGL(k_part_reduce, dim3(3 * SS / 32, nb), blk, 0, s, pl.d.p, G, 3 * SS, c.part.p,
-                               c.small.p, SL_QZ * SS);
+            hipLaunchKernelGGL(k_part_reduce, dim3(3 * SS / 32, nb), blk, 0, s, pl.d.p, G, 3 * SS, c0.part.p,
+                               c0.small.p, SL_QZ * SS);
+        };
+        // one iteration; rr: reduce the recursive residual norms (last of a chunk)
+        auto iteration = [&](bool rr) {
+            tb(PGERT_TIME_SPMM);
+            hipLaunchKernelGGL((k_spmm<VT, AT>), gS, blk, 0, s, pl.d.p, Pw.p, Qw.p);
+            te(PGERT_TIME_SPMM);
+            tb(PGERT_TIME_GRAM_PQ);
+            hipLaunchKernelGGL((k_gram<VT, MODE_PQR>), gG, blk, 0, s, pl.d.p, Pw.p, Qw.p, Rw.p, c0.part.p);
+            te(PGERT_TIME_GRAM_PQ);
+            hipLaunchKernelGGL(k_part_reduce, dim3(2 * SS / 32, nb), blk, 0, s, pl.d.p, G, 2 * SS, c0.part.p,
+                               c0.small.p, SL_G1 * SS);
+            hipLaunchKernelGGL(k_small_alpha, dim3(nb), blk, SMALL_LDS, s, pl.d.p, c0.small.p);
+            tb(PGERT_TIME_UPDATE);
+            hipLaunchKernelGGL(k_update<VT>, gU, blk, 0, s, pl.d.p, c0.small.p, Pw.p, Qw.p, Xp, Rw.p, c0.part.p);
+            te(PGERT_TIME_UPDATE);
+            if (rr)
+                hipLaunchKernelGGL(k_part_reduce, dim3(S / 32, nb), blk, 0, s, pl.d.p, GU, S, c0.part.p, c0.small.p,
+                                   SL_RR * SS);
+            z_grams();
+            hipLaunchKernelGGL(k_small_beta, dim3(nb), blk, SMALL_LDS, s, pl.d.p, c0.small.p, 0);
+            tb(PGERT_TIME_PUPDATE);
+            hipLaunchKernelGGL(k_pupdate<VT>, gU, blk, 0, s, pl.d.p, c0.small.p, Zw.p, Pw.p);
+            te(PGERT_TIME_PUPDATE);
         };
         auto start_block = [&]() {
             // P = 0, Q = 0: Z = M^-1 R and Z'Z only; then P = Z chol(Z'Z)^-T
-            PG_CHECK(hipMemsetAsync(c.small.p, 0, c.small.bytes(), s));
+            PG_CHECK(hipMemsetAsync(c0.small.p, 0, c0.small.bytes(), s));
             z_grams();
-            hipLaunchKernelGGL(k_small_beta, dim3(nb), blk, SMALL_LDS, s, pl.d.p, c.small.p, 1);
-            hipLaunchKernelGGL(k_pupdate<VT>, gU, blk, 0, s, c.n, pl.d.p, c.small.p, Zw.p, Pw.p);
+            hipLaunchKernelGGL(k_small_beta, dim3(nb), blk, SMALL_LDS, s, pl.d.p, c0.small.p, 1);
+            hipLaunchKernelGGL(k_pupdate<VT>, gU, blk, 0, s, pl.d.p, c0.small.p, Zw.p, Pw.p);
             PG_CHECK(hipGetLastError());
         };
         bool done = true;
         for (int b = 0; b < nb; ++b)
             if (pl.h[b].active) done = false;
         if (!done) start_block();
+        // a chunk of check_every iterations is captured once and replayed
+        // (no memsets inside; the kernels read the block flags from memory)
+        hipGraphExec_t gexec = nullptr;
+        const int ce = o.check_every;
+        if (!done && !o.no_graph && !c0.kTiming && ce <= o.max_iter) {
+            hipGraph_t g = nullptr;
+            PG_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            for (int j = 0; j < ce; ++j) iteration(j == ce - 1);
+            PG_CHECK(hipStreamEndCapture(s, &g));
+            PG_CHECK(hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0));
+            PG_CHECK(hipGraphDestroy(g));
+        }
         std::vector<double> hs((size_t)nb * S);
         while (!done && it < o.max_iter) {
-            const int chunk = std::min(o.check_every, o.max_iter - it);
-            for (int j = 0; j < chunk; ++j) {
-                tb(PGERT_TIME_SPMM);
-                hipLaunchKernelGGL((k_spmm<VT, AT>), gS, blk, 0, s, c.n, c.nnz, pl.d.p, c.rowptr.p, c.colidx.p, Aw,
-                                   Pw.p, Qw.p);
-                te(PGERT_TIME_SPMM);
-                tb(PGERT_TIME_GRAM_PQ);
-                hipLaunchKernelGGL((k_gram<VT, MODE_PQR>), gG, blk, 0, s, c.n, pl.d.p, Pw.p, Qw.p, Rw.p, c.part.p);
-                te(PGERT_TIME_GRAM_PQ);
-                hipLaunchKernelGGL(k_part_reduce, dim3(2 * SS / 32, nb), blk, 0, s, pl.d.p, G, 2 * SS, c.part.p,
-                                   c.small.p, SL_G1 * SS);
-                hipLaunchKernelGGL(k_small_alpha, dim3(nb), blk, SMALL_LDS, s, pl.d.p, c.small.p);
-                tb(PGERT_TIME_UPDATE);
-                hipLaunchKernelGGL(k_update<VT>, gU, blk, 0, s, c.n, pl.d.p, c.small.p, Pw.p, Qw.p, Xp, Rw.p,
-                                   c.part.p);
-                te(PGERT_TIME_UPDATE);
-                if (j == chunk - 1)
-                    hipLaunchKernelGGL(k_part_reduce, dim3(S / 32, nb), blk, 0, s, pl.d.p, GU, S, c.part.p,
-                                       c.small.p, SL_RR * SS);
-                z_grams();
-                hipLaunchKernelGGL(k_small_beta, dim3(nb), blk, SMALL_LDS, s, pl.d.p, c.small.p, 0);
-                tb(PGERT_TIME_PUPDATE);
-                hipLaunchKernelGGL(k_pupdate<VT>, gU, blk, 0, s, c.n, pl.d.p, c.small.p, Zw.p, Pw.p);
-                te(PGERT_TIME_PUPDATE);
+            const int chunk = std::min(ce, o.max_iter - it);
+            if (gexec && chunk == ce) {
+                PG_CHECK(hipGraphLaunch(gexec, s));
+            } else {
+                for (int j = 0; j < chunk; ++j) iteration(j == chunk - 1);
             }
             PG_CHECK(hipGetLastError());
             it += chunk;
             // recursive residual norms of the last k_update
             for (int b = 0; b < nb; ++b)
-                PG_CHECK(hipMemcpyAsync(hs.data() + (size_t)b * S, c.small.p + ((size_t)b * NSL + SL_RR) * SS, S * 8,
+                PG_CHECK(hipMemcpyAsync(hs.data() + (size_t)b * S, c0.small.p + ((size_t)b * NSL + SL_RR) * SS, S * 8,
                                         hipMemcpyDeviceToHost, s));
             PG_CHECK(hipStreamSynchronize(s));
-            c.kev_collect();
+            c0.kev_collect();
+            for (int b = 0; b < nb; ++b)
+                if (pl.h[b].active) probIters[pl.h[b].prob] = it;
             double worst = 0;
             bool anyAtTarget = false;
             for (int b = 0; b < nb; ++b) {
@@ -1014,19 +1127,18 @@ void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bo
             bool changed = false;
             if (replaceNow) {
                 lastReplace = it;
-                if constexpr (fp32)
-                    hipLaunchKernelGGL(k_accumulate, gT, blk, 0, s, c.n, pl.d.p, Xw.p, Xb.p);
+                if constexpr (fp32) hipLaunchKernelGGL(k_accumulate, gT, blk, 0, s, pl.d.p, Xw.p, Xb.p);
                 true_resid(Rw.p);
                 ++refine;
                 lastRel = 0.0;
                 for (int b = 0; b < nb; ++b) {
                     if (!pl.h[b].active) continue;
-                    const double tb = block_rel(b);
-                    if (tb <= o.rtol) {
+                    const double tb2 = block_rel(b);
+                    if (tb2 <= o.rtol) {
                         pl.h[b].active = 0;
                         changed = true;
                     } else {
-                        lastRel = std::max(lastRel, tb);
+                        lastRel = std::max(lastRel, tb2);
                     }
                 }
             }
@@ -1036,28 +1148,41 @@ void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bo
             if (changed && !done)
                 PG_CHECK(hipMemcpyAsync(pl.d.p, pl.h.data(), pl.h.size() * sizeof(BlockDesc), hipMemcpyHostToDevice, s));
         }
+        if (gexec) PG_CHECK(hipGraphExecDestroy(gexec));
         if constexpr (fp32) {
-            hipLaunchKernelGGL(k_accumulate, gT, blk, 0, s, c.n, pl.d.p, Xw.p, Xb.p);
+            hipLaunchKernelGGL(k_accumulate, gT, blk, 0, s, pl.d.p, Xw.p, Xb.p);
             PG_CHECK(hipGetLastError());
         }
-        st.iterations = it;
-        st.max_iterations = it;
-        st.refinements = refine;
         PG_CHECK(hipStreamSynchronize(s));  // work buffers are released on return
     };
 
-    if (mixed) {
-        c.fA.alloc((size_t)c.nK * c.nnz);
-        const long nn = (long)c.fA.n;
-        hipLaunchKernelGGL(k_to_float, dim3(div_up(nn, 256)), blk, 0, s, nn, c.valsA.p, c.fA.p);
-        run(float(0), float(0), (const float *)c.fA.p);
-    } else {
-        run(double(0), double(0), (const double *)c.valsA.p);
-    }
-    hipLaunchKernelGGL(k_from_blocks, gT, blk, 0, s, c.n, c.nE, pl.d.p, Xb.p, c.X.p);
+    // the mixed solver keeps A in FP64 inside the FP32 SpMM (products and
+    // row sums in FP64, Q rounded once): rounding A itself to FP32 would make
+    // the inner recurrence solve a different operator, and the FP64 residual
+    // replacement would then stall once cond(A) * 2^-24 nears 1 (the
+    // near-singular small-wavenumber 2.5D systems)
+    if (mixed)
+        run(float(0), double(0));
+    else
+        run(double(0), double(0));
+    // every block back to active for the layout copy
+    for (auto & bd : pl.h) bd.active = 1;
+    PG_CHECK(hipMemcpyAsync(pl.d.p, pl.h.data(), pl.h.size() * sizeof(BlockDesc), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_from_blocks, gT, blk, 0, s, pl.d.p, Xb.p);
     PG_CHECK(hipGetLastError());
-    st.max_rel_residual = residual_check(c);
-    st.converged = st.max_rel_residual <= o.rtol * 1.0001 ? 1 : 0;
+    PG_CHECK(hipEventRecord(c0.solveEv[1], s));
+    c0.stream = user;
+    PG_CHECK(hipStreamWaitEvent(user, c0.solveEv[1], 0));
+    PG_CHECK(hipStreamSynchronize(s));  // the descriptor table is released on return
+    stats.assign(np, pgert_solve_stats{});
+    for (int p = 0; p < np; ++p) {
+        pgert_solve_stats & st = stats[p];
+        st.iterations = probIters[p];
+        st.max_iterations = it;
+        st.refinements = refine;
+        st.max_rel_residual = residual_check(*cs[p]);
+        st.converged = st.max_rel_residual <= o.rtol * 1.0001 ? 1 : 0;
+    }
 }
 
 }  // namespace pgert

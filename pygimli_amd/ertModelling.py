@@ -67,7 +67,7 @@ class ERTModelling:
         self._model = None
         self.jacobianDtype = torch.float64
         self.solverOptions = dict(method="mixed", rtol=1e-6, max_iter=20000, check_every=8,
-                                  inner_rtol=1e-3, learned=False, coarse=0)
+                                  inner_rtol=1e-3, learned=False, coarse=0, graph=True)
         self._coarseSet = None
         self.lastStats = None
         self.warmStart = None
@@ -227,6 +227,7 @@ class ERTModelling:
             self._ctx.check(self._ctx.L.pgert_set_coarse(self._ctx.h, nagg), "set_coarse")
             self._coarseSet = nagg
         opts.coarse = 1 if nagg > 0 else 0
+        opts.no_graph = 0 if o.get("graph", True) else 1
         if self.warmStart is not None:
             ws = self.warmStart
             if ws.dtype != torch.float64 or not ws.is_contiguous():
