@@ -101,6 +101,101 @@ def traffic_from_profiles(kernel):
         return None
 
 
+def batched_cpu_worker(out_path, nthreads, sample):
+    """CPU baseline of the batched 2.5D workload on a bounded sample: for each
+    sampled problem the oracle's FP64 path -- per-wavenumber assembly, sparse
+    direct factorisation (SuperLU, the stand-in for the reference's CHOLMOD),
+    one solve per source, and the calc2 C port for the Jacobian."""
+    import scipy.sparse.linalg as spla
+    from oracle import ert_oracle as eo
+    from oracle import port
+    from pygimli_amd import workloads
+    probs = workloads.batched_2d(count=256, seed=2)
+    res = []
+    for i in sample:
+        mesh, data, rho, k, w = probs[i]
+        fo = eo.Forward(mesh, copy.deepcopy(data), k, w)
+        t = time.time()
+        U = fo.solve_subpotentials(rho)
+        tf = time.time() - t
+        nP = int(mesh.cellMarkers().max()) + 1
+        t = time.time()
+        port.calc2(mesh, U, k, w, data.abmn(), nP, nthreads=nthreads, J=np.zeros((data.size(), nP)))
+        tj = time.time() - t
+        res.append({"i": int(i), "n": mesh.nodeCount(), "nE": len(data.sensorPositions()), "tf": tf, "tj": tj})
+        with open(out_path, "w") as f:
+            json.dump(res, f)
+
+
+def batched_leg(args, log):
+    """Config 3: 256 independent 2.5D problems (seed 2), sub-batches of
+    args.batch_size problems, each ONE batched block solve (every solver
+    launch covers every wavenumber block of every problem of the sub-batch,
+    iterations replayed from a captured HIP graph), then each problem's FP32
+    Jacobian.  Returns the result object."""
+    import torch
+    from pygimli_amd import workloads
+    from pygimli_amd.batch import batch_response_device
+    from pygimli_amd.ertModelling import ERTModelling
+    t0 = time.time()
+    probs = workloads.batched_2d(count=args.batched, seed=2)
+    log(f"[batched] {len(probs)} problems generated in {time.time() - t0:.1f}s")
+    fops, models = [], []
+    for mesh, data, rho, k, w in probs:
+        f = ERTModelling()
+        f.setMesh(mesh)
+        f.setData(copy.deepcopy(data))
+        f.setkValues(k)
+        f.setWeights(w)
+        f.jacobianDtype = torch.float32
+        n = mesh.nodeCount()
+        f.solverOptions.update(method="mixed", rtol=args.rtol, coarse=min(1024, 2 ** int(np.log2(n / 32))))
+        fops.append(f)
+        models.append(torch.as_tensor(rho, device=f.device))
+    log(f"[batched] operators set up at {time.time() - t0:.1f}s")
+    bs = args.batch_size
+    groups = [list(range(i, min(i + bs, len(fops)))) for i in range(0, len(fops), bs)]
+
+    def evaluate(g):
+        batch_response_device([fops[i] for i in g], [models[i] for i in g])
+        its = [fops[i].lastStats["max_iterations"] for i in g]
+        conv = all(fops[i].lastStats["converged"] == 1 for i in g)
+        for i in g:
+            fops[i].createJacobian(models[i])
+            fops[i]._J = None                      # release J before the next problem
+        return max(its), conv
+
+    evaluate(groups[0])                            # warm-up (allocations, code objects)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    iters, ok = [], True
+    for g in groups:
+        it, conv = evaluate(g)
+        iters.append(it)
+        ok = ok and conv
+        log(f"[batched] group of {len(g)}: {it} iterations, converged={conv}, {time.perf_counter() - t:.1f}s")
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    # accuracy on a sample: FP64 block-PCG to 1e-10, one problem at a time
+    errs = []
+    for i in (0, len(fops) // 2, len(fops) - 1):
+        f = fops[i]
+        f.solverOptions.update(method="mixed", rtol=args.rtol)
+        r = batch_response_device([f], [models[i]])[0]
+        f.solverOptions.update(method="block64", rtol=1e-10)
+        r_ref = f.response_device(models[i])
+        errs.append(float(torch.linalg.norm(r - r_ref) / torch.linalg.norm(r_ref)))
+    nodes = sum(f._mesh.nodeCount() for f in fops)
+    rhs = sum(len(f._kv) * len(f._data.sensorPositions()) for f in fops)
+    return {"problems": len(fops), "sub_batch": bs, "seconds": dt, "evals_per_s": 1.0 / dt,
+            "problems_per_s": len(fops) / dt, "iterations_per_group": iters, "converged": ok,
+            "nodes_total": nodes, "rhs_total": rhs, "data_total": sum(f._data.size() for f in fops),
+            "rhoa_rel_err_vs_fp64_1e-10": errs,
+            "workload": "256 x 2.5D: nodes U[2k,40k], electrodes U[24,96] @ U[0.5,2] m, Wenner-Schlumberger, "
+                        "2-4 layers + 0-3 ellipses (log10 rho U[1,3]), 11 wavenumbers; mixed block-PCG 1e-6 with "
+                        "per-problem coarse space, FP32 Jacobian per problem"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +209,10 @@ def main():
     ap.add_argument("--coarse", type=int, default=1024, help="aggregates of the two-level preconditioner (0: off)")
     ap.add_argument("--learned-fields", type=int, default=8,
                     help="random fields the learned warm start is trained on before timing (0: off)")
+    ap.add_argument("--batched", type=int, default=0,
+                    help="also run the batched 2.5D workload with this many problems (256 = the full config)")
+    ap.add_argument("--batch-size", type=int, default=64, help="problems per batched solve")
+    ap.add_argument("--batched-only", action="store_true", help="run only the batched 2.5D leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run one 1 GiB device-to-device copy (source evicted from the last-level cache): "
                          "the known-size kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -137,6 +236,41 @@ def main():
     cpu_proc = None
     cpu_path = os.path.join(ROOT, "gpurun_out", f"cpu_baseline_{os.getpid()}.json")
     nth = min(8, max(1, (os.cpu_count() or 8) - 2))
+    if args.batched_only:
+        import multiprocessing as mp
+        sample = [0, 96, 192]
+        if not args.no_cpu:
+            os.makedirs(os.path.dirname(cpu_path), exist_ok=True)
+            cpu_proc = mp.get_context("spawn").Process(target=batched_cpu_worker, args=(cpu_path, nth, sample),
+                                                       daemon=True)
+            cpu_proc.start()
+        res = batched_leg(args, log)
+        out = {"metric": "batched 2.5D ERT forward+Jacobian, 256 problems per evaluation", "value": res["evals_per_s"],
+               "unit": "evals/s", "n_gpus": 1, "higher_is_better": True, "dtype": "fp32+fp64", "data": "synthetic",
+               "config": {"workload": res["workload"]}, "batched_2d": res}
+        if cpu_proc is not None:
+            cpu_proc.join(args.cpu_timeout)
+            if cpu_proc.is_alive():
+                cpu_proc.terminate()
+            try:
+                with open(cpu_path) as f:
+                    r = json.load(f)
+                from pygimli_amd import workloads
+                probs = workloads.batched_2d(count=args.batched, seed=2)
+                w_all = sum(m.nodeCount() * len(d.sensorPositions()) for m, d, *_ in probs)
+                w_s = sum(x["n"] * x["nE"] for x in r)
+                t_s = sum(x["tf"] + x["tj"] for x in r)
+                t_all = t_s * w_all / w_s
+                out["cpu_baseline"] = {
+                    "value": 1.0 / t_all, "unit": "evals/s", "cores": nth, "kind": "port",
+                    "sample": (f"{len(r)} of {args.batched} problems (indices {[x['i'] for x in r]}) through the oracle: "
+                               f"per-wavenumber FP64 assembly + SuperLU factorisation + one solve per source, calc2 C "
+                               f"port ({nth} threads) = {t_s:.1f} s, scaled by sum(n*nE) to all problems = "
+                               f"{t_all:.0f} s")}
+            except Exception as e:
+                out["cpu_baseline"] = {"error": f"no batched cpu baseline ({e!r})"}
+        print(json.dumps(out))
+        return
     if not args.no_cpu:
         import multiprocessing as mp
         os.makedirs(os.path.dirname(cpu_path), exist_ok=True)
@@ -304,6 +438,8 @@ def main():
                         "potential_rel_err_fro": r["pot_err"], "potential_rel_err_max_col": r["pot_err_max_col"]}
         out["speedup_vs_plain_fp64_block_pcg"] = plain_block["ms"] / best["ms"]
         out["speedup_vs_plain_fp64_pcg"] = plain["ms"] / best["ms"]
+    if args.batched > 0:
+        out["batched_2d"] = batched_leg(args, log)
     if cpu_proc is not None:
         t_wait = time.time()
         while cpu_proc.is_alive() and time.time() - t_wait < args.cpu_timeout:

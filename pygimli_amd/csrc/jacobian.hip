@@ -17,7 +17,7 @@ namespace pgert {
 
 namespace {
 
-constexpr int TILE = 64;
+constexpr int TILE_MAX = 64;  // cells per workgroup (fewer when the electrodes' LDS tile does not fit)
 
 enum { J_STORE = 0, J_ACCUM = 1, J_ATOMIC = 2 };
 
@@ -33,7 +33,9 @@ __device__ inline void jwrite(OutT * J, size_t idx, double v, int mode, bool fir
 }
 
 // One launch per wavenumber.  LDS layout: S[(e * NCOMP + comp) * TILE + cell].
-template <typename OutT, int DIM, int MASS>
+// Lane l of a wavefront takes cell l % TILE of the tile and every
+// (64 / TILE)-th datum starting at its sub-group l / TILE.
+template <typename OutT, int DIM, int MASS, int TILE>
 __global__ void __launch_bounds__(256) k_sens(int n, int nE, int D, int nTile, int nSorted,
                                               const int * __restrict__ sorted, const int * __restrict__ cells,
                                               const int * __restrict__ marker, const double * __restrict__ grads,
@@ -68,14 +70,16 @@ __global__ void __launch_bounds__(256) k_sens(int n, int nE, int D, int nTile, i
             for (int v = 0; v < NV; ++v) S[((size_t)e * NCOMP + DIM + v) * TILE + t] = uv[v];
     }
     __syncthreads();
-    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    constexpr int NSUB = 64 / TILE;
+    const int lane = threadIdx.x % TILE, wave = (threadIdx.x / 64) * NSUB + (threadIdx.x % 64) / TILE;
+    const int nwork = (blockDim.x / 64) * NSUB;
     if (lane >= ncell) return;
     const int c = sorted[base + lane];
     const double V = vol[c];
     const double mfac = MASS ? k2 * V / (NV * (NV + 1)) : 0.0;
     const int col = marker[c];
     const double m2 = scaleModel ? model[col] * model[col] : 1.0;
-    for (int i = wave; i < D; i += blockDim.x / 64) {
+    for (int i = wave; i < D; i += nwork) {
         const int a = abmn[4 * i], b = abmn[4 * i + 1], m = abmn[4 * i + 2], nn = abmn[4 * i + 3];
         double dab[NCOMP], dmn[NCOMP];
         for (int q = 0; q < NCOMP; ++q) {
@@ -121,7 +125,6 @@ __global__ void k_scale_cols(long total, int nParam, const double * __restrict__
 }
 
 template <typename OutT> void run(Ctx & c, const double * model, int nModel, OutT * J) {
-    const int nTile = div_up(c.nSorted, TILE);
     const bool scale = (nModel == c.nParam);
     int mode;
     if (!c.uniqueMarkers) mode = J_ATOMIC;
@@ -134,12 +137,20 @@ template <typename OutT> void run(Ctx & c, const double * model, int nModel, Out
         double k = c.kv[kk];
         const bool mass = k > 0.0;
         const int ncomp = c.dim + (mass ? c.nv : 0);
-        size_t lds = (size_t)c.nE * ncomp * TILE * sizeof(double);
+        int tile = TILE_MAX;
+        while (tile > 16 && (size_t)c.nE * ncomp * tile * sizeof(double) > 160 * 1024) tile /= 2;
+        size_t lds = (size_t)c.nE * ncomp * tile * sizeof(double);
         PG_REQUIRE(lds <= 160 * 1024, "pgert_jacobian: too many electrodes for one LDS tile");
+        const int nTile = div_up(c.nSorted, tile);
         const double * Uk = c.U.p + (size_t)kk * c.n * c.nE;
         int first = kk == 0, last = kk == c.nK - 1;
-        auto kern = c.dim == 2 ? (mass ? k_sens<OutT, 2, 1> : k_sens<OutT, 2, 0>)
-                               : (mass ? k_sens<OutT, 3, 1> : k_sens<OutT, 3, 0>);
+        auto pick = [&](auto tl) {
+            constexpr int T = decltype(tl)::value;
+            return c.dim == 2 ? (mass ? k_sens<OutT, 2, 1, T> : k_sens<OutT, 2, 0, T>)
+                              : (mass ? k_sens<OutT, 3, 1, T> : k_sens<OutT, 3, 0, T>);
+        };
+        auto kern = tile == 64 ? pick(std::integral_constant<int, 64>{})
+                  : tile == 32 ? pick(std::integral_constant<int, 32>{}) : pick(std::integral_constant<int, 16>{});
         PG_CHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL(kern, dim3(nTile), dim3(256), lds, c.stream, c.n, c.nE, c.D, nTile, c.nSorted,
                            c.sortedCells.p, c.cells.p, c.cellMarker.p, c.grads.p, c.vol.p, Uk, c.abmn.p, k * k,

@@ -752,12 +752,15 @@ __global__ void __launch_bounds__(256) k_small_beta(const BlockDesc * __restrict
 }
 
 // ---------------------------------------------------------------- residual and layout kernels
-// R = B - A X (FP64, block layout), R stored as VT; partial |R|^2, |B|^2 per
-// column -> part[(b * G + g) * 2S]
+// True residual r = B - A X (FP64, block layout).  rinv == nullptr: partial
+// |r|^2, |B|^2 per column -> part[(b * G + g) * 2S]; otherwise R = r * rinv
+// per column, stored as VT (the recurrence restarts from a residual of unit
+// scale, so its FP32 products neither underflow nor overflow however small
+// the true residual of a block is).
 template <typename VT>
 __global__ void __launch_bounds__(256) k_bresid(const BlockDesc * __restrict__ blocks, const double * __restrict__ X,
                                                 const double * __restrict__ B, VT * __restrict__ R,
-                                                double * __restrict__ part) {
+                                                double * __restrict__ part, const double * __restrict__ rinv) {
     __shared__ double cn[4][2][S];
     const int b = blockIdx.y;
     const BlockDesc bd = blocks[b];
@@ -776,10 +779,11 @@ __global__ void __launch_bounds__(256) k_bresid(const BlockDesc * __restrict__ b
         const size_t o = vb + (size_t)row * S;
         const double bv = B[o];
         const double r = bv - y;
-        R[o] = (VT)r;
+        if (rinv) R[o] = (VT)(r * rinv[(size_t)b * S + lane]);
         rr += r * r;
         bb += bv * bv;
     }
+    if (rinv) return;
     cn[wave][0][lane] = rr;
     cn[wave][1][lane] = bb;
     __syncthreads();
@@ -817,15 +821,17 @@ __global__ void k_from_blocks(const BlockDesc * __restrict__ blocks, const doubl
     }
 }
 
-// X64 += (double) Xw ; Xw = 0   over the active blocks
-__global__ void k_accumulate(const BlockDesc * __restrict__ blocks, float * __restrict__ Xw,
-                             double * __restrict__ X64) {
+// X64 += scale * Xw ; Xw = 0   over the active blocks (scale: the column's
+// residual norm the FP32 recurrence was normalised by)
+template <typename VT>
+__global__ void k_accumulate(const BlockDesc * __restrict__ blocks, VT * __restrict__ Xw, double * __restrict__ X64,
+                             const double * __restrict__ scale) {
     const BlockDesc bd = blocks[blockIdx.y];
     if (!bd.active) return;
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < (long)bd.n * S; t += (long)gridDim.x * blockDim.x) {
         const size_t o = bd.off + t;
-        X64[o] += (double)Xw[o];
-        Xw[o] = 0.0f;
+        X64[o] += scale[(size_t)blockIdx.y * S + t % S] * (double)Xw[o];
+        Xw[o] = (VT)0;
     }
 }
 
@@ -953,14 +959,39 @@ void solve_blocks(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std
     PG_CHECK(hipGetLastError());
 
     std::vector<double> hn(nb * 2 * S);
-    // FP64 true residual R = B - A X into Rw (VT); norms to the host
+    // per column: the factor the recurrence is normalised by, and its inverse
+    std::vector<double> hscale(nb * S, 1.0), hinv(nb * S, 1.0);
+    DBuf<double> dscale, dinvs;
+    dscale.alloc((size_t)nb * S);
+    dinvs.alloc((size_t)nb * S);
+    // FP64 true residual r = B - A X: norms to the host, then R = r / scale
+    // into Rw (VT)
     auto true_resid = [&](auto * Rw) {
         using VT = std::remove_pointer_t<decltype(Rw)>;
-        hipLaunchKernelGGL(k_bresid<VT>, dim3(GR, nb), blk, 0, s, pl.d.p, Xb.p, Bb.p, Rw, npart.p);
+        hipLaunchKernelGGL(k_bresid<VT>, dim3(GR, nb), blk, 0, s, pl.d.p, Xb.p, Bb.p, Rw, npart.p,
+                           (const double *)nullptr);
         hipLaunchKernelGGL(k_norm_reduce, dim3(nb), dim3(2 * S), 0, s, GR, npart.p, norms.p);
         PG_CHECK(hipGetLastError());
         PG_CHECK(hipMemcpyAsync(hn.data(), norms.p, hn.size() * 8, hipMemcpyDeviceToHost, s));
         PG_CHECK(hipStreamSynchronize(s));
+        // one power-of-two factor per block (about its largest column
+        // residual): exact scaling, so the columns keep their relative weights
+        // and the recurrence its rounding, while its values stay near 1
+        for (int b = 0; b < nb; ++b) {
+            double mx = 0.0;
+            for (int j = 0; j < S; ++j) mx = std::max(mx, hn[(size_t)b * 2 * S + j]);
+            const double sc = mx > 0.0 ? std::ldexp(1.0, std::ilogb(std::sqrt(mx))) : 1.0;
+            for (int j = 0; j < S; ++j) {
+                hscale[b * S + j] = sc;
+                hinv[b * S + j] = 1.0 / sc;
+            }
+        }
+        PG_CHECK(hipMemcpyAsync(dscale.p, hscale.data(), hscale.size() * 8, hipMemcpyHostToDevice, s));
+        PG_CHECK(hipMemcpyAsync(dinvs.p, hinv.data(), hinv.size() * 8, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_bresid<VT>, dim3(GR, nb), blk, 0, s, pl.d.p, Xb.p, Bb.p, Rw, npart.p,
+                           (const double *)dinvs.p);
+        PG_CHECK(hipGetLastError());
+        PG_CHECK(hipStreamSynchronize(s));  // the host scale vectors are reused
     };
     // worst relative residual of block b from the norms just read
     auto block_rel = [&](int b) {
@@ -1005,14 +1036,11 @@ void solve_blocks(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std
         Rw.alloc(tot);
         Pw.alloc(tot);
         Qw.alloc(tot);
-        VT * Xp;
-        if constexpr (fp32) {
-            Xw.alloc(tot);
-            PG_CHECK(hipMemsetAsync(Xw.p, 0, Xw.bytes(), s));
-            Xp = Xw.p;
-        } else {
-            Xp = Xb.p;
-        }
+        // the recurrence's correction in normalised units (see true_resid),
+        // added into the FP64 solution at every residual replacement
+        Xw.alloc(tot);
+        PG_CHECK(hipMemsetAsync(Xw.p, 0, Xw.bytes(), s));
+        VT * Xp = Xw.p;
         PG_CHECK(hipMemsetAsync(Pw.p, 0, Pw.bytes(), s));
         PG_CHECK(hipMemsetAsync(Qw.p, 0, Qw.bytes(), s));
         true_resid(Rw.p);
@@ -1113,7 +1141,7 @@ void solve_blocks(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std
                 double wb = 0.0;
                 for (int j = 0; j < pl.h[b].s; ++j) {
                     const double bb = bnorm2[b * S + j];
-                    if (bb > 0) wb = std::max(wb, std::sqrt(hs[(size_t)b * S + j] / bb));
+                    if (bb > 0) wb = std::max(wb, std::sqrt(hs[(size_t)b * S + j] / bb) * hscale[b * S + j]);
                 }
                 worst = std::max(worst, wb);
                 if (wb <= o.rtol) anyAtTarget = true;
@@ -1127,7 +1155,7 @@ void solve_blocks(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std
             bool changed = false;
             if (replaceNow) {
                 lastReplace = it;
-                if constexpr (fp32) hipLaunchKernelGGL(k_accumulate, gT, blk, 0, s, pl.d.p, Xw.p, Xb.p);
+                hipLaunchKernelGGL(k_accumulate<VT>, gT, blk, 0, s, pl.d.p, Xw.p, Xb.p, (const double *)dscale.p);
                 true_resid(Rw.p);
                 ++refine;
                 lastRel = 0.0;
@@ -1149,10 +1177,8 @@ void solve_blocks(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std
                 PG_CHECK(hipMemcpyAsync(pl.d.p, pl.h.data(), pl.h.size() * sizeof(BlockDesc), hipMemcpyHostToDevice, s));
         }
         if (gexec) PG_CHECK(hipGraphExecDestroy(gexec));
-        if constexpr (fp32) {
-            hipLaunchKernelGGL(k_accumulate, gT, blk, 0, s, pl.d.p, Xw.p, Xb.p);
-            PG_CHECK(hipGetLastError());
-        }
+        hipLaunchKernelGGL(k_accumulate<VT>, gT, blk, 0, s, pl.d.p, Xw.p, Xb.p, (const double *)dscale.p);
+        PG_CHECK(hipGetLastError());
         PG_CHECK(hipStreamSynchronize(s));  // work buffers are released on return
     };
 
