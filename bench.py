@@ -149,7 +149,8 @@ def batched_leg(args, log):
         f.setWeights(w)
         f.jacobianDtype = torch.float32
         n = mesh.nodeCount()
-        f.solverOptions.update(method="mixed", rtol=args.rtol, coarse=min(1024, 2 ** int(np.log2(n / 32))))
+        f.solverOptions.update(method=args.batched_method, rtol=args.rtol,
+                               coarse=min(1024, 2 ** int(np.log2(n / 32))) if args.batched_method != "band" else 0)
         fops.append(f)
         models.append(torch.as_tensor(rho, device=f.device))
     log(f"[batched] operators set up at {time.time() - t0:.1f}s")
@@ -180,7 +181,7 @@ def batched_leg(args, log):
     errs = []
     for i in (0, len(fops) // 2, len(fops) - 1):
         f = fops[i]
-        f.solverOptions.update(method="mixed", rtol=args.rtol)
+        f.solverOptions.update(method=args.batched_method, rtol=args.rtol)
         r = batch_response_device([f], [models[i]])[0]
         f.solverOptions.update(method="block64", rtol=1e-10)
         r_ref = f.response_device(models[i])
@@ -191,9 +192,11 @@ def batched_leg(args, log):
             "problems_per_s": len(fops) / dt, "iterations_per_group": iters, "converged": ok,
             "nodes_total": nodes, "rhs_total": rhs, "data_total": sum(f._data.size() for f in fops),
             "rhoa_rel_err_vs_fp64_1e-10": errs,
+            "method": args.batched_method,
             "workload": "256 x 2.5D: nodes U[2k,40k], electrodes U[24,96] @ U[0.5,2] m, Wenner-Schlumberger, "
-                        "2-4 layers + 0-3 ellipses (log10 rho U[1,3]), 11 wavenumbers; mixed block-PCG 1e-6 with "
-                        "per-problem coarse space, FP32 Jacobian per problem"}
+                        "2-4 layers + 0-3 ellipses (log10 rho U[1,3]), 11 wavenumbers; "
+                        + ("direct banded Cholesky (FP64)" if args.batched_method == "band" else
+                           "mixed block-PCG 1e-6 with per-problem coarse space") + ", FP32 Jacobian per problem"}
 
 
 def main():
@@ -211,7 +214,9 @@ def main():
                     help="random fields the learned warm start is trained on before timing (0: off)")
     ap.add_argument("--batched", type=int, default=0,
                     help="also run the batched 2.5D workload with this many problems (256 = the full config)")
-    ap.add_argument("--batch-size", type=int, default=64, help="problems per batched solve")
+    ap.add_argument("--batch-size", type=int, default=128, help="problems per batched solve")
+    ap.add_argument("--batched-method", default="band",
+                    help="solver of the batched 2.5D leg: band (direct banded Cholesky) or mixed (block-PCG)")
     ap.add_argument("--batched-only", action="store_true", help="run only the batched 2.5D leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run one 1 GiB device-to-device copy (source evicted from the last-level cache): "

@@ -39,6 +39,9 @@ enum pgert_method {
     PGERT_BLOCK_FP64 = 1,
     /* block-PCG with FP32 SpMM/inner iterations and FP64 residual refinement */
     PGERT_BLOCK_MIXED = 2,
+    /* direct: banded Cholesky of every A_k (2D meshes; the reference's
+       sparse direct solve, linSolver.cpp:96-178 / cholmodWrapper.cpp:224-459) */
+    PGERT_BAND_FP64 = 3,
 };
 
 typedef struct pgert_solve_opts {
@@ -142,7 +145,9 @@ int pgert_forward(pgert_ctx * ctx, const double * rho, const pgert_solve_opts * 
  * per-problem steps (assembly, primaries, right-hand sides, response) are the
  * ones of pgert_forward.  rhos[i] / rhoas[i]: DEVICE pointers of problem i;
  * stats: optional array of count entries.  Everything runs on ctxs[0]'s
- * stream.  Block methods only (PGERT_BLOCK_FP64, PGERT_BLOCK_MIXED).
+ * stream.  Block methods and the band solver (PGERT_BLOCK_FP64,
+ * PGERT_BLOCK_MIXED, PGERT_BAND_FP64: one launch factors every wavenumber
+ * of every problem, one launch solves them).
  * The reference runs its problems one after another (each a
  * DCMultiElectrodeModelling::response, dcfemmodelling.cpp:1085-1197); the
  * batch has no counterpart there. */

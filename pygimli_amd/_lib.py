@@ -18,7 +18,9 @@ PGERT_PCG_FP64 = 0
 PGERT_BLOCK_FP64 = 1
 PGERT_BLOCK_MIXED = 2
 
-METHODS = {"pcg64": PGERT_PCG_FP64, "block64": PGERT_BLOCK_FP64, "mixed": PGERT_BLOCK_MIXED}
+PGERT_BAND_FP64 = 3
+METHODS = {"pcg64": PGERT_PCG_FP64, "block64": PGERT_BLOCK_FP64, "mixed": PGERT_BLOCK_MIXED,
+           "band": PGERT_BAND_FP64}
 
 # every symbol include/pgert.h declares
 EXPORTS = (

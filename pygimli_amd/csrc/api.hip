@@ -144,6 +144,7 @@ int pgert_set_mesh(pgert_ctx * c, int dim, int n_nodes, const double * xyz, int 
         hipStream_t s = c->stream;
         c->hXyz.assign(xyz, xyz + (size_t)n_nodes * 3);
         c->nAgg = 0;
+        c->bandValid = false;
         c->xyz.upload(xyz, (size_t)n_nodes * 3, s);
         c->cells.upload(cells, (size_t)n_cells * c->nv, s);
         c->cellMarker.upload(cell_marker, n_cells, s);
@@ -595,7 +596,11 @@ int pgert_forward(pgert_ctx * c, const double * rho, const pgert_solve_opts * op
             solve_block(*c, o, st, false);
         else if (o.method == PGERT_BLOCK_MIXED)
             solve_block(*c, o, st, true);
-        else
+        else if (o.method == PGERT_BAND_FP64) {
+            std::vector<pgert_solve_stats> v;
+            solve_band({c}, o, v);
+            st = v[0];
+        } else
             throw std::runtime_error("pgert_forward: unknown method");
         forward_finish(*c, rhoa);
         sync(*c);
@@ -611,8 +616,8 @@ int pgert_batch_forward(pgert_ctx * const * ctxs, int count, const double * cons
     return guarded(c0, [&] {
         PG_REQUIRE(rhos && rhoas, "pgert_batch_forward: null pointer");
         const pgert_solve_opts o = resolve_opts(opts);
-        PG_REQUIRE(o.method == PGERT_BLOCK_FP64 || o.method == PGERT_BLOCK_MIXED,
-                   "pgert_batch_forward: block methods only");
+        PG_REQUIRE(o.method == PGERT_BLOCK_FP64 || o.method == PGERT_BLOCK_MIXED || o.method == PGERT_BAND_FP64,
+                   "pgert_batch_forward: block or band methods only");
         std::vector<Ctx *> cs(count);
         std::vector<hipStream_t> own(count);
         for (int i = 0; i < count; ++i) {
@@ -628,7 +633,10 @@ int pgert_batch_forward(pgert_ctx * const * ctxs, int count, const double * cons
         try {
             for (int i = 0; i < count; ++i) forward_prepare(*cs[i], rhos[i], o);
             std::vector<pgert_solve_stats> st;
-            solve_blocks(cs, o, st, o.method == PGERT_BLOCK_MIXED);
+            if (o.method == PGERT_BAND_FP64)
+                solve_band(cs, o, st);
+            else
+                solve_blocks(cs, o, st, o.method == PGERT_BLOCK_MIXED);
             for (int i = 0; i < count; ++i) forward_finish(*cs[i], rhoas[i]);
             sync(*c0);
             for (int i = 0; i < count; ++i) {

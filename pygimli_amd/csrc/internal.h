@@ -142,6 +142,11 @@ struct Ctx {
 
     std::shared_ptr<MlpState> mlp;  // learned warm start (mlp.hip)
 
+    // band solver (band.hip): ordering and CSR slot -> band position, per mesh
+    bool bandValid = false;
+    int bandB = 0;
+    DBuf<int> bandPerm, bandSlot;
+
     // block solver's own stream (graph capture needs a non-default stream),
     // ordered after / before the caller's stream with these events
     hipStream_t solveStream = nullptr;
@@ -212,6 +217,9 @@ void solve_block(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bo
 void solve_blocks(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std::vector<pgert_solve_stats> & st,
                   bool mixed);
 double residual_check(Ctx & c);
+// direct banded Cholesky over every wavenumber of several 2D problems (band.hip)
+void band_prepare(Ctx & c);
+void solve_band(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std::vector<pgert_solve_stats> & st);
 
 // two-level preconditioner (coarse.hip)
 void coarse_prepare(Ctx & c);

@@ -39,6 +39,7 @@ def test_batch_matches_single(coarse):
     batched = batch_response(fops, [p[2] for p in probs])
     for f, r1, r0 in zip(fops, batched, single):
         assert f.lastStats["converged"] == 1, f.lastStats
+        assert np.all(np.isfinite(r1)) and np.all(np.isfinite(r0))
         np.testing.assert_allclose(r1, r0, rtol=1e-6)
     # eager launches give the same bits as the graph replays
     for f in fops:
@@ -63,3 +64,25 @@ def test_batch_matches_single(coarse):
         print("J vs oracle", err)
         assert err < 1e-6
     torch.cuda.synchronize()
+
+
+def test_batch_band_direct():
+    """The band solver over a batch (one factor launch, one solve launch for
+    every wavenumber of every problem) equals the single-problem band solves
+    bit for bit and the block solver to its tolerance."""
+    from pygimli_amd.batch import batch_response
+    probs = workloads.batched_2d(count=4, seed=9, n_range=(800, 4000), ne_range=(20, 100))
+    fops = _fops(probs, 0)
+    for f in fops:
+        f.solverOptions.update(method="band", rtol=1e-10)
+    single = [f.response(p[2]) for f, p in zip(fops, probs)]
+    batched = batch_response(fops, [p[2] for p in probs])
+    for f, r1, r0 in zip(fops, batched, single):
+        assert f.lastStats["converged"] == 1 and f.lastStats["max_rel_residual"] < 1e-11, f.lastStats
+        assert np.all(np.isfinite(r1))
+        np.testing.assert_array_equal(r1, r0)
+    for f in fops:
+        f.solverOptions.update(method="block64", rtol=1e-10, coarse=0)
+    ref = batch_response(fops, [p[2] for p in probs])
+    for r1, r2 in zip(batched, ref):
+        np.testing.assert_allclose(r1, r2, rtol=1e-8)

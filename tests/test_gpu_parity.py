@@ -82,6 +82,25 @@ def test_forward_and_jacobian(name, method, oracle_results):
     assert jerr < 1e-5
 
 
+def test_band_direct_small_2d(oracle_results):
+    """Direct banded Cholesky: the direct solution agrees with the oracle's
+    SuperLU solution to rounding."""
+    fo, rhoa_o, J_o = oracle_results["small_2d"]
+    fop, mesh, data, rho = _fop("small_2d", "band")
+    rhoa = fop.response(rho)
+    print("band", fop.lastStats, fop.timings())
+    assert fop.lastStats["converged"] == 1 and fop.lastStats["max_rel_residual"] < 1e-12
+    U = fop.subPotentials().cpu().numpy()
+    assert np.all(np.isfinite(U))
+    pot_err = np.linalg.norm(U - fo.subpots) / np.linalg.norm(fo.subpots)
+    rho_err = np.abs(rhoa - rhoa_o).max() / np.abs(rhoa_o).max()
+    print("potential rel", pot_err, "rhoa rel", rho_err)
+    assert pot_err < 1e-10 and rho_err < 1e-10
+    fop.createJacobian(rho)
+    J = fop.jacobian().cpu().numpy()
+    assert np.linalg.norm(J - J_o) / np.linalg.norm(J_o) < 1e-10
+
+
 def test_config1_2d_vs_oracle():
     """2.5D workload at full size (5203 nodes, 10080 triangles, 41 electrodes,
     888 dipole-dipole data, 11 wavenumbers set explicitly) against the oracle."""
@@ -107,6 +126,17 @@ def test_config1_2d_vs_oracle():
     fop.createJacobian(rho)
     J = fop.jacobian().cpu().numpy()
     assert np.linalg.norm(J - J_o) / np.linalg.norm(J_o) < 1e-5
+    # the direct band solver on the same problem
+    fop.solverOptions.update(method="band", rtol=1e-10)
+    rhoa_b = fop.response(rho)
+    print("band", fop.lastStats, fop.timings())
+    assert fop.lastStats["converged"] == 1
+    U = fop.subPotentials().cpu().numpy()
+    assert np.linalg.norm(U - fo.subpots) / np.linalg.norm(fo.subpots) < 1e-10
+    assert np.abs(rhoa_b - rhoa_o).max() / np.abs(rhoa_o).max() < 1e-10
+    fop.createJacobian(rho)
+    J = fop.jacobian().cpu().numpy()
+    assert np.linalg.norm(J - J_o) / np.linalg.norm(J_o) < 1e-10
 
 
 def test_config2_3d_invariants():
