@@ -168,6 +168,17 @@ int pgert_potentials_node_major(pgert_ctx * ctx, double * out);
  * (dtype 0) or FP32 (dtype 1).  Scaled by k_i / m_j^2 when n_model == n_param. */
 int pgert_jacobian(pgert_ctx * ctx, const double * model, int n_model, void * J, int dtype);
 
+/* Jacobians of several problems after a forward of each (pgert_forward or
+ * pgert_batch_forward), in one launch per wavenumber pass over the cells of
+ * every problem.  The reference computes them one model at a time
+ * (DCMultiElectrodeModelling::createJacobian, dcfemmodelling.cpp:1446-1476);
+ * this is that call for `count` problems.  All contexts on one device, same
+ * dimension; runs on ctxs[0]'s stream.  models[i]: DEVICE [n_models[i]];
+ * Js[i]: DEVICE [n_data_i x n_param_i], FP64 (dtype 0) or FP32 (dtype 1).
+ * The launch time is reported in ctxs[0]'s pgert_last_timings()[4]. */
+int pgert_batch_jacobian(pgert_ctx * const * ctxs, int count, const double * const * models, const int * n_models,
+                         void * const * Js, int dtype);
+
 /* Timing of the last pgert_forward / pgert_jacobian phases in milliseconds
  * (HIP events on the context stream): [0] assembly, [1] primary + SR rhs,
  * [2] solve, [3] response, [4] jacobian. */

@@ -49,6 +49,38 @@ def batch_response_device(fops, models):
     return rhoas
 
 
+def batch_jacobian(fops, models):
+    """Jacobians of every problem after a (batched) forward of the same
+    models, one launch per wavenumber pass over the cells of all problems
+    (pgert_batch_jacobian).  Each fop.jacobian() holds its own afterwards;
+    returns the list of Jacobians."""
+    if len(fops) != len(models) or not fops:
+        raise ValueError("one model per operator")
+    ts, Js = [], []
+    for f, m in zip(fops, models):
+        t = f._model_tensor(m)
+        if f._model is None or f._model.shape != t.shape or not torch.equal(f._model, t):
+            raise ValueError("batch_jacobian needs the forward of the same model first (batch_response)")
+        ts.append(t)
+        Js.append(torch.empty((f._data.size(), f._nParam), dtype=f.jacobianDtype, device=f.device))
+    dt = {f.jacobianDtype for f in fops}
+    if len(dt) != 1:
+        raise ValueError("all operators need the same jacobianDtype")
+    code = 0 if fops[0].jacobianDtype == torch.float64 else 1
+    fops[0]._set_stream()
+    n = len(fops)
+    vp = ctypes.c_void_p
+    ctxs = (vp * n)(*[f._ctx.h for f in fops])
+    mp = (vp * n)(*[_dptr(t) for t in ts])
+    nm = (ctypes.c_int * n)(*[t.numel() for t in ts])
+    jp = (vp * n)(*[_dptr(J) for J in Js])
+    L = fops[0]._ctx.L
+    fops[0]._ctx.check(L.pgert_batch_jacobian(ctxs, n, mp, nm, jp, code), "batch_jacobian")
+    for f, J in zip(fops, Js):
+        f._J = J
+    return Js
+
+
 def batch_response(fops, models):
     """Numpy apparent resistivities of every problem."""
     return [r.cpu().numpy() for r in batch_response_device(fops, models)]

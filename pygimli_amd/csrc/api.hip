@@ -682,6 +682,36 @@ int pgert_jacobian(pgert_ctx * c, const double * model, int n_model, void * J, i
     });
 }
 
+int pgert_batch_jacobian(pgert_ctx * const * ctxs, int count, const double * const * models, const int * n_models,
+                         void * const * Js, int dtype) {
+    if (!ctxs || count <= 0 || !ctxs[0]) return -2;
+    pgert_ctx * c0 = ctxs[0];
+    return guarded(c0, [&] {
+        PG_REQUIRE(models && n_models && Js && (dtype == 0 || dtype == 1), "pgert_batch_jacobian: bad arguments");
+        std::vector<Ctx *> cs(count);
+        std::vector<const double *> ms(count);
+        std::vector<int> nm(count);
+        std::vector<void *> js(count);
+        for (int i = 0; i < count; ++i) {
+            PG_REQUIRE(ctxs[i] && Js[i], "pgert_batch_jacobian: null problem");
+            PG_REQUIRE(ctxs[i]->device == c0->device, "pgert_batch_jacobian: problems on different devices");
+            PG_REQUIRE(ctxs[i]->haveU, "pgert_batch_jacobian: call a forward first");
+            PG_REQUIRE(n_models[i] >= 0 && (n_models[i] == 0 || models[i]), "pgert_batch_jacobian: bad model");
+            cs[i] = ctxs[i];
+            ms[i] = models[i];
+            nm[i] = n_models[i];
+            js[i] = Js[i];
+        }
+        PG_CHECK(hipEventRecord(c0->ev[4], c0->stream));
+        launch_jacobian_batch(cs, ms, nm, js, dtype);
+        PG_CHECK(hipEventRecord(c0->ev[5], c0->stream));
+        sync(*c0);
+        float t;
+        PG_CHECK(hipEventElapsedTime(&t, c0->ev[4], c0->ev[5]));
+        c0->timings[4] = t;
+    });
+}
+
 int pgert_mlp_param_count(void) { return mlp_param_count(); }
 
 int pgert_mlp_set_params(pgert_ctx * c, const float * params, int count) {

@@ -25,8 +25,9 @@ namespace {
 
 constexpr int NB = 32;    // panel width
 constexpr int LDP = NB + 1;
-constexpr int BMAX = 400; // band limit (LDS)
-constexpr int EMAX = 128; // electrodes per solve (LDS)
+constexpr int BMAX = 255; // band limit (one L21 row per solve thread)
+constexpr int EMAX = 128; // electrodes per solve
+constexpr int CG = 16;    // right-hand sides per solve workgroup
 
 struct BandDesc {
     int n, b, nE, nnz;
@@ -37,7 +38,7 @@ struct BandDesc {
     const int * perm;     // band order -> node
     const double * B;     // right-hand sides of wavenumber k, node-major [node][nE]
     double * X;           // solution of wavenumber k, node-major [node][nE]
-    double * Y;           // work, band order [row][nE]
+    double * Y;           // forward-sweep result, band order [row][nE]
 };
 
 __global__ void k_band_scatter(const BandDesc * __restrict__ descs) {
@@ -57,222 +58,329 @@ __device__ inline void tri_tile(int t, int & ti, int & tj) {
     tj = t - i * (i + 1) / 2;
 }
 
-// one workgroup per factorisation; LDS: panel [NB + b + 4][LDP], Li [NB][LDP]
-__global__ void __launch_bounds__(256) k_band_factor(const BandDesc * __restrict__ descs) {
+// One workgroup per factorisation.  Panel rows live in registers (thread r
+// owns rows r, r + 256, ...: RPT rows), so a column step is one barrier: the
+// top NB rows publish their next column in LDS (colb[c][r] = panel(r, c)
+// once columns < c are applied) and every thread scales and updates its own
+// rows.  Then wavefront 0 inverts the diagonal block (a column per lane, in
+// registers) while wavefronts 1-3 apply the rank-NB update to the trailing
+// b x b band window.  LDS: colb [NB][LDP], panel [NB + b][LDP].
+template <int RPT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_band_factor(const BandDesc * __restrict__ descs) {
     extern __shared__ double sm[];
     const BandDesc d = descs[blockIdx.x];
     const int n = d.n, b = d.b, W1 = b + 1;
-    const int PR = NB + b + 4;
-    double * Pm = sm;
-    double * Li = sm + (size_t)PR * LDP;
+    const int tid = threadIdx.x;
+    const auto L = glob(d.L);
+    double * colb = sm;
+    double * Pm = sm + NB * LDP;
     for (int j0 = 0; j0 < n; j0 += NB) {
         const int nc = min(NB, n - j0);
-        // panel: Pm[r][c] = L(j0 + r, j0 + c) inside the band, else 0
-        for (int t = threadIdx.x; t < PR * NB; t += blockDim.x) {
-            const int r = t / NB, c = t % NB;
-            double v = 0.0;
-            if (c < nc && r >= c && r - c <= b && j0 + r < n) v = d.L[(size_t)(j0 + c) * W1 + (r - c)];
-            Pm[r * LDP + c] = v;
-        }
-        __syncthreads();
-        for (int c = 0; c < nc; ++c) {
-            const double piv = Pm[c * LDP + c];
-            const double dd = sqrt(piv > 0.0 ? piv : 1e-300);
-            __syncthreads();
-            for (int r = c + threadIdx.x; r < PR; r += blockDim.x) Pm[r * LDP + c] = r == c ? dd : Pm[r * LDP + c] / dd;
-            __syncthreads();
-            const int m2 = nc - c - 1;
-            if (m2 > 0)
-                for (int t = threadIdx.x; t < PR * m2; t += blockDim.x) {
-                    const int r = t / m2, c2 = c + 1 + t % m2;
-                    if (r >= c2) Pm[r * LDP + c2] -= Pm[r * LDP + c] * Pm[c2 * LDP + c];
-                }
-            __syncthreads();
-        }
-        // factored panel back to the band
-        for (int t = threadIdx.x; t < PR * NB; t += blockDim.x) {
-            const int r = t / NB, c = t % NB;
-            if (c < nc && r >= c && r - c <= b && j0 + r < n) d.L[(size_t)(j0 + c) * W1 + (r - c)] = Pm[r * LDP + c];
-        }
-        // inverse of the diagonal block (lower triangular), a thread per column
-        if (threadIdx.x < NB) {
-            const int col = threadIdx.x;
-            for (int r = 0; r < NB; ++r) {
-                double v = 0.0;
-                if (r < nc && col < nc && r >= col) {
-                    v = r == col ? 1.0 : 0.0;
-                    for (int q = col; q < r; ++q) v -= Pm[r * LDP + q] * Li[q * LDP + col];
-                    v /= Pm[r * LDP + r];
-                }
-                Li[r * LDP + col] = v;
-            }
-        }
-        __syncthreads();
-        double * Di = d.Dinv + (size_t)(j0 / NB) * NB * NB;
-        for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) Di[t] = Li[(t / NB) * LDP + t % NB];
-        // trailing band window: rows / columns j0 + nc .. j0 + nc + W - 1
         const int W = min(b, n - (j0 + nc));
-        const int T = (W + 3) / 4;
-        for (int tile = threadIdx.x; tile < T * (T + 1) / 2; tile += blockDim.x) {
-            int ti, tj;
-            tri_tile(tile, ti, tj);
-            double acc[4][4] = {};
-            for (int c = 0; c < nc; ++c) {
-                double a[4], q[4];
+        const int PRn = nc + W;
+        // panel through LDS: column by column, lanes over rows (coalesced band reads)
+        // (loads are unconditional from a clamped index, so they issue back to back)
+        for (int r = tid; r < PRn; r += 256)
 #pragma unroll
-                for (int x = 0; x < 4; ++x) {
-                    a[x] = Pm[(nc + 4 * ti + x) * LDP + c];
-                    q[x] = Pm[(nc + 4 * tj + x) * LDP + c];
+            for (int c = 0; c < NB; ++c) {
+                const bool in = c < nc && r >= c && r - c <= b;
+                const double v = L[in ? (size_t)(j0 + c) * W1 + (r - c) : 0];
+                Pm[r * LDP + c] = in ? v : 0.0;
+            }
+        __syncthreads();
+        double p[RPT][NB];
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) {
+            const int r = tid + 256 * i;
+#pragma unroll
+            for (int c = 0; c < NB; ++c) p[i][c] = r < PRn ? Pm[r * LDP + c] : 0.0;
+        }
+        if (tid < NB) colb[tid] = p[0][0];
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            if (c < nc) {
+                const double piv = colb[c * LDP + c];
+                const double dd = sqrt(piv > 0.0 ? piv : 1e-300);
+                const double rd = 1.0 / dd;
+#pragma unroll
+                for (int i = 0; i < RPT; ++i) {
+                    const int r = tid + 256 * i;
+                    const double l = r == c ? dd : p[i][c] * rd;
+                    const double lm = r >= c ? l : 0.0;  // rows above c: upper triangle, untouched
+                    p[i][c] = lm;
+#pragma unroll
+                    for (int c2 = c + 1; c2 < NB; ++c2) p[i][c2] -= lm * (colb[c * LDP + c2] * rd);
                 }
+                if (c + 1 < NB && tid < NB) colb[(c + 1) * LDP + tid] = p[0][c + 1];
+            }
+            lds_barrier();
+        }
+        // factored panel: to LDS (L11 for its inverse, L21 for the update) and back to the band
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) {
+            const int r = tid + 256 * i;
+            if (r < PRn)
+#pragma unroll
+                for (int c = 0; c < NB; ++c) Pm[r * LDP + c] = p[i][c];
+        }
+        __syncthreads();
+        for (int c = 0; c < nc; ++c)
+            for (int r = c + tid; r < PRn && r - c <= b; r += 256) L[(size_t)(j0 + c) * W1 + (r - c)] = Pm[r * LDP + c];
+        if (tid < NB) {
+            // inverse of the lower-triangular diagonal block, column `col`
+            const int col = tid;
+            double x[NB];
+#pragma unroll
+            for (int r = 0; r < NB; ++r) {
+                double v = r == col ? 1.0 : 0.0;
+#pragma unroll
+                for (int q = 0; q < r; ++q) v -= Pm[r * LDP + q] * x[q];
+                x[r] = (r >= col && r < nc) ? v / Pm[r * LDP + r] : 0.0;
+            }
+            const auto Di = glob(d.Dinv) + (size_t)(j0 / NB) * NB * NB;
+#pragma unroll
+            for (int r = 0; r < NB; ++r) Di[r * NB + col] = x[r];
+        } else if (tid >= 64) {
+            // trailing band window: rows / columns j0 + nc .. j0 + nc + W - 1
+            const int T = (W + 3) / 4;
+            for (int tile = tid - 64; tile < T * (T + 1) / 2; tile += 192) {
+                int ti, tj;
+                tri_tile(tile, ti, tj);
+                double acc[4][4] = {};
+#pragma unroll 8
+                for (int c = 0; c < NB; ++c) {  // panel columns >= nc are zero
+                    double a[4], q[4];
+#pragma unroll
+                    for (int x = 0; x < 4; ++x) {
+                        a[x] = Pm[(nc + 4 * ti + x) * LDP + c];
+                        q[x] = Pm[(nc + 4 * tj + x) * LDP + c];
+                    }
+#pragma unroll
+                    for (int x = 0; x < 4; ++x)
+#pragma unroll
+                        for (int y = 0; y < 4; ++y) acc[x][y] += a[x] * q[y];
+                }
+                double old[4][4];
 #pragma unroll
                 for (int x = 0; x < 4; ++x)
 #pragma unroll
-                    for (int y = 0; y < 4; ++y) acc[x][y] += a[x] * q[y];
+                    for (int y = 0; y < 4; ++y) {
+                        const int rr = 4 * ti + x, jj = 4 * tj + y;
+                        const bool in = rr < W && jj < W && rr >= jj;
+                        old[x][y] = L[in ? (size_t)(j0 + nc + jj) * W1 + (rr - jj) : 0];
+                    }
+#pragma unroll
+                for (int x = 0; x < 4; ++x)
+#pragma unroll
+                    for (int y = 0; y < 4; ++y) {
+                        const int rr = 4 * ti + x, jj = 4 * tj + y;
+                        if (rr < W && jj < W && rr >= jj) L[(size_t)(j0 + nc + jj) * W1 + (rr - jj)] = old[x][y] - acc[x][y];
+                    }
             }
-#pragma unroll
-            for (int x = 0; x < 4; ++x)
-#pragma unroll
-                for (int y = 0; y < 4; ++y) {
-                    const int rr = 4 * ti + x, jj = 4 * tj + y;
-                    if (rr < W && jj < W && rr >= jj) d.L[(size_t)(j0 + nc + jj) * W1 + (rr - jj)] -= acc[x][y];
-                }
         }
         __syncthreads();
     }
 }
 
-// Y (band order) from the node-major right-hand sides, and back
-__global__ void k_band_gather(const BandDesc * __restrict__ descs) {
-    const BandDesc d = descs[blockIdx.y];
-    const long tot = (long)d.n * d.nE;
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (long)gridDim.x * blockDim.x) {
-        const int r = (int)(t / d.nE), e = (int)(t % d.nE);
-        d.Y[t] = d.B[(size_t)d.perm[r] * d.nE + e];
-    }
-}
-
-__global__ void k_band_scatter_x(const BandDesc * __restrict__ descs) {
-    const BandDesc d = descs[blockIdx.y];
-    const long tot = (long)d.n * d.nE;
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (long)gridDim.x * blockDim.x) {
-        const int r = (int)(t / d.nE), e = (int)(t % d.nE);
-        d.X[(size_t)d.perm[r] * d.nE + e] = d.Y[t];
-    }
-}
-
-// one workgroup per (problem, wavenumber): forward and backward sweeps over
-// all nE right-hand sides.  LDS: Lr [b + 4][LDP] (L21 of the block row),
-// Li [NB][LDP], y [NB][nE + 1], z [NB][nE + 1]
-__global__ void __launch_bounds__(256) k_band_solve(const BandDesc * __restrict__ descs) {
+// One workgroup per (problem, wavenumber, group of CG right-hand sides).
+// Forward (L y = b): the rows still receiving updates live in an LDS ring of
+// NB + b rows; per block row, z = L11^-1 y1 (stored as the forward result),
+// the next W rows of the ring get -= L21 z and the consumed slots are
+// refilled from b.  Backward (L' x = y): the ring holds the last solved rows;
+// t = y1 - L21' x2, x1 = L11^-T t, written to the node-major solution.
+// Software-pipelined: the next block row's L21 row (thread rr holds row rr,
+// b <= 255), its L11^-1 and the refill / forward values are loaded into
+// registers while the current block row computes; only LDS-ordering
+// barriers inside the sweeps, so stores and prefetches stay in flight.
+// LDS: ring [NB + b][CG], Lr [b][LDP], Li [NB][LDP], zs [NB][CG], pt [4][NB][CG].
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_band_solve(const BandDesc * __restrict__ descs) {
     extern __shared__ double sm[];
-    const BandDesc d = descs[blockIdx.x];
-    const int n = d.n, b = d.b, W1 = b + 1, nE = d.nE, LE = nE + 1;
-    double * Lr = sm;
-    double * Li = Lr + (size_t)(b + 4) * LDP;
-    double * ys = Li + (size_t)NB * LDP;
-    double * zs = ys + (size_t)NB * LE;
-    double * Y = d.Y;
-    const int nblk = (n + NB - 1) / NB;
-    const int TE = (nE + 3) / 4;
-    // stage the block row j0: L21 (rows j0 + nc + rr, columns j0 + c) and L11^-1
-    auto stage = [&](int j0, int nc, int W) {
-        for (int t = threadIdx.x; t < (W + 4) * NB; t += blockDim.x) {
-            const int rr = t / NB, c = t % NB;
-            const int dlt = nc + rr - c;
-            double v = 0.0;
-            if (rr < W && c < nc && dlt <= b) v = d.L[(size_t)(j0 + c) * W1 + dlt];
-            Lr[rr * LDP + c] = v;
+    const BandDesc d = descs[blockIdx.y];
+    const int e0 = blockIdx.x * CG;
+    if (e0 >= d.nE) return;
+    const int ncol = min(CG, d.nE - e0);
+    const int n = d.n, b = d.b, W1 = b + 1, nE = d.nE;
+    const int R = NB + b;
+    const int tid = threadIdx.x;
+    double * ring = sm;
+    double * Lr = ring + (size_t)R * CG;
+    double * Li = Lr + (size_t)b * LDP;
+    double * zs = Li + NB * LDP;
+    double * pt = zs + NB * CG;  // [4][NB][CG] partial sums of the backward sweep
+    const auto perm = glob(d.perm);
+    const auto L = glob(d.L);
+    const auto Bv = glob(d.B);
+    const auto Y = glob(d.Y);
+    const auto X = glob(d.X);
+    const auto Dv = glob(d.Dinv);
+    auto slot = [&](int s0, int q) { const int s = s0 + q; return s < R ? s : s - R; };
+    double lp[NB], dp[4];
+    // block row j0's L21 row `tid` and L11^-1 into registers
+    auto fetch = [&](int j0) {
+        const int nc = min(NB, n - j0), W = min(b, n - (j0 + nc));
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            const int dlt = nc + tid - c;
+            const bool in = tid < W && c < nc && dlt <= b;
+            const double v = L[in ? (size_t)(j0 + c) * W1 + dlt : 0];
+            lp[c] = in ? v : 0.0;
         }
-        const double * Di = d.Dinv + (size_t)(j0 / NB) * NB * NB;
-        for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) Li[(t / NB) * LDP + t % NB] = Di[t];
-        for (int t = threadIdx.x; t < NB * nE; t += blockDim.x) {
-            const int r = t / nE, e = t % nE;
-            ys[r * LE + e] = r < nc ? Y[(size_t)(j0 + r) * nE + e] : 0.0;
+        const size_t o = (size_t)(j0 / NB) * NB * NB;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) dp[i] = Dv[o + tid + 256 * i];
+    };
+    auto put = [&]() {
+        if (tid < b)
+#pragma unroll
+            for (int c = 0; c < NB; ++c) Lr[tid * LDP + c] = lp[c];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int t = tid + 256 * i;
+            Li[(t / NB) * LDP + t % NB] = dp[i];
         }
     };
-    // ---- forward: L y = b
+    const int nblk = (n + NB - 1) / NB;
+    // ---- forward
+    for (int t = tid; t < min(n, R) * CG; t += 256) {
+        const int r = t / CG, cc = t % CG;
+        const double v = Bv[(size_t)perm[r] * nE + e0 + min(cc, ncol - 1)];
+        ring[t] = cc < ncol ? v : 0.0;
+    }
+    fetch(0);
     for (int blk = 0; blk < nblk; ++blk) {
         const int j0 = blk * NB, nc = min(NB, n - j0), W = min(b, n - (j0 + nc));
-        stage(j0, nc, W);
-        __syncthreads();
-        // z = L11^-1 y
-        for (int t = threadIdx.x; t < NB * nE; t += blockDim.x) {
-            const int r = t / nE, e = t % nE;
-            double v = 0.0;
-            for (int q = 0; q <= r; ++q) v += Li[r * LDP + q] * ys[q * LE + e];
-            zs[r * LE + e] = v;
-            if (r < nc) Y[(size_t)(j0 + r) * nE + e] = v;
+        const int s0 = j0 % R;
+        put();
+        // values for the slots this block row consumes: rows j0 + R + r
+        double rf[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int t = tid + 256 * k, r = t / CG, cc = t % CG;
+            const int row = min(j0 + R + r, n - 1);
+            rf[k] = Bv[(size_t)perm[row] * nE + e0 + min(cc, ncol - 1)];
         }
-        __syncthreads();
-        // rows below: Y -= L21 z (4 x 4 tiles over W x nE)
-        const int TW = (W + 3) / 4;
-        for (int tile = threadIdx.x; tile < TW * TE; tile += blockDim.x) {
-            const int ti = tile / TE, te = tile % TE;
-            double acc[4][4] = {};
-            for (int c = 0; c < nc; ++c) {
-                double a[4], z[4];
+        lds_barrier();
+        if (blk + 1 < nblk) fetch(j0 + NB);
+        // z = L11^-1 y1 (L11^-1 is zero above the diagonal and beyond nc: fixed-length, unrolled)
 #pragma unroll
-                for (int x = 0; x < 4; ++x) a[x] = Lr[(4 * ti + x) * LDP + c];
+        for (int k = 0; k < 2; ++k) {
+            const int t = tid + 256 * k, r = t / CG, cc = t % CG;
+            double v = 0.0;
+#pragma unroll 8
+            for (int q = 0; q < NB; ++q) v += Li[r * LDP + q] * ring[slot(s0, q) * CG + cc];
+            zs[t] = v;
+            if (r < nc && cc < ncol) Y[(size_t)(j0 + r) * nE + e0 + cc] = v;
+        }
+        lds_barrier();
+        // ring rows j0 + nc + rr -= L21 z, tiles of 4 rows x 2 columns
+        const int TR = (W + 3) / 4;
+        for (int tile = tid; tile < TR * (CG / 2); tile += 256) {
+            const int tr = tile / (CG / 2), tc = tile % (CG / 2);
+            double acc[4][2] = {};
+#pragma unroll 8
+            for (int c = 0; c < NB; ++c) {  // L21 columns >= nc are zero
+                const double2 z = *(const double2 *)(zs + c * CG + 2 * tc);
 #pragma unroll
-                for (int y = 0; y < 4; ++y) z[y] = 4 * te + y < nE ? zs[c * LE + 4 * te + y] : 0.0;
-#pragma unroll
-                for (int x = 0; x < 4; ++x)
-#pragma unroll
-                    for (int y = 0; y < 4; ++y) acc[x][y] += a[x] * z[y];
+                for (int x = 0; x < 4; ++x) {
+                    const double a = Lr[(4 * tr + x) * LDP + c];
+                    acc[x][0] += a * z.x;
+                    acc[x][1] += a * z.y;
+                }
             }
 #pragma unroll
-            for (int x = 0; x < 4; ++x)
-#pragma unroll
-                for (int y = 0; y < 4; ++y) {
-                    const int rr = 4 * ti + x, e = 4 * te + y;
-                    if (rr < W && e < nE) Y[(size_t)(j0 + nc + rr) * nE + e] -= acc[x][y];
+            for (int x = 0; x < 4; ++x) {
+                const int rr = 4 * tr + x;
+                if (rr < W) {
+                    double * rw = ring + slot(s0, nc + rr) * CG + 2 * tc;
+                    rw[0] -= acc[x][0];
+                    rw[1] -= acc[x][1];
                 }
+            }
         }
-        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int t = tid + 256 * k, r = t / CG, cc = t % CG;
+            if (r < nc && j0 + R + r < n) ring[slot(s0, r) * CG + cc] = cc < ncol ? rf[k] : 0.0;
+        }
+        lds_barrier();
     }
-    // ---- backward: L' x = y
+    // ---- backward (the forward results in Y must be visible: full barrier)
+    __syncthreads();
+    double yp[2];
+    int pp[2];
+    auto fetch_y = [&](int j0) {
+        const int nc = min(NB, n - j0);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int t = tid + 256 * k, r = min(t / CG, nc - 1), cc = t % CG;
+            yp[k] = Y[(size_t)(j0 + r) * nE + e0 + min(cc, ncol - 1)];
+            pp[k] = perm[j0 + r];
+        }
+    };
+    fetch((nblk - 1) * NB);
+    fetch_y((nblk - 1) * NB);
     for (int blk = nblk - 1; blk >= 0; --blk) {
         const int j0 = blk * NB, nc = min(NB, n - j0), W = min(b, n - (j0 + nc));
-        stage(j0, nc, W);
-        __syncthreads();
-        // z = y1 - L21' x2 (4 x 4 tiles over NB x nE)
-        for (int tile = threadIdx.x; tile < (NB / 4) * TE; tile += blockDim.x) {
-            const int tc = tile / TE, te = tile % TE;
-            double acc[4][4] = {};
-            for (int rr = 0; rr < W; ++rr) {
-                double a[4], x2[4];
+        const int s0 = j0 % R;
+        put();
+        int pc[2];
 #pragma unroll
-                for (int x = 0; x < 4; ++x) a[x] = Lr[rr * LDP + 4 * tc + x];
+        for (int k = 0; k < 2; ++k) {
+            const int t = tid + 256 * k, r = t / CG, cc = t % CG;
+            zs[t] = (r < nc && cc < ncol) ? yp[k] : 0.0;
+            pc[k] = pp[k];
+        }
+        lds_barrier();
+        if (blk > 0) {
+            fetch(j0 - NB);
+            fetch_y(j0 - NB);
+        }
+        {
+            // L21' x2: wavefront w takes rows rr = w, w + 4, ...; lane = 4 x 2 tile of (r, cc)
+            const int w = tid >> 6, ln = tid & 63, tr = ln >> 3, tc = ln & 7;
+            double acc[4][2] = {};
+#pragma unroll 4
+            for (int rr = w; rr < W; rr += 4) {
+                const double2 xv = *(const double2 *)(ring + slot(s0, nc + rr) * CG + 2 * tc);
 #pragma unroll
-                for (int y = 0; y < 4; ++y) x2[y] = 4 * te + y < nE ? Y[(size_t)(j0 + nc + rr) * nE + 4 * te + y] : 0.0;
-#pragma unroll
-                for (int x = 0; x < 4; ++x)
-#pragma unroll
-                    for (int y = 0; y < 4; ++y) acc[x][y] += a[x] * x2[y];
+                for (int x = 0; x < 4; ++x) {
+                    const double a = Lr[rr * LDP + 4 * tr + x];
+                    acc[x][0] += a * xv.x;
+                    acc[x][1] += a * xv.y;
+                }
             }
 #pragma unroll
-            for (int x = 0; x < 4; ++x)
+            for (int x = 0; x < 4; ++x) {
+                pt[(w * NB + 4 * tr + x) * CG + 2 * tc] = acc[x][0];
+                pt[(w * NB + 4 * tr + x) * CG + 2 * tc + 1] = acc[x][1];
+            }
+        }
+        lds_barrier();
 #pragma unroll
-                for (int y = 0; y < 4; ++y) {
-                    const int c = 4 * tc + x, e = 4 * te + y;
-                    if (e < nE) zs[c * LE + e] = ys[c * LE + e] - acc[x][y];
-                }
+        for (int k = 0; k < 2; ++k) {
+            const int t = tid + 256 * k;
+            zs[t] -= (pt[t] + pt[NB * CG + t]) + (pt[2 * NB * CG + t] + pt[3 * NB * CG + t]);
         }
-        __syncthreads();
-        // x1 = L11^-T z
-        for (int t = threadIdx.x; t < NB * nE; t += blockDim.x) {
-            const int r = t / nE, e = t % nE;
-            double v = 0.0;
-            for (int q = r; q < NB; ++q) v += Li[q * LDP + r] * zs[q * LE + e];
-            if (r < nc) Y[(size_t)(j0 + r) * nE + e] = v;
+        lds_barrier();
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int t = tid + 256 * k, r = t / CG, cc = t % CG;
+            if (r < nc) {
+                double v = 0.0;
+#pragma unroll 8
+                for (int q = 0; q < NB; ++q) v += Li[q * LDP + r] * zs[q * CG + cc];  // zero above / beyond
+                ring[slot(s0, r) * CG + cc] = v;
+                if (cc < ncol) X[(size_t)pc[k] * nE + e0 + cc] = v;
+            }
         }
-        __syncthreads();
+        lds_barrier();
     }
 }
 
-size_t factor_lds(int b) { return ((size_t)(NB + b + 4) * LDP + (size_t)NB * LDP) * sizeof(double); }
-size_t solve_lds(int b, int nE) {
-    return ((size_t)(b + 4) * LDP + (size_t)NB * LDP + 2 * (size_t)NB * (nE + 1)) * sizeof(double);
+size_t factor_lds(int b) { return ((size_t)NB * LDP + (size_t)(NB + b) * LDP) * sizeof(double); }
+size_t solve_lds(int b) {
+    return ((size_t)(NB + b) * CG + (size_t)b * LDP + (size_t)NB * LDP + 5 * (size_t)NB * CG) * sizeof(double);
 }
 
 // bandwidth of an ordering (iperm: node -> position)
@@ -416,15 +524,13 @@ void solve_band(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std::
     const dim3 blk(256);
     hipLaunchKernelGGL(k_band_scatter, dim3(std::max(1, std::min(div_up(maxnnz, 256), 4096 / nd + 1)), nd), blk, 0, s,
                        dd.p);
-    const size_t fl = factor_lds(bmax), sl = solve_lds(bmax, emax);
+    const size_t fl = factor_lds(bmax), sl = solve_lds(bmax);
     PG_REQUIRE(fl <= 160 * 1024 && sl <= 160 * 1024, "band solver: band too wide for LDS");
-    PG_CHECK(hipFuncSetAttribute((const void *)k_band_factor, hipFuncAttributeMaxDynamicSharedMemorySize, (int)fl));
+    auto factor = NB + bmax <= 256 ? k_band_factor<1> : k_band_factor<2>;
+    PG_CHECK(hipFuncSetAttribute((const void *)factor, hipFuncAttributeMaxDynamicSharedMemorySize, (int)fl));
     PG_CHECK(hipFuncSetAttribute((const void *)k_band_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sl));
-    hipLaunchKernelGGL(k_band_factor, dim3(nd), blk, fl, s, dd.p);
-    const int ge = std::max(1, std::min(div_up((long)maxn * emax, 256), 65536 / nd + 1));
-    hipLaunchKernelGGL(k_band_gather, dim3(ge, nd), blk, 0, s, dd.p);
-    hipLaunchKernelGGL(k_band_solve, dim3(nd), blk, sl, s, dd.p);
-    hipLaunchKernelGGL(k_band_scatter_x, dim3(ge, nd), blk, 0, s, dd.p);
+    hipLaunchKernelGGL(factor, dim3(nd), blk, fl, s, dd.p);
+    hipLaunchKernelGGL(k_band_solve, dim3(div_up(emax, CG), nd), blk, sl, s, dd.p);
     PG_CHECK(hipGetLastError());
     PG_CHECK(hipStreamSynchronize(s));  // work buffers are released on return
     stats.assign(cs.size(), pgert_solve_stats{});

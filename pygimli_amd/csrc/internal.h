@@ -142,6 +142,10 @@ struct Ctx {
 
     std::shared_ptr<MlpState> mlp;  // learned warm start (mlp.hip)
 
+    // Jacobian launch descriptors (jacobian.hip), kept until the next launch
+    std::vector<unsigned char> jdescHost;
+    DBuf<unsigned char> jdesc;
+
     // band solver (band.hip): ordering and CSR slot -> band position, per mesh
     bool bandValid = false;
     int bandB = 0;
@@ -208,6 +212,10 @@ void launch_total_potential(Ctx & c);
 void launch_response(Ctx & c, double * rhoa);
 void launch_transpose_pots(Ctx & c, double * out);
 void launch_jacobian(Ctx & c, const double * model, int nModel, void * J, int dtype);
+// one Jacobian launch per wavenumber pass over the cells of several problems
+// (same device and dimension; every context uses cs[0]->stream)
+void launch_jacobian_batch(const std::vector<Ctx *> & cs, const std::vector<const double *> & models,
+                           const std::vector<int> & nModels, const std::vector<void *> & Js, int dtype);
 
 // solvers (solver.hip); return per-solve statistics
 void solve_pcg_fp64(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st);
@@ -235,5 +243,23 @@ float mlp_train_step(Ctx & c, const float * feat, const float * target, int nsam
                      float * grad_host);
 
 inline int div_up(long a, long b) { return (int)((a + b - 1) / b); }
+
+// Global-address-space view of a device pointer.  Pointers read from a
+// descriptor in memory are generic (flat) to the compiler, and a flat access
+// may alias LDS, so it is ordered against every LDS access around it (a wait
+// after each load); through this view the loads and stores are global ones
+// that overlap freely with LDS work.
+template <typename T> __device__ inline __attribute__((address_space(1))) T * glob(T * p) {
+    return (__attribute__((address_space(1))) T *)p;
+}
+
+// Workgroup barrier that orders LDS only: __syncthreads also waits for every
+// outstanding global load and store of the wavefront (vmcnt(0)), which
+// exposes the memory latency of in-flight prefetches and stores.
+__device__ inline void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 
 }  // namespace pgert

@@ -1,116 +1,284 @@
 // Sensitivity (Jacobian) kernel: createSensitivityCol / calc2
 // (core/src/bert/bertJacobian.cpp:170-219, :367-521) re-formulated per cell.
 //
-// For P1 elements grad(u) is constant on a cell, so the reference's per-datum
-// quadratic form (a - b)^T (K_c + k^2 M_c) (m - n) becomes
-//   vol * (g_a - g_b) . (g_m - g_n)
-//   + k^2 vol/((d+1)(d+2)) * (sum_v da_v dm_v + sum_v da_v * sum_v dm_v)
-// with g_e the gradient of electrode e's potential on the cell and da_v the
-// nodal potential differences.  A workgroup owns a tile of 64 cells (sorted
-// by marker, so a tile writes contiguous Jacobian columns); it first stages
-// every electrode's gradient / nodal values of its cells in LDS, then sweeps
-// the data: each lane computes one (datum, cell) entry, so a wavefront
-// writes 64 consecutive floats of a Jacobian row.
+// The reference's entry for datum i = (a, b, m, n) and cell c is
+//   sum_k w_k (u_a,k - u_b,k)_c^T (K_c + k^2 M_c) (u_m,k - u_n,k)_c
+// over the cell's nodal potentials.  For P1 elements it is evaluated in one of
+// two equivalent feature forms:
+//  - gradient form (every wavenumber 0, i.e. 3D): K_c = vol * B B^T, so the
+//    entry is vol * w * (g_a - g_b) . (g_m - g_n) with g_e = grad(u_e) on the
+//    cell (DIM features per electrode, computed in FP64);
+//  - nodal form (2.5D): features f_0 = u_0, f_v = u_v - u_0 (v >= 1, FP64
+//    differences before rounding, so the stiffness part keeps its digits when
+//    the potential is nearly constant over the cell) and the entry is
+//    (f_a - f_b)^T Q' (f_m - f_n), Q' = T^T w (K_c + k^2 M_c) T.
+// A workgroup owns a tile of TILE cells (sorted by marker, so a tile writes
+// contiguous Jacobian columns) of one problem: it stages the features of every
+// electrode of the pass's wavenumbers in LDS (layout [k][cell][comp][electrode],
+// odd cell stride: conflict-free both when lanes run over electrodes (staging)
+// and over cells (sweep); electrode slot nE is zero and stands for index -1),
+// then sweeps the data, each lane one cell, UNR data per step.  Wavenumbers that
+// do not fit into one LDS tile are done in passes that accumulate into J.
+// One launch covers the tiles of every problem of a batch (per-problem
+// descriptors, located by binary search on the first tile).
 #include "internal.h"
+
+#include <algorithm>
 
 namespace pgert {
 
 namespace {
 
-constexpr int TILE_MAX = 64;  // cells per workgroup (fewer when the electrodes' LDS tile does not fit)
+constexpr int KPMAX = 4;        // wavenumbers per pass
+constexpr int LDS_BUDGET = 80 * 1024;  // two workgroups per CU
+constexpr int UNR = 4;          // data per sweep step
 
-enum { J_STORE = 0, J_ACCUM = 1, J_ATOMIC = 2 };
+enum { F_GRAD = 0, F_NODAL = 1 };
 
-template <typename OutT>
-__device__ inline void jwrite(OutT * J, size_t idx, double v, int mode, bool first) {
-    if (mode == J_ATOMIC) {
-        atomicAdd(J + idx, (OutT)v);
-    } else if (mode == J_STORE || first) {
-        J[idx] = (OutT)v;
-    } else {
-        J[idx] = (OutT)((double)J[idx] + v);
-    }
-}
+struct SensDesc {
+    int nE, nEp, cs;            // electrodes, padded electrode count (slot nE = zero), cell stride
+    int D, nSorted, nParam;     // data, cells with marker >= 0, Jacobian columns
+    int tile0, nTile;           // first global tile of this problem in the launch, its tile count
+    int atomic, scale;          // shared markers (atomic adds); apply k_i / m_j^2 on the last pass
+    int first, last, nk;        // first / last pass of this problem; wavenumbers in this pass
+    int pad_;
+    const int * sorted;
+    const int * cells;
+    const int * marker;
+    const int * abmn;
+    const double * grads;
+    const double * vol;
+    const double * kfac;
+    const double * model;
+    const double * U[KPMAX];    // node-major potentials [n][nE] of each wavenumber of the pass
+    double k2[KPMAX], wk[KPMAX];
+    void * J;
+};
 
-// One launch per wavenumber.  LDS layout: S[(e * NCOMP + comp) * TILE + cell].
-// Lane l of a wavefront takes cell l % TILE of the tile and every
-// (64 / TILE)-th datum starting at its sub-group l / TILE.
-template <typename OutT, int DIM, int MASS, int TILE>
-__global__ void __launch_bounds__(256) k_sens(int n, int nE, int D, int nTile, int nSorted,
-                                              const int * __restrict__ sorted, const int * __restrict__ cells,
-                                              const int * __restrict__ marker, const double * __restrict__ grads,
-                                              const double * __restrict__ vol, const double * __restrict__ U,
-                                              const int * __restrict__ abmn, double k2, double wk,
-                                              int mode, int first, int last, const double * __restrict__ kfac,
-                                              const double * __restrict__ model, int scaleModel, int nParam,
-                                              OutT * __restrict__ J) {
+template <typename P, typename T> __device__ inline void store_nt(P p, T v) { __builtin_nontemporal_store(v, p); }
+
+template <typename OutT, typename ST, int DIM, int FORM, int TILE>
+__global__ void __launch_bounds__(256) k_sens(const SensDesc * __restrict__ descs, int count) {
     constexpr int NV = DIM + 1;
-    constexpr int NCOMP = DIM + (MASS ? NV : 0);  // gradient comps (+ nodal values)
-    extern __shared__ double S[];
-    const int tile = blockIdx.x;
-    if (tile >= nTile) return;
+    constexpr int NCOMP = FORM == F_GRAD ? DIM : NV;
+    constexpr int NSUB = 64 / TILE;
+    constexpr int NQ = FORM == F_GRAD ? 1 : NV * (NV + 1) / 2;
+    using CT = ST;  // arithmetic of the sweep
+    extern __shared__ unsigned char smem[];
+    __shared__ int scell[TILE];
+    __shared__ int snode[TILE * NV];
+    __shared__ double sgrad[FORM == F_GRAD ? TILE * NV * DIM : 1];
+
+    const int g = blockIdx.x;
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (descs[mid].tile0 <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    const SensDesc & d = descs[lo];
+    const int tile = g - d.tile0;
+    if (tile < 0 || tile >= d.nTile) return;
     const int base = tile * TILE;
-    const int ncell = min(TILE, nSorted - base);
-    // ---- stage gradients and nodal values of all electrodes for this tile
-    for (int idx = threadIdx.x; idx < TILE * nE; idx += blockDim.x) {
-        int t = idx / nE, e = idx % nE;
-        double g[DIM] = {};
-        double uv[NV] = {};
-        if (t < ncell) {
-            int c = sorted[base + t];
-            for (int v = 0; v < NV; ++v) {
-                int node = cells[(size_t)c * NV + v];
-                double u = U[(size_t)node * nE + e];
-                uv[v] = u;
-                for (int q = 0; q < DIM; ++q) g[q] += u * grads[((size_t)c * NV + v) * 3 + q];
-            }
-        }
-        for (int q = 0; q < DIM; ++q) S[((size_t)e * NCOMP + q) * TILE + t] = g[q];
-        if (MASS)
-            for (int v = 0; v < NV; ++v) S[((size_t)e * NCOMP + DIM + v) * TILE + t] = uv[v];
+    const int ncell = min(TILE, d.nSorted - base);
+    const int nE = d.nE, nEp = d.nEp, CS = d.cs, nk = d.nk;
+    ST * S = (ST *)smem;
+
+    for (int t = threadIdx.x; t < TILE; t += blockDim.x) {
+        const int c = t < ncell ? glob(d.sorted)[base + t] : -1;
+        scell[t] = c;
+        for (int v = 0; v < NV; ++v) snode[t * NV + v] = c >= 0 ? glob(d.cells)[(size_t)c * NV + v] : 0;
+        if (FORM == F_GRAD)
+            for (int v = 0; v < NV; ++v)
+                for (int q = 0; q < DIM; ++q)
+                    sgrad[(t * NV + v) * DIM + q] = c >= 0 ? glob(d.grads)[((size_t)c * NV + v) * 3 + q] : 0.0;
+    }
+    // zero slots: electrodes nE .. nEp-1 of every (wavenumber, cell, component)
+    for (int idx = threadIdx.x; idx < nk * TILE * NCOMP; idx += blockDim.x) {
+        const int row = idx;  // (kp * TILE + t) * NCOMP + q
+        const int kp = row / (TILE * NCOMP), r = row % (TILE * NCOMP);
+        const int t = r / NCOMP, q = r % NCOMP;
+        ST * dst = S + (size_t)(kp * TILE + t) * CS + q * nEp;
+        for (int e = nE; e < nEp; ++e) dst[e] = ST(0);
     }
     __syncthreads();
-    constexpr int NSUB = 64 / TILE;
-    const int lane = threadIdx.x % TILE, wave = (threadIdx.x / 64) * NSUB + (threadIdx.x % 64) / TILE;
-    const int nwork = (blockDim.x / 64) * NSUB;
-    if (lane >= ncell) return;
-    const int c = sorted[base + lane];
-    const double V = vol[c];
-    const double mfac = MASS ? k2 * V / (NV * (NV + 1)) : 0.0;
-    const int col = marker[c];
-    const double m2 = scaleModel ? model[col] * model[col] : 1.0;
-    for (int i = wave; i < D; i += nwork) {
-        const int a = abmn[4 * i], b = abmn[4 * i + 1], m = abmn[4 * i + 2], nn = abmn[4 * i + 3];
-        double dab[NCOMP], dmn[NCOMP];
-        for (int q = 0; q < NCOMP; ++q) {
-            double va = a >= 0 ? S[((size_t)a * NCOMP + q) * TILE + lane] : 0.0;
-            double vb = b >= 0 ? S[((size_t)b * NCOMP + q) * TILE + lane] : 0.0;
-            double vm = m >= 0 ? S[((size_t)m * NCOMP + q) * TILE + lane] : 0.0;
-            double vn = nn >= 0 ? S[((size_t)nn * NCOMP + q) * TILE + lane] : 0.0;
-            dab[q] = va - vb;
-            dmn[q] = vm - vn;
-        }
-        double val = 0.0;
-        for (int q = 0; q < DIM; ++q) val += dab[q] * dmn[q];
-        val *= V;
-        if (MASS) {
-            double s2 = 0, sa = 0, sm = 0;
-            for (int v = 0; v < NV; ++v) {
-                s2 += dab[DIM + v] * dmn[DIM + v];
-                sa += dab[DIM + v];
-                sm += dmn[DIM + v];
+
+    // ---- stage features: lanes run over (cell, electrode) pairs, electrode fastest
+    const float rinv = 1.0f / (float)nE;
+    for (int kp = 0; kp < nk; ++kp) {
+        const auto Uk = glob(d.U[kp]);
+#pragma unroll 4
+        for (int idx = threadIdx.x; idx < TILE * nE; idx += blockDim.x) {
+            const int t = (int)(((float)idx + 0.5f) * rinv);  // exact: idx < 2^15, frac >= 0.5/nE
+            const int e = idx - t * nE;
+            const int c = scell[t];
+            double u[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) u[v] = Uk[(size_t)snode[t * NV + v] * nE + e];  // node 0 for empty cells
+            ST * dst = S + (size_t)(kp * TILE + t) * CS + e;
+            if (FORM == F_GRAD) {
+#pragma unroll
+                for (int q = 0; q < DIM; ++q) {
+                    double gq = 0.0;
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) gq += u[v] * sgrad[(t * NV + v) * DIM + q];
+                    dst[q * nEp] = (ST)gq;
+                }
+            } else {
+                dst[0] = (ST)u[0];
+#pragma unroll
+                for (int v = 1; v < NV; ++v) dst[v * nEp] = (ST)(u[v] - u[0]);
             }
-            val += mfac * (s2 + sa * sm);
         }
-        val *= wk;
-        // createJacobian_ scaling J_ij *= k_i / m_j^2 (dcfemmodelling.cpp:1375-1384)
-        if (last && scaleModel && mode != J_ATOMIC) {
-            // the last wavenumber finishes the entry, so the scale applies to the sum
-            size_t idx = (size_t)i * nParam + col;
-            double prev = (first || mode == J_STORE) ? 0.0 : (double)J[idx];
-            J[idx] = (OutT)((prev + val) * kfac[i] / m2);
+    }
+    __syncthreads();
+
+    // ---- sweep the data
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int t = lane % TILE, sub = lane / TILE;
+    const int c = scell[t];
+    if (c < 0) return;  // no barrier below
+    const double V = glob(d.vol)[c];
+    const int col = glob(d.marker)[c];
+    const double m2 = d.scale ? glob(d.model)[col] * glob(d.model)[col] : 1.0;
+
+    // per-wavenumber coefficients of the quadratic form
+    CT coef[KPMAX][NQ];
+    if (FORM == F_GRAD) {
+#pragma unroll
+        for (int kp = 0; kp < KPMAX; ++kp) coef[kp][0] = (CT)(kp < nk ? V * d.wk[kp] : 0.0);
+    } else {
+        double Sm[NV][NV];  // grad N_v . grad N_w
+        double gr[NV][3];
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) gr[v][q] = glob(d.grads)[((size_t)c * NV + v) * 3 + q];
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+#pragma unroll
+            for (int w = 0; w < NV; ++w) Sm[v][w] = gr[v][0] * gr[w][0] + gr[v][1] * gr[w][1] + gr[v][2] * gr[w][2];
+#pragma unroll
+        for (int kp = 0; kp < KPMAX; ++kp) {
+            const double wk = kp < nk ? d.wk[kp] : 0.0, k2 = kp < nk ? d.k2[kp] : 0.0;
+            const double mf = k2 * V / (NV * (NV + 1));
+            double Q[NV][NV];  // w (K_c + k^2 M_c); M_c = vol / ((d+1)(d+2)) (1 + delta_vw)
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+#pragma unroll
+                for (int w = 0; w < NV; ++w) Q[v][w] = wk * (V * Sm[v][w] + mf * (v == w ? 2.0 : 1.0));
+            // Q' = T^T Q T with u_0 = f_0, u_v = f_0 + f_v
+            int qi = 0;
+#pragma unroll
+            for (int j = 0; j < NV; ++j)
+#pragma unroll
+                for (int l = j; l < NV; ++l) {
+                    double s = 0.0;
+                    if (j == 0 && l == 0) {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v)
+#pragma unroll
+                            for (int w = 0; w < NV; ++w) s += Q[v][w];
+                    } else if (j == 0) {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) s += Q[v][l];
+                    } else {
+                        s = Q[j][l];
+                    }
+                    coef[kp][qi++] = (CT)s;
+                }
+        }
+    }
+
+    const int nwork = (blockDim.x >> 6) * NSUB;
+    const int D = d.D;
+    typedef int iv4 __attribute__((ext_vector_type(4)));
+    const auto ab4 = glob((const iv4 *)d.abmn);
+    OutT * Jf = (OutT *)d.J;  // flat, for the atomics
+    const auto J = glob(Jf);
+    const ST * Sc = S + (size_t)t * CS;
+    for (int i0 = (wave * NSUB + sub) * UNR; i0 < D; i0 += nwork * UNR) {
+        int ia[UNR], ib[UNR], im[UNR], in[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const iv4 q = ab4[min(i0 + u, D - 1)];
+            ia[u] = q.x >= 0 ? q.x : nE;
+            ib[u] = q.y >= 0 ? q.y : nE;
+            im[u] = q.z >= 0 ? q.z : nE;
+            in[u] = q.w >= 0 ? q.w : nE;
+        }
+        CT val[UNR];
+        OutT prev[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) val[u] = CT(0);
+#pragma unroll
+        for (int kp = 0; kp < KPMAX; ++kp) {
+            if (kp >= nk) break;
+            const ST * Sk = Sc + (size_t)kp * TILE * CS;
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) {
+                CT dv[NCOMP], mv[NCOMP];
+#pragma unroll
+                for (int q = 0; q < NCOMP; ++q) {
+                    const ST * r = Sk + q * nEp;
+                    dv[q] = (CT)r[ia[u]] - (CT)r[ib[u]];
+                    mv[q] = (CT)r[im[u]] - (CT)r[in[u]];
+                }
+                if (FORM == F_GRAD) {
+                    CT s = CT(0);
+#pragma unroll
+                    for (int q = 0; q < DIM; ++q) s += dv[q] * mv[q];
+                    val[u] += coef[kp][0] * s;
+                } else {
+                    int qi = 0;
+                    CT y[NV];
+#pragma unroll
+                    for (int j = 0; j < NV; ++j) y[j] = CT(0);
+#pragma unroll
+                    for (int j = 0; j < NV; ++j)
+#pragma unroll
+                        for (int l = j; l < NV; ++l) {
+                            const CT qq = coef[kp][qi++];
+                            y[j] += qq * mv[l];
+                            if (l != j) y[l] += qq * mv[j];
+                        }
+                    CT s = CT(0);
+#pragma unroll
+                    for (int j = 0; j < NV; ++j) s += dv[j] * y[j];
+                    val[u] += s;
+                }
+            }
+        }
+        // earlier passes' partial sums and the data's geometric factors,
+        // loaded together from clamped rows
+        double fac[UNR];
+        if (!d.first && !d.atomic) {
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) prev[u] = J[(size_t)min(i0 + u, D - 1) * d.nParam + col];
         } else {
-            jwrite(J, (size_t)i * nParam + col, val, mode, first);
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) prev[u] = OutT(0);
+        }
+        if (d.last && d.scale) {
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) fac[u] = glob(d.kfac)[min(i0 + u, D - 1)] / m2;
+        } else {
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) fac[u] = 1.0;
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const int i = i0 + u;
+            if (i >= D) break;
+            const size_t idx = (size_t)i * d.nParam + col;
+            if (d.atomic) {
+                atomicAdd(Jf + idx, (OutT)val[u]);
+            } else {
+                // createJacobian_ scaling J_ij *= k_i / m_j^2 (dcfemmodelling.cpp:1375-1384)
+                const double r = ((double)val[u] + (double)prev[u]) * fac[u];
+                if (d.last) store_nt(J + idx, (OutT)r);
+                else J[idx] = (OutT)r;
+            }
         }
     }
 }
@@ -124,52 +292,168 @@ __global__ void k_scale_cols(long total, int nParam, const double * __restrict__
     J[t] = (OutT)((double)J[t] * kfac[i] / (model[j] * model[j]));
 }
 
-template <typename OutT> void run(Ctx & c, const double * model, int nModel, OutT * J) {
-    const bool scale = (nModel == c.nParam);
-    int mode;
-    if (!c.uniqueMarkers) mode = J_ATOMIC;
-    else mode = (c.nK == 1) ? J_STORE : J_ACCUM;
-    const size_t jsize = (size_t)c.D * c.nParam;
-    // columns without any cell stay zero (markers not covering 0..max)
-    if (mode == J_ATOMIC || c.nSorted < c.nParam)
-        PG_CHECK(hipMemsetAsync(J, 0, jsize * sizeof(OutT), c.stream));
-    for (int kk = 0; kk < c.nK; ++kk) {
-        double k = c.kv[kk];
-        const bool mass = k > 0.0;
-        const int ncomp = c.dim + (mass ? c.nv : 0);
-        int tile = TILE_MAX;
-        while (tile > 16 && (size_t)c.nE * ncomp * tile * sizeof(double) > 160 * 1024) tile /= 2;
-        size_t lds = (size_t)c.nE * ncomp * tile * sizeof(double);
-        PG_REQUIRE(lds <= 160 * 1024, "pgert_jacobian: too many electrodes for one LDS tile");
-        const int nTile = div_up(c.nSorted, tile);
-        const double * Uk = c.U.p + (size_t)kk * c.n * c.nE;
-        int first = kk == 0, last = kk == c.nK - 1;
-        auto pick = [&](auto tl) {
-            constexpr int T = decltype(tl)::value;
-            return c.dim == 2 ? (mass ? k_sens<OutT, 2, 1, T> : k_sens<OutT, 2, 0, T>)
-                              : (mass ? k_sens<OutT, 3, 1, T> : k_sens<OutT, 3, 0, T>);
-        };
-        auto kern = tile == 64 ? pick(std::integral_constant<int, 64>{})
-                  : tile == 32 ? pick(std::integral_constant<int, 32>{}) : pick(std::integral_constant<int, 16>{});
-        PG_CHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(kern, dim3(nTile), dim3(256), lds, c.stream, c.n, c.nE, c.D, nTile, c.nSorted,
-                           c.sortedCells.p, c.cells.p, c.cellMarker.p, c.grads.p, c.vol.p, Uk, c.abmn.p, k * k,
-                           c.wv[kk], mode, first, last, c.kfac.p, model, scale ? 1 : 0, c.nParam, J);
-        PG_CHECK(hipGetLastError());
+struct Plan {
+    int tile = 64, kp = 1, form = F_NODAL;
+    size_t lds = 0;
+};
+
+int odd_stride(int ncomp, int nEp) {
+    int cs = ncomp * nEp;
+    return (cs & 1) ? cs : cs + 1;
+}
+
+// Tile width and wavenumbers per pass for the largest electrode count of a
+// launch: fewest passes (each pass after the first is a read-modify-write
+// sweep of J), then a tile within LDS_BUDGET (two workgroups per CU), then
+// the widest tile (longer row segments per store).
+Plan make_plan(int dim, int nEmax, int nK, bool allZeroK, size_t st) {
+    Plan p;
+    p.form = allZeroK ? F_GRAD : F_NODAL;
+    const int ncomp = p.form == F_GRAD ? dim : dim + 1;
+    const int cs = odd_stride(ncomp, nEmax + 1);
+    long best = -1;
+    for (int tile : {64, 32, 16}) {
+        const size_t perk = (size_t)tile * cs * st;
+        int kp = (int)std::min<size_t>(std::min(KPMAX, nK), LDS_BUDGET / perk);
+        int over = 0;
+        if (kp < 1) {
+            if (perk > 150 * 1024) continue;
+            kp = 1;   // one wavenumber beyond the budget: one workgroup per CU
+            over = 1;
+        }
+        const long passes = (nK + kp - 1) / kp;
+        const long score = passes * 4 + over * 2;  // ties: the first (widest) tile
+        if (best < 0 || score < best) {
+            best = score;
+            p.tile = tile;
+            p.kp = kp;
+            p.lds = (size_t)kp * perk;
+        }
     }
-    if (mode == J_ATOMIC && scale) {
-        long total = (long)jsize;
-        hipLaunchKernelGGL(k_scale_cols<OutT>, dim3(div_up(total, 256)), dim3(256), 0, c.stream, total, c.nParam,
-                           c.kfac.p, model, J);
+    PG_REQUIRE(best >= 0, "pgert_jacobian: too many electrodes for one LDS tile");
+    return p;
+}
+
+template <typename OutT, typename ST, int DIM, int FORM>
+void launch_k(const Plan & p, const SensDesc * descs, int count, int grid, hipStream_t s) {
+    auto go = [&](auto kern) {
+        PG_CHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds));
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(256), p.lds, s, descs, count);
         PG_CHECK(hipGetLastError());
+    };
+    if (p.tile == 64) go(k_sens<OutT, ST, DIM, FORM, 64>);
+    else if (p.tile == 32) go(k_sens<OutT, ST, DIM, FORM, 32>);
+    else go(k_sens<OutT, ST, DIM, FORM, 16>);
+}
+
+template <typename OutT>
+void run(const std::vector<Ctx *> & cs, const std::vector<const double *> & models, const std::vector<int> & nModels,
+         const std::vector<void *> & Js) {
+    Ctx & c0 = *cs[0];
+    hipStream_t s = c0.stream;
+    const int count = (int)cs.size();
+    int nEmax = 0, nKmax = 0;
+    bool allZeroK = true;
+    for (Ctx * c : cs) {
+        PG_REQUIRE(c->dim == c0.dim, "pgert_batch_jacobian: problems of different dimension");
+        nEmax = std::max(nEmax, c->nE);
+        nKmax = std::max(nKmax, c->nK);
+        for (double k : c->kv) allZeroK = allZeroK && k == 0.0;
+    }
+    const Plan p = make_plan(c0.dim, nEmax, nKmax, allZeroK, sizeof(OutT));
+    const int npass = (nKmax + p.kp - 1) / p.kp;
+    const int ncomp = p.form == F_GRAD ? c0.dim : c0.dim + 1;
+
+    std::vector<char> atomicP(count), scaleP(count);
+    for (int j = 0; j < count; ++j) {
+        Ctx & c = *cs[j];
+        atomicP[j] = !c.uniqueMarkers;
+        scaleP[j] = nModels[j] == c.nParam;
+        const size_t jsize = (size_t)c.D * c.nParam;
+        // columns without any cell stay zero (markers not covering 0..max)
+        if (atomicP[j] || c.nSorted < c.nParam) PG_CHECK(hipMemsetAsync(Js[j], 0, jsize * sizeof(OutT), s));
+    }
+    // descriptors of every pass, uploaded once
+    std::vector<SensDesc> hd;
+    std::vector<int> passStart(npass + 1, 0), passGrid(npass, 0);
+    for (int ps = 0; ps < npass; ++ps) {
+        passStart[ps] = (int)hd.size();
+        int tile0 = 0;
+        for (int j = 0; j < count; ++j) {
+            Ctx & c = *cs[j];
+            const int k0 = ps * p.kp, k1 = std::min(c.nK, k0 + p.kp);
+            if (k0 >= k1 || c.nSorted == 0 || c.D == 0) continue;
+            SensDesc sd{};
+            sd.nE = c.nE;
+            sd.nEp = c.nE + 1;
+            sd.cs = odd_stride(ncomp, sd.nEp);
+            sd.D = c.D;
+            sd.nSorted = c.nSorted;
+            sd.nParam = c.nParam;
+            sd.tile0 = tile0;
+            sd.nTile = div_up(c.nSorted, p.tile);
+            sd.atomic = atomicP[j];
+            sd.scale = scaleP[j];
+            sd.first = k0 == 0;
+            sd.last = k1 == c.nK;
+            sd.nk = k1 - k0;
+            sd.sorted = c.sortedCells.p;
+            sd.cells = c.cells.p;
+            sd.marker = c.cellMarker.p;
+            sd.abmn = c.abmn.p;
+            sd.grads = c.grads.p;
+            sd.vol = c.vol.p;
+            sd.kfac = c.kfac.p;
+            sd.model = models[j];
+            for (int kk = k0; kk < k1; ++kk) {
+                sd.U[kk - k0] = c.U.p + (size_t)kk * c.n * c.nE;
+                sd.k2[kk - k0] = c.kv[kk] * c.kv[kk];
+                sd.wk[kk - k0] = c.wv[kk];
+            }
+            sd.J = Js[j];
+            tile0 += sd.nTile;
+            hd.push_back(sd);
+        }
+        passGrid[ps] = tile0;
+    }
+    passStart[npass] = (int)hd.size();
+    if (hd.empty()) return;
+    c0.jdescHost.assign((const unsigned char *)hd.data(), (const unsigned char *)(hd.data() + hd.size()));
+    c0.jdesc.upload(c0.jdescHost, s);
+    const SensDesc * dd = (const SensDesc *)c0.jdesc.p;
+    for (int ps = 0; ps < npass; ++ps) {
+        const int n = passStart[ps + 1] - passStart[ps];
+        if (n == 0 || passGrid[ps] == 0) continue;
+        const SensDesc * dp = dd + passStart[ps];
+        if (c0.dim == 2) {
+            if (p.form == F_GRAD) launch_k<OutT, OutT, 2, F_GRAD>(p, dp, n, passGrid[ps], s);
+            else launch_k<OutT, OutT, 2, F_NODAL>(p, dp, n, passGrid[ps], s);
+        } else {
+            if (p.form == F_GRAD) launch_k<OutT, OutT, 3, F_GRAD>(p, dp, n, passGrid[ps], s);
+            else launch_k<OutT, OutT, 3, F_NODAL>(p, dp, n, passGrid[ps], s);
+        }
+    }
+    for (int j = 0; j < count; ++j) {
+        Ctx & c = *cs[j];
+        if (atomicP[j] && scaleP[j]) {
+            const long total = (long)c.D * c.nParam;
+            hipLaunchKernelGGL(k_scale_cols<OutT>, dim3(div_up(total, 256)), dim3(256), 0, s, total, c.nParam,
+                               c.kfac.p, models[j], (OutT *)Js[j]);
+            PG_CHECK(hipGetLastError());
+        }
     }
 }
 
 }  // namespace
 
+void launch_jacobian_batch(const std::vector<Ctx *> & cs, const std::vector<const double *> & models,
+                           const std::vector<int> & nModels, const std::vector<void *> & Js, int dtype) {
+    if (dtype == 0) run<double>(cs, models, nModels, Js);
+    else run<float>(cs, models, nModels, Js);
+}
+
 void launch_jacobian(Ctx & c, const double * model, int nModel, void * J, int dtype) {
-    if (dtype == 0) run<double>(c, model, nModel, (double *)J);
-    else run<float>(c, model, nModel, (float *)J);
+    launch_jacobian_batch({&c}, {model}, {nModel}, {J}, dtype);
 }
 
 }  // namespace pgert

@@ -73,29 +73,55 @@ __global__ void k_rho_source(int nE, const int * __restrict__ ptr, const int * _
 
 // b = S1 p - rho_s A p for every (k, e) column (dcfemmodelling.cpp:2253-2254
 // with prim = rho_s p): one wavefront per (row, k), lanes over electrodes.
+// The row's column indices and coefficients are read once, one per lane, and
+// broadcast with readlane, so the gathers of a row are independent loads.
+__device__ inline double readlane_d(double v, int j) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), j);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), j);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 __global__ void __launch_bounds__(256) k_sr_rhs(int n, int nE, int nnz, const int * __restrict__ rowptr,
                                                 const int * __restrict__ colidx,
                                                 const double * __restrict__ valsS1,
                                                 const double * __restrict__ valsA,
                                                 const double * __restrict__ rhoS,
                                                 const double * __restrict__ prim, double * __restrict__ b) {
-    int row = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
-    int lane = threadIdx.x % 64;
-    int kk = blockIdx.y;
+    const int row = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    const int lane = threadIdx.x % 64;
+    const int kk = blockIdx.y;
     if (row >= n) return;
     const double * s1 = valsS1 + (size_t)kk * nnz;
     const double * a = valsA + (size_t)kk * nnz;
     const double * P = prim + (size_t)kk * n * nE;
-    for (int e0 = 0; e0 < nE; e0 += 64) {
-        int e = e0 + lane;
-        if (e >= nE) break;
-        double rs = rhoS[e];
-        double acc = 0.0;
-        for (int q = rowptr[row]; q < rowptr[row + 1]; ++q) {
-            double coef = s1[q] - rs * a[q];
-            acc += coef * P[(size_t)colidx[q] * nE + e];
+    double * out = b + ((size_t)kk * n + row) * nE;
+    const int q0 = rowptr[row], q1 = rowptr[row + 1];
+    for (int base = q0; base < q1; base += 64) {
+        const int m = min(64, q1 - base);
+        int ci = 0;
+        double cs = 0.0, ca = 0.0;
+        if (lane < m) {
+            ci = colidx[base + lane];
+            cs = s1[base + lane];
+            ca = a[base + lane];
         }
-        b[((size_t)kk * n + row) * nE + e] = acc;
+        for (int e0 = 0; e0 < nE; e0 += 64) {
+            const int e = e0 + lane;
+            const bool ok = e < nE;
+            const double rs = ok ? rhoS[e] : 0.0;
+            double acc = 0.0;
+            // groups of 8 entries (lanes >= m hold column 0 with coefficient 0)
+            for (int j0 = 0; j0 < m; j0 += 8) {
+#pragma unroll
+                for (int jj = 0; jj < 8; ++jj) {
+                    const int col = __builtin_amdgcn_readlane(ci, j0 + jj);
+                    const double coef = readlane_d(cs, j0 + jj) - rs * readlane_d(ca, j0 + jj);
+                    if (ok) acc += coef * P[(size_t)col * nE + e];
+                }
+            }
+            if (ok) out[e] = base == q0 ? acc : out[e] + acc;
+        }
     }
 }
 

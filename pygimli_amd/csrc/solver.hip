@@ -190,6 +190,15 @@ __global__ void k_pcg_reduce(int nK, int nE, int G, int stage, double tol2, cons
 }
 
 // partials of ||B - A X||^2 and ||B||^2 per column
+__device__ inline double readlane_d(double v, int j) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), j);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), j);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// partial ||b - A x||^2 and ||b||^2 per column; a wavefront per row, the
+// row's entries broadcast from lane registers (independent gathers)
 __global__ void __launch_bounds__(256) k_resid(int n, int nE, int nnz, const int * __restrict__ rowptr,
                                                const int * __restrict__ colidx, const double * __restrict__ vals,
                                                const double * __restrict__ X, const double * __restrict__ B,
@@ -205,12 +214,26 @@ __global__ void __launch_bounds__(256) k_resid(int n, int nE, int nnz, const int
     for (int row = blockIdx.x * WAVES + wave; row < n; row += gridDim.x * WAVES) {
         const int q0 = rowptr[row], q1 = rowptr[row + 1];
         for (int ch = 0; ch < nch; ++ch) {
-            int e = ch * 64 + lane;
-            if (e < nE) {
-                double y = 0.0;
-                for (int q = q0; q < q1; ++q) y += vals[q] * X[(size_t)colidx[q] * nE + e];
-                double b = B[(size_t)row * nE + e];
-                double r = b - y;
+            const int e = ch * 64 + lane;
+            const bool ok = e < nE;
+            double y = 0.0;
+            for (int base = q0; base < q1; base += 64) {
+                const int m = min(64, q1 - base);
+                const int ci = lane < m ? colidx[base + lane] : 0;
+                const double cv = lane < m ? vals[base + lane] : 0.0;
+                // groups of 8 entries (lanes >= m hold column 0 with value 0)
+                for (int j0 = 0; j0 < m; j0 += 8) {
+#pragma unroll
+                    for (int jj = 0; jj < 8; ++jj) {
+                        const int col = __builtin_amdgcn_readlane(ci, j0 + jj);
+                        const double v = readlane_d(cv, j0 + jj);
+                        if (ok) y += v * X[(size_t)col * nE + e];
+                    }
+                }
+            }
+            if (ok) {
+                const double b = B[(size_t)row * nE + e];
+                const double r = b - y;
                 acc[ch][0] += r * r;
                 acc[ch][1] += b * b;
             }
@@ -219,18 +242,27 @@ __global__ void __launch_bounds__(256) k_resid(int n, int nE, int nnz, const int
     store_partials<2>(acc, nE, part, lds);
 }
 
-__global__ void k_resid_reduce(int nK, int nE, int G, const double * __restrict__ part, double * __restrict__ out) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+// one wavefront per (k, e) column: lanes over the G partials, then a
+// butterfly over the wavefront (fixed order, deterministic)
+__global__ void __launch_bounds__(64) k_resid_reduce(int nK, int nE, int G, const double * __restrict__ part,
+                                                     double * __restrict__ out) {
+    const int t = blockIdx.x, lane = threadIdx.x;
     if (t >= nK * nE) return;
-    int kk = t / nE, e = t % nE;
+    const int kk = t / nE, e = t % nE;
     double rr = 0, bb = 0;
-    for (int g = 0; g < G; ++g) {
+    for (int g = lane; g < G; g += 64) {
         const double * p = part + ((size_t)kk * G + g) * nE * 2 + (size_t)e * 2;
         rr += p[0];
         bb += p[1];
     }
-    out[2 * t] = rr;
-    out[2 * t + 1] = bb;
+    for (int off = 32; off > 0; off >>= 1) {
+        rr += __shfl_xor(rr, off);
+        bb += __shfl_xor(bb, off);
+    }
+    if (lane == 0) {
+        out[2 * t] = rr;
+        out[2 * t + 1] = bb;
+    }
 }
 
 }  // namespace
@@ -240,11 +272,11 @@ double residual_check(Ctx & c) {
     const int nK = c.nK, nE = c.nE;
     c.part.alloc((size_t)nK * NGROUPS * nE * 3);
     c.scal.alloc((size_t)nK * nE * NS);
-    hipLaunchKernelGGL(k_resid, dim3(NGROUPS, nK), dim3(256), 0, c.stream, c.n, nE, c.nnz, c.rowptr.p,
+    const int G = std::max(1, std::min(NGROUPS, div_up(c.n, 16 * WAVES)));  // >= 16 rows per wavefront
+    hipLaunchKernelGGL(k_resid, dim3(G, nK), dim3(256), 0, c.stream, c.n, nE, c.nnz, c.rowptr.p,
                        c.colidx.p, c.valsA.p, c.X.p, c.B.p, c.part.p);
     PG_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_resid_reduce, dim3(div_up((long)nK * nE, 64)), dim3(64), 0, c.stream, nK, nE, NGROUPS,
-                       c.part.p, c.scal.p);
+    hipLaunchKernelGGL(k_resid_reduce, dim3(nK * nE), dim3(64), 0, c.stream, nK, nE, G, c.part.p, c.scal.p);
     PG_CHECK(hipGetLastError());
     std::vector<double> h((size_t)nK * nE * 2);
     PG_CHECK(hipMemcpyAsync(h.data(), c.scal.p, h.size() * 8, hipMemcpyDeviceToHost, c.stream));

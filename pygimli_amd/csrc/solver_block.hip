@@ -106,12 +106,6 @@ template <typename VT> __device__ inline int drow(int lane, int r) {
 // Workgroup barrier ordering LDS only.  __syncthreads() also orders global
 // memory, which makes every wavefront wait for all of its outstanding loads
 // (vmcnt(0)) and would drain a software-pipelined load ring at each step.
-__device__ inline void lds_barrier() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
 template <typename VT> __device__ inline V4<VT> ld4(const VT * p) { return *reinterpret_cast<const V4<VT> *>(p); }
 template <typename VT> __device__ inline void st4(VT * p, V4<VT> v) { *reinterpret_cast<V4<VT> *>(p) = v; }
 

@@ -86,3 +86,35 @@ def test_batch_band_direct():
     ref = batch_response(fops, [p[2] for p in probs])
     for r1, r2 in zip(batched, ref):
         np.testing.assert_allclose(r1, r2, rtol=1e-8)
+
+
+def test_batch_jacobian():
+    """One Jacobian launch over several problems (pgert_batch_jacobian)
+    against each problem's own createJacobian, FP64 and FP32, and problem 0
+    against the oracle."""
+    import torch
+    from pygimli_amd.batch import batch_jacobian, batch_response
+    probs = workloads.batched_2d(count=4, seed=11, n_range=(800, 3000), ne_range=(12, 100))
+    print("problems", [(m.nodeCount(), d.size(), len(d.sensorPositions())) for m, d, *_ in probs], flush=True)
+    fops = _fops(probs, 0)
+    models = [p[2] for p in probs]
+    for f in fops:
+        f.solverOptions.update(method="band", rtol=1e-10)
+    batch_response(fops, models)
+    for dt, tol in ((torch.float64, 1e-13), (torch.float32, 1e-6)):
+        for f in fops:
+            f.jacobianDtype = dt
+        Js = [J.cpu().numpy().astype(np.float64) for J in batch_jacobian(fops, models)]
+        for f, m, Jb in zip(fops, models, Js):
+            f.createJacobian(m)
+            Js_ = f.jacobian().cpu().numpy().astype(np.float64)
+            assert np.all(np.isfinite(Jb))
+            err = np.linalg.norm(Jb - Js_) / np.linalg.norm(Js_)
+            assert err <= tol, (dt, err)
+        if dt == torch.float32:
+            from oracle import ert_oracle as eo
+            mesh, data, rho, k, w = probs[0]
+            J_o = eo.Forward(mesh, copy.deepcopy(data), k, w).jacobian(rho)
+            err = np.linalg.norm(Js[0] - J_o) / np.linalg.norm(J_o)
+            print("batched fp32 J vs oracle", err)
+            assert err < 1e-5

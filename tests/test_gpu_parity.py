@@ -82,6 +82,44 @@ def test_forward_and_jacobian(name, method, oracle_results):
     assert jerr < 1e-5
 
 
+@pytest.mark.parametrize("name", CASES)
+def test_jacobian_fp32_and_regions(name, oracle_results):
+    """FP32 Jacobian (FP32 features in LDS) against the oracle, and a region
+    model: 13 parameters shared by many cells (atomic accumulation into
+    J columns, scaling k_i / m_j^2 after the sum), FP64 and FP32."""
+    import torch
+    fo, rhoa_o, J_o = oracle_results[name]
+    fop, mesh, data, rho = _fop(name, "block64")
+    fop.response(rho)
+    fop.jacobianDtype = torch.float32
+    fop.createJacobian(rho)
+    J = fop.jacobian().cpu().numpy().astype(np.float64)
+    err = np.linalg.norm(J - J_o) / np.linalg.norm(J_o)
+    print(name, "fp32 J rel fro", err)
+    assert err < 1e-5
+    # regions
+    mesh = copy.deepcopy(mesh)
+    cm = (np.arange(mesh.cellCount()) // 5) % 13
+    mesh.setCellMarkers(cm)
+    model = 20.0 + 10.0 * np.arange(13)
+    fo2 = eo.Forward(mesh, copy.deepcopy(data))
+    fo2.response(model)
+    J_o2 = fo2.jacobian(model)
+    from pygimli_amd.ertModelling import ERTModelling
+    fop2 = ERTModelling()
+    fop2.setMesh(mesh)
+    fop2.setData(copy.deepcopy(data))
+    fop2.solverOptions.update(method="block64", rtol=1e-11, coarse=0)
+    for dt, tol in ((torch.float64, 1e-9), (torch.float32, 1e-5)):
+        fop2.jacobianDtype = dt
+        fop2.createJacobian(model)
+        J2 = fop2.jacobian().cpu().numpy().astype(np.float64)
+        assert J2.shape == J_o2.shape
+        err = np.linalg.norm(J2 - J_o2) / np.linalg.norm(J_o2)
+        print(name, dt, "regions J rel fro", err)
+        assert err < tol
+
+
 def test_band_direct_small_2d(oracle_results):
     """Direct banded Cholesky: the direct solution agrees with the oracle's
     SuperLU solution to rounding."""
