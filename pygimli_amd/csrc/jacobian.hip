@@ -24,14 +24,21 @@
 #include "internal.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace pgert {
 
 namespace {
 
-constexpr int KPMAX = 4;        // wavenumbers per pass
-constexpr int LDS_BUDGET = 80 * 1024;  // two workgroups per CU
-constexpr int UNR = 4;          // data per sweep step
+#ifndef PGERT_SENS_LDS
+#define PGERT_SENS_LDS (80 * 1024)
+#endif
+#ifndef PGERT_SENS_UNR
+#define PGERT_SENS_UNR 4
+#endif
+constexpr int KPMAX = 4;                   // wavenumbers per pass
+constexpr int LDS_BUDGET = PGERT_SENS_LDS;  // feature tile bytes per workgroup
+constexpr int UNR = PGERT_SENS_UNR;         // data per sweep step
 
 enum { F_GRAD = 0, F_NODAL = 1 };
 
@@ -192,95 +199,112 @@ __global__ void __launch_bounds__(256) k_sens(const SensDesc * __restrict__ desc
 
     const int nwork = (blockDim.x >> 6) * NSUB;
     const int D = d.D;
+    const int step = nwork * UNR;
     typedef int iv4 __attribute__((ext_vector_type(4)));
     const auto ab4 = glob((const iv4 *)d.abmn);
+    const auto kf = glob(d.kfac);
     OutT * Jf = (OutT *)d.J;  // flat, for the atomics
     const auto J = glob(Jf);
+    const size_t ldj = d.nParam;
     const ST * Sc = S + (size_t)t * CS;
-    for (int i0 = (wave * NSUB + sub) * UNR; i0 < D; i0 += nwork * UNR) {
-        int ia[UNR], ib[UNR], im[UNR], in[UNR];
-#pragma unroll
-        for (int u = 0; u < UNR; ++u) {
-            const iv4 q = ab4[min(i0 + u, D - 1)];
-            ia[u] = q.x >= 0 ? q.x : nE;
-            ib[u] = q.y >= 0 ? q.y : nE;
-            im[u] = q.z >= 0 ? q.z : nE;
-            in[u] = q.w >= 0 ? q.w : nE;
-        }
-        CT val[UNR];
-        OutT prev[UNR];
-#pragma unroll
-        for (int u = 0; u < UNR; ++u) val[u] = CT(0);
-#pragma unroll
-        for (int kp = 0; kp < KPMAX; ++kp) {
-            if (kp >= nk) break;
-            const ST * Sk = Sc + (size_t)kp * TILE * CS;
+    // createJacobian_ scaling J_ij *= k_i / m_j^2 (dcfemmodelling.cpp:1375-1384), on the last pass
+    const bool scl = d.last && d.scale;
+    const CT rm2 = (CT)(1.0 / m2);
+    // MODE 0: first pass, store; 1: later pass, add to the stored partial sums; 2: atomic adds.
+    // A step's indices, geometric factors and partial sums are loaded one step ahead,
+    // before the previous step's stores: on this architecture a wait for a load also
+    // waits for the stores issued before it, so loads issued after the stores would
+    // put the store latency on every step.
+    auto sweep = [&](auto mode_c) {
+        constexpr int MODE = decltype(mode_c)::value;
+        int i0 = (wave * NSUB + sub) * UNR;
+        iv4 qn[UNR];
+        CT kn[UNR];
+        OutT pn[UNR];
+        auto fetch = [&](int i) {
 #pragma unroll
             for (int u = 0; u < UNR; ++u) {
-                CT dv[NCOMP], mv[NCOMP];
+                const int ii = min(i + u, D - 1);
+                qn[u] = ab4[ii];
+                kn[u] = (CT)kf[ii];
+                if (MODE == 1) pn[u] = J[(size_t)ii * ldj + col];
+            }
+        };
+        fetch(i0);
+        for (; i0 < D; i0 += step) {
+            int ia[UNR], ib[UNR], im[UNR], in[UNR];
+            CT fac[UNR];
+            OutT prev[UNR];
 #pragma unroll
-                for (int q = 0; q < NCOMP; ++q) {
-                    const ST * r = Sk + q * nEp;
-                    dv[q] = (CT)r[ia[u]] - (CT)r[ib[u]];
-                    mv[q] = (CT)r[im[u]] - (CT)r[in[u]];
+            for (int u = 0; u < UNR; ++u) {
+                ia[u] = qn[u].x >= 0 ? qn[u].x : nE;
+                ib[u] = qn[u].y >= 0 ? qn[u].y : nE;
+                im[u] = qn[u].z >= 0 ? qn[u].z : nE;
+                in[u] = qn[u].w >= 0 ? qn[u].w : nE;
+                fac[u] = scl ? kn[u] * rm2 : CT(1);
+                prev[u] = MODE == 1 ? pn[u] : OutT(0);
+            }
+            fetch(i0 + step);
+            CT val[UNR];
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) val[u] = CT(0);
+#pragma unroll
+            for (int kp = 0; kp < KPMAX; ++kp) {
+                if (kp >= nk) break;
+                const ST * Sk = Sc + (size_t)kp * TILE * CS;
+#pragma unroll
+                for (int u = 0; u < UNR; ++u) {
+                    CT dv[NCOMP], mv[NCOMP];
+#pragma unroll
+                    for (int q = 0; q < NCOMP; ++q) {
+                        const ST * r = Sk + q * nEp;
+                        dv[q] = (CT)r[ia[u]] - (CT)r[ib[u]];
+                        mv[q] = (CT)r[im[u]] - (CT)r[in[u]];
+                    }
+                    if (FORM == F_GRAD) {
+                        CT sum = CT(0);
+#pragma unroll
+                        for (int q = 0; q < DIM; ++q) sum += dv[q] * mv[q];
+                        val[u] += coef[kp][0] * sum;
+                    } else {
+                        int qi = 0;
+                        CT y[NV];
+#pragma unroll
+                        for (int j = 0; j < NV; ++j) y[j] = CT(0);
+#pragma unroll
+                        for (int j = 0; j < NV; ++j)
+#pragma unroll
+                            for (int l = j; l < NV; ++l) {
+                                const CT qq = coef[kp][qi++];
+                                y[j] += qq * mv[l];
+                                if (l != j) y[l] += qq * mv[j];
+                            }
+                        CT sum = CT(0);
+#pragma unroll
+                        for (int j = 0; j < NV; ++j) sum += dv[j] * y[j];
+                        val[u] += sum;
+                    }
                 }
-                if (FORM == F_GRAD) {
-                    CT s = CT(0);
+            }
 #pragma unroll
-                    for (int q = 0; q < DIM; ++q) s += dv[q] * mv[q];
-                    val[u] += coef[kp][0] * s;
-                } else {
-                    int qi = 0;
-                    CT y[NV];
-#pragma unroll
-                    for (int j = 0; j < NV; ++j) y[j] = CT(0);
-#pragma unroll
-                    for (int j = 0; j < NV; ++j)
-#pragma unroll
-                        for (int l = j; l < NV; ++l) {
-                            const CT qq = coef[kp][qi++];
-                            y[j] += qq * mv[l];
-                            if (l != j) y[l] += qq * mv[j];
-                        }
-                    CT s = CT(0);
-#pragma unroll
-                    for (int j = 0; j < NV; ++j) s += dv[j] * y[j];
-                    val[u] += s;
+            for (int u = 0; u < UNR; ++u) {
+                const int i = i0 + u;
+                if (i < D) {
+                    const size_t idx = (size_t)i * ldj + col;
+                    if (MODE == 2) {
+                        atomicAdd(Jf + idx, (OutT)val[u]);
+                    } else {
+                        const OutT r = (OutT)((val[u] + (CT)prev[u]) * fac[u]);
+                        if (d.last) store_nt(J + idx, r);
+                        else J[idx] = r;
+                    }
                 }
             }
         }
-        // earlier passes' partial sums and the data's geometric factors,
-        // loaded together from clamped rows
-        double fac[UNR];
-        if (!d.first && !d.atomic) {
-#pragma unroll
-            for (int u = 0; u < UNR; ++u) prev[u] = J[(size_t)min(i0 + u, D - 1) * d.nParam + col];
-        } else {
-#pragma unroll
-            for (int u = 0; u < UNR; ++u) prev[u] = OutT(0);
-        }
-        if (d.last && d.scale) {
-#pragma unroll
-            for (int u = 0; u < UNR; ++u) fac[u] = glob(d.kfac)[min(i0 + u, D - 1)] / m2;
-        } else {
-#pragma unroll
-            for (int u = 0; u < UNR; ++u) fac[u] = 1.0;
-        }
-#pragma unroll
-        for (int u = 0; u < UNR; ++u) {
-            const int i = i0 + u;
-            if (i >= D) break;
-            const size_t idx = (size_t)i * d.nParam + col;
-            if (d.atomic) {
-                atomicAdd(Jf + idx, (OutT)val[u]);
-            } else {
-                // createJacobian_ scaling J_ij *= k_i / m_j^2 (dcfemmodelling.cpp:1375-1384)
-                const double r = ((double)val[u] + (double)prev[u]) * fac[u];
-                if (d.last) store_nt(J + idx, (OutT)r);
-                else J[idx] = (OutT)r;
-            }
-        }
-    }
+    };
+    if (d.atomic) sweep(std::integral_constant<int, 2>{});
+    else if (!d.first) sweep(std::integral_constant<int, 1>{});
+    else sweep(std::integral_constant<int, 0>{});
 }
 
 template <typename OutT>
@@ -303,9 +327,11 @@ int odd_stride(int ncomp, int nEp) {
 }
 
 // Tile width and wavenumbers per pass for the largest electrode count of a
-// launch: fewest passes (each pass after the first is a read-modify-write
-// sweep of J), then a tile within LDS_BUDGET (two workgroups per CU), then
-// the widest tile (longer row segments per store).
+// launch, by measurement on the 3D (1 pass) and 2.5D (11 wavenumbers)
+// workloads: fewest passes first (each pass after the first is a
+// read-modify-write sweep of J), then a tile of at most LDS_SMALL bytes
+// (four and more workgroups per CU), then 32- over 16-cell tiles.
+constexpr int LDS_SMALL = 40 * 1024;
 Plan make_plan(int dim, int nEmax, int nK, bool allZeroK, size_t st) {
     Plan p;
     p.form = allZeroK ? F_GRAD : F_NODAL;
@@ -322,12 +348,13 @@ Plan make_plan(int dim, int nEmax, int nK, bool allZeroK, size_t st) {
             over = 1;
         }
         const long passes = (nK + kp - 1) / kp;
-        const long score = passes * 4 + over * 2;  // ties: the first (widest) tile
+        const size_t lds = (size_t)kp * perk;
+        const long score = passes * 16 + over * 8 + (lds > LDS_SMALL ? 4 : 0) + (tile == 16 ? 2 : 0);
         if (best < 0 || score < best) {
             best = score;
             p.tile = tile;
             p.kp = kp;
-            p.lds = (size_t)kp * perk;
+            p.lds = lds;
         }
     }
     PG_REQUIRE(best >= 0, "pgert_jacobian: too many electrodes for one LDS tile");
