@@ -34,7 +34,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline_worker(out_path, nthreads, sample_frac, pcg_iters):
+def cpu_baseline_worker(out_path, nthreads, sample_frac, pcg_iters, workload="ert3d_grid_grf"):
     """Runs in a child process (started before the GPU work): the oracle's C
     ports on the host cores, on a bounded sample of the same workload.
     Forward = FP64 assembly of the system (oracle) + the plain FP64 Jacobi-PCG
@@ -47,7 +47,7 @@ def cpu_baseline_worker(out_path, nthreads, sample_frac, pcg_iters):
     from oracle import ert_oracle as eo
     from oracle import port
     from pygimli_amd import workloads
-    mesh, data, rho = workloads.ert3d_grid_grf()
+    mesh, data, rho = getattr(workloads, workload)()
     fo = eo.Forward(mesh, copy.deepcopy(data))
     kv, wv = fo.k, fo.w
     t = time.time()
@@ -58,12 +58,14 @@ def cpu_baseline_worker(out_path, nthreads, sample_frac, pcg_iters):
     _, it = port.pcg(S, B, maxit=pcg_iters, rtol=0.0, nthreads=nthreads)
     t_it = (time.time() - t) / max(it, 1)
     C = mesh.cellCount()
-    nP = int(mesh.cellMarkers().max()) + 1
     c1 = max(1, int(C * sample_frac))
     U = fo.primaries()
+    # the sampled cells get their own columns 0..c1-1 (J of the sample only)
+    mk = np.full(C, -1, np.int32)
+    mk[:c1] = np.arange(c1)
     t = time.time()
-    port.calc2(mesh, U, kv, wv, data.abmn(), nP, nthreads=nthreads, cell_range=(0, c1),
-               J=np.zeros((data.size(), nP)))
+    port.calc2(mesh, U, kv, wv, data.abmn(), c1, nthreads=nthreads, cell_range=(0, c1),
+               J=np.zeros((data.size(), c1)), markers=mk)
     tj = (time.time() - t) * C / c1
     with open(out_path, "w") as f:
         json.dump({"ta": ta, "t_it": t_it, "it_sampled": it, "tj": tj, "c1": c1, "C": C, "nrhs": B.shape[1]}, f)
@@ -211,6 +213,86 @@ def batched_leg(args, log):
                            "mixed block-PCG 1e-6 with per-problem coarse space") + ", FP32 Jacobian per problem"}
 
 
+def survey_large_leg(args, log):
+    """Large 3D survey: 256 electrodes (16x16 @ 1 m), 10000 dipole-dipole and
+    gradient quadrupoles, 1.5M-tet mesh, log10-rho GRF; FP32 Jacobian of
+    10000 x 1.5M (60 GB) in HBM.  A step = response + createJacobian with
+    the mixed block-PCG + two-level preconditioner (four 64-column blocks).
+    Plain FP64 PCG / block-PCG are timed once each for comparison, potentials
+    are checked against an FP64 block-PCG solve to 1e-10.  Returns the result
+    object."""
+    import torch
+    from pygimli_amd import workloads
+    from pygimli_amd.ertModelling import ERTModelling
+    t0 = time.time()
+    mesh, data, rho = workloads.ert3d_survey_large()
+    log(f"[large] workload {mesh.nodeCount()} nodes, {mesh.cellCount()} cells, {data.size()} data, "
+        f"{len(data.sensorPositions())} electrodes ({time.time() - t0:.1f}s)")
+    fop = ERTModelling()
+    fop.setMesh(mesh)
+    fop.setData(copy.deepcopy(data))
+    fop.jacobianDtype = torch.float32
+    model = torch.as_tensor(rho, device=fop.device)
+    fop.solverOptions.update(method="block64", rtol=1e-10, coarse=args.coarse)
+    fop.response_device(model)
+    U_ref = fop.subPotentials(node_major=True).clone()
+    log(f"[large] reference solve {fop.lastStats} {fop.timings()} at {time.time() - t0:.1f}s")
+
+    def run(method, steps, warmup, coarse):
+        fop.solverOptions.update(method=method, rtol=args.rtol, coarse=coarse)
+        for _ in range(warmup):
+            fop.response_device(model)
+            fop.createJacobian(model)
+            fop._J = None
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        jms = []
+        for _ in range(steps):
+            fop.response_device(model)
+            fop.createJacobian(model)
+            jms.append(fop.timings()["jacobian"])
+            fop._J = None
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / steps
+        U = fop.subPotentials(node_major=True)
+        err = float(torch.linalg.norm(U - U_ref) / torch.linalg.norm(U_ref))
+        r = dict(ms=dt * 1e3, iters=fop.lastStats["iterations"], pot_err=err, stats=dict(fop.lastStats),
+                 timings_ms=fop.timings(), jacobian_ms=min(jms))
+        log(f"[large] {method} coarse={coarse}: {r}")
+        return r
+
+    plain = run("pcg64", 1, 0, 0)
+    plain_block = run("block64", 1, 0, 0)
+    best = run(args.method, args.steps, args.warmup, args.coarse)
+    n, C, D = mesh.nodeCount(), mesh.cellCount(), data.size()
+    nE = len(data.sensorPositions())
+    # Jacobian compulsory bytes: J written once, potentials read once, cells + geometry read once
+    jbytes = D * C * 4.0 + n * nE * 8.0 + C * (4 * 4 + 4 * 3 * 8 + 8 + 4)
+    jgbs = jbytes / (best["jacobian_ms"] * 1e-3) / 1e9
+    return {
+        "metric": "ERT forward+Jacobian/s, 3D large survey (256 electrodes, 10000 data, 1.5M tets), 1 GPU",
+        "value": 1e3 / best["ms"], "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": best["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32+fp64" if args.method == "mixed" else "fp64", "data": "synthetic",
+        "config": {"workload": f"3D ERT: {n}-node / {C}-tet graded mesh, {nE} electrodes (16x16 @ 1 m), "
+                               f"{D} dipole-dipole + gradient quadrupoles, log10-rho GRF (2.0, 0.5, 5 m); "
+                               f"J {D} x {C} FP32 ({D * C * 4 / 1e9:.0f} GB)",
+                   "method": args.method, "rtol": args.rtol, "coarse_aggregates": args.coarse,
+                   "learned_warm_start": False, "nodes": n, "cells": C, "electrodes": nE, "data": D, "rhs": nE},
+        "accuracy": {"potential_rel_err_fro": best["pot_err"], "max_rel_residual": best["stats"]["max_rel_residual"],
+                     "reference": "FP64 block-PCG to rtol 1e-10"},
+        "iterations": best["iters"], "phase_ms": best["timings_ms"],
+        "roofline": {"bound": "hbm", "kernel": "k_sens (Jacobian)", "achieved": jgbs, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": jgbs / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": jbytes, "avg_launch_us": best["jacobian_ms"] * 1e3},
+        "plain_fp64_pcg": {"ms_per_step": plain["ms"], "iterations": plain["iters"], "potential_rel_err_fro": plain["pot_err"]},
+        "plain_fp64_block_pcg": {"ms_per_step": plain_block["ms"], "iterations": plain_block["iters"],
+                                 "potential_rel_err_fro": plain_block["pot_err"]},
+        "speedup_vs_plain_fp64_block_pcg": plain_block["ms"] / best["ms"],
+        "speedup_vs_plain_fp64_pcg": plain["ms"] / best["ms"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -230,6 +312,8 @@ def main():
     ap.add_argument("--batched-method", default="band",
                     help="solver of the batched 2.5D leg: band (direct banded Cholesky) or mixed (block-PCG)")
     ap.add_argument("--batched-only", action="store_true", help="run only the batched 2.5D leg")
+    ap.add_argument("--survey-large", action="store_true",
+                    help="run only the large 3D survey leg (256 electrodes, 10000 data, 1.5M tets, 60 GB Jacobian)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run one 1 GiB device-to-device copy (source evicted from the last-level cache): "
                          "the known-size kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -286,6 +370,21 @@ def main():
                                f"{t_all:.0f} s")}
             except Exception as e:
                 out["cpu_baseline"] = {"error": f"no batched cpu baseline ({e!r})"}
+        print(json.dumps(out))
+        return
+    if args.survey_large:
+        if not args.no_cpu:
+            import multiprocessing as mp
+            os.makedirs(os.path.dirname(cpu_path), exist_ok=True)
+            cpu_proc = mp.get_context("spawn").Process(
+                target=cpu_baseline_worker, args=(cpu_path, nth, 0.002, 10, "ert3d_survey_large"), daemon=True)
+            cpu_proc.start()
+        out = survey_large_leg(args, log)
+        if cpu_proc is not None:
+            cpu_proc.join(args.cpu_timeout)
+            if cpu_proc.is_alive():
+                cpu_proc.terminate()
+            out["cpu_baseline"] = cpu_baseline_result(cpu_path, nth, out["plain_fp64_pcg"]["iterations"])
         print(json.dumps(out))
         return
     if not args.no_cpu:

@@ -8,13 +8,13 @@ import numpy as np
 LIB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_build", "liboracle.so")
 
 
-def calc2(mesh, U, kvals, weights, abmn, nParam, nthreads=1, cell_range=None, J=None):
+def calc2(mesh, U, kvals, weights, abmn, nParam, nthreads=1, cell_range=None, J=None, markers=None):
     L = ctypes.CDLL(LIB)
     D = ctypes.POINTER(ctypes.c_double)
     I = ctypes.POINTER(ctypes.c_int)
     xyz = np.ascontiguousarray(mesh.nodes, np.float64)
     cells = np.ascontiguousarray(mesh.cells, np.int32)
-    mk = np.ascontiguousarray(mesh.cellMarkers(), np.int32)
+    mk = np.ascontiguousarray(mesh.cellMarkers() if markers is None else markers, np.int32)
     U = np.ascontiguousarray(U, np.float64)
     kv = np.ascontiguousarray(kvals, np.float64)
     wv = np.ascontiguousarray(weights, np.float64)

@@ -133,3 +133,34 @@ def test_batched_2d_workload():
         assert len(k) == 11 and len(w) == 11 and np.all(k > 0)
         assert np.all(rho >= 10.0 - 1e-9) and np.all(rho <= 1000.0 + 1e-9)
         np.testing.assert_array_equal(rho, rho2)
+
+
+def test_large_survey_schemes():
+    """Grid dipole-dipole and gradient schemes: indices on the grid, four
+    distinct electrodes, every potential dipole of neighbouring electrodes;
+    the large-survey generator keeps only quadrupoles with a non-vanishing
+    half-space signal and is deterministic."""
+    from pygimli_amd import workloads
+    ns = 6
+    dd = workloads.line_dipole_dipole_grid(ns, spacings=(1, 2), seps=(1, 2))
+    per_line = len(workloads.dipole_dipole(ns, (1, 2), (1, 2)))
+    assert len(dd) == 2 * ns * per_line
+    gr = workloads.gradient_grid(ns, insets=(0,), reach=1)
+    for q in (dd, gr):
+        assert q.min() >= 0 and q.max() < ns * ns
+        assert all(len(set(r)) == 4 for r in q.tolist())
+    # potential dipoles of the gradient arrays are neighbours on one grid line
+    m, n = gr[:, 2], gr[:, 3]
+    assert np.all((np.abs(m - n) == 1) | (np.abs(m - n) == ns))
+    mesh, data, rho = workloads.ert3d_survey_large(n_side=4, spacing=1.0, n_data=50, h=1.0, growth=2.0, pad=40.0,
+                                                   z_h=1.0, z_growth=2.0)
+    s = data.sensorPositions()
+    ab = data.abmn()
+    d = np.sqrt(((s[:, None] - s[None]) ** 2).sum(-1))
+    g = 1 / d[ab[:, 0], ab[:, 2]] - 1 / d[ab[:, 0], ab[:, 3]] - 1 / d[ab[:, 1], ab[:, 2]] + 1 / d[ab[:, 1], ab[:, 3]]
+    assert np.all(np.abs(g) * d[d > 0].min() > 1e-3)
+    assert len(s) == 16 and data.size() <= 50 and len(rho) == mesh.cellCount()
+    _, data2, rho2 = workloads.ert3d_survey_large(n_side=4, spacing=1.0, n_data=50, h=1.0, growth=2.0, pad=40.0,
+                                                  z_h=1.0, z_growth=2.0)
+    np.testing.assert_array_equal(ab, data2.abmn())
+    np.testing.assert_array_equal(rho, rho2)
