@@ -213,6 +213,81 @@ def batched_leg(args, log):
                            "mixed block-PCG 1e-6 with per-problem coarse space") + ", FP32 Jacobian per problem"}
 
 
+def _section_arrays(out_path, name, what):
+    """Reference results of the section leg: large, so kept in the temp dir."""
+    import tempfile
+    return os.path.join(tempfile.gettempdir(), f"{os.path.basename(out_path)}.{name}.{what}.npy")
+
+
+def section_ref_worker(out_path, nthreads, reps):
+    """Config-1 CPU baseline in a child process: the reference's own C++
+    core (oracle/_ref, DCSRMultiElectrodeModelling with its solver plug-in
+    served by an envelope Cholesky) on the full workload, response and
+    createJacobian, for the explicit 11 wavenumbers and the default list;
+    median of `reps` runs after one warm-up."""
+    os.dup2(2, 1)  # the reference core prints its warnings to stdout; keep stdout for the JSON line
+    from oracle import ref
+    from pygimli_amd import bert, workloads
+    mesh, data, rho = workloads.ert2d_layers_block()
+    an = bert.MeshAnalysis(mesh, data.sensorPositions())
+    d = np.sqrt(((an.elecPos[:, None] - an.elecPos[None]) ** 2).sum(-1))
+    k11, w11 = bert.kwave_from_range(d[d > 0].min() / 2, d.max() * 2, 7, 4)
+    res = {}
+    for name, kw in (("k11", (k11, w11)), ("default", (None, None))):
+        ses = ref.Session(mesh, copy.deepcopy(data), kw[0], kw[1], nthreads=nthreads)
+        ts = []
+        for r in range(reps + 1):
+            t = time.perf_counter()
+            rhoa = ses.response(rho)
+            J = ses.jacobian(rho)
+            ts.append(time.perf_counter() - t)
+        res[name] = {"s": float(np.median(ts[1:])), "nk": len(ses.kw()[0])}
+        np.save(_section_arrays(out_path, name, "rhoa"), rhoa)
+        np.save(_section_arrays(out_path, name, "J"), J)
+    with open(out_path, "w") as f:
+        json.dump(res, f)
+
+
+def section_leg(args, log):
+    """Config 1: one 2.5D section (3-layer + block model, 41 electrodes,
+    dipole-dipole), the reference's CPU-runnable case.  A step = response
+    (direct band solver, the reference's own direct approach) +
+    createJacobian (FP64, as the reference's), with the explicit 11
+    wavenumbers and with the default list.  CPU baseline: the reference's
+    own C++ core (oracle/_ref) on the same inputs, results compared."""
+    import torch
+    from pygimli_amd import bert, workloads
+    from pygimli_amd.ertModelling import ERTModelling
+    mesh, data, rho = workloads.ert2d_layers_block()
+    an = bert.MeshAnalysis(mesh, data.sensorPositions())
+    d = np.sqrt(((an.elecPos[:, None] - an.elecPos[None]) ** 2).sum(-1))
+    k11, w11 = bert.kwave_from_range(d[d > 0].min() / 2, d.max() * 2, 7, 4)
+    out = {}
+    for name, kw in (("k11", (k11, w11)), ("default", (None, None))):
+        fop = ERTModelling()
+        fop.setMesh(mesh)
+        fop.setData(copy.deepcopy(data))
+        if kw[0] is not None:
+            fop.setkValues(kw[0])
+            fop.setWeights(kw[1])
+        fop.solverOptions.update(method="band", rtol=1e-10)
+        model = torch.as_tensor(rho, device=fop.device)
+        for _ in range(args.warmup + 1):
+            fop.response_device(model)
+            fop.createJacobian(model)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(max(args.steps, 5)):
+            r = fop.response_device(model)
+            fop.createJacobian(model)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / max(args.steps, 5)
+        out[name] = {"ms_per_step": dt * 1e3, "evals_per_s": 1.0 / dt, "nk": len(fop._kv),
+                     "phase_ms": fop.timings(), "rhoa": r.cpu().numpy(), "J": fop.jacobian().cpu().numpy()}
+        log(f"[section] {name}: {dt * 1e3:.2f} ms, phases {fop.timings()}")
+    return out, mesh, data
+
+
 def survey_large_leg(args, log):
     """Large 3D survey: 256 electrodes (16x16 @ 1 m), 10000 dipole-dipole and
     gradient quadrupoles, 1.5M-tet mesh, log10-rho GRF; FP32 Jacobian of
@@ -312,6 +387,9 @@ def main():
     ap.add_argument("--batched-method", default="band",
                     help="solver of the batched 2.5D leg: band (direct banded Cholesky) or mixed (block-PCG)")
     ap.add_argument("--batched-only", action="store_true", help="run only the batched 2.5D leg")
+    ap.add_argument("--section", action="store_true",
+                    help="run only the single 2.5D section leg (41 electrodes, 11 and default wavenumbers) "
+                         "beside the reference's own C++ core")
     ap.add_argument("--survey-large", action="store_true",
                     help="run only the large 3D survey leg (256 electrodes, 10000 data, 1.5M tets, 60 GB Jacobian)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -370,6 +448,50 @@ def main():
                                f"{t_all:.0f} s")}
             except Exception as e:
                 out["cpu_baseline"] = {"error": f"no batched cpu baseline ({e!r})"}
+        print(json.dumps(out))
+        return
+    if args.section:
+        import multiprocessing as mp
+        cpu_proc = None
+        if not args.no_cpu:
+            os.makedirs(os.path.dirname(cpu_path), exist_ok=True)
+            cpu_proc = mp.get_context("spawn").Process(target=section_ref_worker, args=(cpu_path, nth, 3),
+                                                       daemon=True)
+            cpu_proc.start()
+        res, mesh, data = section_leg(args, log)
+        g = res["k11"]
+        out = {"metric": "ERT forward+Jacobian/s, 2.5D section (41 electrodes, 11 wavenumbers), 1 GPU",
+               "value": g["evals_per_s"], "unit": "evals/s", "n_gpus": 1, "steps": max(args.steps, 5),
+               "warmup": args.warmup + 1, "ms_per_step": g["ms_per_step"], "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None, "dtype": "fp64", "data": "synthetic",
+               "config": {"workload": f"2.5D section: {mesh.nodeCount()} nodes / {mesh.cellCount()} triangles, "
+                                      f"41 electrodes @ 1 m, {data.size()} dipole-dipole data, 3 layers + block; "
+                                      f"direct band solver (FP64), FP64 Jacobian"},
+               "section": {k: {kk: vv for kk, vv in v.items() if kk not in ("rhoa", "J")} for k, v in res.items()}}
+        if cpu_proc is not None:
+            cpu_proc.join(args.cpu_timeout)
+            if cpu_proc.is_alive():
+                cpu_proc.terminate()
+            try:
+                with open(cpu_path) as f:
+                    r = json.load(f)
+                cmp = {}
+                for name in ("k11", "default"):
+                    rr = np.load(_section_arrays(cpu_path, name, "rhoa"))
+                    Jr = np.load(_section_arrays(cpu_path, name, "J"))
+                    cmp[name] = {"reference_s": r[name]["s"], "nk": r[name]["nk"],
+                                 "speedup": r[name]["s"] / (res[name]["ms_per_step"] * 1e-3),
+                                 "rhoa_max_rel_diff": float(np.abs(res[name]["rhoa"] - rr).max() / np.abs(rr).max()),
+                                 "J_rel_fro_diff": float(np.linalg.norm(res[name]["J"] - Jr) / np.linalg.norm(Jr))}
+                out["cpu_baseline"] = {
+                    "value": 1.0 / r["k11"]["s"], "unit": "evals/s", "cores": nth, "kind": "reference",
+                    "sample": (f"the whole workload (response + createJacobian) through the reference's own C++ "
+                               f"core built from its sources (oracle/_ref), median of 3 after a warm-up, "
+                               f"{nth} Jacobian threads; its solver plug-in is an RCM envelope Cholesky "
+                               f"(CHOLMOD is absent)"),
+                    "comparison": cmp}
+            except Exception as e:
+                out["cpu_baseline"] = {"error": f"no reference baseline ({e!r})"}
         print(json.dumps(out))
         return
     if args.survey_large:
