@@ -288,6 +288,101 @@ def section_leg(args, log):
     return out, mesh, data
 
 
+def learned_large_leg(args, log):
+    """Learned warm start at scale: 3D mesh of 1.09M nodes / 6.3M tets, 128
+    electrodes (8x8 grid + two 32-electrode lines), log-normal training
+    fields (std U[0.3, 0.8], correlation U[2, 20] m; seeds 100..).  Trains
+    the warm-start MLP (BF16 matrix cores) on samples of the mixed solver's
+    own solutions of the training fields, timing the epochs (samples/s and
+    TFLOP/s of forward + backward), then solves a held-out field (seed 5000)
+    with and without the warm start.  Returns the result object."""
+    import torch
+    from pygimli_amd import workloads
+    from pygimli_amd.ertModelling import ERTModelling
+    from pygimli_amd.learned import LearnedWarmStart
+    t0 = time.time()
+    mesh, data = workloads.ert3d_lines_grid()
+    n, C, nE = mesh.nodeCount(), mesh.cellCount(), len(data.sensorPositions())
+    log(f"[learned] workload {n} nodes, {C} cells, {nE} electrodes ({time.time() - t0:.1f}s)")
+    fop = ERTModelling()
+    fop.setMesh(mesh)
+    fop.setData(copy.deepcopy(data))
+    fop.solverOptions.update(method=args.method, rtol=args.rtol, coarse=args.coarse, learned=False)
+    dev = fop.device
+    nf = args.learned_fields if args.learned_fields > 0 else 8
+    train = [torch.as_tensor(f, device=dev) for f in workloads.learned_fields(mesh, range(100, 100 + nf))]
+    test = torch.as_tensor(workloads.learned_fields(mesh, [5000])[0], device=dev)
+    lw = LearnedWarmStart(fop, seed=0)
+    rng = np.random.default_rng(0)
+    per = args.learned_samples
+    feats, tgts, solve_s, iters = [], [], [], []
+    for m in train:
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        fop.response_device(m)
+        torch.cuda.synchronize()
+        solve_s.append(time.perf_counter() - t)
+        iters.append(fop.lastStats["iterations"])
+        f, tg = lw.collect(per, rng)
+        feats.append(f)
+        tgts.append(tg)
+    log(f"[learned] {nf} training solves: {iters} iterations, {np.mean(solve_s):.2f} s each")
+    F = torch.cat(feats)
+    T = torch.clamp(torch.cat(tgts), -5.0, 5.0)
+    batch, epochs = args.learned_batch, args.learned_epochs
+    nb = F.shape[0] // batch
+    losses, ep_s = [], []
+    for ep in range(epochs):
+        perm = torch.as_tensor(rng.permutation(F.shape[0]), device=dev)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        tot = 0.0
+        for b in range(nb):
+            sel = perm[b * batch:(b + 1) * batch]
+            loss, _ = lw.step(F[sel].contiguous(), T[sel].contiguous(), 1e-3)
+            tot += loss
+        torch.cuda.synchronize()
+        ep_s.append(time.perf_counter() - t)
+        losses.append(tot / max(nb, 1))
+        log(f"[learned] epoch {ep}: loss {losses[-1]:.4e}, {ep_s[-1]:.2f} s")
+    samples = nb * batch
+    sps = samples / np.median(ep_s)
+    flop_fwd = 2.0 * (32 * 128 + 128 * 128 * 2 + 128)
+    tflops = sps * 3.0 * flop_fwd / 1e12
+
+    def solve(learned):
+        fop.solverOptions.update(learned=learned)
+        fop.response_device(test)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        fop.response_device(test)
+        torch.cuda.synchronize()
+        return {"ms": (time.perf_counter() - t) * 1e3, "iterations": fop.lastStats["iterations"],
+                "max_rel_residual": fop.lastStats["max_rel_residual"], "phase_ms": fop.timings()}
+
+    without = solve(False)
+    with_ = solve(True)
+    log(f"[learned] held-out field: without {without}, with {with_}")
+    return {
+        "metric": "ERT forward/s on a held-out field with the learned warm start, 3D 1.09M nodes / 128 electrodes, 1 GPU",
+        "value": 1e3 / with_["ms"], "unit": "evals/s", "n_gpus": 1, "higher_is_better": True,
+        "ms_per_step": with_["ms"], "dtype": "fp32+fp64 (solver), bf16 (MLP)", "data": "synthetic",
+        "config": {"workload": f"3D ERT: {n}-node / {C}-tet graded mesh, {nE} electrodes (8x8 @ 2 m + 2 lines of 32 @ 1 m), "
+                               f"log-normal GRF fields (std U[0.3,0.8], corr U[2,20] m)",
+                   "method": args.method, "rtol": args.rtol, "coarse_aggregates": args.coarse,
+                   "training_fields": nf, "samples_per_field": per, "batch": batch, "epochs": epochs},
+        "held_out": {"without_warm_start": without, "with_warm_start": with_,
+                     "iteration_ratio": with_["iterations"] / max(without["iterations"], 1),
+                     "speedup": without["ms"] / with_["ms"]},
+        "training": {"solves_s_mean": float(np.mean(solve_s)), "solve_iterations": iters,
+                     "samples": int(samples), "epoch_s": ep_s, "samples_per_s": sps,
+                     "loss_first": losses[0], "loss_last": losses[-1]},
+        "roofline": {"bound": "mfma", "kernel": "MLP train step (fwd + bwd, BF16)", "achieved": tflops,
+                     "peak": 2500.0, "unit": "TFLOP/s", "frac": tflops / 2500.0, "traffic": None,
+                     "flops_per_sample": 3.0 * flop_fwd},
+    }
+
+
 def survey_large_leg(args, log):
     """Large 3D survey: 256 electrodes (16x16 @ 1 m), 10000 dipole-dipole and
     gradient quadrupoles, 1.5M-tet mesh, log10-rho GRF; FP32 Jacobian of
@@ -390,6 +485,11 @@ def main():
     ap.add_argument("--section", action="store_true",
                     help="run only the single 2.5D section leg (41 electrodes, 11 and default wavenumbers) "
                          "beside the reference's own C++ core")
+    ap.add_argument("--learned-large", action="store_true",
+                    help="run only the learned-warm-start leg on the 1.09M-node / 128-electrode mesh")
+    ap.add_argument("--learned-samples", type=int, default=1 << 20, help="training samples per field")
+    ap.add_argument("--learned-batch", type=int, default=16384)
+    ap.add_argument("--learned-epochs", type=int, default=3)
     ap.add_argument("--survey-large", action="store_true",
                     help="run only the large 3D survey leg (256 electrodes, 10000 data, 1.5M tets, 60 GB Jacobian)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -493,6 +593,9 @@ def main():
             except Exception as e:
                 out["cpu_baseline"] = {"error": f"no reference baseline ({e!r})"}
         print(json.dumps(out))
+        return
+    if args.learned_large:
+        print(json.dumps(learned_large_leg(args, log)))
         return
     if args.survey_large:
         if not args.no_cpu:
