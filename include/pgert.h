@@ -179,6 +179,29 @@ int pgert_jacobian(pgert_ctx * ctx, const double * model, int n_model, void * J,
 int pgert_batch_jacobian(pgert_ctx * const * ctxs, int count, const double * const * models, const int * n_models,
                          void * const * Js, int dtype);
 
+/* Constraint matrix C of the Gauss-Newton step (ModellingBase::constraints,
+ * used by RInversion::oneStep, inversion.cpp:308): HOST CSR [n_const x
+ * n_model], sorted or unsorted columns.  Copied with its transpose to the
+ * device; replaces the previous one.  n_const = 0: no constraints. */
+int pgert_set_constraints(pgert_ctx * ctx, int n_const, int n_model, const int * rowptr, const int * colidx,
+                          const double * vals);
+
+/* Regularised, transformed CGLS of one Gauss-Newton step
+ * (solveCGLSCDWWhtrans, core/src/solver.cpp:27-224):
+ *   min || dW (b - td * J (x / tm)) ||^2 + lambda || wc (C (wm x)) + wc roughness ||^2
+ * by conjugate gradients on the normal equations, starting from x, with the
+ * reference's stopping rule (|r|^2 <= tol; tol < 0: max(1e-12, 1e-8 |r0|^2),
+ * r0 = J'(b dW^2 td)/tm - lambda wm C'(wc roughness)) and its rounding of
+ * C (p wm) to 1e-10.  J: DEVICE [n_data x n_model] row-major, FP64 (dtype 0)
+ * or FP32 (dtype 1), e.g. the output of pgert_jacobian.  d_weight, b, td:
+ * DEVICE [n_data]; x (in/out), wm, tm: DEVICE [n_model]; wc, roughness:
+ * DEVICE [n_const] (C from pgert_set_constraints).  All FP64.  Returns the
+ * iteration count and the final |r|^2. */
+int pgert_cgls(pgert_ctx * ctx, const void * J, int dtype, int n_data, int n_model, const double * d_weight,
+               const double * b, double * x, const double * wc, const double * wm, const double * tm,
+               const double * td, double lambda, const double * roughness, int max_iter, double tol,
+               int * iterations, double * norm_r2);
+
 /* Timing of the last pgert_forward / pgert_jacobian phases in milliseconds
  * (HIP events on the context stream): [0] assembly, [1] primary + SR rhs,
  * [2] solve, [3] response, [4] jacobian. */

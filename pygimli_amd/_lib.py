@@ -30,7 +30,7 @@ EXPORTS = (
     "pgert_potentials", "pgert_potentials_node_major", "pgert_jacobian", "pgert_last_timings",
     "pgert_kernel_timing", "pgert_mlp_param_count", "pgert_mlp_set_params", "pgert_mlp_get_params",
     "pgert_mlp_collect", "pgert_mlp_predict", "pgert_mlp_train_step", "pgert_set_coarse",
-    "pgert_batch_forward", "pgert_batch_jacobian",
+    "pgert_batch_forward", "pgert_batch_jacobian", "pgert_set_constraints", "pgert_cgls",
 )
 
 
@@ -83,6 +83,9 @@ def load():
                                       ctypes.POINTER(SolveOpts), ctypes.POINTER(vp), ctypes.POINTER(SolveStats)]
     L.pgert_batch_jacobian.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp), _c_int_p,
                                        ctypes.POINTER(vp), ctypes.c_int]
+    L.pgert_set_constraints.argtypes = [vp, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p, _c_double_p]
+    L.pgert_cgls.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp, vp, vp,
+                             ctypes.c_double, vp, ctypes.c_int, ctypes.c_double, _c_int_p, _c_double_p]
     L.pgert_potentials.argtypes = [vp, vp]
     L.pgert_potentials_node_major.argtypes = [vp, vp]
     L.pgert_jacobian.argtypes = [vp, vp, ctypes.c_int, vp, ctypes.c_int]

@@ -712,6 +712,27 @@ int pgert_batch_jacobian(pgert_ctx * const * ctxs, int count, const double * con
     });
 }
 
+int pgert_set_constraints(pgert_ctx * c, int n_const, int n_model, const int * rowptr, const int * colidx,
+                          const double * vals) {
+    return guarded(c, [&] { set_constraints(*c, n_const, n_model, rowptr, colidx, vals); });
+}
+
+int pgert_cgls(pgert_ctx * c, const void * J, int dtype, int n_data, int n_model, const double * d_weight,
+               const double * b, double * x, const double * wc, const double * wm, const double * tm,
+               const double * td, double lambda, const double * roughness, int max_iter, double tol,
+               int * iterations, double * norm_r2) {
+    return guarded(c, [&] {
+        PG_REQUIRE(J && (dtype == 0 || dtype == 1) && n_data > 0 && n_model > 0, "pgert_cgls: bad Jacobian");
+        PG_REQUIRE(d_weight && b && x && wm && tm && td, "pgert_cgls: null vector");
+        PG_REQUIRE(c->nConst == 0 || (wc && roughness), "pgert_cgls: constraint weights / roughness missing");
+        PG_REQUIRE(max_iter >= 0, "pgert_cgls: bad iteration limit");
+        const int it = cgls(*c, J, dtype, n_data, n_model, d_weight, b, x, wc, wm, tm, td, lambda, roughness,
+                            max_iter, tol);
+        if (iterations) *iterations = it;
+        if (norm_r2) *norm_r2 = c->cglsNormR2;
+    });
+}
+
 int pgert_mlp_param_count(void) { return mlp_param_count(); }
 
 int pgert_mlp_set_params(pgert_ctx * c, const float * params, int count) {

@@ -142,6 +142,12 @@ struct Ctx {
 
     std::shared_ptr<MlpState> mlp;  // learned warm start (mlp.hip)
 
+    // constraint matrix C of the Gauss-Newton step and its transpose, CSR (cgls.hip)
+    int nConst = 0, cCols = 0;
+    DBuf<int> cRowptr, cColidx, ctRowptr, ctColidx;
+    DBuf<double> cVals, ctVals;
+    double cglsNormR2 = 0.0;  // |r|^2 at the end of the last pgert_cgls
+
     // Jacobian launch descriptors (jacobian.hip), kept until the next launch
     std::vector<unsigned char> jdescHost;
     DBuf<unsigned char> jdesc;
@@ -216,6 +222,12 @@ void launch_jacobian(Ctx & c, const double * model, int nModel, void * J, int dt
 // (same device and dimension; every context uses cs[0]->stream)
 void launch_jacobian_batch(const std::vector<Ctx *> & cs, const std::vector<const double *> & models,
                            const std::vector<int> & nModels, const std::vector<void *> & Js, int dtype);
+
+// Gauss-Newton step (cgls.hip)
+void set_constraints(Ctx & c, int nConst, int nModel, const int * rowptr, const int * colidx, const double * vals);
+int cgls(Ctx & c, const void * J, int dtype, int nData, int nModel, const double * dW, const double * b, double * x,
+         const double * wc, const double * wm, const double * tm, const double * td, double lambda,
+         const double * roughness, int maxIter, double tol);
 
 // solvers (solver.hip); return per-solve statistics
 void solve_pcg_fp64(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st);

@@ -327,3 +327,39 @@ def test_port_pcg_matches_direct_solve():
     ref = spla.spsolve(A.tocsc(), B)
     assert 0 < it < 5000
     assert np.linalg.norm(X - ref) / np.linalg.norm(ref) < 1e-9
+
+
+def _cgls_problem(seed, D=40, P=25, nC=30):
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    S = rng.standard_normal((D, P))
+    C = sp.random(nC, P, density=0.12, random_state=seed, format="csr") + sp.eye(nC, P, format="csr")
+    vec = dict(dW=rng.uniform(0.5, 2.0, D), b=rng.standard_normal(D), wc=rng.uniform(0.5, 1.5, nC),
+               wm=rng.uniform(0.5, 1.5, P), tm=rng.uniform(0.5, 2.0, P), td=rng.uniform(0.5, 2.0, D),
+               roughness=0.1 * rng.standard_normal(nC))
+    return S, C, vec
+
+
+def test_cgls_oracle_matches_normal_equations():
+    """The CGLS restatement (solveCGLSCDWWhtrans) converges to the solution of
+    the normal equations derived by hand from its objective; the reference's
+    default stopping rule (tol < 0) stops it early with a small gradient."""
+    from oracle import cgls as cg
+    S, C, v = _cgls_problem(0)
+    lam = 3.0
+    x0 = np.zeros(S.shape[1])
+    # (past |r|^2 ~ 1e-24 |r0|^2 the 1e-10 rounding of C (p wm) dominates and CG breaks down)
+    x, it, hist = cg.cgls(S, C, v["dW"], v["b"], x0, v["wc"], v["wm"], v["tm"], v["td"], lam, v["roughness"],
+                          max_iter=500, tol=1e-20)
+    xn = cg.normal_equations(S, C, v["dW"], v["b"], v["wc"], v["wm"], v["tm"], v["td"], lam, v["roughness"])
+    assert np.linalg.norm(x - xn) / np.linalg.norm(xn) < 1e-8
+    # a plausible mistake (dropping the roughness term) fails the same check
+    xr = cg.normal_equations(S, C, v["dW"], v["b"], v["wc"], v["wm"], v["tm"], v["td"], lam, 0.0 * v["roughness"])
+    assert np.linalg.norm(xr - xn) / np.linalg.norm(xn) > 1e-4
+    x2, it2, hist2 = cg.cgls(S, C, v["dW"], v["b"], x0, v["wc"], v["wm"], v["tm"], v["td"], lam, v["roughness"],
+                             max_iter=500, tol=-1.0)
+    assert it2 < it and hist2[-1] <= max(1e-12, 1e-8 * hist2[0]) * 10
+    # starting from the solution, no iteration is needed beyond rounding
+    x3, it3, _ = cg.cgls(S, C, v["dW"], v["b"], xn, v["wc"], v["wm"], v["tm"], v["td"], lam, v["roughness"],
+                         max_iter=5, tol=1e-20)
+    assert np.linalg.norm(x3 - xn) / np.linalg.norm(xn) < 1e-10
