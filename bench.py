@@ -434,6 +434,34 @@ def survey_large_leg(args, log):
     plain = run("pcg64", 1, 0, 0)
     plain_block = run("block64", 1, 0, 0)
     best = run(args.method, args.steps, args.warmup, args.coarse)
+    # one Gauss-Newton step on the Jacobian in HBM (solveCGLSCDWWhtrans):
+    # log-model / log-data transformation, 3 % data error, first-order smoothness
+    from pygimli_amd.inversion import smoothness_constraints, solveCGLSCDWWhtrans
+    fop.solverOptions.update(method=args.method, rtol=args.rtol, coarse=args.coarse)
+    resp = fop.response(model)
+    fop.createJacobian(model)
+    Jd = fop.jacobian()
+    Cm = smoothness_constraints(mesh)
+    rng = np.random.default_rng(3)
+    dobs = resp * (1.0 + 0.03 * rng.standard_normal(len(resp)))
+    gn = dict(dW=np.full(len(resp), 1.0 / 0.03), b=np.log(dobs) - np.log(resp), tm=1.0 / rho, td=1.0 / resp,
+              wc=np.ones(Cm.shape[0]), wm=np.ones(len(rho)), rough=Cm @ np.log(rho))
+    solveCGLSCDWWhtrans(Jd, Cm, gn["dW"], gn["b"], np.zeros(len(rho)), gn["wc"], gn["wm"], gn["tm"], gn["td"], 20.0,
+                        gn["rough"], maxIter=2, tol=0.0)  # set-up: constraints to the device
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    _, cg_it = solveCGLSCDWWhtrans(Jd, Cm, gn["dW"], gn["b"], np.zeros(len(rho)), gn["wc"], gn["wm"], gn["tm"],
+                                   gn["td"], 20.0, gn["rough"], maxIter=args.cgls_iters, tol=0.0)
+    torch.cuda.synchronize()
+    cg_s = time.perf_counter() - t
+    jb = Jd.numel() * Jd.element_size()
+    cgls = {"iterations": cg_it, "seconds": cg_s, "ms_per_iteration": cg_s / max(cg_it, 1) * 1e3,
+            "constraints": int(Cm.shape[0]),
+            "jacobian_GB_per_s": 2.0 * jb * cg_it / cg_s / 1e9,
+            "note": "each iteration reads J twice (J u and J' v); vectors FP64"}
+    log(f"[large] CGLS: {cgls}")
+    del Jd
+    fop._J = None
     n, C, D = mesh.nodeCount(), mesh.cellCount(), data.size()
     nE = len(data.sensorPositions())
     # Jacobian compulsory bytes: J written once, potentials read once, cells + geometry read once
@@ -460,6 +488,7 @@ def survey_large_leg(args, log):
                                  "potential_rel_err_fro": plain_block["pot_err"]},
         "speedup_vs_plain_fp64_block_pcg": plain_block["ms"] / best["ms"],
         "speedup_vs_plain_fp64_pcg": plain["ms"] / best["ms"],
+        "gauss_newton_cgls": cgls,
     }
 
 
@@ -490,6 +519,7 @@ def main():
     ap.add_argument("--learned-samples", type=int, default=1 << 20, help="training samples per field")
     ap.add_argument("--learned-batch", type=int, default=16384)
     ap.add_argument("--learned-epochs", type=int, default=3)
+    ap.add_argument("--cgls-iters", type=int, default=20, help="CGLS iterations of the Gauss-Newton step timed")
     ap.add_argument("--survey-large", action="store_true",
                     help="run only the large 3D survey leg (256 electrodes, 10000 data, 1.5M tets, 60 GB Jacobian)")
     ap.add_argument("--copy-calibration", action="store_true",
