@@ -41,11 +41,18 @@ __device__ inline float bf2f(unsigned short h) {
     memcpy(&f, &u, 4);
     return f;
 }
+// tanh(y) = 1 - 2 / (e^{2y} + 1) with the hardware exponential and
+// reciprocal (relative error ~1e-7 in e^{2y}; absolute error ~1e-7 in the
+// result, far below the BF16 activations it feeds); exact limits +-1
+__device__ inline float fast_tanh(float y) {
+    const float e = __expf(2.0f * y);
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+}
 __device__ inline float gelu(float x) {
-    return 0.5f * x * (1.0f + tanhf(C_GELU * (x + 0.044715f * x * x * x)));
+    return 0.5f * x * (1.0f + fast_tanh(C_GELU * (x + 0.044715f * x * x * x)));
 }
 __device__ inline float gelu_grad(float x) {
-    float t = tanhf(C_GELU * (x + 0.044715f * x * x * x));
+    float t = fast_tanh(C_GELU * (x + 0.044715f * x * x * x));
     return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * C_GELU * (1.0f + 3.0f * 0.044715f * x * x);
 }
 
@@ -154,33 +161,72 @@ struct MlpW {
     const float * prm;           // FP32 master (biases)
 };
 
-__device__ inline v4s load4(const unsigned short * p) {
-    v4s r;
-    r.x = (short)p[0]; r.y = (short)p[1]; r.z = (short)p[2]; r.w = (short)p[3];
-    return r;
+// four consecutive bf16 (8-byte aligned: every fragment offset is a multiple of 4 elements)
+__device__ inline v4s load4(const unsigned short * p) { return *reinterpret_cast<const v4s *>(p); }
+
+// Network weights staged in LDS once per workgroup (the forward kernels run
+// 8 wavefronts over many 16-sample tiles): W^T rows padded by WPAD bf16 so the
+// 16 rows of a B fragment fall into different LDS banks.
+constexpr int WPAD = 8, KW1 = NF + WPAD, KW2 = NH + WPAD;
+constexpr int MLP_WAVES = 8;
+constexpr size_t MLP_W_LDS = (size_t)(NH * KW1 + 2 * NH * KW2) * 2 + 3 * NH * 4 + NH * 2;
+
+struct MlpL {
+    const unsigned short *w1t, *w2t, *w3t, *w4;
+    const float *b1, *b2, *b3;
+    float b4;
+};
+
+__device__ MlpL stage_weights(const MlpW & w, unsigned char * smem) {
+    unsigned short * s1 = (unsigned short *)smem;
+    unsigned short * s2 = s1 + NH * KW1;
+    unsigned short * s3 = s2 + NH * KW2;
+    float * bb = (float *)(s3 + NH * KW2);
+    unsigned short * s4 = (unsigned short *)(bb + 3 * NH);
+    const auto g1 = glob(w.w1t), g2 = glob(w.w2t), g3 = glob(w.w3t), g4 = glob(w.w4);
+    const auto gp = glob(w.prm);
+    for (int t = threadIdx.x; t < NH * NF; t += blockDim.x) s1[(t / NF) * KW1 + t % NF] = g1[t];
+    for (int t = threadIdx.x; t < NH * NH; t += blockDim.x) {
+        s2[(t / NH) * KW2 + t % NH] = g2[t];
+        s3[(t / NH) * KW2 + t % NH] = g3[t];
+    }
+    for (int t = threadIdx.x; t < NH; t += blockDim.x) {
+        bb[t] = gp[OB1 + t];
+        bb[NH + t] = gp[OB2 + t];
+        bb[2 * NH + t] = gp[OB3 + t];
+        s4[t] = g4[t];
+    }
+    __syncthreads();
+    return MlpL{s1, s2, s3, s4, bb, bb + NH, bb + 2 * NH, gp[OB4]};
 }
 
-// dense layer: act (16 x K bf16 in LDS, row stride K) x W^T -> 16 x NH, + bias,
-// optional GELU; writes bf16 activations to `out` (16 x NH) and, when zsave is
-// given, the FP32 pre-activations
-template <int K>
+// dense layer: act (16 x K bf16 in LDS, row stride K) x W^T (LDS, row stride KW)
+// -> 16 x NH, + bias, optional GELU; writes bf16 activations to `out` (16 x NH)
+// and, when zsave is given, the FP32 pre-activations
+template <int K, int KW>
 __device__ void dense(const unsigned short * act, const unsigned short * wt, const float * bias, bool act_gelu,
                       unsigned short * out, float * zsave, int lane) {
     const int r16 = lane % 16, q = lane / 16;
     v4f acc[8];
+#pragma unroll
     for (int t = 0; t < 8; ++t) acc[t] = (v4f){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
     for (int s = 0; s < K / 16; ++s) {
         v4s a = load4(act + r16 * K + 16 * s + 4 * q);
+#pragma unroll
         for (int t = 0; t < 8; ++t) {
-            v4s b = load4(wt + (size_t)(16 * t + r16) * K + 16 * s + 4 * q);
+            v4s b = load4(wt + (16 * t + r16) * KW + 16 * s + 4 * q);
             acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
         }
     }
+#pragma unroll
     for (int t = 0; t < 8; ++t) {
         const int col = 16 * t + r16;
+        const float bc = bias[col];
+#pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int row = 4 * q + j;
-            float z = acc[t][j] + bias[col];
+            float z = acc[t][j] + bc;
             if (zsave) zsave[row * NH + col] = z;
             out[row * NH + col] = f2bf(act_gelu ? gelu(z) : z);
         }
@@ -189,16 +235,16 @@ __device__ void dense(const unsigned short * act, const unsigned short * wt, con
 
 // Forward of a 16-sample tile whose bf16 inputs are in x0 (16 x NF, LDS).
 // h1/h2 scratch are 16 x NH bf16 LDS buffers.  Returns y for sample lane (lanes 0..15).
-__device__ float mlp_tile(const MlpW & w, unsigned short * x0, unsigned short * h1, unsigned short * h2, int lane,
+__device__ float mlp_tile(const MlpL & w, unsigned short * x0, unsigned short * h1, unsigned short * h2, int lane,
                           float * z1, float * z2, float * z3, unsigned short * a1, unsigned short * a2,
                           unsigned short * a3) {
-    dense<NF>(x0, w.w1t, w.prm + OB1, true, h1, z1, lane);
+    dense<NF, KW1>(x0, w.w1t, w.b1, true, h1, z1, lane);
     __builtin_amdgcn_wave_barrier();
-    dense<NH>(h1, w.w2t, w.prm + OB2, true, h2, z2, lane);
+    dense<NH, KW2>(h1, w.w2t, w.b2, true, h2, z2, lane);
     __builtin_amdgcn_wave_barrier();
     if (a1)
         for (int t = lane; t < 16 * NH; t += 64) a1[t] = h1[t];
-    dense<NH>(h2, w.w3t, w.prm + OB3, true, h1, z3, lane);
+    dense<NH, KW2>(h2, w.w3t, w.b3, true, h1, z3, lane);
     __builtin_amdgcn_wave_barrier();
     if (a2)
         for (int t = lane; t < 16 * NH; t += 64) a2[t] = h2[t];
@@ -207,19 +253,22 @@ __device__ float mlp_tile(const MlpW & w, unsigned short * x0, unsigned short * 
     float y = 0.f;
     if (lane < 16) {
         for (int c = 0; c < NH; ++c) y += bf2f(h1[lane * NH + c]) * bf2f(w.w4[c]);
-        y += w.prm[OB4];
+        y += w.b4;
     }
     return y;
 }
 
 // Inference over every (node, source) sample of one wavenumber: features on
 // the fly, MLP, and X0 = t_hat rho_s p written in the solver's node-major layout.
-__global__ void __launch_bounds__(256) k_mlp_warmstart(long nsamp, FeatArgs a, MlpW w, double * __restrict__ X0) {
-    __shared__ unsigned short xs[4][16 * NF];
-    __shared__ unsigned short hA[4][16 * NH], hB[4][16 * NH];
+__global__ void __launch_bounds__(64 * MLP_WAVES) k_mlp_warmstart(long nsamp, FeatArgs a, MlpW w,
+                                                                  double * __restrict__ X0) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    __shared__ __attribute__((aligned(16))) unsigned short xs[MLP_WAVES][16 * NF];
+    __shared__ __attribute__((aligned(16))) unsigned short hA[MLP_WAVES][16 * NH], hB[MLP_WAVES][16 * NH];
+    const MlpL wl = stage_weights(w, smem);
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const long ntile = (nsamp + 15) / 16;
-    for (long tile = (long)blockIdx.x * 4 + wave; tile < ntile; tile += (long)gridDim.x * 4) {
+    for (long tile = (long)blockIdx.x * MLP_WAVES + wave; tile < ntile; tile += (long)gridDim.x * MLP_WAVES) {
         const long s0 = tile * 16;
         if (lane < 16) {
             float f[NF];
@@ -230,7 +279,7 @@ __global__ void __launch_bounds__(256) k_mlp_warmstart(long nsamp, FeatArgs a, M
             for (int q = 0; q < NF; ++q) xs[wave][lane * NF + q] = f2bf(f[q]);
         }
         __builtin_amdgcn_wave_barrier();
-        float y = mlp_tile(w, xs[wave], hA[wave], hB[wave], lane, nullptr, nullptr, nullptr, nullptr, nullptr,
+        float y = mlp_tile(wl, xs[wave], hA[wave], hB[wave], lane, nullptr, nullptr, nullptr, nullptr, nullptr,
                            nullptr);
         if (lane < 16) {
             long s = s0 + lane;
@@ -245,14 +294,16 @@ __global__ void __launch_bounds__(256) k_mlp_warmstart(long nsamp, FeatArgs a, M
 
 // Forward on given features (training / tests); optionally stores the
 // activations needed by the backward pass.
-__global__ void __launch_bounds__(256) k_mlp_forward(int nsamp, const float * __restrict__ feat, MlpW w,
-                                                     float * __restrict__ y, float * __restrict__ Z,
-                                                     unsigned short * __restrict__ Aact) {
-    __shared__ unsigned short xs[4][16 * NF];
-    __shared__ unsigned short hA[4][16 * NH], hB[4][16 * NH];
+__global__ void __launch_bounds__(64 * MLP_WAVES) k_mlp_forward(int nsamp, const float * __restrict__ feat, MlpW w,
+                                                                float * __restrict__ y, float * __restrict__ Z,
+                                                                unsigned short * __restrict__ Aact) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    __shared__ __attribute__((aligned(16))) unsigned short xs[MLP_WAVES][16 * NF];
+    __shared__ __attribute__((aligned(16))) unsigned short hA[MLP_WAVES][16 * NH], hB[MLP_WAVES][16 * NH];
+    const MlpL wl = stage_weights(w, smem);
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const int ntile = (nsamp + 15) / 16;
-    for (int tile = blockIdx.x * 4 + wave; tile < ntile; tile += gridDim.x * 4) {
+    for (int tile = blockIdx.x * MLP_WAVES + wave; tile < ntile; tile += gridDim.x * MLP_WAVES) {
         const int s0 = tile * 16;
         for (int t = lane; t < 16 * NF; t += 64) {
             int s = s0 + t / NF;
@@ -262,7 +313,7 @@ __global__ void __launch_bounds__(256) k_mlp_forward(int nsamp, const float * __
         // Z: [3][nsamp][NH] FP32 pre-activations; Aact: [4][nsamp][NH] bf16
         // inputs of each layer (layer 0 uses the first NF columns)
         const bool keep = Z != nullptr && s0 + 16 <= nsamp;
-        float yy = mlp_tile(w, xs[wave], hA[wave], hB[wave], lane,
+        float yy = mlp_tile(wl, xs[wave], hA[wave], hB[wave], lane,
                             keep ? Z + (size_t)s0 * NH : nullptr,
                             keep ? Z + ((size_t)nsamp + s0) * NH : nullptr,
                             keep ? Z + ((size_t)2 * nsamp + s0) * NH : nullptr,
@@ -288,7 +339,7 @@ __global__ void __launch_bounds__(256) k_mlp_backward(int nsamp, const float * _
                                                       const unsigned short * __restrict__ w3,
                                                       const unsigned short * __restrict__ w4,
                                                       float * __restrict__ G, float * __restrict__ g4) {
-    __shared__ unsigned short gs[4][16 * NH];
+    __shared__ __attribute__((aligned(16))) unsigned short gs[4][16 * NH];
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const int r16 = lane % 16, q = lane / 16;
     const int ntile = nsamp / 16;  // the caller passes a multiple of 16
@@ -338,9 +389,9 @@ __global__ void __launch_bounds__(256) k_mlp_backward(int nsamp, const float * _
 constexpr int KSPLIT = 64;
 __global__ void __launch_bounds__(256) k_mlp_wgrad(int nsamp, const unsigned short * __restrict__ Aact,
                                                    const float * __restrict__ G, const float * __restrict__ g4,
-                                                   float * __restrict__ part) {
-    // blockIdx.x: output row block of 8 (in-features), blockIdx.y: split, blockIdx.z: layer (0..3)
-    const int layer = blockIdx.z;
+                                                   float * __restrict__ part, int layer0) {
+    // blockIdx.x: output row block of 8 (in-features), blockIdx.y: split, blockIdx.z: layer - layer0
+    const int layer = layer0 + blockIdx.z;
     const int nin = layer == 0 ? NF : NH;
     const int nout = layer == 3 ? 1 : NH;
     const int i0 = blockIdx.x * 8;
@@ -362,6 +413,72 @@ __global__ void __launch_bounds__(256) k_mlp_wgrad(int nsamp, const unsigned sho
     for (int r = 0; r < 4; ++r) {
         int i = i0 + ii + 2 * r;
         if (i <= nin && j < nout) P[(size_t)i * NH + j] = acc[r];
+    }
+}
+
+// Weight gradients of layers 0..2 on the matrix cores (FP32 MFMA 16x16x4:
+// A fragment = activations^T (in-feature x sample), B fragment = gz (sample x
+// out-feature), K = samples).  blockIdx.x: split of the samples, blockIdx.y:
+// layer.  Chunks of 64 samples are staged in LDS (activations widened to
+// FP32, the bias column as 1.0); each wavefront owns a quarter of the 16 x 16
+// output tiles.  Same partial layout as k_mlp_wgrad.
+constexpr int WG_CHUNK = 64, WG_LD = 144, WG_TILES = 18;  // 9 x 8 tiles / 4 wavefronts
+typedef float v4f_ __attribute__((ext_vector_type(4)));
+__global__ void __launch_bounds__(256) k_mlp_wgrad_mfma(int nsamp, const unsigned short * __restrict__ Aact,
+                                                        const float * __restrict__ G, float * __restrict__ part) {
+    __shared__ float As[WG_CHUNK * WG_LD], Gs[WG_CHUNK * WG_LD];
+    const int layer = blockIdx.y;
+    const int nin = layer == 0 ? NF : NH;
+    const int TI = (nin + 1 + 15) / 16, ntile = TI * 8;
+    const int s_lo = (int)((long)nsamp * blockIdx.x / KSPLIT), s_hi = (int)((long)nsamp * (blockIdx.x + 1) / KSPLIT);
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    v4f_ acc[WG_TILES];
+#pragma unroll
+    for (int k = 0; k < WG_TILES; ++k) acc[k] = (v4f_){0.f, 0.f, 0.f, 0.f};
+    const auto A = glob(Aact) + (size_t)layer * nsamp * NH;
+    const auto Gl = glob(G) + (size_t)layer * nsamp * NH;
+    for (int c0 = s_lo; c0 < s_hi; c0 += WG_CHUNK) {
+        const int cn = min(WG_CHUNK, s_hi - c0);
+        __syncthreads();
+        for (int t = threadIdx.x; t < WG_CHUNK * NH; t += 256) {
+            const int r = t / NH, col = t % NH;
+            const bool in = r < cn;
+            const int s = c0 + (in ? r : 0);
+            const float a = bf2f(A[(size_t)s * NH + col]);
+            const float g = Gl[(size_t)s * NH + col];
+            As[r * WG_LD + col] = !in ? 0.f : col < nin ? a : col == nin ? 1.f : 0.f;  // column nin: bias
+            Gs[r * WG_LD + col] = in ? g : 0.f;
+        }
+        if (nin == NH)  // the bias column lies beyond the loaded columns
+            for (int r = threadIdx.x; r < WG_CHUNK; r += 256) As[r * WG_LD + NH] = r < cn ? 1.f : 0.f;
+        __syncthreads();
+#pragma unroll 4
+        for (int ks = 0; ks < WG_CHUNK / 4; ++ks) {
+            const int kr = 4 * ks + lane / 16;
+#pragma unroll
+            for (int k = 0; k < WG_TILES; ++k) {
+                const int tile = wave * WG_TILES + k;
+                if (tile < ntile) {
+                    const int ti = tile / 8, tj = tile % 8;
+                    const float a = As[kr * WG_LD + ti * 16 + lane % 16];
+                    const float b = Gs[kr * WG_LD + tj * 16 + lane % 16];
+                    acc[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[k], 0, 0, 0);
+                }
+            }
+        }
+    }
+    float * P = part + ((size_t)layer * KSPLIT + blockIdx.x) * (NH + 1) * NH;
+#pragma unroll
+    for (int k = 0; k < WG_TILES; ++k) {
+        const int tile = wave * WG_TILES + k;
+        if (tile < ntile) {
+            const int ti = tile / 8, tj = tile % 8;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = ti * 16 + 4 * (lane / 16) + r, j = tj * 16 + lane % 16;
+                if (i <= nin) P[(size_t)i * NH + j] = acc[k][r];
+            }
+        }
     }
 }
 
@@ -439,6 +556,14 @@ static MlpState & state(Ctx & c) {
 int mlp_param_count() { return NPARAM; }
 
 static MlpW weights(MlpState & m) {
+    static bool attr = false;  // the forward kernels stage the weights in dynamic LDS
+    if (!attr) {
+        PG_CHECK(hipFuncSetAttribute((const void *)k_mlp_warmstart, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)MLP_W_LDS));
+        PG_CHECK(hipFuncSetAttribute((const void *)k_mlp_forward, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)MLP_W_LDS));
+        attr = true;
+    }
     return MlpW{m.w1t.p, m.w2t.p, m.w3t.p, m.w4.p, m.prm.p};
 }
 
@@ -514,7 +639,8 @@ void mlp_warm_start(Ctx & c) {
     for (int kk = 0; kk < c.nK; ++kk) {
         FeatArgs a = feat_args(c, kk);
         long nsamp = (long)c.n * c.nE;
-        hipLaunchKernelGGL(k_mlp_warmstart, dim3(2048), dim3(256), 0, c.stream, nsamp, a, weights(m),
+        hipLaunchKernelGGL(k_mlp_warmstart, dim3(std::max(1L, std::min(512L, (nsamp + 127) / 128))),
+                           dim3(64 * MLP_WAVES), MLP_W_LDS, c.stream, nsamp, a, weights(m),
                            c.X.p + (size_t)kk * c.n * c.nE);
         PG_CHECK(hipGetLastError());
     }
@@ -531,8 +657,9 @@ void mlp_collect(Ctx & c, int kk, const long * idx, long count, float * feat, fl
 void mlp_predict(Ctx & c, const float * feat, int nsamp, float * out) {
     MlpState & m = state(c);
     PG_REQUIRE(m.ready, "mlp: no parameters");
-    hipLaunchKernelGGL(k_mlp_forward, dim3(std::max(1, std::min(1024, div_up(nsamp, 64)))), dim3(256), 0, c.stream,
-                       nsamp, feat, weights(m), out, (float *)nullptr, (unsigned short *)nullptr);
+    hipLaunchKernelGGL(k_mlp_forward, dim3(std::max(1, std::min(512, div_up(nsamp, 16 * MLP_WAVES)))),
+                       dim3(64 * MLP_WAVES), MLP_W_LDS, c.stream, nsamp, feat, weights(m), out, (float *)nullptr,
+                       (unsigned short *)nullptr);
     PG_CHECK(hipGetLastError());
 }
 
@@ -550,12 +677,13 @@ float mlp_train_step(Ctx & c, const float * feat, const float * target, int nsam
     m.part.alloc((size_t)4 * KSPLIT * (NH + 1) * NH);
     PG_CHECK(hipMemsetAsync(m.part.p, 0, m.part.bytes(), c.stream));
     const int grid = std::max(1, std::min(1024, div_up(nsamp, 64)));
-    hipLaunchKernelGGL(k_mlp_forward, dim3(grid), dim3(256), 0, c.stream, nsamp, feat, weights(m), m.y.p, m.Z.p,
-                       m.A.p);
+    hipLaunchKernelGGL(k_mlp_forward, dim3(std::max(1, std::min(512, div_up(nsamp, 16 * MLP_WAVES)))),
+                       dim3(64 * MLP_WAVES), MLP_W_LDS, c.stream, nsamp, feat, weights(m), m.y.p, m.Z.p, m.A.p);
     hipLaunchKernelGGL(k_mlp_backward, dim3(grid), dim3(256), 0, c.stream, nsamp, m.y.p, target, m.Z.p, m.w2.p,
                        m.w3.p, m.w4.p, m.G.p, m.g4.p);
-    hipLaunchKernelGGL(k_mlp_wgrad, dim3((NH + 1 + 7) / 8, KSPLIT, 4), dim3(256), 0, c.stream, nsamp, m.A.p, m.G.p,
-                       m.g4.p, m.part.p);
+    hipLaunchKernelGGL(k_mlp_wgrad_mfma, dim3(KSPLIT, 3), dim3(256), 0, c.stream, nsamp, m.A.p, m.G.p, m.part.p);
+    hipLaunchKernelGGL(k_mlp_wgrad, dim3((NH + 1 + 7) / 8, KSPLIT, 1), dim3(256), 0, c.stream, nsamp, m.A.p, m.G.p,
+                       m.g4.p, m.part.p, 3);
     hipLaunchKernelGGL(k_mlp_gather_grad, dim3(div_up(NPARAM, 256)), dim3(256), 0, c.stream, m.part.p, m.grad.p);
     PG_CHECK(hipGetLastError());
     if (apply) {

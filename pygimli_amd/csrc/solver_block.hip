@@ -134,13 +134,13 @@ __global__ void __launch_bounds__(256) k_spmm(const BlockDesc * __restrict__ blo
     const BlockDesc bd = blocks[b];
     if (!bd.active) return;
     const int n = bd.n;
-    const int * __restrict__ rowptr = bd.rowptr;
-    const int * __restrict__ colidx = bd.colidx;
+    const auto rowptr = glob(bd.rowptr);
+    const auto colidx = glob(bd.colidx);
     const int lane = threadIdx.x % 64;
     const VT * Pb = P + bd.off + 4 * (lane % 16);
     VT * Qb = Q + bd.off + 4 * (lane % 16);
     static_assert(sizeof(AT) == 8, "the matrix stays FP64");
-    const AT * __restrict__ A = bd.valsD;
+    const auto A = glob(bd.valsD);
     const int x = blockIdx.x % NXCD, nw = gridDim.x / NXCD;
     const int lo = (int)((long)n * x / NXCD), hi = (int)((long)n * (x + 1) / NXCD);
     const int g0 = (blockIdx.x / NXCD) * 4 + threadIdx.x / 64;
@@ -296,10 +296,10 @@ __global__ void __launch_bounds__(256) k_zform(const BlockDesc * __restrict__ bl
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
         const int row = (int)(t / (S / 4)), c4 = 4 * (int)(t % (S / 4));
         const size_t o = bd.off + (size_t)row * S + c4;
-        const double d = bd.dinv[row];
+        const double d = glob(bd.dinv)[row];
         const V4<VT> r = ld4(R + o);
         f64x4 cv = {0, 0, 0, 0};
-        if (cc && bd.m > 0) cv = ld4(cc + bd.coff + (size_t)bd.agg[row] * S + c4);
+        if (cc && bd.m > 0) cv = ld4(cc + bd.coff + (size_t)glob(bd.agg)[row] * S + c4);
         V4<VT> z;
 #pragma unroll
         for (int e = 0; e < 4; ++e) z[e] = (VT)(d * (double)r[e] + cv[e]);
@@ -516,7 +516,9 @@ __global__ void __launch_bounds__(256) k_restrict(const BlockDesc * __restrict__
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const VT * Rb = R + bd.off + lane;
     double s = 0.0;
-    for (int q = bd.aggPtr[I] + wave; q < bd.aggPtr[I + 1]; q += 4) s += (double)Rb[(size_t)bd.aggNodes[q] * S];
+    const auto aggPtr = glob(bd.aggPtr);
+    const auto aggNodes = glob(bd.aggNodes);
+    for (int q = aggPtr[I] + wave; q < aggPtr[I + 1]; q += 4) s += (double)Rb[(size_t)aggNodes[q] * S];
     cn[wave][lane] = s;
     __syncthreads();
     if (wave == 0) y[bd.coff + (size_t)I * S + lane] = cn[0][lane] + cn[1][lane] + cn[2][lane] + cn[3][lane];
@@ -761,9 +763,9 @@ __global__ void __launch_bounds__(256) k_bresid(const BlockDesc * __restrict__ b
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const int n = bd.n;
     const size_t vb = bd.off + lane;
-    const double * __restrict__ A = bd.valsD;
-    const int * __restrict__ rowptr = bd.rowptr;
-    const int * __restrict__ colidx = bd.colidx;
+    const auto A = glob(bd.valsD);
+    const auto rowptr = glob(bd.rowptr);
+    const auto colidx = glob(bd.colidx);
     int lo, hi, first, stride;
     xcd_rows(n, lo, hi, first, stride);
     double rr = 0.0, bb = 0.0;
