@@ -195,7 +195,8 @@ __global__ void __launch_bounds__(256) k_gram(const BlockDesc * __restrict__ blo
     constexpr bool F32 = sizeof(VT) == 4;
     constexpr int DEPTH = F32 ? (MODE == MODE_Z ? 12 : 16) : 8;  // steps in flight per wavefront
     using V = V4<VT>;
-    __shared__ V sv[2][3][64];
+    constexpr int GS = 4;          // 4-row steps per LDS exchange
+    __shared__ V sv[2][GS][3][64];  // [buffer][step of the group][vector][lane]
     const int b = blockIdx.y;
     if (!blocks[b].active) return;
     const int n = blocks[b].n;
@@ -235,35 +236,43 @@ __global__ void __launch_bounds__(256) k_gram(const BlockDesc * __restrict__ blo
     V ring[DEPTH];
 #pragma unroll
     for (int i = 0; i < DEPTH; ++i) ring[i] = fetch(st0 + i);
+    // GS 4-row steps per LDS exchange (one barrier per group)
+    static_assert(DEPTH % GS == 0 && FLUSH % GS == 0, "steps are exchanged in groups");
     for (int s0 = st0; s0 < st1; s0 += DEPTH) {
 #pragma unroll
-        for (int i = 0; i < DEPTH; ++i) {
-            const int buf = i & 1;
-            if (wave < 3) sv[buf][wave][lane] = ring[i];
-            ring[i] = fetch(s0 + i + DEPTH);
+        for (int i = 0; i < DEPTH; i += GS) {
+            const int buf = (i / GS) & 1;
+            if (wave < 3)
+#pragma unroll
+                for (int h = 0; h < GS; ++h) sv[buf][h][wave][lane] = ring[i + h];
+#pragma unroll
+            for (int h = 0; h < GS; ++h) ring[i + h] = fetch(s0 + i + h + DEPTH);
             lds_barrier();
-            const V x0 = sv[buf][0][lane], x1 = sv[buf][1][lane], x2 = sv[buf][2][lane];
-            V a[NG], bo[NG];
-            if constexpr (MODE == MODE_PQR) {
-                a[0] = x0;  // P'Q
-                a[1] = x0;  // P'R
-                bo[0] = x1;
-                bo[1] = x2;
-            } else {
-                a[0] = x1;  // Q'Z
-                a[1] = x2;  // Z'Z
-                a[2] = x0;  // P'Z
-                bo[0] = x2;
-                bo[1] = x2;
-                bo[2] = x2;
-            }
 #pragma unroll
-            for (int g = 0; g < NG; ++g) {
-                const VT bw = wave == 0 ? bo[g][0] : wave == 1 ? bo[g][1] : wave == 2 ? bo[g][2] : bo[g][3];
+            for (int h = 0; h < GS; ++h) {
+                const V x0 = sv[buf][h][0][lane], x1 = sv[buf][h][1][lane], x2 = sv[buf][h][2][lane];
+                V a[NG], bo[NG];
+                if constexpr (MODE == MODE_PQR) {
+                    a[0] = x0;  // P'Q
+                    a[1] = x0;  // P'R
+                    bo[0] = x1;
+                    bo[1] = x2;
+                } else {
+                    a[0] = x1;  // Q'Z
+                    a[1] = x2;  // Z'Z
+                    a[2] = x0;  // P'Z
+                    bo[0] = x2;
+                    bo[1] = x2;
+                    bo[2] = x2;
+                }
 #pragma unroll
-                for (int ta = 0; ta < 4; ++ta) acc[g][ta] = mfma4(a[g][ta], bw, acc[g][ta]);
+                for (int g = 0; g < NG; ++g) {
+                    const VT bw = wave == 0 ? bo[g][0] : wave == 1 ? bo[g][1] : wave == 2 ? bo[g][2] : bo[g][3];
+#pragma unroll
+                    for (int ta = 0; ta < 4; ++ta) acc[g][ta] = mfma4(a[g][ta], bw, acc[g][ta]);
+                }
             }
-            if ((i % FLUSH) == FLUSH - 1) flush();
+            if (((i + GS - 1) % FLUSH) == FLUSH - 1) flush();
         }
     }
     flush();
