@@ -507,7 +507,7 @@ def main():
                     help="random fields the learned warm start is trained on before timing (0: off)")
     ap.add_argument("--batched", type=int, default=0,
                     help="also run the batched 2.5D workload with this many problems (256 = the full config)")
-    ap.add_argument("--batch-size", type=int, default=128, help="problems per batched solve")
+    ap.add_argument("--batch-size", type=int, default=256, help="problems per batched solve")
     ap.add_argument("--batched-method", default="band",
                     help="solver of the batched 2.5D leg: band (direct banded Cholesky) or mixed (block-PCG)")
     ap.add_argument("--batched-only", action="store_true", help="run only the batched 2.5D leg")

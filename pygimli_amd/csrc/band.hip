@@ -488,11 +488,11 @@ void solve_band(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std::
         bmax = std::max(bmax, c->bandB);
         emax = std::max(emax, c->nE);
     }
-    DBuf<double> Lp, Dp, Yp;
-    Lp.alloc(Lsz);
-    Dp.alloc(Dsz);
-    Yp.alloc(Ysz);
-    PG_CHECK(hipMemsetAsync(Lp.p, 0, Lp.bytes(), s));
+    DBuf<double> &Lp = c0.bandL, &Dp = c0.bandDinv, &Yp = c0.bandY;
+    Lp.reserve(Lsz);
+    Dp.reserve(Dsz);
+    Yp.reserve(Ysz);
+    PG_CHECK(hipMemsetAsync(Lp.p, 0, Lsz * sizeof(double), s));
     std::vector<BandDesc> h;
     size_t lo = 0, dof = 0, yo = 0;
     int maxn = 0, maxnnz = 0;
@@ -518,8 +518,10 @@ void solve_band(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std::
             maxnnz = std::max(maxnnz, c->nnz);
             h.push_back(d);
         }
-    DBuf<BandDesc> dd;
-    dd.upload(h, s);
+    DBuf<unsigned char> & ddb = c0.bandDesc;
+    ddb.reserve(h.size() * sizeof(BandDesc));
+    PG_CHECK(hipMemcpyAsync(ddb.p, h.data(), h.size() * sizeof(BandDesc), hipMemcpyHostToDevice, s));
+    struct { const BandDesc * p; } dd{(const BandDesc *)ddb.p};
     const int nd = (int)h.size();
     const dim3 blk(256);
     hipLaunchKernelGGL(k_band_scatter, dim3(std::max(1, std::min(div_up(maxnnz, 256), 4096 / nd + 1)), nd), blk, 0, s,
@@ -532,7 +534,7 @@ void solve_band(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std::
     hipLaunchKernelGGL(factor, dim3(nd), blk, fl, s, dd.p);
     hipLaunchKernelGGL(k_band_solve, dim3(div_up(emax, CG), nd), blk, sl, s, dd.p);
     PG_CHECK(hipGetLastError());
-    PG_CHECK(hipStreamSynchronize(s));  // work buffers are released on return
+    PG_CHECK(hipStreamSynchronize(s));  // the descriptors' host copy lives until here
     stats.assign(cs.size(), pgert_solve_stats{});
     for (size_t p = 0; p < cs.size(); ++p) {
         pgert_solve_stats & st = stats[p];

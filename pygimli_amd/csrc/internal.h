@@ -51,6 +51,13 @@ template <class T> struct DBuf {
         if (count) PG_CHECK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
     }
     void upload(const std::vector<T> & v, hipStream_t s) { upload(v.data(), v.size(), s); }
+    // at least `count` elements, keeping a larger allocation (work buffers reused across calls)
+    void reserve(size_t count) {
+        if (p && n >= count) return;
+        release();
+        if (count) PG_CHECK(hipMalloc((void **)&p, count * sizeof(T)));
+        n = count;
+    }
     size_t bytes() const { return n * sizeof(T); }
 };
 
@@ -156,6 +163,9 @@ struct Ctx {
     bool bandValid = false;
     int bandB = 0;
     DBuf<int> bandPerm, bandSlot;
+    // band work buffers of a (batched) solve led by this context, reused across calls
+    DBuf<double> bandL, bandDinv, bandY;
+    DBuf<unsigned char> bandDesc;
 
     // block solver's own stream (graph capture needs a non-default stream),
     // ordered after / before the caller's stream with these events
