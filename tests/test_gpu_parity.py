@@ -209,3 +209,29 @@ def test_config2_3d_invariants():
     r2 = fop.response(2.0 * rho)
     fop.response(rho)
     np.testing.assert_allclose(r2, 2.0 * fop.response(rho), rtol=1e-6)
+
+
+def test_config1_vs_reference_build():
+    """The full 2.5D workload (5203 nodes, 41 electrodes, 888 data) against
+    the reference's own C++ core built from its sources (oracle/_ref): the
+    default wavenumber list (17), direct band solver, FP64 Jacobian."""
+    from oracle import ref
+    if not ref.available():
+        pytest.skip("reference build oracle/_ref not present")
+    from pygimli_amd.ertModelling import ERTModelling
+    mesh, data, rho = workloads.ert2d_layers_block()
+    ses = ref.Session(mesh, copy.deepcopy(data), nthreads=4)
+    rhoa_r = ses.response(rho)
+    J_r = ses.jacobian(rho)
+    fop = ERTModelling()
+    fop.setMesh(mesh)
+    fop.setData(copy.deepcopy(data))
+    fop.solverOptions.update(method="band", rtol=1e-10)
+    rhoa = fop.response(rho)
+    fop.createJacobian(rho)
+    J = fop.jacobian().cpu().numpy()
+    assert len(fop._kv) == len(ses.kw()[0]) == 17
+    e1 = np.abs(rhoa - rhoa_r).max() / np.abs(rhoa_r).max()
+    e2 = np.linalg.norm(J - J_r) / np.linalg.norm(J_r)
+    print("vs reference build: rhoa", e1, "J", e2)
+    assert e1 < 1e-10 and e2 < 1e-10
