@@ -57,6 +57,10 @@ def lib():
         L.ref_mixed_bc.argtypes = [ctypes.c_void_p, _D, ctypes.c_double, _D]
         L.ref_boundary_count.argtypes = [ctypes.c_void_p]
         L.ref_boundary_nodes.argtypes = [ctypes.c_void_p, ctypes.c_int, _I, _I]
+        L.ref_mesh_convert.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+        L.ref_session_save_mesh.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+        L.ref_data_convert.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                                       ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p]
         _lib = L
     return _lib
 
@@ -106,6 +110,22 @@ def kwave(rmin, rmax, nleg, nlag):
 def exact_dc(v, src, k=0.0, surfaceZ=0.0, fallback=1.0):
     v, pv = _d(v); s, ps = _d(src)
     return lib().ref_exact_dc(pv, ps, k, surfaceZ, fallback)
+
+
+def mesh_convert(src, dst):
+    """Mesh::load(src, no neighbour infos) then Mesh::save(dst) (binary v3)."""
+    _check(lib().ref_mesh_convert(str(src).encode(), str(dst).encode()))
+
+
+def data_convert(src, dst, fmt_data="all", fmt_sensor="x y z", remove_invalid=True,
+                 no_filter=False):
+    """DataContainerERT(src, removeInvalid) then save(dst, fmt_data, fmt_sensor,
+    noFilter); the load runs in dst's folder (it may write invalid.data there)."""
+    import os
+    work = os.path.dirname(os.path.abspath(str(dst)))
+    _check(lib().ref_data_convert(str(src).encode(), str(dst).encode(), fmt_data.encode(),
+                                  fmt_sensor.encode(), int(remove_invalid), int(no_filter),
+                                  work.encode()))
 
 
 class Session:
@@ -187,6 +207,10 @@ class Session:
         _check(lib().ref_session_jacobian(self.h, pm, len(model), out.ctypes.data_as(_D),
                                           ctypes.byref(r), ctypes.byref(c)))
         return out
+
+    def save_mesh(self, path):
+        """The reference mesh (with every cell face as a boundary) as .bms."""
+        _check(lib().ref_session_save_mesh(self.h, str(path).encode()))
 
     def _csr(self, fn, *args):
         nnz = ctypes.c_int()

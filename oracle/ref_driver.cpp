@@ -32,6 +32,7 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+#include <unistd.h>
 
 namespace GIMLI {
 // Defined with external linkage in core/src/bert/dcfemmodelling.cpp:295-300
@@ -448,6 +449,42 @@ void ref_boundary_nodes(void * h, int i, int * ids, int * marker) {
     const Boundary & b = s->mesh.boundary(i);
     for (Index j = 0; j < b.nodeCount(); ++j) ids[j] = (int)b.node(j).id();
     *marker = b.marker();
+}
+
+// File formats, through the reference's own reader and writer.
+// Mesh::load without neighbour infos keeps exactly the boundaries the file
+// holds (core/src/mesh_io.cpp:30-49); Mesh::save writes binary v3 (:391-538).
+int ref_mesh_convert(const char * in, const char * out) {
+    return guarded([&] {
+        Mesh m;
+        m.load(in, false);
+        m.save(out);
+    });
+}
+
+// The session's mesh after Mesh::createNeighborInfos, which adds a boundary
+// for every cell face (core/src/mesh.cpp:1684-1740).
+int ref_session_save_mesh(void * h, const char * out) {
+    return guarded([&] { ((Session *)h)->mesh.save(out); });
+}
+
+// DataContainerERT(in, removeInvalid) then save(out, fmtData, fmtSensor, noFilter)
+// (core/src/datacontainer.cpp:209-419 load, :519-633 write).  The validity check
+// writes invalid.data into the working directory, so the load runs in workdir.
+int ref_data_convert(const char * in, const char * out, const char * fmtData,
+                     const char * fmtSensor, int removeInvalid, int noFilter,
+                     const char * workdir) {
+    char old[4096];
+    if (!getcwd(old, sizeof(old))) return -1;
+    if (chdir(workdir) != 0) return -1;
+    int rc = guarded([&] {
+        DataContainerERT d(in, removeInvalid != 0);
+        // std::string arguments: a const char * would bind to the
+        // save(name, fmtData, bool noFilter) overload instead.
+        d.save(std::string(out), std::string(fmtData), std::string(fmtSensor), noFilter != 0);
+    });
+    if (chdir(old) != 0) return -1;
+    return rc;
 }
 
 } // extern "C"

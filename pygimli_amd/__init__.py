@@ -16,4 +16,7 @@ def __getattr__(name):
     if name in ("createStiffnessMatrix", "createMassMatrix"):
         from . import solver
         return getattr(solver, name)
+    if name in ("load", "load_data", "save_data", "load_mesh", "save_mesh"):
+        from . import fileio
+        return getattr(fileio, name)
     raise AttributeError(name)

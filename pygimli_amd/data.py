@@ -108,5 +108,16 @@ class DataContainerERT:
             self._tok[name][i] = v
         return i
 
+    def save(self, fileName, formatData="all", formatSensor="x y z", noFilter=False):
+        """Unified data format file (DataContainer::save)."""
+        from .fileio import save_data
+        return save_data(self, fileName, formatData, formatSensor, noFilter)
+
+    @staticmethod
+    def load(fileName, sensorIndicesFromOne=True, removeInvalid=True):
+        """DataContainerERT(fileName, removeInvalid) of the unified data format."""
+        from .fileio import load_data
+        return load_data(fileName, sensorIndicesFromOne, removeInvalid)
+
     def abmn(self):
         return np.stack([self._tok[x] for x in "abmn"], axis=1).astype(np.int32)

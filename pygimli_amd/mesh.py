@@ -52,6 +52,7 @@ class Mesh:
         self.boundMarkers_ = np.ascontiguousarray(np.asarray(boundaryMarkers, np.int32))
         self._leftCell = None
         self._cellAttributes = np.ones(nc)
+        self._data = {}
 
     # -- pyGIMLi Mesh API subset --------------------------------------------
     def dim(self):
@@ -100,6 +101,29 @@ class Mesh:
         if a.shape != (self.cellCount(),):
             raise ValueError("attribute size mismatch")
         self._cellAttributes = a.copy()
+
+    def addData(self, name, v):
+        """Named vector stored with the mesh (Mesh::addData, mesh.h)."""
+        v = np.ascontiguousarray(np.asarray(v, np.float64).ravel())
+        if not name or v.size == 0:
+            raise ValueError("mesh data needs a name and at least one value")
+        self._data[str(name)] = v
+
+    def data(self, name):
+        return self._data[name]
+
+    def haveData(self, name):
+        return name in self._data
+
+    def dataMap(self):
+        return dict(self._data)
+
+    def save(self, fileName):
+        """Binary mesh .bms (Mesh::save -> saveBinaryV2)."""
+        from .fileio import save_mesh
+        save_mesh(self, fileName)
+
+    saveBinaryV2 = save
 
     def cellCenters(self):
         return self.nodes[self.cells].mean(axis=1)

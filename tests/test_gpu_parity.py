@@ -235,3 +235,28 @@ def test_config1_vs_reference_build():
     e2 = np.linalg.norm(J - J_r) / np.linalg.norm(J_r)
     print("vs reference build: rhoa", e1, "J", e2)
     assert e1 < 1e-10 and e2 < 1e-10
+
+
+def test_forward_from_files(tmp_path):
+    """A survey written to the unified data format and a .bms mesh, read back
+    and modelled: the same apparent resistivities and Jacobian as the
+    in-memory survey (sensor positions pass through 14-digit text, so to
+    1e-11 rather than bit for bit)."""
+    from pygimli_amd import fileio
+    from pygimli_amd.ertModelling import ERTModelling
+    mesh, data, rho = workloads.small_3d()
+    mesh.save(tmp_path / "mesh.bms")
+    data.save(tmp_path / "survey.dat", "a b m n k")
+    out = []
+    for m, d in ((mesh, data), (fileio.load_mesh(tmp_path / "mesh.bms"), fileio.load_data(tmp_path / "survey.dat"))):
+        fop = ERTModelling()
+        fop.setMesh(m)
+        fop.setData(copy.deepcopy(d))
+        fop.solverOptions.update(method="block64", rtol=1e-12)
+        r = fop.response(rho)
+        fop.createJacobian(rho)
+        out.append((r, fop.jacobian().cpu().numpy()))
+    (r0, J0), (r1, J1) = out
+    print("from files: rhoa", np.abs(r1 - r0).max() / np.abs(r0).max())
+    assert np.abs(r1 - r0).max() <= 1e-11 * np.abs(r0).max()
+    assert np.linalg.norm(J1 - J0) <= 1e-11 * np.linalg.norm(J0)
