@@ -159,6 +159,28 @@ int pgert_batch_forward(pgert_ctx * const * ctxs, int count, const double * cons
  * (dcfemmodelling.cpp:1681-1682).  DEVICE [nK * nE * n]. */
 int pgert_potentials(pgert_ctx * ctx, double * out);
 
+/* Complex resistivity, DCMultiElectrodeModelling with setComplex(true)
+ * (ertModelling.py:217-238 with sr=False; dcfemmodelling.cpp:1103-1120 response,
+ * :163-292 assembly with 1/rho complex, :1755-1928 calculateK_ total-field
+ * potentials of unit point sources).  rho_re, rho_im: DEVICE [n_cells]
+ * (pgert_map_model of the real and imaginary model halves); resp_re, resp_im:
+ * DEVICE [n_data], k * ((uAM - uAN) - (uBM - uBN)) of the real and imaginary
+ * potentials (no rounding, no reciprocity, as the reference's complex branch).
+ * The complex symmetric systems are solved by real/imaginary block Gauss-Seidel
+ * sweeps, each half with opts' real solver; fails when cell phases are too large
+ * for the sweeps to converge. */
+int pgert_forward_complex(pgert_ctx * ctx, const double * rho_re, const double * rho_im,
+                          const pgert_solve_opts * opts, double * resp_re, double * resp_im,
+                          pgert_solve_stats * stats);
+
+/* Complex Jacobian of the last pgert_forward_complex (createJacobian,
+ * dcfemmodelling.cpp:1446-1463 -> createJacobian_ :1410-1444): sensitivities of
+ * the complex potentials (no conjugation), divided by m_j^2 / k_i (complex m)
+ * when n_model == n_param.  model_re, model_im: DEVICE [n_model]; J_re, J_im:
+ * DEVICE [n_data x n_param] FP64 row-major. */
+int pgert_jacobian_complex(pgert_ctx * ctx, const double * model_re, const double * model_im, int n_model,
+                           double * J_re, double * J_im);
+
 /* Same potentials, node-major as held in HBM: [nK][n][nE].  DEVICE. */
 int pgert_potentials_node_major(pgert_ctx * ctx, double * out);
 

@@ -379,7 +379,8 @@ def assemble(mesh, rho, k, source, geo=None):
         geo = element_matrices(mesh)
     K, M, vol, _ = geo
     n = mesh.nodeCount()
-    rho = np.asarray(rho, np.float64)
+    rho = np.asarray(rho)
+    rho = rho.astype(np.complex128 if np.iscomplexobj(rho) else np.float64)   # complex: 1 / rho complex
     use = np.abs(rho) > TOLERANCE
     sig = np.where(use, 1.0 / np.where(use, rho, 1.0), 0.0)
     Ae = (K + (k * k) * M) * sig[:, None, None] if k > 0.0 else K * sig[:, None, None]
@@ -412,7 +413,7 @@ def assemble(mesh, rho, k, source, geo=None):
         raise NotImplementedError("Dirichlet boundaries are not part of this path")
     A.sum_duplicates()
     A.sort_indices()
-    diag = A.diagonal()
+    diag = A.diagonal().real
     if np.any(diag < TOLERANCE):
         raise RuntimeError("zero diagonal (all cells around a node have rho = 0)")
     return A.tocsr()
@@ -511,14 +512,15 @@ class Forward:
         """Non-SR total-field potentials: unit point source at the electrode
         node (DCMultiElectrodeModelling::calculateK_, dcfemmodelling.cpp:1755-1928
         with ElectrodeShapeNode::assembleRHS, electrode.cpp:133-151)."""
-        rho = np.asarray(rho, np.float64)
+        rho = np.asarray(rho)
         n = self.mesh.nodeCount()
-        U = np.empty((len(self.k) * self.nE, n))
+        dt = np.complex128 if np.iscomplexobj(rho) else np.float64
+        U = np.empty((len(self.k) * self.nE, n), dt)
         for ki, k in enumerate(self.k):
             S = assemble(self.mesh, rho, k, self.setup.sourceCenter, self.geo)
             lu = spla.splu(S.tocsc())
             for i in range(self.nE):
-                rhs = np.zeros(n)
+                rhs = np.zeros(n, dt)
                 rhs[self.setup.elecNode[i]] = 1.0
                 U[i + ki * self.nE] = lu.solve(rhs)
         return U
@@ -567,6 +569,35 @@ class Forward:
         return J
 
 
+def response_complex(fwd, model_re, model_im):
+    """DCMultiElectrodeModelling::response, complex branch
+    (dcfemmodelling.cpp:1103-1120): complex cell resistivities (mapERTModel of
+    both halves, :1199-1209), total-field potentials of the complex system
+    (calculateK_, :1755-1928), k * ((uAM - uAN) - (uBM - uBN)) of the real and
+    the imaginary potential maps, no rounding, no reciprocity.  Returns (re, im)."""
+    re = map_model(fwd.mesh, np.asarray(model_re, np.float64))
+    im = map_model(fwd.mesh, np.asarray(model_im, np.float64))
+    kf = fwd.data["k"]
+    if not fwd.data.haveData("k") or np.min(np.abs(kf)) < TOLERANCE:
+        kf = fwd.geometric_factors()
+        fwd.data.set("k", kf)
+    fwd.subpots = fwd.solve_point_source(re + 1j * im)
+    emap = fwd.electrode_map(fwd.potentials())
+    u = collect_data(emap, fwd.data.abmn(), False) * kf
+    return u.real, u.imag
+
+
+def jacobian_complex(fwd, model_re, model_im):
+    """createJacobian_ complex (dcfemmodelling.cpp:1410-1444): calc2 on the
+    complex potentials without conjugation, divided by m^2 / k (m complex)."""
+    J = sensitivity_calc2(fwd.mesh, fwd.geo, fwd.data.abmn(), fwd.subpots, fwd.k, fwd.w,
+                          fwd.data.sensorCount())
+    m = np.asarray(model_re, np.float64) + 1j * np.asarray(model_im, np.float64)
+    if len(m) == J.shape[1]:
+        J = J / ((m * m)[None, :] / fwd.data["k"][:, None])
+    return J.real, J.imag
+
+
 def collect_data(emap, abmn, reciprocity):
     """DataMap::data (datamap.cpp:159-231)."""
     a, b, m, n = [abmn[:, i].copy() for i in range(4)]
@@ -594,7 +625,7 @@ def sensitivity_calc2(mesh, geo, abmn, U, kvals, weights, nE):
     K = K[use]
     M = M[use]
     D = len(abmn)
-    J = np.zeros((D, nModel))
+    J = np.zeros((D, nModel), np.result_type(U.dtype, np.float64))
     nn = U.shape[1]
     zero = np.zeros(nn)
     for ki, k in enumerate(kvals):
@@ -609,5 +640,9 @@ def sensitivity_calc2(mesh, geo, abmn, U, kvals, weights, nE):
             dab = (va - vb)[cells]
             dmn = (vm - vn)[cells]
             val = np.einsum("ci,cij,cj->c", dmn, S, dab)
-            J[i] += weights[ki] * np.bincount(mk, weights=val, minlength=nModel)
+            if np.iscomplexobj(val):
+                J[i] += weights[ki] * (np.bincount(mk, weights=val.real, minlength=nModel)
+                                       + 1j * np.bincount(mk, weights=val.imag, minlength=nModel))
+            else:
+                J[i] += weights[ki] * np.bincount(mk, weights=val, minlength=nModel)
     return J

@@ -57,6 +57,9 @@ def lib():
         L.ref_mixed_bc.argtypes = [ctypes.c_void_p, _D, ctypes.c_double, _D]
         L.ref_boundary_count.argtypes = [ctypes.c_void_p]
         L.ref_boundary_nodes.argtypes = [ctypes.c_void_p, ctypes.c_int, _I, _I]
+        L.ref_session_init_fop_complex.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, ctypes.c_int]
+        L.ref_session_response_complex.argtypes = [ctypes.c_void_p, _D, _D, ctypes.c_int, _D, _D]
+        L.ref_session_jacobian_complex.argtypes = [ctypes.c_void_p, _D, _D, ctypes.c_int, _D, _D, _I, _I]
         L.ref_mesh_convert.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
         L.ref_session_save_mesh.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
         L.ref_data_convert.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
@@ -131,7 +134,7 @@ def data_convert(src, dst, fmt_data="all", fmt_sensor="x y z", remove_invalid=Tr
 class Session:
     """The reference DCSRMultiElectrodeModelling on one mesh + data set."""
 
-    def __init__(self, mesh, data=None, kvalues=None, weights=None, nthreads=0):
+    def __init__(self, mesh, data=None, kvalues=None, weights=None, nthreads=0, complex_=False):
         L = lib()
         self._keep = []
         nodes, pn = _d(mesh.nodes)
@@ -156,11 +159,12 @@ class Session:
                 k, pk = _d(k)
             _check(L.ref_session_set_data(self.h, data.sensorCount(), ps, data.size(),
                                           pa, pbb, pm, pnn, pk))
+            init = L.ref_session_init_fop_complex if complex_ else L.ref_session_init_fop
             if kvalues is not None:
                 kv, pkv = _d(kvalues); wv, pwv = _d(weights)
-                _check(L.ref_session_init_fop(self.h, len(kv), pkv, pwv, nthreads))
+                _check(init(self.h, len(kv), pkv, pwv, nthreads))
             else:
-                _check(L.ref_session_init_fop(self.h, -1, None, None, nthreads))
+                _check(init(self.h, -1, None, None, nthreads))
 
     def __del__(self):
         try:
@@ -185,6 +189,24 @@ class Session:
         out = np.empty(self.data.size())
         _check(lib().ref_session_response(self.h, pm, len(model), out.ctypes.data_as(_D)))
         return out
+
+    def response_complex(self, re, im):
+        """Complex response (setComplex(True) session): (real, imag) of k * U."""
+        re, pr = _d(re); im, pi = _d(im)
+        out_re = np.empty(self.data.size()); out_im = np.empty(self.data.size())
+        _check(lib().ref_session_response_complex(self.h, pr, pi, len(re), out_re.ctypes.data_as(_D),
+                                                  out_im.ctypes.data_as(_D)))
+        return out_re, out_im
+
+    def jacobian_complex(self, re, im):
+        re, pr = _d(re); im, pi = _d(im)
+        r = ctypes.c_int(); c = ctypes.c_int()
+        _check(lib().ref_session_jacobian_complex(self.h, pr, pi, len(re), None, None,
+                                                  ctypes.byref(r), ctypes.byref(c)))
+        Jr = np.empty((r.value, c.value)); Ji = np.empty((r.value, c.value))
+        _check(lib().ref_session_jacobian_complex(self.h, pr, pi, len(re), Jr.ctypes.data_as(_D),
+                                                  Ji.ctypes.data_as(_D), ctypes.byref(r), ctypes.byref(c)))
+        return Jr, Ji
 
     def data_k(self):
         out = np.empty(self.data.size())

@@ -56,10 +56,38 @@ public:
     EnvelopeCholesky() : SolverWrapper(false) { name_ = "envelope-cholesky"; }
 
     void setMatrix(const RSparseMatrix & S) override {
-        const std::vector<int> & ptr = S.vecColPtr();
-        const std::vector<int> & idx = S.vecRowIdx();
-        const RVector & val = S.vecVals();
-        n_ = (int)S.rows();
+        pattern_(S.vecColPtr(), S.vecRowIdx(), S.rows());
+        factor_(S.vecColPtr(), S.vecRowIdx(), &S.vecVals()[0], L_);
+    }
+
+    // complex symmetric systems (setComplex(true)): L L^T without conjugation
+    void setMatrix(const CSparseMatrix & S) override {
+        pattern_(S.vecColPtr(), S.vecRowIdx(), S.rows());
+        factor_(S.vecColPtr(), S.vecRowIdx(), &S.vecVals()[0], LC_);
+    }
+
+    void solve(const RVector & rhs, RVector & sol) override {
+        std::vector<double> y(n_);
+        for (int pi = 0; pi < n_; ++pi) y[pi] = rhs[perm_[pi]];
+        subst_(L_, y);
+        if (sol.size() != (Index)n_) sol.resize(n_);
+        for (int pi = 0; pi < n_; ++pi) sol[perm_[pi]] = y[pi];
+    }
+
+    void solve(const CVector & rhs, CVector & sol) override {
+        std::vector<Complex> y(n_);
+        for (int pi = 0; pi < n_; ++pi) y[pi] = rhs[perm_[pi]];
+        subst_(LC_, y);
+        if (sol.size() != (Index)n_) sol.resize(n_);
+        for (int pi = 0; pi < n_; ++pi) sol[perm_[pi]] = y[pi];
+    }
+
+private:
+    static bool positive_(double d) { return d > 0.0; }
+    static bool positive_(const Complex & d) { return std::abs(d) > 0.0; }
+
+    void pattern_(const std::vector<int> & ptr, const std::vector<int> & idx, Index rows) {
+        n_ = (int)rows;
         rcm_(ptr, idx);
         // envelope start per permuted row
         first_.assign(n_, 0);
@@ -74,54 +102,53 @@ public:
         }
         off_.assign(n_ + 1, 0);
         for (int i = 0; i < n_; ++i) off_[i + 1] = off_[i] + (i - first_[i] + 1);
-        L_.assign(off_[n_], 0.0);
+    }
+
+    template <class T>
+    void factor_(const std::vector<int> & ptr, const std::vector<int> & idx, const T * val, std::vector<T> & L) {
+        L.assign(off_[n_], T(0.0));
         for (int pi = 0; pi < n_; ++pi) {
             int i = perm_[pi];
             for (int e = ptr[i]; e < ptr[i + 1]; ++e) {
                 int pj = iperm_[idx[e]];
-                if (pj <= pi) L_[off_[pi] + pj - first_[pi]] += val[e];
+                if (pj <= pi) L[off_[pi] + pj - first_[pi]] += val[e];
             }
         }
         // row-oriented envelope Cholesky: L(i,j) for first_[i] <= j <= i
         for (int i = 0; i < n_; ++i) {
-            double * Li = &L_[off_[i]];
+            T * Li = &L[off_[i]];
             int fi = first_[i];
             for (int j = fi; j < i; ++j) {
-                double * Lj = &L_[off_[j]];
+                T * Lj = &L[off_[j]];
                 int fj = first_[j];
                 int s = std::max(fi, fj);
-                double acc = Li[j - fi];
+                T acc = Li[j - fi];
                 for (int q = s; q < j; ++q) acc -= Li[q - fi] * Lj[q - fj];
                 Li[j - fi] = acc / Lj[j - fj];
             }
-            double d = Li[i - fi];
+            T d = Li[i - fi];
             for (int q = fi; q < i; ++q) d -= Li[q - fi] * Li[q - fi];
-            if (!(d > 0.0)) throw std::runtime_error("EnvelopeCholesky: matrix not SPD");
+            if (!positive_(d)) throw std::runtime_error("EnvelopeCholesky: matrix not SPD");
             Li[i - fi] = std::sqrt(d);
         }
     }
 
-    void solve(const RVector & rhs, RVector & sol) override {
-        std::vector<double> y(n_);
-        for (int pi = 0; pi < n_; ++pi) y[pi] = rhs[perm_[pi]];
+    template <class T> void subst_(const std::vector<T> & L, std::vector<T> & y) const {
         for (int i = 0; i < n_; ++i) {
-            const double * Li = &L_[off_[i]];
+            const T * Li = &L[off_[i]];
             int fi = first_[i];
-            double acc = y[i];
+            T acc = y[i];
             for (int q = fi; q < i; ++q) acc -= Li[q - fi] * y[q];
             y[i] = acc / Li[i - fi];
         }
         for (int i = n_ - 1; i >= 0; --i) {
-            const double * Li = &L_[off_[i]];
+            const T * Li = &L[off_[i]];
             int fi = first_[i];
             y[i] /= Li[i - fi];
             for (int q = fi; q < i; ++q) y[q] -= Li[q - fi] * y[i];
         }
-        if (sol.size() != (Index)n_) sol.resize(n_);
-        for (int pi = 0; pi < n_; ++pi) sol[perm_[pi]] = y[pi];
     }
 
-private:
     void rcm_(const std::vector<int> & ptr, const std::vector<int> & idx) {
         std::vector<int> deg(n_);
         for (int i = 0; i < n_; ++i) deg[i] = ptr[i + 1] - ptr[i];
@@ -173,12 +200,13 @@ private:
     int n_ = 0;
     std::vector<int> perm_, iperm_, first_, off_;
     std::vector<double> L_;
+    std::vector<Complex> LC_;
 };
 
 struct Session {
     Mesh mesh;
     DataContainerERT data;
-    DCSRMultiElectrodeModelling * fop = nullptr;
+    DCMultiElectrodeModelling * fop = nullptr;
     EnvelopeCholesky solver;
     RMatrix subpots;
     int dim = 2;
@@ -312,6 +340,62 @@ int ref_session_init_fop(void * h, int nk, const double * kv, const double * wv,
         }
         if (nthreads > 0) s->fop->setThreadCount(nthreads);
         s->fop->setSolver(&s->solver);
+    });
+}
+
+// ERTModelling(sr=False) + setComplex(True) (ertModelling.py:103-121): the
+// total-field DCMultiElectrodeModelling on complex resistivities.
+int ref_session_init_fop_complex(void * h, int nk, const double * kv, const double * wv, int nthreads) {
+    Session * s = (Session *)h;
+    return guarded([&] {
+        delete s->fop;
+        s->fop = new DCMultiElectrodeModelling(false);
+        s->fop->setComplex(true);
+        s->fop->setMesh(s->mesh, true);
+        s->fop->setData(s->data);
+        if (nk > 0) {
+            RVector k(nk), w(nk);
+            for (int i = 0; i < nk; ++i) { k[i] = kv[i]; w[i] = wv[i]; }
+            s->fop->setkValues(k);
+            s->fop->setWeights(w);
+        }
+        if (nthreads > 0) s->fop->setThreadCount(nthreads);
+        s->fop->setSolver(&s->solver);
+    });
+}
+
+// complex response (dcfemmodelling.cpp:1103-1120): model = [re, im], result
+// [real, imag] of k * U
+int ref_session_response_complex(void * h, const double * re, const double * im, int nm, double * out_re,
+                                 double * out_im) {
+    Session * s = (Session *)h;
+    return guarded([&] {
+        RVector mod(2 * nm);
+        for (int i = 0; i < nm; ++i) { mod[i] = re[i]; mod[nm + i] = im[i]; }
+        RVector r(s->fop->response(mod));
+        Index D = r.size() / 2;
+        for (Index i = 0; i < D; ++i) { out_re[i] = r[i]; out_im[i] = r[D + i]; }
+    });
+}
+
+// complex Jacobian (dcfemmodelling.cpp:1446-1463, :1410-1444)
+int ref_session_jacobian_complex(void * h, const double * re, const double * im, int nm, double * out_re,
+                                 double * out_im, int * rows, int * cols) {
+    Session * s = (Session *)h;
+    return guarded([&] {
+        RVector mod(2 * nm);
+        for (int i = 0; i < nm; ++i) { mod[i] = re[i]; mod[nm + i] = im[i]; }
+        s->fop->createJacobian(mod);
+        CMatrix * J = dynamic_cast<CMatrix *>(s->fop->jacobian());
+        if (!J) throw std::runtime_error("complex Jacobian expected");
+        *rows = (int)J->rows();
+        *cols = (int)J->cols();
+        if (out_re)
+            for (Index i = 0; i < J->rows(); ++i)
+                for (Index j = 0; j < J->cols(); ++j) {
+                    out_re[i * J->cols() + j] = (*J)[i][j].real();
+                    out_im[i * J->cols() + j] = (*J)[i][j].imag();
+                }
     });
 }
 

@@ -573,6 +573,22 @@ void forward_finish(Ctx & c, double * rhoa) {
     PG_CHECK(hipEventRecord(c.ev[4], c.stream));
 }
 
+// one real solve of c.B into c.X (c.X holds the initial guess)
+void solve_one(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st) {
+    if (o.method == PGERT_PCG_FP64)
+        solve_pcg_fp64(c, o, st);
+    else if (o.method == PGERT_BLOCK_FP64)
+        solve_block(c, o, st, false);
+    else if (o.method == PGERT_BLOCK_MIXED)
+        solve_block(c, o, st, true);
+    else if (o.method == PGERT_BAND_FP64) {
+        std::vector<pgert_solve_stats> v;
+        solve_band({&c}, o, v);
+        st = v[0];
+    } else
+        throw std::runtime_error("pgert_forward: unknown method");
+}
+
 void read_timings(Ctx & c) {
     float t;
     for (int i = 0; i < 4; ++i) {
@@ -606,6 +622,103 @@ int pgert_forward(pgert_ctx * c, const double * rho, const pgert_solve_opts * op
         sync(*c);
         read_timings(*c);
         if (stats) *stats = st;
+    });
+}
+
+int pgert_forward_complex(pgert_ctx * c, const double * rho_re, const double * rho_im,
+                          const pgert_solve_opts * opts, double * resp_re, double * resp_im,
+                          pgert_solve_stats * stats) {
+    return guarded(c, [&] {
+        PG_REQUIRE(rho_re && rho_im && resp_re && resp_im, "pgert_forward_complex: null pointer");
+        Ctx & C = *c;
+        PG_REQUIRE(C.C > 0 && C.nE > 0 && C.nK > 0 && C.D > 0,
+                   "pgert_forward_complex: mesh, electrodes, wavenumbers and data must be set");
+        pgert_solve_opts o = resolve_opts(opts);
+        o.warm_start = nullptr;
+        o.learned_warm_start = 0;
+        hipStream_t s = C.stream;
+        const size_t nk = C.nK, blk = nk * C.n * C.nE;
+        PG_CHECK(hipEventRecord(C.ev[0], s));
+        C.valsA.alloc(nk * C.nnz);
+        C.valsAi.alloc(nk * C.nnz);
+        for (size_t kk = 0; kk < nk; ++kk)
+            launch_assemble_complex(C, rho_re, rho_im, C.kv[kk], C.valsA.p + kk * C.nnz, C.valsAi.p + kk * C.nnz);
+        launch_diag_inv(C);
+        C.Bpt.alloc(blk);
+        launch_point_rhs(C, C.Bpt.p);
+        C.B.alloc(blk);
+        C.X.alloc(blk);
+        C.Xre.alloc(blk);
+        C.Xim.alloc(blk);
+        C.chg.alloc(4);
+        PG_CHECK(hipMemsetAsync(C.Xre.p, 0, C.Xre.bytes(), s));
+        PG_CHECK(hipMemsetAsync(C.Xim.p, 0, C.Xim.bytes(), s));
+        C.haveU = false;
+        C.haveUc = false;
+        pgert_solve_stats st{}, tot{};
+        tot.converged = 1;
+        const int maxSweeps = 60;
+        // the sweeps stop once the potentials change by less than rtol (relative);
+        // an inner solve from a converged guess moves them by about rtol itself
+        const double stop = 10.0 * o.rtol;
+        bool done = false;
+        double hchg[4];
+        for (int it = 0; it < maxSweeps && !done; ++it) {
+            for (int half = 0; half < 2; ++half) {
+                double * Xh = half == 0 ? C.Xre.p : C.Xim.p;
+                if (half == 0)
+                    launch_spmm_add(C, C.valsAi.p, C.Xim.p, C.Bpt.p, 1.0, C.B.p);
+                else
+                    launch_spmm_add(C, C.valsAi.p, C.Xre.p, nullptr, -1.0, C.B.p);
+                PG_CHECK(hipMemcpyAsync(C.X.p, Xh, C.X.bytes(), hipMemcpyDeviceToDevice, s));
+                st = pgert_solve_stats{};
+                solve_one(C, o, st);
+                tot.iterations += st.iterations;
+                tot.max_iterations += st.max_iterations;
+                tot.refinements += st.refinements;
+                tot.converged = tot.converged && st.converged;
+                tot.max_rel_residual = std::max(tot.max_rel_residual, st.max_rel_residual);
+                launch_change(C, (long)blk, C.X.p, Xh, C.chg.p + 2 * half);
+                PG_CHECK(hipMemcpyAsync(Xh, C.X.p, C.X.bytes(), hipMemcpyDeviceToDevice, s));
+            }
+            PG_CHECK(hipMemcpyAsync(hchg, C.chg.p, sizeof(hchg), hipMemcpyDeviceToHost, s));
+            sync(C);
+            const double rel = std::sqrt((hchg[0] + hchg[2]) / std::max(hchg[1] + hchg[3], 1e-300));
+            done = it > 0 && rel < stop;
+        }
+        PG_REQUIRE(done, "pgert_forward_complex: the real/imaginary sweeps did not converge "
+                         "(cell phases too large for this solver)");
+        PG_CHECK(hipEventRecord(C.ev[3], s));
+        launch_response_complex(C, C.Xre.p, C.Xim.p, resp_re, resp_im);
+        PG_CHECK(hipEventRecord(C.ev[4], s));
+        C.haveUc = true;
+        sync(C);
+        if (stats) *stats = tot;
+    });
+}
+
+int pgert_jacobian_complex(pgert_ctx * c, const double * model_re, const double * model_im, int n_model,
+                           double * J_re, double * J_im) {
+    return guarded(c, [&] {
+        Ctx & C = *c;
+        PG_REQUIRE(C.haveUc, "pgert_jacobian_complex: call pgert_forward_complex first");
+        PG_REQUIRE(J_re && J_im, "pgert_jacobian_complex: bad output");
+        PG_REQUIRE(n_model >= 0 && (n_model == 0 || (model_re && model_im)), "pgert_jacobian_complex: bad model");
+        hipStream_t s = C.stream;
+        const size_t blk = (size_t)C.nK * C.n * C.nE, jsz = (size_t)C.D * C.nParam;
+        C.Jw.reserve(3 * jsz);
+        C.U.alloc(blk);
+        // S(U) is the quadratic form of the real kernel: S(re) - S(im) is the
+        // real part, S(re + im) - S(re) - S(im) the imaginary part
+        PG_CHECK(hipMemcpyAsync(C.U.p, C.Xre.p, C.U.bytes(), hipMemcpyDeviceToDevice, s));
+        launch_jacobian(C, nullptr, 0, C.Jw.p, 0);
+        PG_CHECK(hipMemcpyAsync(C.U.p, C.Xim.p, C.U.bytes(), hipMemcpyDeviceToDevice, s));
+        launch_jacobian(C, nullptr, 0, C.Jw.p + jsz, 0);
+        launch_add(C, (long)blk, C.Xre.p, C.Xim.p, C.U.p);
+        launch_jacobian(C, nullptr, 0, C.Jw.p + 2 * jsz, 0);
+        launch_jacobian_complex_combine(C, C.Jw.p, C.Jw.p + jsz, C.Jw.p + 2 * jsz, model_re, model_im,
+                                        n_model == C.nParam, J_re, J_im);
+        sync(C);
     });
 }
 

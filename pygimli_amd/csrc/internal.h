@@ -166,6 +166,10 @@ struct Ctx {
     // band work buffers of a (batched) solve led by this context, reused across calls
     DBuf<double> bandL, bandDinv, bandY;
     DBuf<unsigned char> bandDesc;
+    // complex resistivity: imaginary half of A, unit point sources, the two
+    // halves of the potentials, Jacobian work blocks, change accumulator
+    DBuf<double> valsAi, Bpt, Xre, Xim, Jw, chg;
+    bool haveUc = false;
 
     // block solver's own stream (graph capture needs a non-default stream),
     // ordered after / before the caller's stream with these events
@@ -230,6 +234,17 @@ void launch_transpose_pots(Ctx & c, double * out);
 void launch_jacobian(Ctx & c, const double * model, int nModel, void * J, int dtype);
 // one Jacobian launch per wavenumber pass over the cells of several problems
 // (same device and dimension; every context uses cs[0]->stream)
+void launch_assemble_complex(Ctx & c, const double * rre, const double * rim, double k, double * vre,
+                             double * vim);
+void launch_point_rhs(Ctx & c, double * B);
+void launch_spmm_add(Ctx & c, const double * vals, const double * X, const double * base, double sign,
+                     double * out);
+void launch_change(Ctx & c, long total, const double * a, const double * b, double * acc);
+void launch_response_complex(Ctx & c, const double * Ure, const double * Uim, double * out_re, double * out_im);
+void launch_add(Ctx & c, long total, const double * a, const double * b, double * out);
+void launch_jacobian_complex_combine(Ctx & c, const double * Srr, const double * Sii, const double * Sss,
+                                     const double * mr, const double * mi, bool scale, double * Jre,
+                                     double * Jim);
 void launch_jacobian_batch(const std::vector<Ctx *> & cs, const std::vector<const double *> & models,
                            const std::vector<int> & nModels, const std::vector<void *> & Js, int dtype);
 

@@ -41,8 +41,9 @@ class ERTModelling:
     Parameters
     ----------
     sr : bool
-        Singularity removal (the reference default).  Only the SR operator is
-        provided.
+        Singularity removal (the reference default).  sr=False gives the
+        total-field operator, provided for complex resistivities
+        (setComplex(True)), the only case the reference computes without SR.
     verbose : bool
         Print solver statistics.
     device : int
@@ -50,11 +51,12 @@ class ERTModelling:
     """
 
     def __init__(self, sr=True, verbose=False, device=0):
-        if not sr:
-            raise NotImplementedError("only the singularity-removal operator (sr=True) is provided")
         if not torch.cuda.is_available():
             raise RuntimeError("ERTModelling needs a HIP device (no GPU visible)")
         self.verbose = verbose
+        self._sr = bool(sr)
+        self._complex = False
+        self._Jc = None
         self.device = torch.device("cuda", device)
         self._ctx = _lib.Context(device)
         self._mesh = None
@@ -74,6 +76,21 @@ class ERTModelling:
         self._dirty = True
 
     # ------------------------------------------------------------------ set-up
+    def setComplex(self, c):
+        """Complex resistivity (DCMultiElectrodeModelling::setComplex,
+        dcfemmodelling.cpp:573-582); needs the total-field operator (sr=False),
+        as in the reference, whose SR operator has no complex branch."""
+        c = bool(c)
+        if c and self._sr:
+            raise NotImplementedError("complex resistivity needs ERTModelling(sr=False)")
+        if c != self._complex:
+            self._model = None
+            self._J = None
+        self._complex = c
+
+    def complex(self):
+        return self._complex
+
     def setVerbose(self, v):
         self.verbose = bool(v)
 
@@ -236,9 +253,15 @@ class ERTModelling:
         return opts
 
     def response_device(self, model):
-        """Apparent resistivities as a float64 device tensor."""
+        """Apparent resistivities as a float64 device tensor; for complex
+        resistivities model = [re, im] and the result [re, im] of the complex
+        apparent resistivities."""
         self._sync_setup()
         t = self._model_tensor(model)
+        if self._complex:
+            return self._response_complex(t)
+        if not self._sr:
+            raise NotImplementedError("sr=False is provided for complex resistivities (setComplex(True)) only")
         if float(t.min()) < 1e-12:
             raise ValueError("response for model with negative or zero resistivity is not defined")
         rho = self.mapERTModel(t)
@@ -253,6 +276,29 @@ class ERTModelling:
         if self.verbose:
             print("ERTModelling.response:", self.lastStats, self.timings())
         return rhoa
+
+    def _response_complex(self, t):
+        """DCMultiElectrodeModelling::response, complex branch
+        (dcfemmodelling.cpp:1103-1120)."""
+        if t.numel() % 2:
+            raise ValueError("complex model: [real part, imaginary part] expected")
+        h = t.numel() // 2
+        re, im = t[:h].contiguous(), t[h:].contiguous()
+        if float(torch.sqrt(re * re + im * im).min()) < 1e-12:
+            raise ValueError("complex response for abs model with negative or zero resistivity is not defined")
+        rre, rim = self.mapERTModel(re), self.mapERTModel(im)
+        D = self._data.size()
+        out = torch.empty(2 * D, dtype=torch.float64, device=self.device)
+        stats = _lib.SolveStats()
+        opts = self._opts()
+        opts.learned_warm_start = 0
+        self._set_stream()
+        self._ctx.check(self._ctx.L.pgert_forward_complex(self._ctx.h, _dptr(rre), _dptr(rim), ctypes.byref(opts),
+                                                          _dptr(out), ctypes.c_void_p(out.data_ptr() + 8 * D),
+                                                          ctypes.byref(stats)), "response")
+        self.lastStats = stats.as_dict()
+        self._model = t
+        return out
 
     def response(self, mod):
         """Forward response (apparent resistivity), numpy float64 [nData]
@@ -293,6 +339,8 @@ class ERTModelling:
         t = self._model_tensor(mod)
         if self._model is None or self._model.shape != t.shape or not torch.equal(self._model, t):
             self.response_device(t)
+        if self._complex:
+            return self._jacobian_complex(t)
         dtype = self.jacobianDtype
         J = torch.empty((self._data.size(), self._nParam), dtype=dtype, device=self.device)
         code = 0 if dtype == torch.float64 else 1
@@ -302,13 +350,35 @@ class ERTModelling:
         self._J = J
         return None
 
+    def _jacobian_complex(self, t):
+        """createJacobian complex branch (dcfemmodelling.cpp:1446-1463,
+        :1410-1444) and its real block form (ERTModelling.createJacobian,
+        ertModelling.py:224-236 -> pg.utils.toRealMatrix, utils/complex.py:79-110):
+        [[Re J, -Im J], [Im J, Re J]]."""
+        h = t.numel() // 2
+        re, im = t[:h].contiguous(), t[h:].contiguous()
+        D, P = self._data.size(), self._nParam
+        Jr = torch.empty((D, P), dtype=torch.float64, device=self.device)
+        Ji = torch.empty((D, P), dtype=torch.float64, device=self.device)
+        self._set_stream()
+        self._ctx.check(self._ctx.L.pgert_jacobian_complex(self._ctx.h, _dptr(re), _dptr(im), h, _dptr(Jr),
+                                                           _dptr(Ji)), "createJacobian")
+        self._Jc = (Jr, Ji)
+        self._J = torch.cat([torch.cat([Jr, -Ji], 1), torch.cat([Ji, Jr], 1)], 0)
+        return None
+
     def jacobian(self):
-        """The Jacobian of the last createJacobian, [nData, nParam] device tensor."""
+        """The Jacobian of the last createJacobian, [nData, nParam] device
+        tensor ([2 nData, 2 nParam] real block form for complex resistivities)."""
         return self._J
+
+    def jacobianComplex(self):
+        """(real, imaginary) parts of the complex Jacobian, [nData, nParam] each."""
+        return self._Jc
 
     @property
     def parameterCount(self):
-        return self._nParam
+        return self._nParam * (1 + int(self._complex))
 
     def close(self):
         self._ctx.close()

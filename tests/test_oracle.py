@@ -15,6 +15,7 @@ import pytest
 import scipy.special as ss
 
 from oracle import ert_oracle as eo
+from oracle import ref
 from pygimli_amd import workloads
 from pygimli_amd.meshgen import grid_mesh_2d, grid_mesh_3d
 
@@ -363,3 +364,48 @@ def test_cgls_oracle_matches_normal_equations():
     x3, it3, _ = cg.cgls(S, C, v["dW"], v["b"], xn, v["wc"], v["wm"], v["tm"], v["td"], lam, v["roughness"],
                          max_iter=5, tol=1e-20)
     assert np.linalg.norm(x3 - xn) / np.linalg.norm(xn) < 1e-10
+
+
+@pytest.mark.parametrize("name", ["small_2d", "small_3d"])
+def test_complex_oracle_vs_reference_build(name):
+    """Complex resistivity (DCMultiElectrodeModelling with setComplex(true)):
+    the oracle's restatement against the reference's own C++ core."""
+    if not ref.available():
+        pytest.skip("oracle/_ref not built")
+    mesh, data, rho = getattr(workloads, name)()
+    rim = -rho * np.random.default_rng(1).uniform(0.005, 0.05, len(rho))
+    s = ref.Session(mesh, copy.deepcopy(data), complex_=True)
+    k, w = s.kw()
+    rr, ri = s.response_complex(rho, rim)
+    Jr, Ji = s.jacobian_complex(rho, rim)
+    f = eo.Forward(mesh, copy.deepcopy(data), k, w)
+    orr, ori = eo.response_complex(f, rho, rim)
+    oJr, oJi = eo.jacobian_complex(f, rho, rim)
+    assert np.abs(orr - rr).max() <= 1e-12 * np.abs(rr).max()
+    assert np.abs(ori - ri).max() <= 1e-12 * np.abs(ri).max()
+    assert np.linalg.norm(oJr - Jr) <= 1e-12 * np.linalg.norm(Jr)
+    assert np.linalg.norm(oJi - Ji) <= 1e-12 * np.linalg.norm(Ji)
+
+
+def test_complex_oracle_uniform_phase():
+    """Scaling every resistivity by one complex z scales the potentials and
+    the response by z (the system matrix by 1/z) and leaves J = S(U)/(m^2/k)
+    real and equal to the real-model Jacobian.  A conjugated product in the
+    sensitivity (a plausible slip) breaks the second identity."""
+    mesh, data, rho = workloads.small_2d()
+    f = eo.Forward(mesh, copy.deepcopy(data))
+    z = np.exp(-0.04j)
+    r0, i0 = eo.response_complex(f, rho, 0 * rho)
+    J0, J0i = eo.jacobian_complex(f, rho, 0 * rho)
+    assert np.abs(i0).max() == 0.0 and np.abs(J0i).max() == 0.0
+    rz = rho * z
+    r1, i1 = eo.response_complex(f, rz.real, rz.imag)
+    assert np.abs((r1 + 1j * i1) - z * r0).max() <= 1e-12 * np.abs(r0).max()
+    Jr, Ji = eo.jacobian_complex(f, rz.real, rz.imag)
+    assert np.linalg.norm(Ji) <= 1e-12 * np.linalg.norm(J0)
+    assert np.linalg.norm(Jr - J0) <= 1e-12 * np.linalg.norm(J0)
+    # planted mistake: a conjugated factor in the sensitivity gives |z|^2 S
+    # instead of z^2 S, whose scaled Jacobian is far from real
+    S0 = J0 * (rho * rho)[None, :] / f.data["k"][:, None]
+    Jbad = S0 * abs(z) ** 2 / ((rz * rz)[None, :] / f.data["k"][:, None])
+    assert np.linalg.norm(Jbad.imag) > 1e-2 * np.linalg.norm(J0)
