@@ -288,6 +288,89 @@ def section_leg(args, log):
     return out, mesh, data
 
 
+def complex_ref_worker(out_path, nthreads, reps):
+    """CPU baseline of the complex leg in a child process: the reference's
+    own C++ core (oracle/_ref, DCMultiElectrodeModelling with setComplex(true),
+    complex envelope Cholesky in its solver plug-in), response +
+    createJacobian of the 2.5D section with chargeable block, default
+    wavenumbers; median of `reps` runs after one warm-up."""
+    os.dup2(2, 1)
+    from oracle import ref
+    from pygimli_amd import workloads
+    mesh, data, rho = workloads.ert2d_layers_block()
+    rim = workloads.chargeable_block_phases(mesh, rho)
+    ses = ref.Session(mesh, copy.deepcopy(data), nthreads=nthreads, complex_=True)
+    ts = []
+    for r in range(reps + 1):
+        t = time.perf_counter()
+        rr, ri = ses.response_complex(rho, rim)
+        Jr, Ji = ses.jacobian_complex(rho, rim)
+        ts.append(time.perf_counter() - t)
+    for name, a in (("re", rr), ("im", ri), ("Jre", Jr), ("Jim", Ji)):
+        np.save(_section_arrays(out_path, "complex", name), a)
+    with open(out_path, "w") as f:
+        json.dump({"s": float(np.median(ts[1:])), "nk": len(ses.kw()[0])}, f)
+
+
+def complex_leg(args, log):
+    """Complex resistivity: the 2.5D section with a chargeable block
+    (-30 mrad in the block, -5 mrad elsewhere), ERTModelling(sr=False) +
+    setComplex(True), direct band solver, default wavenumbers; a step =
+    complex response + complex Jacobian (FP64).  Also the 3D survey of
+    8 x 8 electrodes (202k nodes, 2000 data) with the same phases inside a
+    box, block-PCG mixed + two-level preconditioner, one step."""
+    import torch
+    from pygimli_amd import workloads
+    from pygimli_amd.ertModelling import ERTModelling
+    mesh, data, rho = workloads.ert2d_layers_block()
+    rim = workloads.chargeable_block_phases(mesh, rho)
+    fop = ERTModelling(sr=False)
+    fop.setComplex(True)
+    fop.setMesh(mesh)
+    fop.setData(copy.deepcopy(data))
+    fop.solverOptions.update(method="band", rtol=1e-10)
+    model = torch.as_tensor(np.concatenate([rho, rim]), device=fop.device)
+    for _ in range(args.warmup + 1):
+        fop.response_device(model)
+        fop.createJacobian(model)
+    torch.cuda.synchronize()
+    steps = max(args.steps, 5)
+    t = time.perf_counter()
+    for _ in range(steps):
+        r = fop.response_device(model)
+        fop.createJacobian(model)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / steps
+    Jr, Ji = fop.jacobianComplex()
+    res2 = {"ms_per_step": dt * 1e3, "evals_per_s": 1.0 / dt, "nk": len(fop._kv), "stats": fop.lastStats,
+            "resp": r.cpu().numpy(), "Jre": Jr.cpu().numpy(), "Jim": Ji.cpu().numpy(),
+            "workload": f"2.5D section: {mesh.nodeCount()} nodes / {mesh.cellCount()} triangles, 41 electrodes, "
+                        f"{data.size()} dipole-dipole data, 3 layers + block, phases -30 mrad (block) / -5 mrad"}
+    log(f"[complex] 2.5D: {dt * 1e3:.2f} ms, {fop.lastStats}")
+    mesh3, data3, rho3 = workloads.ert3d_grid_grf()
+    c = mesh3.cellCenters()
+    inside = (np.abs(c[:, 0]) < 4) & (np.abs(c[:, 1]) < 4) & (c[:, 2] > -6) & (c[:, 2] < -2)
+    rim3 = rho3 * np.tan(np.where(inside, -0.03, -0.005))
+    f3 = ERTModelling(sr=False)
+    f3.setComplex(True)
+    f3.setMesh(mesh3)
+    f3.setData(copy.deepcopy(data3))
+    f3.solverOptions.update(method="mixed", rtol=1e-6, coarse=args.coarse)
+    m3 = torch.as_tensor(np.concatenate([rho3, rim3]), device=f3.device)
+    f3.response_device(m3)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    f3.response_device(m3)
+    f3.createJacobian(m3)
+    torch.cuda.synchronize()
+    dt3 = time.perf_counter() - t
+    res3 = {"ms_per_step": dt3 * 1e3, "stats": f3.lastStats,
+            "workload": f"3D: {mesh3.nodeCount()} nodes / {mesh3.cellCount()} tetrahedra, 64 electrodes, "
+                        f"{data3.size()} data, phases -30 mrad (box) / -5 mrad, mixed + coarse, rtol 1e-6"}
+    log(f"[complex] 3D: {dt3 * 1e3:.1f} ms, {f3.lastStats}")
+    return res2, res3
+
+
 def learned_large_leg(args, log):
     """Learned warm start at scale: 3D mesh of 1.09M nodes / 6.3M tets, 128
     electrodes (8x8 grid + two 32-electrode lines), log-normal training
@@ -522,6 +605,8 @@ def main():
     ap.add_argument("--cgls-iters", type=int, default=20, help="CGLS iterations of the Gauss-Newton step timed")
     ap.add_argument("--survey-large", action="store_true",
                     help="run only the large 3D survey leg (256 electrodes, 10000 data, 1.5M tets, 60 GB Jacobian)")
+    ap.add_argument("--complex", action="store_true",
+                    help="complex-resistivity leg: 2.5D section with a chargeable block (plus a 3D step)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run one 1 GiB device-to-device copy (source evicted from the last-level cache): "
                          "the known-size kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -539,7 +624,7 @@ def main():
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported "
-                                   "(the path shards by RHS / cells, see DESIGN.md)"}))
+                                   "(the path would shard by right-hand sides and by cells)"}))
         sys.exit(2)
 
     cpu_proc = None
@@ -620,6 +705,46 @@ def main():
                                f"{nth} Jacobian threads; its solver plug-in is an RCM envelope Cholesky "
                                f"(CHOLMOD is absent)"),
                     "comparison": cmp}
+            except Exception as e:
+                out["cpu_baseline"] = {"error": f"no reference baseline ({e!r})"}
+        print(json.dumps(out))
+        return
+    if args.complex:
+        import multiprocessing as mp
+        cpu_proc = None
+        if not args.no_cpu:
+            os.makedirs(os.path.dirname(cpu_path), exist_ok=True)
+            cpu_proc = mp.get_context("spawn").Process(target=complex_ref_worker, args=(cpu_path, nth, 1),
+                                                       daemon=True)
+            cpu_proc.start()
+        r2, r3 = complex_leg(args, log)
+        out = {"metric": "complex-resistivity ERT forward+Jacobian/s, 2.5D section (41 electrodes), 1 GPU",
+               "value": r2["evals_per_s"], "unit": "evals/s", "n_gpus": 1, "steps": max(args.steps, 5),
+               "warmup": args.warmup + 1, "ms_per_step": r2["ms_per_step"], "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None, "dtype": "fp64", "data": "synthetic",
+               "config": {"workload": r2["workload"] + "; direct band solver (FP64), FP64 complex Jacobian"},
+               "complex_2d": {k: v for k, v in r2.items() if k not in ("resp", "Jre", "Jim")},
+               "complex_3d": r3}
+        if cpu_proc is not None:
+            cpu_proc.join(args.cpu_timeout)
+            if cpu_proc.is_alive():
+                cpu_proc.terminate()
+            try:
+                with open(cpu_path) as f:
+                    r = json.load(f)
+                D = len(r2["resp"]) // 2
+                rr, ri, Jr, Ji = (np.load(_section_arrays(cpu_path, "complex", n)) for n in ("re", "im", "Jre", "Jim"))
+                out["cpu_baseline"] = {
+                    "value": 1.0 / r["s"], "unit": "evals/s", "cores": nth, "kind": "reference",
+                    "sample": (f"the whole 2.5D workload (complex response + createJacobian) through the "
+                               f"reference's own C++ core (oracle/_ref, setComplex(true)), one run after a "
+                               f"warm-up, {nth} Jacobian threads, complex envelope Cholesky as its solver"),
+                    "comparison": {
+                        "speedup": r["s"] / (r2["ms_per_step"] * 1e-3), "nk": r["nk"],
+                        "re_max_rel_diff": float(np.abs(r2["resp"][:D] - rr).max() / np.abs(rr).max()),
+                        "im_max_rel_diff": float(np.abs(r2["resp"][D:] - ri).max() / np.abs(ri).max()),
+                        "Jre_rel_fro_diff": float(np.linalg.norm(r2["Jre"] - Jr) / np.linalg.norm(Jr)),
+                        "Jim_rel_fro_diff": float(np.linalg.norm(r2["Jim"] - Ji) / np.linalg.norm(Ji))}}
             except Exception as e:
                 out["cpu_baseline"] = {"error": f"no reference baseline ({e!r})"}
         print(json.dumps(out))

@@ -574,7 +574,7 @@ void forward_finish(Ctx & c, double * rhoa) {
 }
 
 // one real solve of c.B into c.X (c.X holds the initial guess)
-void solve_one(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st) {
+void solve_one(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st, bool refactor = true) {
     if (o.method == PGERT_PCG_FP64)
         solve_pcg_fp64(c, o, st);
     else if (o.method == PGERT_BLOCK_FP64)
@@ -583,7 +583,7 @@ void solve_one(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st) {
         solve_block(c, o, st, true);
     else if (o.method == PGERT_BAND_FP64) {
         std::vector<pgert_solve_stats> v;
-        solve_band({&c}, o, v);
+        solve_band({&c}, o, v, refactor);
         st = v[0];
     } else
         throw std::runtime_error("pgert_forward: unknown method");
@@ -672,7 +672,7 @@ int pgert_forward_complex(pgert_ctx * c, const double * rho_re, const double * r
                     launch_spmm_add(C, C.valsAi.p, C.Xre.p, nullptr, -1.0, C.B.p);
                 PG_CHECK(hipMemcpyAsync(C.X.p, Xh, C.X.bytes(), hipMemcpyDeviceToDevice, s));
                 st = pgert_solve_stats{};
-                solve_one(C, o, st);
+                solve_one(C, o, st, it == 0 && half == 0);   // A_re is the same in every solve
                 tot.iterations += st.iterations;
                 tot.max_iterations += st.max_iterations;
                 tot.refinements += st.refinements;

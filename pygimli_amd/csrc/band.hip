@@ -471,7 +471,8 @@ void band_prepare(Ctx & c) {
     c.bandValid = true;
 }
 
-void solve_band(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std::vector<pgert_solve_stats> & stats) {
+void solve_band(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std::vector<pgert_solve_stats> & stats,
+                bool refactor) {
     PG_REQUIRE(!cs.empty(), "band solver: no problems");
     Ctx & c0 = *cs[0];
     hipStream_t s = c0.stream;
@@ -492,7 +493,7 @@ void solve_band(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std::
     Lp.reserve(Lsz);
     Dp.reserve(Dsz);
     Yp.reserve(Ysz);
-    PG_CHECK(hipMemsetAsync(Lp.p, 0, Lsz * sizeof(double), s));
+    if (refactor) PG_CHECK(hipMemsetAsync(Lp.p, 0, Lsz * sizeof(double), s));
     std::vector<BandDesc> h;
     size_t lo = 0, dof = 0, yo = 0;
     int maxn = 0, maxnnz = 0;
@@ -524,14 +525,18 @@ void solve_band(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std::
     struct { const BandDesc * p; } dd{(const BandDesc *)ddb.p};
     const int nd = (int)h.size();
     const dim3 blk(256);
-    hipLaunchKernelGGL(k_band_scatter, dim3(std::max(1, std::min(div_up(maxnnz, 256), 4096 / nd + 1)), nd), blk, 0, s,
-                       dd.p);
     const size_t fl = factor_lds(bmax), sl = solve_lds(bmax);
     PG_REQUIRE(fl <= 160 * 1024 && sl <= 160 * 1024, "band solver: band too wide for LDS");
-    auto factor = NB + bmax <= 256 ? k_band_factor<1> : k_band_factor<2>;
-    PG_CHECK(hipFuncSetAttribute((const void *)factor, hipFuncAttributeMaxDynamicSharedMemorySize, (int)fl));
+    // refactor == false: the factors of the previous call on the same problems
+    // and matrices are still in L / Dinv (new right-hand sides only)
+    if (refactor) {
+        hipLaunchKernelGGL(k_band_scatter, dim3(std::max(1, std::min(div_up(maxnnz, 256), 4096 / nd + 1)), nd), blk,
+                           0, s, dd.p);
+        auto factor = NB + bmax <= 256 ? k_band_factor<1> : k_band_factor<2>;
+        PG_CHECK(hipFuncSetAttribute((const void *)factor, hipFuncAttributeMaxDynamicSharedMemorySize, (int)fl));
+        hipLaunchKernelGGL(factor, dim3(nd), blk, fl, s, dd.p);
+    }
     PG_CHECK(hipFuncSetAttribute((const void *)k_band_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sl));
-    hipLaunchKernelGGL(factor, dim3(nd), blk, fl, s, dd.p);
     hipLaunchKernelGGL(k_band_solve, dim3(div_up(emax, CG), nd), blk, sl, s, dd.p);
     PG_CHECK(hipGetLastError());
     PG_CHECK(hipStreamSynchronize(s));  // the descriptors' host copy lives until here

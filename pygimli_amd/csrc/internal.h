@@ -264,7 +264,8 @@ void solve_blocks(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std
 double residual_check(Ctx & c);
 // direct banded Cholesky over every wavenumber of several 2D problems (band.hip)
 void band_prepare(Ctx & c);
-void solve_band(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std::vector<pgert_solve_stats> & st);
+void solve_band(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std::vector<pgert_solve_stats> & st,
+                bool refactor = true);
 
 // two-level preconditioner (coarse.hip)
 void coarse_prepare(Ctx & c);
