@@ -159,6 +159,25 @@ int pgert_batch_forward(pgert_ctx * const * ctxs, int count, const double * cons
  * (dcfemmodelling.cpp:1681-1682).  DEVICE [nK * nE * n]. */
 int pgert_potentials(pgert_ctx * ctx, double * out);
 
+/* Total-field operator, DCMultiElectrodeModelling (ertModelling.py:103-106 with
+ * sr=False): the system of pgert_forward without singularity removal, unit
+ * point sources at the electrode nodes (calculateK_, dcfemmodelling.cpp:1755-1928,
+ * ElectrodeShapeNode::assembleRHS electrode.cpp:133-151), potentials = the
+ * solution, then the same response (rounding to 1e-10, reciprocal geometric
+ * mean, :1140-1196).  pgert_jacobian and pgert_potentials work on its
+ * potentials as on pgert_forward's.  Arguments as pgert_forward (warm_start:
+ * a total-field guess). */
+int pgert_forward_total(pgert_ctx * ctx, const double * rho, const pgert_solve_opts * opts, double * rhoa,
+                        pgert_solve_stats * stats);
+
+/* Bypass resistors of the total-field operator
+ * (assembleStiffnessMatrixDCFEMByPass_, dcfemmodelling.cpp:601-707, applied in
+ * calculateK_ :1833 only): conductance[i] = 1/R between nodes a[i] and b[i],
+ * added to every wavenumber's matrix by pgert_forward_total.  The pairs must
+ * share a cell (matrix pattern of the mesh).  n_pairs = 0 removes them. HOST
+ * arrays. */
+int pgert_set_bypass(pgert_ctx * ctx, int n_pairs, const int * a, const int * b, const double * conductance);
+
 /* Complex resistivity, DCMultiElectrodeModelling with setComplex(true)
  * (ertModelling.py:217-238 with sr=False; dcfemmodelling.cpp:1103-1120 response,
  * :163-292 assembly with 1/rho complex, :1755-1928 calculateK_ total-field

@@ -255,11 +255,27 @@ class Setup:
             yconst = not (np.ptp(ePos[:, 1]) > 0) and np.abs(ePos[:, 1]).max() < 1e-8
             if zvar and yconst:
                 ePos[:, [1, 2]] = ePos[:, [2, 1]]
+        # bypass electrodes: nodes with one marker <= -10000 (searchElectrodes_
+        # :815-826), matched to a sensor at the group's first node (:886-897)
+        nm = mesh.nodeMarkers()
+        self.bypass = {}
+        for nid in range(len(nm)):
+            if nm[nid] <= -10000:
+                self.bypass.setdefault(int(nm[nid]), []).append(nid)
+        free = sorted(self.bypass, key=lambda m: -10000 - m)
         cand = list(mesh.findNodesIdxByMarker(-99))
         nodes = mesh.nodes
         self.elecNode = np.empty(len(ePos), np.int64)
         for i, p in enumerate(ePos):
             hit = None
+            for m in free:
+                if np.linalg.norm(p - nodes[self.bypass[m][0]]) < 0.01:
+                    hit = m
+                    break
+            if hit is not None:
+                free.remove(hit)
+                self.elecNode[i] = self.bypass[hit][0]
+                continue
             for j, nid in enumerate(cand):
                 if np.linalg.norm(p - nodes[nid]) < 0.01:
                     hit = j
@@ -477,6 +493,7 @@ class Forward:
         self.nE = len(self.setup.elecNode)
         self.subpots = None
         self.prim = None
+        self.bypassR = {}      # node marker -> bypass resistance (electrodeBypassResistances.map)
 
     def primaries(self):
         """checkPrimpotentials_ (dcfemmodelling.cpp:2086-2110): rows
@@ -518,6 +535,18 @@ class Forward:
         U = np.empty((len(self.k) * self.nE, n), dt)
         for ki, k in enumerate(self.k):
             S = assemble(self.mesh, rho, k, self.setup.sourceCenter, self.geo)
+            if dt == np.float64 and self.setup.bypass:
+                # assembleStiffnessMatrixDCFEMByPass_ (:671-707): 1e-6 Ohm between
+                # consecutive nodes of a group (real systems only, :597-599)
+                S = S.tolil()
+                for m, ids in self.setup.bypass.items():
+                    for a1, a2 in zip(ids[:-1], ids[1:]):
+                        g = 1.0 / self.bypassR.get(m, 1e-6)
+                        S[a1, a1] += g
+                        S[a2, a2] += g
+                        S[a1, a2] -= g
+                        S[a2, a1] -= g
+                S = S.tocsr()
             lu = spla.splu(S.tocsc())
             for i in range(self.nE):
                 rhs = np.zeros(n, dt)
@@ -567,6 +596,22 @@ class Forward:
         if len(model) == J.shape[1]:
             J *= self.data["k"][:, None] / (model * model)[None, :]
         return J
+
+
+def response_total(fwd, model):
+    """DCMultiElectrodeModelling::response (dcfemmodelling.cpp:1085-1197) with
+    the total-field calculateK_ (:1755-1928): unit point sources, no
+    singularity removal; the response rounding and reciprocity of the SR path."""
+    model = np.asarray(model, np.float64)
+    kf = fwd.data["k"]
+    if not fwd.data.haveData("k") or np.min(np.abs(kf)) < TOLERANCE:
+        kf = fwd.geometric_factors()
+        fwd.data.set("k", kf)
+    fwd.subpots = fwd.solve_point_source(map_model(fwd.mesh, model))
+    emap = fwd.electrode_map(fwd.potentials())
+    u = collect_data(emap, fwd.data.abmn(), False)
+    ur = collect_data(emap, fwd.data.abmn(), True)
+    return np.sqrt(np.abs(np.rint(u / 1e-10) * 1e-10 * kf * (np.rint(ur / 1e-10) * 1e-10 * kf)))
 
 
 def response_complex(fwd, model_re, model_im):

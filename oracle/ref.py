@@ -57,7 +57,7 @@ def lib():
         L.ref_mixed_bc.argtypes = [ctypes.c_void_p, _D, ctypes.c_double, _D]
         L.ref_boundary_count.argtypes = [ctypes.c_void_p]
         L.ref_boundary_nodes.argtypes = [ctypes.c_void_p, ctypes.c_int, _I, _I]
-        L.ref_session_init_fop_complex.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, ctypes.c_int]
+        L.ref_session_init_fop_total.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, ctypes.c_int, ctypes.c_int]
         L.ref_session_response_complex.argtypes = [ctypes.c_void_p, _D, _D, ctypes.c_int, _D, _D]
         L.ref_session_jacobian_complex.argtypes = [ctypes.c_void_p, _D, _D, ctypes.c_int, _D, _D, _I, _I]
         L.ref_mesh_convert.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
@@ -134,7 +134,9 @@ def data_convert(src, dst, fmt_data="all", fmt_sensor="x y z", remove_invalid=Tr
 class Session:
     """The reference DCSRMultiElectrodeModelling on one mesh + data set."""
 
-    def __init__(self, mesh, data=None, kvalues=None, weights=None, nthreads=0, complex_=False):
+    def __init__(self, mesh, data=None, kvalues=None, weights=None, nthreads=0, complex_=False, sr=True):
+        """sr=False: the total-field DCMultiElectrodeModelling (complex_=True
+        implies it)."""
         L = lib()
         self._keep = []
         nodes, pn = _d(mesh.nodes)
@@ -159,7 +161,11 @@ class Session:
                 k, pk = _d(k)
             _check(L.ref_session_set_data(self.h, data.sensorCount(), ps, data.size(),
                                           pa, pbb, pm, pnn, pk))
-            init = L.ref_session_init_fop_complex if complex_ else L.ref_session_init_fop
+            if complex_ or not sr:
+                def init(h, nk, pk, pw, nt):
+                    return L.ref_session_init_fop_total(h, nk, pk, pw, nt, 1 if complex_ else 0)
+            else:
+                init = L.ref_session_init_fop
             if kvalues is not None:
                 kv, pkv = _d(kvalues); wv, pwv = _d(weights)
                 _check(init(self.h, len(kv), pkv, pwv, nthreads))

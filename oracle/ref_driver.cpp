@@ -343,14 +343,16 @@ int ref_session_init_fop(void * h, int nk, const double * kv, const double * wv,
     });
 }
 
-// ERTModelling(sr=False) + setComplex(True) (ertModelling.py:103-121): the
-// total-field DCMultiElectrodeModelling on complex resistivities.
-int ref_session_init_fop_complex(void * h, int nk, const double * kv, const double * wv, int nthreads) {
+// ERTModelling(sr=False) (ertModelling.py:103-106): the total-field
+// DCMultiElectrodeModelling, real (complex = 0) or on complex resistivities
+// (complex = 1, setComplex(True)).
+int ref_session_init_fop_total(void * h, int nk, const double * kv, const double * wv, int nthreads,
+                               int complex) {
     Session * s = (Session *)h;
     return guarded([&] {
         delete s->fop;
         s->fop = new DCMultiElectrodeModelling(false);
-        s->fop->setComplex(true);
+        s->fop->setComplex(complex != 0);
         s->fop->setMesh(s->mesh, true);
         s->fop->setData(s->data);
         if (nk > 0) {

@@ -12,6 +12,9 @@ import numpy as np
 
 MAX_DOUBLE = float(np.finfo(np.float64).max)
 TOLERANCE = 1e-12
+MARKER_BOUND_ELECTRODE = -10000        # core/src/bert/bert.h:33
+MARKER_FIXEDVALUE_REGION = -1000000    # core/src/gimli.h:241
+BYPASS_RESISTANCE = 1e-6               # dcfemmodelling.cpp:684-688 default
 
 
 def _gauss_legendre_nr(n):
@@ -134,7 +137,9 @@ def geometric_factors(sensors, abmn, dim=3):
 class MeshAnalysis:
     """Boundary/topography analysis and electrode search
     (updateMeshDependency_ dcfemmodelling.cpp:711-773, searchElectrodes_ :790-1073)
-    for node electrodes (marker -99)."""
+    for node electrodes (marker -99) and bypass electrodes: nodes sharing a
+    marker <= -10000 form one electrode at the group's first node, joined
+    node to node by bypass resistors (:815-826, :886-897)."""
 
     def __init__(self, mesh, sensors):
         dim = mesh.dim()
@@ -158,11 +163,30 @@ class MeshAnalysis:
             ystill = not (np.ptp(pos[:, 1]) > 0) and np.abs(pos[:, 1]).max() < 1e-8
             if zvar and ystill:
                 pos[:, [1, 2]] = pos[:, [2, 1]]
+        cm = mesh.cellMarkers()
+        if np.any(bm <= MARKER_BOUND_ELECTRODE + 1) or np.any((cm <= MARKER_BOUND_ELECTRODE) &
+                                                               (cm > MARKER_FIXEDVALUE_REGION)):
+            raise NotImplementedError("complete electrode model (electrode boundaries / cells with marker "
+                                      "<= -10000) is not supported")
+        nmk = np.asarray(mesh.nodeMarkers())
+        self.bypassGroups = {}             # marker -> node ids ascending, markers in order of appearance
+        for i in np.nonzero(nmk <= MARKER_BOUND_ELECTRODE)[0]:
+            self.bypassGroups.setdefault(int(nmk[i]), []).append(int(i))
+        free = sorted(self.bypassGroups, key=lambda m: MARKER_BOUND_ELECTRODE - m)   # map order
         cand = list(mesh.findNodesIdxByMarker(-99))
         nodes = mesh.positions()
         self.elecNode = np.empty(len(pos), np.int32)
         for i, p in enumerate(pos):
             hit = None
+            for m in free:
+                first = self.bypassGroups[m][0]
+                if np.sqrt(((nodes[first] - p) ** 2).sum()) < 0.01:
+                    hit = m
+                    break
+            if hit is not None:
+                free.remove(hit)
+                self.elecNode[i] = self.bypassGroups[hit][0]
+                continue
             if cand:
                 d = np.sqrt(((nodes[cand] - p) ** 2).sum(-1))
                 close = np.nonzero(d < 0.01)[0]
@@ -175,3 +199,17 @@ class MeshAnalysis:
             self.elecNode[i] = cand.pop(hit)
         self.elecPos = nodes[self.elecNode]
         self.sourceCenter = self.elecPos.sum(0) / len(self.elecPos)
+
+    def bypass_pairs(self, resistances=None):
+        """Node pairs (i, i+1) of every bypass group with their conductance
+        1 / R, R from `resistances` (marker -> Ohm, the reference's
+        electrodeBypassResistances.map) or 1e-6 Ohm
+        (assembleStiffnessMatrixDCFEMByPass_, dcfemmodelling.cpp:671-707)."""
+        a, b, g = [], [], []
+        for m, ids in self.bypassGroups.items():
+            r = BYPASS_RESISTANCE if resistances is None else float(resistances.get(m, BYPASS_RESISTANCE))
+            for i in range(len(ids) - 1):
+                a.append(ids[i])
+                b.append(ids[i + 1])
+                g.append(1.0 / r)
+        return np.array(a, np.int32), np.array(b, np.int32), np.array(g, np.float64)

@@ -697,6 +697,72 @@ int pgert_forward_complex(pgert_ctx * c, const double * rho_re, const double * r
     });
 }
 
+int pgert_forward_total(pgert_ctx * c, const double * rho, const pgert_solve_opts * opts, double * rhoa,
+                        pgert_solve_stats * stats) {
+    return guarded(c, [&] {
+        PG_REQUIRE(rho && rhoa, "pgert_forward_total: null pointer");
+        Ctx & C = *c;
+        PG_REQUIRE(C.C > 0 && C.nE > 0 && C.nK > 0 && C.D > 0,
+                   "pgert_forward_total: mesh, electrodes, wavenumbers and data must be set");
+        pgert_solve_opts o = resolve_opts(opts);
+        o.learned_warm_start = 0;
+        hipStream_t s = C.stream;
+        const size_t nk = C.nK, blk = nk * C.n * C.nE;
+        PG_CHECK(hipEventRecord(C.ev[0], s));
+        C.rho.alloc(C.C);
+        PG_CHECK(hipMemcpyAsync(C.rho.p, rho, (size_t)C.C * 8, hipMemcpyDeviceToDevice, s));
+        C.valsA.alloc(nk * C.nnz);
+        for (size_t kk = 0; kk < nk; ++kk) launch_assemble(C, C.rho.p, C.kv[kk], C.valsA.p + kk * C.nnz, false);
+        launch_bypass(C, C.valsA.p);
+        launch_diag_inv(C);
+        PG_CHECK(hipEventRecord(C.ev[1], s));
+        C.B.alloc(blk);
+        launch_point_rhs(C, C.B.p);
+        C.X.alloc(blk);
+        if (o.warm_start)
+            PG_CHECK(hipMemcpyAsync(C.X.p, o.warm_start, C.X.bytes(), hipMemcpyDeviceToDevice, s));
+        else
+            PG_CHECK(hipMemsetAsync(C.X.p, 0, C.X.bytes(), s));
+        PG_CHECK(hipEventRecord(C.ev[2], s));
+        pgert_solve_stats st{};
+        solve_one(C, o, st);
+        PG_CHECK(hipEventRecord(C.ev[3], s));
+        C.U.alloc(blk);
+        PG_CHECK(hipMemcpyAsync(C.U.p, C.X.p, C.U.bytes(), hipMemcpyDeviceToDevice, s));
+        C.haveU = true;
+        launch_response(C, rhoa);
+        PG_CHECK(hipEventRecord(C.ev[4], s));
+        sync(C);
+        read_timings(C);
+        if (stats) *stats = st;
+    });
+}
+
+int pgert_set_bypass(pgert_ctx * c, int n_pairs, const int * a, const int * b, const double * conductance) {
+    return guarded(c, [&] {
+        Ctx & C = *c;
+        PG_REQUIRE(C.n > 0, "pgert_set_bypass: set the mesh first");
+        PG_REQUIRE(n_pairs >= 0 && (n_pairs == 0 || (a && b && conductance)), "pgert_set_bypass: bad arguments");
+        std::vector<int> slot;
+        std::vector<double> val;
+        for (int i = 0; i < n_pairs; ++i) {
+            PG_REQUIRE(a[i] >= 0 && a[i] < C.n && b[i] >= 0 && b[i] < C.n && a[i] != b[i],
+                       "pgert_set_bypass: bad node pair " + std::to_string(i));
+            const int sab = find_slot(C, a[i], b[i]), sba = find_slot(C, b[i], a[i]);
+            PG_REQUIRE(sab >= 0 && sba >= 0, "pgert_set_bypass: nodes " + std::to_string(a[i]) + " and " +
+                                                 std::to_string(b[i]) + " share no cell (only bypass resistors "
+                                                 "along mesh edges are supported)");
+            const double g = conductance[i];
+            slot.insert(slot.end(), {find_slot(C, a[i], a[i]), find_slot(C, b[i], b[i]), sab, sba});
+            val.insert(val.end(), {g, g, -g, -g});
+        }
+        C.nByp = n_pairs;
+        C.bypSlot.upload(slot, C.stream);
+        C.bypVal.upload(val, C.stream);
+        sync(C);
+    });
+}
+
 int pgert_jacobian_complex(pgert_ctx * c, const double * model_re, const double * model_im, int n_model,
                            double * J_re, double * J_im) {
     return guarded(c, [&] {

@@ -209,6 +209,16 @@ __global__ void k_jacobian_c(long total, int nParam, const double * __restrict__
     Jim[t] = (zi * qr - zr * qi) / d;
 }
 
+// bypass conductances into every wavenumber's matrix, one thread per matrix
+// (a node shared by two resistors takes both, in order)
+__global__ void k_bypass(int nK, int nnz, int nSlot, const int * __restrict__ slot, const double * __restrict__ val,
+                         double * __restrict__ vals) {
+    const int kk = blockIdx.x * blockDim.x + threadIdx.x;
+    if (kk >= nK) return;
+    double * A = vals + (size_t)kk * nnz;
+    for (int i = 0; i < nSlot; ++i) A[slot[i]] += val[i];
+}
+
 __global__ void k_add(long total, const double * __restrict__ a, const double * __restrict__ b,
                       double * __restrict__ out) {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -266,6 +276,13 @@ void launch_response_complex(Ctx & c, const double * Ure, const double * Uim, do
     PG_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_response_c, dim3(div_up(c.D, 256)), dim3(256), 0, c.stream, c.D, c.nE, c.abmn.p,
                        c.kfac.p, c.emap.p, c.emap.p + ne2, out_re, out_im);
+    PG_CHECK(hipGetLastError());
+}
+
+void launch_bypass(Ctx & c, double * vals) {
+    if (c.nByp == 0) return;
+    hipLaunchKernelGGL(k_bypass, dim3(div_up(c.nK, 64)), dim3(64), 0, c.stream, c.nK, c.nnz, 4 * c.nByp,
+                       c.bypSlot.p, c.bypVal.p, vals);
     PG_CHECK(hipGetLastError());
 }
 

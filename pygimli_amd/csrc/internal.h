@@ -169,6 +169,10 @@ struct Ctx {
     // complex resistivity: imaginary half of A, unit point sources, the two
     // halves of the potentials, Jacobian work blocks, change accumulator
     DBuf<double> valsAi, Bpt, Xre, Xim, Jw, chg;
+    // bypass resistors of the total-field operator: matrix slots and values
+    DBuf<int> bypSlot;
+    DBuf<double> bypVal;
+    int nByp = 0;
     bool haveUc = false;
 
     // block solver's own stream (graph capture needs a non-default stream),
@@ -237,6 +241,7 @@ void launch_jacobian(Ctx & c, const double * model, int nModel, void * J, int dt
 void launch_assemble_complex(Ctx & c, const double * rre, const double * rim, double k, double * vre,
                              double * vim);
 void launch_point_rhs(Ctx & c, double * B);
+void launch_bypass(Ctx & c, double * vals);
 void launch_spmm_add(Ctx & c, const double * vals, const double * X, const double * base, double sign,
                      double * out);
 void launch_change(Ctx & c, long total, const double * a, const double * b, double * acc);

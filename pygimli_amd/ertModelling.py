@@ -42,8 +42,8 @@ class ERTModelling:
     ----------
     sr : bool
         Singularity removal (the reference default).  sr=False gives the
-        total-field operator, provided for complex resistivities
-        (setComplex(True)), the only case the reference computes without SR.
+        total-field operator (DCMultiElectrodeModelling), also the one for
+        complex resistivities (setComplex(True)).
     verbose : bool
         Print solver statistics.
     device : int
@@ -55,6 +55,7 @@ class ERTModelling:
             raise RuntimeError("ERTModelling needs a HIP device (no GPU visible)")
         self.verbose = verbose
         self._sr = bool(sr)
+        self._bypassR = None
         self._complex = False
         self._Jc = None
         self.device = torch.device("cuda", device)
@@ -90,6 +91,15 @@ class ERTModelling:
 
     def complex(self):
         return self._complex
+
+    def setBypassResistances(self, resistances):
+        """Resistance (Ohm) of the bypass resistors per node marker <= -10000
+        (the reference reads electrodeBypassResistances.map; default 1e-6).
+        Bypass resistors act in the total-field operator (sr=False), as in the
+        reference."""
+        self._bypassR = {int(k): float(v) for k, v in dict(resistances).items()}
+        if self._mesh is not None and self._data is not None:
+            self._setup_electrodes()
 
     def setVerbose(self, v):
         self.verbose = bool(v)
@@ -160,6 +170,11 @@ class ERTModelling:
         sc, psc = _host_d(an.sourceCenter)
         ctx = self._ctx
         ctx.check(ctx.L.pgert_set_electrodes(ctx.h, len(en), pen, psc, an.surfaceZ), "setElectrodes")
+        ba, bb, bg = an.bypass_pairs(self._bypassR)
+        ba, pba = _host_i(ba)
+        bb, pbb = _host_i(bb)
+        bg, pbg = _host_d(bg)
+        ctx.check(ctx.L.pgert_set_bypass(ctx.h, len(ba), pba, pbb, pbg), "bypass")
         if not self._kUser or self._kv is None:
             self._kv, self._wv = kwave_list(mesh.dim(), an.elecPos)
         self._dirty = True
@@ -260,8 +275,6 @@ class ERTModelling:
         t = self._model_tensor(model)
         if self._complex:
             return self._response_complex(t)
-        if not self._sr:
-            raise NotImplementedError("sr=False is provided for complex resistivities (setComplex(True)) only")
         if float(t.min()) < 1e-12:
             raise ValueError("response for model with negative or zero resistivity is not defined")
         rho = self.mapERTModel(t)
@@ -269,8 +282,13 @@ class ERTModelling:
         stats = _lib.SolveStats()
         opts = self._opts()
         self._set_stream()
-        self._ctx.check(self._ctx.L.pgert_forward(self._ctx.h, _dptr(rho), ctypes.byref(opts), _dptr(rhoa),
-                                                  ctypes.byref(stats)), "response")
+        if self._sr:
+            self._ctx.check(self._ctx.L.pgert_forward(self._ctx.h, _dptr(rho), ctypes.byref(opts), _dptr(rhoa),
+                                                      ctypes.byref(stats)), "response")
+        else:
+            opts.learned_warm_start = 0
+            self._ctx.check(self._ctx.L.pgert_forward_total(self._ctx.h, _dptr(rho), ctypes.byref(opts),
+                                                            _dptr(rhoa), ctypes.byref(stats)), "response")
         self.lastStats = stats.as_dict()
         self._model = t
         if self.verbose:

@@ -117,7 +117,10 @@ def test_complex_errors():
         ERTModelling().setComplex(True)
     mesh, data, rho = workloads.small_2d()
     fop = ERTModelling(sr=False)
+    fop.setComplex(True)
     fop.setMesh(mesh)
     fop.setData(copy.deepcopy(data))
-    with pytest.raises(NotImplementedError):
-        fop.response(rho)
+    with pytest.raises(ValueError):
+        fop.response(rho[:-1])                     # not [re, im] of one length
+    with pytest.raises(ValueError):
+        fop.response(np.concatenate([rho * 0, rho * 0]))   # |rho| = 0

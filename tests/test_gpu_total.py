@@ -1,0 +1,108 @@
+"""Total-field operator on the GPU: ERTModelling(sr=False), the reference's
+DCMultiElectrodeModelling (core/src/bert/dcfemmodelling.cpp:1755-1928
+calculateK_ with unit point sources, :1085-1197 response, :1318-1408
+Jacobian), against the oracle's restatement (oracle/ert_oracle.py
+response_total, equal to the reference's own C++ core in tests/test_oracle.py)
+and against the reference build directly."""
+import copy
+
+import numpy as np
+import pytest
+
+from pygimli_amd import workloads
+from oracle import ert_oracle as eo
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name", ["small_2d", "small_3d"])
+@pytest.mark.parametrize("method", ["band", "block64", "mixed"])
+def test_total_field_vs_oracle(name, method):
+    from pygimli_amd.ertModelling import ERTModelling
+    mesh, data, rho = getattr(workloads, name)()
+    if method == "band" and mesh.dim() == 3:
+        pytest.skip("direct band solver: 2D meshes")
+    fop = ERTModelling(sr=False)
+    fop.setMesh(mesh)
+    fop.setData(copy.deepcopy(data))
+    fop.solverOptions.update(method=method, rtol=1e-12, coarse=0)
+    r = fop.response(rho)
+    fop.createJacobian(rho)
+    J = fop.jacobian().cpu().numpy()
+    f = eo.Forward(mesh, copy.deepcopy(data), fop.kValues(), fop.weights())
+    o = eo.response_total(f, rho)
+    oJ = f.jacobian(rho)
+    e1 = np.abs(r - o).max() / np.abs(o).max()
+    e2 = np.linalg.norm(J - oJ) / np.linalg.norm(oJ)
+    print(name, method, e1, e2, fop.lastStats)
+    tol = 1e-9 if method != "mixed" else 1e-8
+    assert e1 < tol and e2 < tol
+
+
+def test_total_field_vs_reference_build():
+    from oracle import ref
+    if not ref.available():
+        pytest.skip("reference build oracle/_ref not present")
+    from pygimli_amd.ertModelling import ERTModelling
+    mesh, data, rho = workloads.ert2d_layers_block()
+    ses = ref.Session(mesh, copy.deepcopy(data), nthreads=4, sr=False)
+    r_ref = ses.response(rho)
+    J_ref = ses.jacobian(rho)
+    fop = ERTModelling(sr=False)
+    fop.setMesh(mesh)
+    fop.setData(copy.deepcopy(data))
+    fop.solverOptions.update(method="band", rtol=1e-10)
+    r = fop.response(rho)
+    fop.createJacobian(rho)
+    J = fop.jacobian().cpu().numpy()
+    e1 = np.abs(r - r_ref).max() / np.abs(r_ref).max()
+    e2 = np.linalg.norm(J - J_ref) / np.linalg.norm(J_ref)
+    print("total field vs reference build:", e1, e2)
+    assert e1 < 1e-10 and e2 < 1e-10
+
+
+def test_bypass_vs_oracle_and_reference(tmp_path, monkeypatch):
+    """Bypass electrodes (node groups with marker <= -10000) in the
+    total-field operator: against the oracle with the same resistances, and
+    against the reference build, whose resistors act as R / 2 (each added once
+    per electrode search, two searches)."""
+    from pygimli_amd.ertModelling import ERTModelling
+    mesh, data, rho = workloads.small_2d_bypass()
+    R = {-10000: 50.0, -10001: 100.0}
+    out = {}
+    for method in ("band", "block64"):
+        fop = ERTModelling(sr=False)
+        fop.setBypassResistances(R)
+        fop.setMesh(mesh)
+        fop.setData(copy.deepcopy(data))
+        fop.solverOptions.update(method=method, rtol=1e-12, coarse=0)
+        r = fop.response(rho)
+        fop.createJacobian(rho)
+        out[method] = (r, fop.jacobian().cpu().numpy(), fop.kValues(), fop.weights())
+    r, J, k, w = out["band"]
+    f = eo.Forward(mesh, copy.deepcopy(data), k, w)
+    f.bypassR = R
+    o = eo.response_total(f, rho)
+    oJ = f.jacobian(rho)
+    for method, (rm, Jm, _, _) in out.items():
+        e1 = np.abs(rm - o).max() / np.abs(o).max()
+        e2 = np.linalg.norm(Jm - oJ) / np.linalg.norm(oJ)
+        print("bypass", method, e1, e2)
+        assert e1 < 1e-9 and e2 < 1e-9
+    from oracle import ref
+    if ref.available():
+        (tmp_path / "electrodeBypassResistances.map").write_text("-10000 100\n-10001 200\n")
+        monkeypatch.chdir(tmp_path)
+        s = ref.Session(mesh, copy.deepcopy(data), k, w, sr=False)
+        rr = s.response(rho)
+        assert np.abs(r - rr).max() <= 1e-9 * np.abs(rr).max()
+    # bypass electrodes need the electrode at the group's first node
+    with pytest.raises(Exception):
+        from pygimli_amd.mesh import Mesh
+        nm = mesh.nodeMarkers().copy()
+        g = np.nonzero(nm == -10000)[0]
+        nm[g[0]] = 0
+        bad = Mesh(2, mesh.nodes, mesh.cells, mesh.cellMarkers(), mesh.bounds, mesh.boundaryMarkers(), nm)
+        fop = ERTModelling(sr=False)
+        fop.setMesh(bad)
+        fop.setData(copy.deepcopy(data))

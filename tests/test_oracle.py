@@ -409,3 +409,52 @@ def test_complex_oracle_uniform_phase():
     S0 = J0 * (rho * rho)[None, :] / f.data["k"][:, None]
     Jbad = S0 * abs(z) ** 2 / ((rz * rz)[None, :] / f.data["k"][:, None])
     assert np.linalg.norm(Jbad.imag) > 1e-2 * np.linalg.norm(J0)
+
+
+@pytest.mark.parametrize("name", ["small_2d", "small_3d"])
+def test_total_field_oracle_vs_reference_build(name):
+    """ERTModelling(sr=False): the total-field DCMultiElectrodeModelling,
+    oracle restatement against the reference's own C++ core."""
+    if not ref.available():
+        pytest.skip("oracle/_ref not built")
+    mesh, data, rho = getattr(workloads, name)()
+    s = ref.Session(mesh, copy.deepcopy(data), sr=False)
+    k, w = s.kw()
+    r = s.response(rho)
+    J = s.jacobian(rho)
+    f = eo.Forward(mesh, copy.deepcopy(data), k, w)
+    o = eo.response_total(f, rho)
+    oJ = f.jacobian(rho)
+    assert np.abs(o - r).max() <= 1e-12 * np.abs(r).max()
+    assert np.linalg.norm(oJ - J) <= 1e-12 * np.linalg.norm(J)
+    # without singularity removal the potentials differ from the SR ones
+    s2 = ref.Session(mesh, copy.deepcopy(data), k, w)
+    assert np.abs(s2.response(rho) - r).max() > 1e-6 * np.abs(r).max()
+
+
+def test_bypass_oracle_vs_reference_build(tmp_path, monkeypatch):
+    """Bypass resistors of the total-field operator
+    (assembleStiffnessMatrixDCFEMByPass_, dcfemmodelling.cpp:601-707) with
+    resistances from electrodeBypassResistances.map.  The reference adds each
+    resistor once per electrode search and searches twice (setMesh, then
+    setData; bypassNodeIdx_ is never cleared, :832), so its resistors act as
+    R / 2: the oracle, given R / 2, matches it to rounding."""
+    if not ref.available():
+        pytest.skip("oracle/_ref not built")
+    mesh, data, rho = workloads.small_2d_bypass()
+    (tmp_path / "electrodeBypassResistances.map").write_text("-10000 100\n-10001 200\n")
+    monkeypatch.chdir(tmp_path)
+    s = ref.Session(mesh, copy.deepcopy(data), sr=False)
+    k, w = s.kw()
+    r = s.response(rho)
+    J = s.jacobian(rho)
+    f = eo.Forward(mesh, copy.deepcopy(data), k, w)
+    f.bypassR = {-10000: 50.0, -10001: 100.0}
+    o = eo.response_total(f, rho)
+    oJ = f.jacobian(rho)
+    assert np.abs(o - r).max() <= 1e-12 * np.abs(r).max()
+    assert np.linalg.norm(oJ - J) <= 1e-12 * np.linalg.norm(J)
+    # the resistors matter: once each (as written) is another result
+    f1 = eo.Forward(mesh, copy.deepcopy(data), k, w)
+    f1.bypassR = {-10000: 100.0, -10001: 200.0}
+    assert np.abs(eo.response_total(f1, rho) - r).max() > 1e-3 * np.abs(r).max()
