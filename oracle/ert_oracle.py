@@ -479,11 +479,13 @@ def map_model(mesh, model):
 class Forward:
     """DCSRMultiElectrodeModelling restated: response() and createJacobian()."""
 
-    def __init__(self, mesh, data, kvalues=None, weights=None):
+    def __init__(self, mesh, data, kvalues=None, weights=None, total_field=False):
+        """total_field: the operator without singularity removal, which needs
+        no primary potentials and so admits topography."""
         self.mesh = mesh
         self.data = data
         self.setup = Setup(mesh, data.sensorPositions())
-        if self.setup.topography:
+        if self.setup.topography and not total_field:
             raise NotImplementedError("topography / Neumann-only domains are not part of this path")
         if kvalues is None:
             kvalues, weights = kwave_list(mesh.dim(), self.setup.elecPos)

@@ -162,9 +162,13 @@ class ERTModelling:
     def _setup_electrodes(self):
         mesh, data = self._mesh, self._data
         an = MeshAnalysis(mesh, data.sensorPositions())
-        if an.topography:
-            raise NotImplementedError("topography / Neumann-only domains need numerical primary "
-                                      "potentials, which this path does not provide")
+        if not np.isin(mesh.boundaryMarkers(), (-2,)).any():
+            raise NotImplementedError("Neumann-only domains (no mixed boundary) need a calibration node, "
+                                      "which this path does not provide")
+        if an.topography and self._sr:
+            raise NotImplementedError("topography with singularity removal needs numerical primary potentials, "
+                                      "which this path does not provide; the total-field operator "
+                                      "ERTModelling(sr=False) models topography")
         self._analysis = an
         en, pen = _host_i(an.elecNode)
         sc, psc = _host_d(an.sourceCenter)
@@ -202,6 +206,8 @@ class ERTModelling:
         d = self._data
         kf = d["k"]
         if not d.haveData("k") or np.min(np.abs(kf)) < 1e-12:
+            if self._analysis is not None and self._analysis.topography:
+                raise ValueError("data contains no K-factors (needed with topography)")
             d.set("k", self.calcGeometricFactor(d))
 
     def _upload_data(self):

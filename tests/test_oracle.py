@@ -458,3 +458,23 @@ def test_bypass_oracle_vs_reference_build(tmp_path, monkeypatch):
     f1 = eo.Forward(mesh, copy.deepcopy(data), k, w)
     f1.bypassR = {-10000: 100.0, -10001: 200.0}
     assert np.abs(eo.response_total(f1, rho) - r).max() > 1e-3 * np.abs(r).max()
+
+
+def test_topography_total_field_oracle_vs_reference_build():
+    """Topography in the total-field operator (no primary potentials needed):
+    oracle against the reference build on a mesh with a surface bump."""
+    if not ref.available():
+        pytest.skip("oracle/_ref not built")
+    mesh, data, rho = workloads.small_2d_topo()
+    s = ref.Session(mesh, copy.deepcopy(data), sr=False)
+    k, w = s.kw()
+    r = s.response(rho)
+    J = s.jacobian(rho)
+    f = eo.Forward(mesh, copy.deepcopy(data), k, w, total_field=True)
+    assert f.setup.topography
+    o = eo.response_total(f, rho)
+    oJ = f.jacobian(rho)
+    assert np.abs(o - r).max() <= 1e-12 * np.abs(r).max()
+    assert np.linalg.norm(oJ - J) <= 1e-12 * np.linalg.norm(J)
+    with pytest.raises(NotImplementedError):
+        eo.Forward(mesh, copy.deepcopy(data), k, w)
