@@ -119,5 +119,10 @@ class DataContainerERT:
         from .fileio import load_data
         return load_data(fileName, sensorIndicesFromOne, removeInvalid)
 
+    def additionalPoints(self):
+        """Topography points read with the data (DataContainer::additionalPoints);
+        they are not written back (DataContainer::write writes a zero count)."""
+        return getattr(self, "_topoPoints", np.zeros((0, 3)))
+
     def abmn(self):
         return np.stack([self._tok[x] for x in "abmn"], axis=1).astype(np.int32)

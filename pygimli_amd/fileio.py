@@ -90,6 +90,7 @@ def translate_alias(tok):
 
 
 _NUM = re.compile(rb"\s*[+-]?(?:(?:\d+\.?\d*|\.\d+)(?:[eE][+-]?\d+)?|inf(?:inity)?|nan)", re.I)
+_HEX = re.compile(rb"\s*([+-]?)0[xX]((?:[0-9a-fA-F]+\.?[0-9a-fA-F]*|\.[0-9a-fA-F]+)(?:[pP][+-]?\d+)?)")
 _INT = re.compile(rb"\s*[+-]?\d+")
 
 
@@ -100,6 +101,9 @@ def _strtod(tok):
             return float(tok)
     except ValueError:
         pass
+    h = _HEX.match(tok)
+    if h:
+        return float.fromhex((h.group(1) + b"0x" + h.group(2)).decode())
     m = _NUM.match(tok)
     return float(m.group(0)) if m else 0.0
 
@@ -291,9 +295,34 @@ def load_data(fileName, sensorIndicesFromOne=True, removeInvalid=True):
         keep = valid == 1
         tok = {k: v[keep] for k, v in tok.items()}
 
+    # optional topography points after the data (datacontainer.cpp:421-474)
+    topo = np.zeros((0, 3))
+    row = rows.nonempty()
+    if len(row) == 1:
+        nTopo = _atoi(row[0])
+        if nTopo > 0:
+            tf = rows.format_line()
+            if tf is None or not tf or tf[0] not in (b"x", b"X"):
+                tf = [b"x", b"y", b"z"]
+            topo = np.zeros((nTopo, 3))
+            for i in range(nTopo):
+                row = rows.nonempty()
+                if not row:
+                    raise ValueError(f"{fileName}: too few topo data. {nTopo} topopoints expected and {i} found.")
+                for j, t in enumerate(row):
+                    if j == len(tf):
+                        break
+                    ax = {b"x": 0, b"X": 0, b"y": 1, b"Y": 1, b"z": 2, b"Z": 2}.get(tf[j])
+                    if ax is None:
+                        raise ValueError(f"{fileName}: format description unknown: topo electrode "
+                                         f"format[{j}] = {tf[j].decode('latin-1')}")
+                    topo[i, ax] = _strtod(t)
+            topo = _round(topo, 1e-12)
+
     data = DataContainerERT()
     data.setSensorPositions(sensors)
     data._fromOne = bool(sensorIndicesFromOne)
+    data._topoPoints = topo
     n = len(tok[b"valid"])
     data.resize(n)
     for name, v in tok.items():

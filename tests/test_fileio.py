@@ -60,6 +60,15 @@ POLE = b"""4
 """
 
 
+# a topography section after the data, with a format line and hex / exponent forms
+TOPO = b"""3
+# x y z
+0 0 0x1p-2
+1.5 0 2e-1
+3.0 0 -.5
+"""
+
+
 def _write(tmp_path, name, raw):
     p = tmp_path / name
     p.write_bytes(raw)
@@ -124,6 +133,20 @@ def test_save_load_round_trip(tmp_path):
     assert (tmp_path / "f.dat").read_bytes() == (tmp_path / "g.dat").read_bytes()
 
 
+def test_topography_points_and_number_forms(tmp_path):
+    raw = (b"2\n0 0 0\n1 0 0\n1\n# a b m n u i\n1 0 2 0 0x1p-1 -1e-1\n"
+           b"2\n# x z\n0.5 0x10\n1.5 -2.25   # second\n")
+    d = fileio.load_data(_write(tmp_path, "t.dat", raw), removeInvalid=False)
+    assert d["u"][0] == 0.5 and d["i"][0] == -0.1           # hexadecimal and signed exponent forms
+    assert np.array_equal(d.additionalPoints(), [[0.5, 0, 16], [1.5, 0, -2.25]])
+    d.save(tmp_path / "t2.dat")
+    assert (tmp_path / "t2.dat").read_bytes().endswith(b"\n0\n")   # not written back
+    with pytest.raises(ValueError, match="topo electrode format"):
+        fileio.load_data(_write(tmp_path, "u.dat", raw.replace(b"# x z", b"# x q")))
+    with pytest.raises(ValueError, match="too few topo"):
+        fileio.load_data(_write(tmp_path, "v.dat", raw.replace(b"1.5 -2.25   # second\n", b"")))
+
+
 def test_data_errors(tmp_path):
     with pytest.raises(ValueError, match="sensor count"):
         fileio.load_data(_write(tmp_path, "a.dat", b"0\n0\n"))
@@ -145,11 +168,11 @@ def test_data_errors(tmp_path):
                                            ("a b m n rhoa u i err ip valid", True),
                                            ("c1 c2 p1 p2 rhoa/Ohmm", False)])
 @pytest.mark.parametrize("fmt_sensor", ["x y z", "x z", "X Y q"])
-@pytest.mark.parametrize("name", ["survey", "pole"])
+@pytest.mark.parametrize("name", ["survey", "pole", "topo"])
 def test_data_bytes_match_reference(tmp_path, remove, fmt, no_filter, fmt_sensor, name):
-    if name == "pole" and fmt.startswith("a b m n rhoa u i err ip"):
+    if name != "survey" and fmt.startswith("a b m n rhoa u i err ip"):
         fmt = "a b m n u i r k valid"
-    src = _write(tmp_path, "in.dat", SURVEY if name == "survey" else POLE)
+    src = _write(tmp_path, "in.dat", {"survey": SURVEY, "pole": POLE, "topo": POLE + TOPO}[name])
     d = fileio.load_data(src, removeInvalid=remove)
     fileio.save_data(d, tmp_path / "ours.dat", fmt, fmt_sensor, no_filter)
     ref.data_convert(src, tmp_path / "theirs.dat", fmt, fmt_sensor, remove_invalid=remove, no_filter=no_filter)
