@@ -60,6 +60,15 @@ def smoothness_constraints(mesh):
     return sp.csr_matrix((vals, (rows, cols)), shape=(n, int(mk.max()) + 1))
 
 
+def complex_constraints(C):
+    """Constraints of a complex-resistivity inversion: the real-model matrix
+    repeated for the real and the imaginary parameter halves
+    (ERTModelling.createConstraints, ertModelling.py:150-162 ->
+    pg.matrix.RepeatDMatrix(C1, 2)).  Returns scipy CSR."""
+    import scipy.sparse as sp
+    return sp.block_diag([C, C], format="csr")
+
+
 def solveCGLSCDWWhtrans(S, C, dWeight, b, x, wc, wm, tm, td, lam, roughness, maxIter=200, tol=-1.0,
                         verbose=False):
     """Regularised CGLS of one Gauss-Newton step (solver.cpp:27-224):

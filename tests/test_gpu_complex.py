@@ -124,3 +124,33 @@ def test_complex_errors():
         fop.response(rho[:-1])                     # not [re, im] of one length
     with pytest.raises(ValueError):
         fop.response(np.concatenate([rho * 0, rho * 0]))   # |rho| = 0
+
+
+def test_complex_gauss_newton_step():
+    """The complex inversion step as the reference sets it up: the real block
+    Jacobian [[Re, -Im], [Im, Re]] (toRealMatrix), constraints repeated for
+    both halves (RepeatDMatrix(C, 2)), model / data = [re, im]; the GPU CGLS
+    (pgert_cgls) against the oracle's CGLS on the same inputs."""
+    from oracle import cgls as cg
+    from pygimli_amd.inversion import complex_constraints, smoothness_constraints, solveCGLSCDWWhtrans
+    mesh, data, rho = workloads.small_2d()
+    rim = _phases(rho)
+    fop = _fop(mesh, data, "band")
+    m = np.concatenate([rho, rim])
+    resp = fop.response(m)
+    fop.createJacobian(m)
+    J = fop.jacobian()
+    D2, P2 = J.shape
+    C = complex_constraints(smoothness_constraints(mesh))
+    assert C.shape[1] == P2 == fop.parameterCount
+    rng = np.random.default_rng(2)
+    b = 0.01 * resp * rng.standard_normal(D2)
+    dW = 1.0 / (0.03 * np.abs(resp) + 1e-3)
+    tm, td = np.ones(P2), np.ones(D2)
+    wc, wm = np.ones(C.shape[0]), np.ones(P2)
+    rough = C @ m
+    x, it = solveCGLSCDWWhtrans(J, C, dW, b, np.zeros(P2), wc, wm, tm, td, 10.0, rough, maxIter=5, tol=0.0)
+    xo, ito, _ = cg.cgls(J.cpu().numpy(), C, dW, b, np.zeros(P2), wc, wm, tm, td, 10.0, rough, max_iter=5, tol=0.0)
+    err = np.linalg.norm(x.cpu().numpy() - xo) / np.linalg.norm(xo)
+    print("complex GN step: 5-iteration rel err", err)
+    assert it == ito == 5 and err < 1e-9
