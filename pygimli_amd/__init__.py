@@ -3,6 +3,11 @@
     from pygimli_amd import ERTModelling, DataContainerERT, Mesh
     fop = ERTModelling(); fop.setMesh(mesh); fop.setData(data)
     rhoa = fop.response(rho); fop.createJacobian(rho); J = fop.jacobian()
+
+ERTModelling(sr=False) is the total-field operator (topography, bypass
+electrodes) and, with setComplex(True), the complex-resistivity one;
+load / save_data / save_mesh read and write the unified data format and .bms
+meshes; inversion.solveCGLSCDWWhtrans runs the Gauss-Newton step's CGLS.
 """
 from .data import DataContainerERT  # noqa: F401
 from .mesh import Mesh  # noqa: F401
