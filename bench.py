@@ -308,8 +308,18 @@ def complex_ref_worker(out_path, nthreads, reps):
         ts.append(time.perf_counter() - t)
     for name, a in (("re", rr), ("im", ri), ("Jre", Jr), ("Jim", Ji)):
         np.save(_section_arrays(out_path, "complex", name), a)
+    # the real total-field operator (ERTModelling(sr=False)) on the same section
+    st = ref.Session(mesh, copy.deepcopy(data), nthreads=nthreads, sr=False)
+    tt = []
+    for r in range(reps + 1):
+        t = time.perf_counter()
+        rt = st.response(rho)
+        Jt = st.jacobian(rho)
+        tt.append(time.perf_counter() - t)
+    np.save(_section_arrays(out_path, "total", "rhoa"), rt)
+    np.save(_section_arrays(out_path, "total", "J"), Jt)
     with open(out_path, "w") as f:
-        json.dump({"s": float(np.median(ts[1:])), "nk": len(ses.kw()[0])}, f)
+        json.dump({"s": float(np.median(ts[1:])), "nk": len(ses.kw()[0]), "total_s": float(np.median(tt[1:]))}, f)
 
 
 def complex_leg(args, log):
@@ -347,6 +357,26 @@ def complex_leg(args, log):
             "workload": f"2.5D section: {mesh.nodeCount()} nodes / {mesh.cellCount()} triangles, 41 electrodes, "
                         f"{data.size()} dipole-dipole data, 3 layers + block, phases -30 mrad (block) / -5 mrad"}
     log(f"[complex] 2.5D: {dt * 1e3:.2f} ms, {fop.lastStats}")
+    # the real total-field operator on the same section
+    ft = ERTModelling(sr=False)
+    ft.setMesh(mesh)
+    ft.setData(copy.deepcopy(data))
+    ft.solverOptions.update(method="band", rtol=1e-10)
+    mt = torch.as_tensor(rho, device=ft.device)
+    for _ in range(args.warmup + 1):
+        ft.response_device(mt)
+        ft.createJacobian(mt)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        rt = ft.response_device(mt)
+        ft.createJacobian(mt)
+    torch.cuda.synchronize()
+    dtt = (time.perf_counter() - t) / steps
+    res2["total"] = {"ms_per_step": dtt * 1e3, "evals_per_s": 1.0 / dtt, "rhoa": rt.cpu().numpy(),
+                     "J": ft.jacobian().cpu().numpy(),
+                     "workload": "same section, real resistivities, ERTModelling(sr=False), band solver, FP64 J"}
+    log(f"[total] 2.5D: {dtt * 1e3:.2f} ms")
     mesh3, data3, rho3 = workloads.ert3d_grid_grf()
     c = mesh3.cellCenters()
     inside = (np.abs(c[:, 0]) < 4) & (np.abs(c[:, 1]) < 4) & (c[:, 2] > -6) & (c[:, 2] < -2)
@@ -723,7 +753,8 @@ def main():
                "warmup": args.warmup + 1, "ms_per_step": r2["ms_per_step"], "higher_is_better": True,
                "scaling": "weak", "vs_baseline": None, "dtype": "fp64", "data": "synthetic",
                "config": {"workload": r2["workload"] + "; direct band solver (FP64), FP64 complex Jacobian"},
-               "complex_2d": {k: v for k, v in r2.items() if k not in ("resp", "Jre", "Jim")},
+               "complex_2d": {k: v for k, v in r2.items() if k not in ("resp", "Jre", "Jim", "total")},
+               "total_2d": {k: v for k, v in r2["total"].items() if k not in ("rhoa", "J")},
                "complex_3d": r3}
         if cpu_proc is not None:
             cpu_proc.join(args.cpu_timeout)
@@ -745,6 +776,14 @@ def main():
                         "im_max_rel_diff": float(np.abs(r2["resp"][D:] - ri).max() / np.abs(ri).max()),
                         "Jre_rel_fro_diff": float(np.linalg.norm(r2["Jre"] - Jr) / np.linalg.norm(Jr)),
                         "Jim_rel_fro_diff": float(np.linalg.norm(r2["Jim"] - Ji) / np.linalg.norm(Ji))}}
+                if "total_s" in r:
+                    ra = np.load(_section_arrays(cpu_path, "total", "rhoa"))
+                    JA = np.load(_section_arrays(cpu_path, "total", "J"))
+                    tot = r2["total"]
+                    out["total_2d"]["reference"] = {
+                        "reference_s": r["total_s"], "speedup": r["total_s"] / (tot["ms_per_step"] * 1e-3),
+                        "rhoa_max_rel_diff": float(np.abs(tot["rhoa"] - ra).max() / np.abs(ra).max()),
+                        "J_rel_fro_diff": float(np.linalg.norm(tot["J"] - JA) / np.linalg.norm(JA))}
             except Exception as e:
                 out["cpu_baseline"] = {"error": f"no reference baseline ({e!r})"}
         print(json.dumps(out))
