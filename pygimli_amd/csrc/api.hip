@@ -662,7 +662,7 @@ int pgert_forward_complex(pgert_ctx * c, const double * rho_re, const double * r
         // an inner solve from a converged guess moves them by about rtol itself
         const double stop = 10.0 * o.rtol;
         bool done = false;
-        double hchg[4];
+        double hchg[4] = {0.0, 0.0, 0.0, 0.0};
         for (int it = 0; it < maxSweeps && !done; ++it) {
             for (int half = 0; half < 2; ++half) {
                 double * Xh = half == 0 ? C.Xre.p : C.Xim.p;
@@ -672,7 +672,13 @@ int pgert_forward_complex(pgert_ctx * c, const double * rho_re, const double * r
                     launch_spmm_add(C, C.valsAi.p, C.Xre.p, nullptr, -1.0, C.B.p);
                 PG_CHECK(hipMemcpyAsync(C.X.p, Xh, C.X.bytes(), hipMemcpyDeviceToDevice, s));
                 st = pgert_solve_stats{};
-                solve_one(C, o, st, it == 0 && half == 0);   // A_re is the same in every solve
+                // the imaginary half needs the real half's absolute accuracy: its
+                // relative tolerance is looser by |X_re| / |X_im| (known from the
+                // previous sweep), at most 1e-2
+                pgert_solve_opts oh = o;
+                if (half == 1 && it > 0 && hchg[3] > 0.0)
+                    oh.rtol = std::min(1e-2, std::max(o.rtol, o.rtol * std::sqrt(hchg[1] / hchg[3])));
+                solve_one(C, oh, st, it == 0 && half == 0);   // A_re is the same in every solve
                 tot.iterations += st.iterations;
                 tot.max_iterations += st.max_iterations;
                 tot.refinements += st.refinements;
