@@ -22,6 +22,8 @@ def batch_response_device(fops, models):
     fops[0]'s solverOptions are the batch's."""
     if len(fops) != len(models) or not fops:
         raise ValueError("one model per operator")
+    if any(not f._sr or f.complex() for f in fops):
+        raise NotImplementedError("batched forward: singularity-removal operators with real resistivities only")
     rhos, rhoas = [], []
     for f, m in zip(fops, models):
         f._sync_setup()
@@ -56,6 +58,8 @@ def batch_jacobian(fops, models):
     returns the list of Jacobians."""
     if len(fops) != len(models) or not fops:
         raise ValueError("one model per operator")
+    if any(f.complex() for f in fops):
+        raise NotImplementedError("batched Jacobian: real resistivities only")
     ts, Js = [], []
     for f, m in zip(fops, models):
         t = f._model_tensor(m)

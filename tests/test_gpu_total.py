@@ -142,3 +142,17 @@ def test_topography_total_field():
     f2.setData(nok)
     with pytest.raises(ValueError):
         f2.response(rho)
+
+
+def test_batched_api_refuses_total_field_and_complex():
+    from pygimli_amd.batch import batch_jacobian, batch_response
+    from pygimli_amd.ertModelling import ERTModelling
+    mesh, data, rho = workloads.small_2d()
+    f = ERTModelling(sr=False)
+    f.setMesh(mesh)
+    f.setData(copy.deepcopy(data))
+    with pytest.raises(NotImplementedError):
+        batch_response([f], [rho])
+    f.setComplex(True)
+    with pytest.raises(NotImplementedError):
+        batch_jacobian([f], [np.concatenate([rho, 0 * rho])])
