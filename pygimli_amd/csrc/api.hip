@@ -569,6 +569,7 @@ void forward_finish(Ctx & c, double * rhoa) {
     c.U.alloc((size_t)c.nK * c.n * c.nE);
     launch_total_potential(c);
     c.haveU = true;
+    c.srState = true;
     launch_response(c, rhoa);
     PG_CHECK(hipEventRecord(c.ev[4], c.stream));
 }
@@ -655,6 +656,7 @@ int pgert_forward_complex(pgert_ctx * c, const double * rho_re, const double * r
         PG_CHECK(hipMemsetAsync(C.Xim.p, 0, C.Xim.bytes(), s));
         C.haveU = false;
         C.haveUc = false;
+        C.srState = false;
         pgert_solve_stats st{}, tot{};
         tot.converged = 1;
         const int maxSweeps = 60;
@@ -714,6 +716,7 @@ int pgert_forward_total(pgert_ctx * c, const double * rho, const pgert_solve_opt
         o.learned_warm_start = 0;
         hipStream_t s = C.stream;
         const size_t nk = C.nK, blk = nk * C.n * C.nE;
+        C.srState = false;
         PG_CHECK(hipEventRecord(C.ev[0], s));
         C.rho.alloc(C.C);
         PG_CHECK(hipMemcpyAsync(C.rho.p, rho, (size_t)C.C * 8, hipMemcpyDeviceToDevice, s));
@@ -939,6 +942,8 @@ int pgert_mlp_collect(pgert_ctx * c, int k_index, const long * idx, long count, 
     return guarded(c, [&] {
         PG_REQUIRE(k_index >= 0 && k_index < c->nK, "pgert_mlp_collect: bad wavenumber index");
         PG_REQUIRE(count > 0 && idx && feat, "pgert_mlp_collect: bad arguments");
+        PG_REQUIRE(c->srState && c->primValid,
+                   "pgert_mlp_collect: needs the potentials of a singularity-removal forward (pgert_forward)");
         mlp_collect(*c, k_index, idx, count, feat, target);
         sync(*c);
     });

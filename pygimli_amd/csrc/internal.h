@@ -140,6 +140,7 @@ struct Ctx {
     bool primValid = false;
     DBuf<double> U;         // nK * n * nE   total potentials (secondary + rho_s * primary)
     bool haveU = false;
+    bool srState = false;   // X / prim / rhoS / rho hold a singularity-removal forward
     DBuf<double> emap;      // nE * nE
     // solver work
     DBuf<double> B, X, R, P, Q, Z;

@@ -36,6 +36,9 @@ class LearnedWarmStart:
     """Owns the parameters on one ERTModelling's context."""
 
     def __init__(self, fop, params=None, seed=0):
+        if not fop._sr or fop.complex():
+            raise NotImplementedError("the learned warm start predicts singularity-removal (secondary) "
+                                      "potentials: ERTModelling(sr=True) with real resistivities only")
         self.fop = fop
         L = fop._ctx.L
         self.n = L.pgert_mlp_param_count()

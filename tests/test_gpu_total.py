@@ -156,3 +156,26 @@ def test_batched_api_refuses_total_field_and_complex():
     f.setComplex(True)
     with pytest.raises(NotImplementedError):
         batch_jacobian([f], [np.concatenate([rho, 0 * rho])])
+
+
+def test_learned_warm_start_refuses_total_field():
+    """The learned warm start predicts SR secondary potentials: a
+    total-field operator is refused on the host, and collecting training
+    samples after a total-field forward is refused by the library."""
+    import ctypes
+    import torch
+    from pygimli_amd.ertModelling import ERTModelling
+    from pygimli_amd.learned import LearnedWarmStart
+    mesh, data, rho = workloads.small_2d()
+    f = ERTModelling(sr=False)
+    f.setMesh(mesh)
+    f.setData(copy.deepcopy(data))
+    with pytest.raises(NotImplementedError):
+        LearnedWarmStart(f)
+    f.response(rho)
+    idx = torch.zeros(4, dtype=torch.int64, device=f.device)
+    feat = torch.empty((4, 64), dtype=torch.float32, device=f.device)
+    tgt = torch.empty(4, dtype=torch.float32, device=f.device)
+    rc = f._ctx.L.pgert_mlp_collect(f._ctx.h, 0, ctypes.c_void_p(idx.data_ptr()), 4, ctypes.c_void_p(feat.data_ptr()),
+                                    ctypes.c_void_p(tgt.data_ptr()))
+    assert rc != 0 and b"singularity-removal" in f._ctx.L.pgert_last_error(f._ctx.h)
