@@ -269,7 +269,8 @@ int pgert_set_mesh(pgert_ctx * c, int dim, int n_nodes, const double * xyz, int 
 
         c->hElecNode.clear();
         c->nE = 0;
-        c->s1Valid = c->primValid = c->haveU = false;
+        c->nByp = 0;   // bypass slots index the previous pattern
+        c->s1Valid = c->primValid = c->haveU = c->haveUc = c->srState = false;
         sync(*c);
     });
 }
@@ -348,7 +349,7 @@ int pgert_set_electrodes(pgert_ctx * c, int n_elec, const int * elec_node,
             std::nth_element(nn.begin(), nn.begin() + nn.size() / 2, nn.end());
             c->featureScale = (float)(4.0 * nn[nn.size() / 2]);
         }
-        c->s1Valid = c->primValid = c->haveU = false;
+        c->s1Valid = c->primValid = c->haveU = c->haveUc = c->srState = false;
         sync(*c);
     });
 }
@@ -363,7 +364,7 @@ int pgert_set_wavenumbers(pgert_ctx * c, int nk, const double * k, const double 
         c->wv.assign(w, w + nk);
         c->dk.upload(c->kv, c->stream);
         c->dw.upload(c->wv, c->stream);
-        c->s1Valid = c->primValid = c->haveU = false;
+        c->s1Valid = c->primValid = c->haveU = c->haveUc = c->srState = false;
         sync(*c);
     });
 }
