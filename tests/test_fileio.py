@@ -168,11 +168,12 @@ def test_data_errors(tmp_path):
                                            ("a b m n rhoa u i err ip valid", True),
                                            ("c1 c2 p1 p2 rhoa/Ohmm", False)])
 @pytest.mark.parametrize("fmt_sensor", ["x y z", "x z", "X Y q"])
-@pytest.mark.parametrize("name", ["survey", "pole", "topo"])
+@pytest.mark.parametrize("name", ["survey", "pole", "topo", "crlf"])
 def test_data_bytes_match_reference(tmp_path, remove, fmt, no_filter, fmt_sensor, name):
-    if name != "survey" and fmt.startswith("a b m n rhoa u i err ip"):
+    if name in ("pole", "topo") and fmt.startswith("a b m n rhoa u i err ip"):
         fmt = "a b m n u i r k valid"
-    src = _write(tmp_path, "in.dat", {"survey": SURVEY, "pole": POLE, "topo": POLE + TOPO}[name])
+    src = _write(tmp_path, "in.dat", {"survey": SURVEY, "pole": POLE, "topo": POLE + TOPO,
+                                      "crlf": SURVEY.replace(b"\n", b"\r\n")}[name])
     d = fileio.load_data(src, removeInvalid=remove)
     fileio.save_data(d, tmp_path / "ours.dat", fmt, fmt_sensor, no_filter)
     ref.data_convert(src, tmp_path / "theirs.dat", fmt, fmt_sensor, remove_invalid=remove, no_filter=no_filter)
