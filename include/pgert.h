@@ -55,6 +55,10 @@ typedef struct pgert_solve_opts {
     int coarse;                /* 1: two-level preconditioner (block methods, see pgert_set_coarse) */
     int no_graph;              /* 1: launch the block iterations one by one instead of replaying a
                                   captured HIP graph of check_every iterations */
+    int report_only;           /* 0: a solve that misses rtol is repeated from the analytic initial
+                                  guess (when it started from a learned or given guess), then with
+                                  FP64 block-PCG (when it was mixed), and the call fails if that
+                                  also misses; 1: return with stats.converged = 0 instead */
 } pgert_solve_opts;
 
 typedef struct pgert_solve_stats {
@@ -63,6 +67,7 @@ typedef struct pgert_solve_stats {
     int refinements;       /* outer FP64 refinement steps (mixed) */
     int converged;         /* 1 if every column met rtol */
     double max_rel_residual; /* final FP64 ||b - A x|| / ||b||, max over columns */
+    int fallback;          /* bit 0: re-solved from the analytic initial guess; bit 1: re-solved in FP64 */
 } pgert_solve_stats;
 
 /* Context ----------------------------------------------------------------- */
@@ -116,7 +121,11 @@ int pgert_pattern(const pgert_ctx * ctx, int * rowptr, int * colidx);
 
 /* Cell resistivity from a model vector (mapERTModel, dcfemmodelling.cpp:1211;
  * createMappedModel, modellingbase.cpp:401-470): len == n_cells -> copy,
- * else rho_c = model[marker_c].  DEVICE pointers. */
+ * else rho_c = model[marker_c] for cells with a marker >= 0 and cells with a
+ * negative marker prolongated from their face neighbours, weighted by the
+ * horizontal part of the face normal (Mesh::prolongateEmptyCellsValues,
+ * mesh.cpp:2247-2318, background -9e99 as in response(model)).  Fails for
+ * fixed-value regions (marker <= -1000000, region manager).  DEVICE pointers. */
 int pgert_map_model(pgert_ctx * ctx, const double * model, int n_model, double * rho);
 
 /* System matrix values of one wavenumber on the CSR pattern, exactly as
@@ -145,7 +154,8 @@ int pgert_forward(pgert_ctx * ctx, const double * rho, const pgert_solve_opts * 
  * per-problem steps (assembly, primaries, right-hand sides, response) are the
  * ones of pgert_forward.  rhos[i] / rhoas[i]: DEVICE pointers of problem i;
  * stats: optional array of count entries.  Everything runs on ctxs[0]'s
- * stream.  Block methods and the band solver (PGERT_BLOCK_FP64,
+ * stream.  A problem that misses rtol fails the call (no fallback solves in a
+ * batch) unless opts.report_only is set.  Block methods and the band solver (PGERT_BLOCK_FP64,
  * PGERT_BLOCK_MIXED, PGERT_BAND_FP64: one launch factors every wavenumber
  * of every problem, one launch solves them).
  * The reference runs its problems one after another (each a

@@ -462,17 +462,82 @@ def _scatter(mesh, Ae):
 # --------------------------------------------------------------------------- a13
 def map_model(mesh, model):
     """mapERTModel (dcfemmodelling.cpp:1211-1218) + ModellingBase::
-    createMappedModel (modellingbase.cpp:401-470) for meshes whose cells all
-    carry a parameter marker >= 0."""
+    createMappedModel (modellingbase.cpp:401-470) with the default background
+    -9e99 of response(model) (dcfemmodelling.h:164): parameter cells take
+    model[marker], cells with a negative marker are prolongated."""
     model = np.asarray(model, np.float64)
     if len(model) == mesh.cellCount():
         return model.copy()
     cm = mesh.cellMarkers()
-    if np.any(cm < 0):
-        raise NotImplementedError("background (marker -1) prolongation is not part of this path")
+    if np.any(cm <= -1000000):
+        raise NotImplementedError("fixed-value regions need the region manager")
     if cm.max() >= len(model):
         raise ValueError("marker >= model.size()")
-    return model[cm]
+    vals = np.where(cm >= 0, model[np.maximum(cm, 0)], 0.0)
+    if np.all(cm < 0):
+        raise ValueError("too many empty cells")
+    return prolongate_empty_cells(mesh, vals)
+
+
+def cell_neighbours(mesh):
+    """Cell::findNeighborCell (meshentities.cpp:553-573): neighbour j of a cell
+    is the unique other cell sharing the face opposite node j (Triangle /
+    Tetrahedron::boundaryNodes, meshentities.cpp:1025-1033, :1153-1165), -1 if
+    none.  Returns (nb [C, nv], face nodes [C, nv, dim])."""
+    c = mesh.cells.astype(np.int64)
+    C, nv = c.shape
+    dim = nv - 1
+    faces = np.stack([np.delete(c, j, axis=1) for j in range(nv)], axis=1)  # [C, nv, dim]
+    key = np.sort(faces, axis=2).reshape(-1, dim)
+    order = np.lexsort(key.T[::-1])
+    ks = key[order]
+    same = np.all(ks[1:] == ks[:-1], axis=1)
+    nb = -np.ones(C * nv, np.int64)
+    for q in np.nonzero(same)[0]:
+        nb[order[q]] = order[q + 1] // nv
+        nb[order[q + 1]] = order[q] // nv
+    return nb.reshape(C, nv), faces
+
+
+def prolongate_empty_cells(mesh, vals):
+    """Mesh::prolongateEmptyCellsValues with background -9e99 (mesh.cpp:2247-2318):
+    sweeps in which every empty cell (|v| < 1e-12) with filled neighbours takes
+    their mean weighted by |n * (1, 1, 0)| + 1e-6 (3D) or |n_x| + 1e-6 (2D), n the
+    unit normal of the common face (RVector3::normXY for edges, pos.cpp:185-188;
+    TriangleShape::norm, shape.cpp:535-540); values of a sweep use the state
+    before it."""
+    vals = np.array(vals, np.float64)
+    nb, faces = cell_neighbours(mesh)
+    xyz = mesh.nodes
+    dim = mesh.dim()
+    while True:
+        empty = np.nonzero(np.abs(vals) < TOLERANCE)[0]
+        if len(empty) == 0:
+            return vals
+        new = {}
+        for i in empty:
+            val = 0.0
+            weight = 0.0
+            for j in range(nb.shape[1]):
+                o = nb[i, j]
+                if o < 0 or abs(vals[o]) <= TOLERANCE:
+                    continue
+                p = xyz[faces[i, j]]
+                if dim == 2:
+                    d = p[0] - p[1]
+                    w = abs(d[1]) / np.hypot(d[0], d[1]) + 1e-6
+                else:
+                    n = np.cross(p[1] - p[0], p[2] - p[0])
+                    n = n / np.linalg.norm(n)
+                    w = np.hypot(n[0], n[1]) + 1e-6
+                val += vals[o] * w
+                weight += w
+            if weight > 1e-8:
+                new[i] = val / weight
+        if not new:
+            raise RuntimeError("cannot fill the empty cells")
+        for i, v in new.items():
+            vals[i] = v
 
 
 # --------------------------------------------------------------------------- a7/a9/a10

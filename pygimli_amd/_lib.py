@@ -39,13 +39,13 @@ class SolveOpts(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("rtol", ctypes.c_double), ("max_iter", ctypes.c_int),
                 ("check_every", ctypes.c_int), ("inner_rtol", ctypes.c_double),
                 ("warm_start", ctypes.c_void_p), ("learned_warm_start", ctypes.c_int), ("coarse", ctypes.c_int),
-                ("no_graph", ctypes.c_int)]
+                ("no_graph", ctypes.c_int), ("report_only", ctypes.c_int)]
 
 
 class SolveStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("max_iterations", ctypes.c_int),
                 ("refinements", ctypes.c_int), ("converged", ctypes.c_int),
-                ("max_rel_residual", ctypes.c_double)]
+                ("max_rel_residual", ctypes.c_double), ("fallback", ctypes.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -121,7 +121,7 @@ __global__ void k_spmm_add(int n, int nE, int nnz, const int * __restrict__ rowp
 
 // acc[0] += |a - b|^2, acc[1] += |a|^2 (one atomic per block)
 __global__ void k_change(long total, const double * __restrict__ a, const double * __restrict__ b,
-                         double * __restrict__ acc) {
+                         double * __restrict__ part) {
     __shared__ double s0[256], s1[256];
     double d2 = 0.0, a2 = 0.0;
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
@@ -140,13 +140,33 @@ __global__ void k_change(long total, const double * __restrict__ a, const double
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        atomicAdd(acc, s0[0]);
-        atomicAdd(acc + 1, s1[0]);
+        part[2 * blockIdx.x] = s0[0];
+        part[2 * blockIdx.x + 1] = s1[0];
     }
 }
 
-__global__ void k_zero2(double * acc) {
-    if (threadIdx.x < 2) acc[threadIdx.x] = 0.0;
+// fixed-order sum of the per-workgroup partials of k_change (one workgroup)
+__global__ void k_change_sum(int nPart, const double * __restrict__ part, double * __restrict__ acc) {
+    __shared__ double s0[256], s1[256];
+    double d2 = 0.0, a2 = 0.0;
+    for (int p = threadIdx.x; p < nPart; p += blockDim.x) {
+        d2 += part[2 * p];
+        a2 += part[2 * p + 1];
+    }
+    s0[threadIdx.x] = d2;
+    s1[threadIdx.x] = a2;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            s0[threadIdx.x] += s0[threadIdx.x + w];
+            s1[threadIdx.x] += s1[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        acc[0] = s0[0];
+        acc[1] = s1[0];
+    }
 }
 
 // response (:1103-1120): k * ((uAM - uAN) - (uBM - uBN)) of the real and the
@@ -183,24 +203,27 @@ __global__ void k_electrode_map_c(int n, int nE, int nK, const double * __restri
     emap[t] = s;
 }
 
-// J = (S(re) - S(im)) + i (S(re + im) - S(re) - S(im)), then J / (m^2 / k)
-// (createJacobian_, dcfemmodelling.cpp:1410-1444); m = mr + i mi
-__global__ void k_jacobian_c(long total, int nParam, const double * __restrict__ Srr,
-                             const double * __restrict__ Sii, const double * __restrict__ Sss,
-                             const double * __restrict__ kfac, const double * __restrict__ mr,
-                             const double * __restrict__ mi, int scale, double * __restrict__ Jre,
-                             double * __restrict__ Jim) {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+// The complex Jacobian in the two output arrays, without scratch blocks:
+// J_re = S(re), J_im = S(im) on entry; afterwards J_re = S(re) - S(im) and
+// J_im = -(S(re) + S(im)), to which the sensitivity launch of re + im adds
+// S(re + im), leaving J = (S(re) - S(im)) + i (S(re + im) - S(re) - S(im)).
+__global__ void k_jacobian_c_pre(long total, double * __restrict__ Jre, double * __restrict__ Jim) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const double a = Jre[t], b = Jim[t];
+    Jre[t] = a - b;
+    Jim[t] = -(a + b);
+}
+
+// J / (m^2 / k) in place (createJacobian_, dcfemmodelling.cpp:1410-1444); m = mr + i mi
+__global__ void k_jacobian_c_scale(long total, int nParam, const double * __restrict__ kfac,
+                                   const double * __restrict__ mr, const double * __restrict__ mi,
+                                   double * __restrict__ Jre, double * __restrict__ Jim) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
     const int j = (int)(t % nParam);
     const long i = t / nParam;
-    const double zr = Srr[t] - Sii[t];
-    const double zi = Sss[t] - Srr[t] - Sii[t];
-    if (!scale) {
-        Jre[t] = zr;
-        Jim[t] = zi;
-        return;
-    }
+    const double zr = Jre[t], zi = Jim[t];
     // q = m^2 / k
     const double a = mr[j], b = mi[j];
     const double qr = (a * a - b * b) / kfac[i], qi = (2.0 * a * b) / kfac[i];
@@ -257,10 +280,11 @@ void launch_spmm_add(Ctx & c, const double * vals, const double * X, const doubl
 
 // |a - b|^2 and |a|^2 into acc[0..1] (device), zeroed first
 void launch_change(Ctx & c, long total, const double * a, const double * b, double * acc) {
-    hipLaunchKernelGGL(k_zero2, dim3(1), dim3(64), 0, c.stream, acc);
+    const int grid = std::max(1, (int)std::min<long>(1024, div_up(total, 256)));
+    c.chgPart.reserve(2 * 1024);
+    hipLaunchKernelGGL(k_change, dim3(grid), dim3(256), 0, c.stream, total, a, b, c.chgPart.p);
     PG_CHECK(hipGetLastError());
-    const int grid = (int)std::min<long>(1024, div_up(total, 256));
-    hipLaunchKernelGGL(k_change, dim3(std::max(grid, 1)), dim3(256), 0, c.stream, total, a, b, acc);
+    hipLaunchKernelGGL(k_change_sum, dim3(1), dim3(256), 0, c.stream, grid, c.chgPart.p, acc);
     PG_CHECK(hipGetLastError());
 }
 
@@ -291,12 +315,16 @@ void launch_add(Ctx & c, long total, const double * a, const double * b, double 
     PG_CHECK(hipGetLastError());
 }
 
-void launch_jacobian_complex_combine(Ctx & c, const double * Srr, const double * Sii, const double * Sss,
-                                     const double * mr, const double * mi, bool scale, double * Jre,
-                                     double * Jim) {
+void launch_jacobian_complex_pre(Ctx & c, double * Jre, double * Jim) {
     const long total = (long)c.D * c.nParam;
-    hipLaunchKernelGGL(k_jacobian_c, dim3(div_up(total, 256)), dim3(256), 0, c.stream, total, c.nParam, Srr, Sii,
-                       Sss, c.kfac.p, mr, mi, scale ? 1 : 0, Jre, Jim);
+    hipLaunchKernelGGL(k_jacobian_c_pre, dim3(div_up(total, 256)), dim3(256), 0, c.stream, total, Jre, Jim);
+    PG_CHECK(hipGetLastError());
+}
+
+void launch_jacobian_complex_scale(Ctx & c, const double * mr, const double * mi, double * Jre, double * Jim) {
+    const long total = (long)c.D * c.nParam;
+    hipLaunchKernelGGL(k_jacobian_c_scale, dim3(div_up(total, 256)), dim3(256), 0, c.stream, total, c.nParam,
+                       c.kfac.p, mr, mi, Jre, Jim);
     PG_CHECK(hipGetLastError());
 }
 

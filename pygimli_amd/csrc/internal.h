@@ -77,6 +77,18 @@ struct Ctx {
     DBuf<double> vol;            // C
     DBuf<int> sortedCells;       // cells with marker >= 0, sorted by marker
     int nSorted = 0;
+    int nDistinct = 0;           // distinct markers >= 0
+    std::vector<int> hSortedMarker;  // marker of each sorted cell
+    // shared markers: Jacobian tiles of whole markers (built for tile width segTile)
+    int segTile = 0, segNTile = 0, segNBig = 0, segNChunk = 0;
+    DBuf<int> segTiles;          // segNTile * 4
+    DBuf<int> segBig;            // segNBig * 3: marker, first chunk, chunk count
+    DBuf<unsigned char> segPart; // chunk partial sums [chunk][D]
+    // background (marker < 0) cells: prolongation sweeps planned per mesh (prolong.hip)
+    int prolStatus = 0;              // 0 ok, 2 fixed-value regions, 3 no parameter cell, 4 unreachable cells
+    std::vector<int> prolLayerPtr;   // sweep l fills prolCell[ptr[l] .. ptr[l+1])
+    DBuf<int> prolCell, prolPtr, prolNbr;
+    DBuf<double> prolW;
 
     // ---- CSR pattern and atomics-free assembly map
     int nnz = 0;
@@ -169,7 +181,7 @@ struct Ctx {
     DBuf<unsigned char> bandDesc;
     // complex resistivity: imaginary half of A, unit point sources, the two
     // halves of the potentials, Jacobian work blocks, change accumulator
-    DBuf<double> valsAi, Bpt, Xre, Xim, Jw, chg;
+    DBuf<double> valsAi, Bpt, Xre, Xim, chg, chgPart;
     // bypass resistors of the total-field operator: matrix slots and values
     DBuf<int> bypSlot;
     DBuf<double> bypVal;
@@ -229,6 +241,9 @@ void launch_contrib_geometry(Ctx & c, const int * codes);
 void launch_assemble(Ctx & c, const double * rho, double k, double * vals, bool unitRho);
 void launch_fill_matrix(Ctx & c, int mass, const double * a, double * vals);
 void launch_map_model(Ctx & c, const double * model, int nModel, double * rho);
+void build_prolongation(Ctx & c, const double * xyz, const int * cells, const int * marker);
+void check_prolongation(const Ctx & c);
+void launch_prolongation(Ctx & c, double * rho);
 void launch_rho_source(Ctx & c, const double * rho);
 void launch_primary(Ctx & c);
 void launch_sr_rhs(Ctx & c);
@@ -236,7 +251,8 @@ void launch_diag_inv(Ctx & c);
 void launch_total_potential(Ctx & c);
 void launch_response(Ctx & c, double * rhoa);
 void launch_transpose_pots(Ctx & c, double * out);
-void launch_jacobian(Ctx & c, const double * model, int nModel, void * J, int dtype);
+// accumulate: add the sensitivities to the values J holds (no model scaling then)
+void launch_jacobian(Ctx & c, const double * model, int nModel, void * J, int dtype, bool accumulate = false);
 // one Jacobian launch per wavenumber pass over the cells of several problems
 // (same device and dimension; every context uses cs[0]->stream)
 void launch_assemble_complex(Ctx & c, const double * rre, const double * rim, double k, double * vre,
@@ -248,9 +264,8 @@ void launch_spmm_add(Ctx & c, const double * vals, const double * X, const doubl
 void launch_change(Ctx & c, long total, const double * a, const double * b, double * acc);
 void launch_response_complex(Ctx & c, const double * Ure, const double * Uim, double * out_re, double * out_im);
 void launch_add(Ctx & c, long total, const double * a, const double * b, double * out);
-void launch_jacobian_complex_combine(Ctx & c, const double * Srr, const double * Sii, const double * Sss,
-                                     const double * mr, const double * mi, bool scale, double * Jre,
-                                     double * Jim);
+void launch_jacobian_complex_pre(Ctx & c, double * Jre, double * Jim);
+void launch_jacobian_complex_scale(Ctx & c, const double * mr, const double * mi, double * Jre, double * Jim);
 void launch_jacobian_batch(const std::vector<Ctx *> & cs, const std::vector<const double *> & models,
                            const std::vector<int> & nModels, const std::vector<void *> & Js, int dtype);
 

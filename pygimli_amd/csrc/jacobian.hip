@@ -46,9 +46,11 @@ struct SensDesc {
     int nE, nEp, cs;            // electrodes, padded electrode count (slot nE = zero), cell stride
     int D, nSorted, nParam;     // data, cells with marker >= 0, Jacobian columns
     int tile0, nTile;           // first global tile of this problem in the launch, its tile count
-    int atomic, scale;          // shared markers (atomic adds); apply k_i / m_j^2 on the last pass
+    int seg, scale;             // shared markers (segmented sums over a tile's cells); apply k_i / m_j^2 on the last pass
     int first, last, nk;        // first / last pass of this problem; wavenumbers in this pass
-    int pad_;
+    int accum;                  // add to the values J holds on entry (every pass reads and adds)
+    const int * tiles;          // seg: per tile {first sorted cell, cell count, chunk slot or -1, 0}
+    void * part;                // seg: partial sums [chunk][D] of markers with more cells than a tile
     const int * sorted;
     const int * cells;
     const int * marker;
@@ -73,6 +75,7 @@ __global__ void __launch_bounds__(256) k_sens(const SensDesc * __restrict__ desc
     using CT = ST;  // arithmetic of the sweep
     extern __shared__ unsigned char smem[];
     __shared__ int scell[TILE];
+    __shared__ int smk[TILE];
     __shared__ int snode[TILE * NV];
     __shared__ double sgrad[FORM == F_GRAD ? TILE * NV * DIM : 1];
 
@@ -86,14 +89,23 @@ __global__ void __launch_bounds__(256) k_sens(const SensDesc * __restrict__ desc
     const SensDesc & d = descs[lo];
     const int tile = g - d.tile0;
     if (tile < 0 || tile >= d.nTile) return;
-    const int base = tile * TILE;
-    const int ncell = min(TILE, d.nSorted - base);
+    // shared markers: tiles hold whole markers (or one chunk of a marker with more cells than a tile)
+    int base, ncell, chunk = -1;
+    if (d.seg) {
+        base = glob(d.tiles)[4 * tile];
+        ncell = glob(d.tiles)[4 * tile + 1];
+        chunk = glob(d.tiles)[4 * tile + 2];
+    } else {
+        base = tile * TILE;
+        ncell = min(TILE, d.nSorted - base);
+    }
     const int nE = d.nE, nEp = d.nEp, CS = d.cs, nk = d.nk;
     ST * S = (ST *)smem;
 
     for (int t = threadIdx.x; t < TILE; t += blockDim.x) {
         const int c = t < ncell ? glob(d.sorted)[base + t] : -1;
         scell[t] = c;
+        smk[t] = c >= 0 ? glob(d.marker)[c] : -1;
         for (int v = 0; v < NV; ++v) snode[t * NV + v] = c >= 0 ? glob(d.cells)[(size_t)c * NV + v] : 0;
         if (FORM == F_GRAD)
             for (int v = 0; v < NV; ++v)
@@ -143,8 +155,24 @@ __global__ void __launch_bounds__(256) k_sens(const SensDesc * __restrict__ desc
     // ---- sweep the data
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int t = lane % TILE, sub = lane / TILE;
-    const int c = scell[t];
-    if (c < 0) return;  // no barrier below
+    // Shared markers: the TILE lanes of a sub-group hold consecutive cells of the
+    // marker-sorted order; a segmented scan over the lanes sums each marker's
+    // cells in a fixed order and the marker's last lane stores the sum.  Lanes
+    // of empty tile slots take part in the scan with a zero value.
+    const bool seg = d.seg != 0;
+    int c = scell[t];
+    const bool live = c >= 0;
+    if (!live) {
+        if (!seg) return;  // no barrier below
+        c = scell[0];
+    }
+    int segStart = t;
+    bool tail = live;
+    if (seg && live) {
+        const int mk = smk[t];
+        while (segStart > 0 && smk[segStart - 1] == mk) --segStart;
+        tail = t + 1 >= ncell || smk[t + 1] != mk;
+    }
     const double V = glob(d.vol)[c];
     const int col = glob(d.marker)[c];
     const double m2 = d.scale ? glob(d.model)[col] * glob(d.model)[col] : 1.0;
@@ -203,20 +231,23 @@ __global__ void __launch_bounds__(256) k_sens(const SensDesc * __restrict__ desc
     typedef int iv4 __attribute__((ext_vector_type(4)));
     const auto ab4 = glob((const iv4 *)d.abmn);
     const auto kf = glob(d.kfac);
-    OutT * Jf = (OutT *)d.J;  // flat, for the atomics
-    const auto J = glob(Jf);
-    const size_t ldj = d.nParam;
+    // destination of this lane's sums: column `col` of J, or the chunk's row of partial sums
+    OutT * dst0 = chunk >= 0 ? (OutT *)d.part + (size_t)chunk * d.D : (OutT *)d.J + col;
+    const auto J = glob(dst0);
+    const size_t ldj = chunk >= 0 ? 1 : (size_t)d.nParam;
     const ST * Sc = S + (size_t)t * CS;
     // createJacobian_ scaling J_ij *= k_i / m_j^2 (dcfemmodelling.cpp:1375-1384), on the last pass
-    const bool scl = d.last && d.scale;
+    // (chunk sums are scaled when they are added up)
+    const bool scl = d.last && d.scale && chunk < 0;
     const CT rm2 = (CT)(1.0 / m2);
-    // MODE 0: first pass, store; 1: later pass, add to the stored partial sums; 2: atomic adds.
+    // MODE 0: first pass, store; 1: later pass, add to the stored partial sums.
     // A step's indices, geometric factors and partial sums are loaded one step ahead,
     // before the previous step's stores: on this architecture a wait for a load also
     // waits for the stores issued before it, so loads issued after the stores would
     // put the store latency on every step.
-    auto sweep = [&](auto mode_c) {
+    auto sweep = [&](auto mode_c, auto seg_c) {
         constexpr int MODE = decltype(mode_c)::value;
+        constexpr bool SEG = decltype(seg_c)::value;
         int i0 = (wave * NSUB + sub) * UNR;
         iv4 qn[UNR];
         CT kn[UNR];
@@ -227,7 +258,7 @@ __global__ void __launch_bounds__(256) k_sens(const SensDesc * __restrict__ desc
                 const int ii = min(i + u, D - 1);
                 qn[u] = ab4[ii];
                 kn[u] = (CT)kf[ii];
-                if (MODE == 1) pn[u] = J[(size_t)ii * ldj + col];
+                if (MODE == 1) pn[u] = J[(size_t)ii * ldj];
             }
         };
         fetch(i0);
@@ -288,32 +319,93 @@ __global__ void __launch_bounds__(256) k_sens(const SensDesc * __restrict__ desc
             }
 #pragma unroll
             for (int u = 0; u < UNR; ++u) {
-                const int i = i0 + u;
-                if (i < D) {
-                    const size_t idx = (size_t)i * ldj + col;
-                    if (MODE == 2) {
-                        atomicAdd(Jf + idx, (OutT)val[u]);
-                    } else {
-                        const OutT r = (OutT)((val[u] + (CT)prev[u]) * fac[u]);
-                        if (d.last) store_nt(J + idx, r);
-                        else J[idx] = r;
+                CT v = val[u];
+                if (SEG) {
+                    // segmented inclusive scan over the sub-group's TILE lanes (fixed order)
+#pragma unroll
+                    for (int off = 1; off < TILE; off <<= 1) {
+                        const CT o = __shfl_up(v, off, TILE);
+                        if (t - off >= segStart) v += o;
                     }
+                }
+                const int i = i0 + u;
+                if (i < D && (!SEG || tail)) {
+                    const size_t idx = (size_t)i * ldj;
+                    const OutT r = (OutT)((v + (CT)prev[u]) * fac[u]);
+                    if (d.last) store_nt(J + idx, r);
+                    else J[idx] = r;
                 }
             }
         }
     };
-    if (d.atomic) sweep(std::integral_constant<int, 2>{});
-    else if (!d.first) sweep(std::integral_constant<int, 1>{});
-    else sweep(std::integral_constant<int, 0>{});
+    // chunk partials start from zero on the first pass; J itself is read back when accumulating
+    const bool addPrev = !d.first || (d.accum && chunk < 0);
+    if (seg) {
+        if (addPrev) sweep(std::integral_constant<int, 1>{}, std::true_type{});
+        else sweep(std::integral_constant<int, 0>{}, std::true_type{});
+    } else {
+        if (addPrev) sweep(std::integral_constant<int, 1>{}, std::false_type{});
+        else sweep(std::integral_constant<int, 0>{}, std::false_type{});
+    }
 }
 
+// Sum of the chunk partials of every marker with more cells than a tile, in
+// chunk order, scaled by k_i / m_j^2: J[i][marker] for big[g] = {marker,
+// first chunk, chunk count}.
 template <typename OutT>
-__global__ void k_scale_cols(long total, int nParam, const double * __restrict__ kfac,
-                             const double * __restrict__ model, OutT * __restrict__ J) {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= total) return;
-    int i = (int)(t / nParam), j = (int)(t % nParam);
-    J[t] = (OutT)((double)J[t] * kfac[i] / (model[j] * model[j]));
+__global__ void k_chunk_sum(int nBig, int D, int nParam, const int * __restrict__ big,
+                            const OutT * __restrict__ part, int scale, int accum, const double * __restrict__ kfac,
+                            const double * __restrict__ model, OutT * __restrict__ J) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nBig * D) return;
+    const int g = (int)(t / D), i = (int)(t % D);
+    const int mk = big[3 * g], c0 = big[3 * g + 1], nc = big[3 * g + 2];
+    OutT s = OutT(0);
+    for (int ch = 0; ch < nc; ++ch) s += part[(size_t)(c0 + ch) * D + i];
+    if (scale) s = (OutT)((double)s * kfac[i] / (model[mk] * model[mk]));
+    const size_t idx = (size_t)i * nParam + mk;
+    J[idx] = accum ? J[idx] + s : s;
+}
+
+// Tiles of the marker-sorted cells for shared markers: whole markers packed
+// into tiles of at most `tile` cells, a marker with more cells than a tile
+// split into chunks of its own tiles whose sums go to a partial buffer
+// [chunk][D] that k_chunk_sum adds up.  Kept per context for the tile width.
+void seg_tiles(Ctx & c, int tile, size_t elem) {
+    if (c.segTile != tile) {
+        std::vector<int> tl, big;
+        int nChunk = 0, curStart = 0, curCount = 0;
+        auto flush = [&] {
+            if (curCount > 0) tl.insert(tl.end(), {curStart, curCount, -1, 0});
+            curCount = 0;
+        };
+        const int ns = (int)c.hSortedMarker.size();
+        for (int g0 = 0; g0 < ns;) {
+            int g1 = g0 + 1;
+            while (g1 < ns && c.hSortedMarker[g1] == c.hSortedMarker[g0]) ++g1;
+            const int len = g1 - g0;
+            if (len > tile) {
+                flush();
+                const int nch = div_up(len, tile);
+                big.insert(big.end(), {c.hSortedMarker[g0], nChunk, nch});
+                for (int ch = 0; ch < nch; ++ch)
+                    tl.insert(tl.end(), {g0 + ch * tile, std::min(tile, len - ch * tile), nChunk++, 0});
+            } else {
+                if (curCount + len > tile) flush();
+                if (curCount == 0) curStart = g0;
+                curCount += len;
+            }
+            g0 = g1;
+        }
+        flush();
+        c.segTiles.upload(tl, c.stream);
+        c.segBig.upload(big, c.stream);
+        c.segNTile = (int)tl.size() / 4;
+        c.segNBig = (int)big.size() / 3;
+        c.segNChunk = nChunk;
+        c.segTile = tile;
+    }
+    c.segPart.reserve((size_t)c.segNChunk * c.D * elem);
 }
 
 struct Plan {
@@ -375,7 +467,7 @@ void launch_k(const Plan & p, const SensDesc * descs, int count, int grid, hipSt
 
 template <typename OutT>
 void run(const std::vector<Ctx *> & cs, const std::vector<const double *> & models, const std::vector<int> & nModels,
-         const std::vector<void *> & Js) {
+         const std::vector<void *> & Js, bool accum) {
     Ctx & c0 = *cs[0];
     hipStream_t s = c0.stream;
     const int count = (int)cs.size();
@@ -391,14 +483,15 @@ void run(const std::vector<Ctx *> & cs, const std::vector<const double *> & mode
     const int npass = (nKmax + p.kp - 1) / p.kp;
     const int ncomp = p.form == F_GRAD ? c0.dim : c0.dim + 1;
 
-    std::vector<char> atomicP(count), scaleP(count);
+    std::vector<char> segP(count), scaleP(count);
     for (int j = 0; j < count; ++j) {
         Ctx & c = *cs[j];
-        atomicP[j] = !c.uniqueMarkers;
+        segP[j] = !c.uniqueMarkers;
         scaleP[j] = nModels[j] == c.nParam;
+        if (segP[j]) seg_tiles(c, p.tile, sizeof(OutT));
         const size_t jsize = (size_t)c.D * c.nParam;
         // columns without any cell stay zero (markers not covering 0..max)
-        if (atomicP[j] || c.nSorted < c.nParam) PG_CHECK(hipMemsetAsync(Js[j], 0, jsize * sizeof(OutT), s));
+        if (c.nDistinct < c.nParam && !accum) PG_CHECK(hipMemsetAsync(Js[j], 0, jsize * sizeof(OutT), s));
     }
     // descriptors of every pass, uploaded once
     std::vector<SensDesc> hd;
@@ -418,10 +511,13 @@ void run(const std::vector<Ctx *> & cs, const std::vector<const double *> & mode
             sd.nSorted = c.nSorted;
             sd.nParam = c.nParam;
             sd.tile0 = tile0;
-            sd.nTile = div_up(c.nSorted, p.tile);
-            sd.atomic = atomicP[j];
+            sd.nTile = segP[j] ? c.segNTile : div_up(c.nSorted, p.tile);
+            sd.seg = segP[j];
+            sd.tiles = segP[j] ? c.segTiles.p : nullptr;
+            sd.part = segP[j] ? (void *)c.segPart.p : nullptr;
             sd.scale = scaleP[j];
             sd.first = k0 == 0;
+            sd.accum = accum;
             sd.last = k1 == c.nK;
             sd.nk = k1 - k0;
             sd.sorted = c.sortedCells.p;
@@ -462,10 +558,11 @@ void run(const std::vector<Ctx *> & cs, const std::vector<const double *> & mode
     }
     for (int j = 0; j < count; ++j) {
         Ctx & c = *cs[j];
-        if (atomicP[j] && scaleP[j]) {
-            const long total = (long)c.D * c.nParam;
-            hipLaunchKernelGGL(k_scale_cols<OutT>, dim3(div_up(total, 256)), dim3(256), 0, s, total, c.nParam,
-                               c.kfac.p, models[j], (OutT *)Js[j]);
+        if (segP[j] && c.segNBig > 0 && c.D > 0) {
+            const long total = (long)c.segNBig * c.D;
+            hipLaunchKernelGGL(k_chunk_sum<OutT>, dim3(div_up(total, 256)), dim3(256), 0, s, c.segNBig, c.D,
+                               c.nParam, c.segBig.p, (const OutT *)c.segPart.p, (int)scaleP[j], (int)accum, c.kfac.p,
+                               models[j], (OutT *)Js[j]);
             PG_CHECK(hipGetLastError());
         }
     }
@@ -475,12 +572,13 @@ void run(const std::vector<Ctx *> & cs, const std::vector<const double *> & mode
 
 void launch_jacobian_batch(const std::vector<Ctx *> & cs, const std::vector<const double *> & models,
                            const std::vector<int> & nModels, const std::vector<void *> & Js, int dtype) {
-    if (dtype == 0) run<double>(cs, models, nModels, Js);
-    else run<float>(cs, models, nModels, Js);
+    if (dtype == 0) run<double>(cs, models, nModels, Js, false);
+    else run<float>(cs, models, nModels, Js, false);
 }
 
-void launch_jacobian(Ctx & c, const double * model, int nModel, void * J, int dtype) {
-    launch_jacobian_batch({&c}, {model}, {nModel}, {J}, dtype);
+void launch_jacobian(Ctx & c, const double * model, int nModel, void * J, int dtype, bool accumulate) {
+    if (dtype == 0) run<double>({&c}, {model}, {nModel}, {J}, accumulate);
+    else run<float>({&c}, {model}, {nModel}, {J}, accumulate);
 }
 
 }  // namespace pgert

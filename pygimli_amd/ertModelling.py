@@ -69,8 +69,11 @@ class ERTModelling:
         self._J = None
         self._model = None
         self.jacobianDtype = torch.float64
+        # report_only=False: a solve that misses rtol is repeated from the analytic
+        # start, then in FP64, and raises if that also misses; True: lastStats
+        # carries converged = 0 instead
         self.solverOptions = dict(method="mixed", rtol=1e-6, max_iter=20000, check_every=8,
-                                  inner_rtol=1e-3, learned=False, coarse=0, graph=True)
+                                  inner_rtol=1e-3, learned=False, coarse=0, graph=True, report_only=False)
         self._coarseSet = None
         self.lastStats = None
         self.warmStart = None
@@ -87,6 +90,7 @@ class ERTModelling:
         if c != self._complex:
             self._model = None
             self._J = None
+            self._Jc = None
         self._complex = c
 
     def complex(self):
@@ -235,13 +239,17 @@ class ERTModelling:
 
     def mapERTModel(self, model, background=-9e99):
         """Cell resistivities from a model (dcfemmodelling.cpp:1211-1218,
-        modellingbase.cpp:401-470).  Returns a device tensor."""
+        modellingbase.cpp:401-470): a cell-length model is copied, otherwise
+        rho_c = model[marker_c] and cells with a negative marker are
+        prolongated from their neighbours (Mesh::prolongateEmptyCellsValues,
+        mesh.cpp:2247-2318; only the default background -9e99 of
+        response(model) is supported).  Returns a device tensor."""
+        if background > -9e99:
+            raise NotImplementedError("only the prolongating background -9e99 is supported")
         t = self._model_tensor(model)
         nm = t.numel()
         C = self._mesh.cellCount()
         if nm != C:
-            if np.any(self._mesh.cellMarkers() < 0):
-                raise NotImplementedError("background (marker -1) prolongation is not supported")
             if self._mesh.cellMarkers().max() >= nm:
                 raise IndexError("marker >= model.size()")
         rho = torch.empty(C, dtype=torch.float64, device=self.device)
@@ -266,6 +274,7 @@ class ERTModelling:
             self._coarseSet = nagg
         opts.coarse = 1 if nagg > 0 else 0
         opts.no_graph = 0 if o.get("graph", True) else 1
+        opts.report_only = 1 if o.get("report_only", False) else 0
         if self.warmStart is not None:
             ws = self.warmStart
             if ws.dtype != torch.float64 or not ws.is_contiguous():
@@ -367,6 +376,7 @@ class ERTModelling:
             return self._jacobian_complex(t)
         dtype = self.jacobianDtype
         J = torch.empty((self._data.size(), self._nParam), dtype=dtype, device=self.device)
+        self._Jc = None
         code = 0 if dtype == torch.float64 else 1
         self._set_stream()
         self._ctx.check(self._ctx.L.pgert_jacobian(self._ctx.h, _dptr(t), t.numel(), _dptr(J), code),
@@ -388,12 +398,17 @@ class ERTModelling:
         self._ctx.check(self._ctx.L.pgert_jacobian_complex(self._ctx.h, _dptr(re), _dptr(im), h, _dptr(Jr),
                                                            _dptr(Ji)), "createJacobian")
         self._Jc = (Jr, Ji)
-        self._J = torch.cat([torch.cat([Jr, -Ji], 1), torch.cat([Ji, Jr], 1)], 0)
+        self._J = None   # the real block form is built on demand (4 D x P)
         return None
 
     def jacobian(self):
         """The Jacobian of the last createJacobian, [nData, nParam] device
-        tensor ([2 nData, 2 nParam] real block form for complex resistivities)."""
+        tensor ([2 nData, 2 nParam] real block form for complex resistivities,
+        materialised on the first call: it holds four D x P FP64 blocks, while
+        jacobianComplex() returns the two parts without a copy)."""
+        if self._J is None and self._Jc is not None:
+            Jr, Ji = self._Jc
+            self._J = torch.cat([torch.cat([Jr, -Ji], 1), torch.cat([Ji, Jr], 1)], 0)
         return self._J
 
     def jacobianComplex(self):

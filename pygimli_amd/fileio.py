@@ -301,8 +301,12 @@ def load_data(fileName, sensorIndicesFromOne=True, removeInvalid=True):
     if len(row) == 1:
         nTopo = _atoi(row[0])
         if nTopo > 0:
+            # without a '#' line the reference keeps the format of the section before
+            # (the data or sensor format) and refuses its non-coordinate tokens
             tf = rows.format_line()
-            if tf is None or not tf or tf[0] not in (b"x", b"X"):
+            if tf is None:
+                tf = fmt
+            elif not tf or tf[0] not in (b"x", b"X"):
                 tf = [b"x", b"y", b"z"]
             topo = np.zeros((nTopo, 3))
             for i in range(nTopo):

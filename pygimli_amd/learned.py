@@ -32,6 +32,34 @@ def init_params(seed=0):
     return p
 
 
+FORMAT = "pgert-mlp-warmstart-1"
+
+
+def save_params(path, params, meta=None):
+    """Write trained parameters (packed FP32, pgert_mlp_set_params layout) with
+    the layer sizes and a format tag to an .npz file; `meta` adds scalar
+    entries (training fields, epochs, loss, ...)."""
+    params = np.ascontiguousarray(params, np.float32)
+    n = sum(fi * fo + fo for fi, fo in zip(SIZES[:-1], SIZES[1:]))
+    if params.size != n:
+        raise ValueError(f"expected {n} parameters, got {params.size}")
+    extra = {f"meta_{k}": np.asarray(v) for k, v in (meta or {}).items()}
+    np.savez(path, format=np.asarray(FORMAT), sizes=np.asarray(SIZES, np.int32), params=params, **extra)
+
+
+def load_params(path):
+    """Read a file written by save_params: (params, meta dict).  Raises on a
+    foreign format or other layer sizes."""
+    with np.load(path, allow_pickle=False) as z:
+        if "format" not in z or str(z["format"]) != FORMAT:
+            raise ValueError(f"{path}: not a warm-start parameter file ({FORMAT})")
+        if tuple(int(v) for v in z["sizes"]) != SIZES:
+            raise ValueError(f"{path}: layer sizes {tuple(z['sizes'])} differ from {SIZES}")
+        params = np.array(z["params"], np.float32)
+        meta = {k[5:]: z[k][()] for k in z.files if k.startswith("meta_")}
+    return params, meta
+
+
 class LearnedWarmStart:
     """Owns the parameters on one ERTModelling's context."""
 
@@ -59,6 +87,16 @@ class LearnedWarmStart:
         ctx.check(ctx.L.pgert_mlp_get_params(ctx.h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), p.size),
                   "mlp_get_params")
         return p
+
+    def save(self, path, **meta):
+        """Checkpoint the current parameters (save_params)."""
+        save_params(path, self.params(), meta)
+
+    def load(self, path):
+        """Restore parameters written by save(); returns the stored meta entries."""
+        p, meta = load_params(path)
+        self.set_params(p)
+        return meta
 
     def collect(self, n_samples, rng, k_index=0):
         """Features/targets of random (node, source) samples of the last response()."""

@@ -50,10 +50,17 @@ def _context_for(mesh, device=0):
 
 
 def _fill(mesh, a, mass, device=0):
-    ctx, rowptr, colidx = _context_for(mesh, device)
+    if mesh.cellCount() == 0:
+        raise Exception("Mesh invalid")
     if a is None:
         a = 1.0
-    a = np.broadcast_to(np.asarray(a, np.float64), (mesh.cellCount(),))
+    a = np.asarray(a)
+    if np.iscomplexobj(a):
+        raise NotImplementedError("complex cell values: the GPU fill is real-valued")
+    if a.ndim == 1 and len(a) != mesh.cellCount():
+        raise ValueError("Number of cell values need to match cell count")
+    a = np.broadcast_to(a.astype(np.float64), (mesh.cellCount(),))
+    ctx, rowptr, colidx = _context_for(mesh, device)
     dev = torch.device("cuda", device)
     at = torch.as_tensor(np.ascontiguousarray(a), device=dev)
     vals = torch.empty(len(colidx), dtype=torch.float64, device=dev)

@@ -256,3 +256,15 @@ def test_mesh_bytes_match_reference(tmp_path, name):
     m._fileNeighbors = None
     l2, r2 = fileio._neighbors(m)
     assert np.array_equal(np.sort(np.stack([left, right]), 0), np.sort(np.stack([l2, r2]), 0))
+
+
+@needs_ref
+def test_topography_without_format_line_refused_like_reference(tmp_path):
+    """A topography section with no '#' line keeps the data section's format in
+    the reference (datacontainer.cpp:431-441), whose first token is no
+    coordinate: both readers refuse the file."""
+    src = _write(tmp_path, "in.dat", POLE + TOPO.replace(b"# x y z\n", b""))
+    with pytest.raises(ValueError, match="topo electrode format"):
+        fileio.load_data(src)
+    with pytest.raises(RuntimeError):
+        ref.data_convert(src, tmp_path / "theirs.dat")

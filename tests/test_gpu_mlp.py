@@ -100,3 +100,22 @@ def test_learned_warm_start_gain():
         assert fop.lastStats["converged"] == 1
         np.testing.assert_allclose(r1, r0, rtol=1e-6)
     print("iterations without / with learned warm start", it0, it1)
+    # the network must earn its place: at least a 20 % cut in iterations
+    assert np.mean(it1) <= 0.8 * np.mean(it0), (it0, it1)
+    # a checkpoint restores the same predictions on a fresh context
+    import os
+    import tempfile
+    from pygimli_amd.ertModelling import ERTModelling
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "warmstart.npz")
+        lw.save(path, epochs=40)
+        fop2 = ERTModelling()
+        fop2.setMesh(mesh)
+        fop2.setData(copy.deepcopy(data))
+        fop2.solverOptions.update(method="mixed", rtol=1e-8, learned=True)
+        lw2 = LearnedWarmStart(fop2, seed=1)
+        meta = lw2.load(path)
+        assert int(meta["epochs"]) == 40
+        np.testing.assert_array_equal(lw2.params(), lw.params())
+        fop2.response(field(101))
+        assert fop2.lastStats["iterations"] == it1[0]

@@ -85,8 +85,9 @@ def test_forward_and_jacobian(name, method, oracle_results):
 @pytest.mark.parametrize("name", CASES)
 def test_jacobian_fp32_and_regions(name, oracle_results):
     """FP32 Jacobian (FP32 features in LDS) against the oracle, and a region
-    model: 13 parameters shared by many cells (atomic accumulation into
-    J columns, scaling k_i / m_j^2 after the sum), FP64 and FP32."""
+    model: 13 parameters shared by many cells (segmented sums over the
+    marker-sorted cells, markers larger than a tile summed from chunk
+    partials, scaling k_i / m_j^2 after the sum), FP64 and FP32."""
     import torch
     fo, rhoa_o, J_o = oracle_results[name]
     fop, mesh, data, rho = _fop(name, "block64")
@@ -260,3 +261,123 @@ def test_forward_from_files(tmp_path):
     print("from files: rhoa", np.abs(r1 - r0).max() / np.abs(r0).max())
     assert np.abs(r1 - r0).max() <= 1e-11 * np.abs(r0).max()
     assert np.linalg.norm(J1 - J0) <= 1e-11 * np.linalg.norm(J0)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_shared_markers_vs_reference_deterministic(dim):
+    """Parameters shared by groups of 4 (2D) / 8 (3D) scattered cells, plus one
+    marker over 150 cells (more than a Jacobian tile): rhoa and J against the
+    reference's own C++ core (oracle/_ref) and the oracle, FP64 and FP32, and
+    two Jacobian calls give identical bits (fixed-order sums, no atomics)."""
+    import torch
+    from oracle import ref
+    from pygimli_amd.ertModelling import ERTModelling
+    if dim == 2:
+        mesh, data, _ = workloads.small_2d(n_elec=6, seed=7)
+        grp = 4
+    else:
+        mesh, data, _ = workloads.small_3d()
+        grp = 8
+    rng = np.random.default_rng(3)
+    C = mesh.cellCount()
+    mk = (rng.permutation(C) // grp).astype(np.int32)
+    big = int(mk.max()) + 1
+    mk[rng.choice(C, 150, replace=False)] = big
+    mesh.setCellMarkers(mk)
+    model = np.exp(rng.normal(3.0, 0.5, big + 1))
+    fo = eo.Forward(mesh, copy.deepcopy(data))
+    rhoa_o = fo.response(model)
+    J_o = fo.jacobian(model)
+    if ref.available():
+        s = ref.Session(mesh, copy.deepcopy(data))
+        J_r = s.jacobian(model)
+        assert np.linalg.norm(J_r - J_o) / np.linalg.norm(J_r) < 1e-11
+    fop = ERTModelling()
+    fop.setMesh(mesh)
+    fop.setData(copy.deepcopy(data))
+    fop.solverOptions.update(method="block64", rtol=1e-11, coarse=0)
+    rhoa = fop.response(model)
+    assert np.abs(rhoa - rhoa_o).max() / np.abs(rhoa_o).max() < 1e-9
+    for dt, tol in ((torch.float64, 1e-9), (torch.float32, 1e-5)):
+        fop.jacobianDtype = dt
+        fop.createJacobian(model)
+        J1 = fop.jacobian().clone()
+        fop.createJacobian(model)
+        J2 = fop.jacobian()
+        assert torch.equal(J1, J2), "shared-marker Jacobian differs between two calls"
+        J = J1.cpu().numpy().astype(np.float64)
+        err = np.linalg.norm(J - J_o) / np.linalg.norm(J_o)
+        print(dim, dt, "shared-marker J rel fro", err)
+        assert J.shape == J_o.shape and err < tol
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_pg_solver_stiffness_and_mass(dim):
+    """pg.solver.createStiffnessMatrix / createMassMatrix (solver.py:1908-2038)
+    -> RSparseMatrix::fillStiffnessMatrix / fillMassMatrix (sparsematrix.h:1039-1065)
+    on the GPU against the reference build (oracle/_ref) and the oracle: identical
+    pattern (row pointer, sorted column indices), values within 1e-13."""
+    from oracle import ref
+    from pygimli_amd import solver
+    mesh = workloads.small_2d()[0] if dim == 2 else workloads.small_3d()[0]
+    rng = np.random.default_rng(5 + dim)
+    a = rng.uniform(0.1, 5.0, mesh.cellCount())
+    s = ref.Session(mesh) if ref.available() else None
+    for mass, fn, ofn in ((False, solver.createStiffnessMatrix, eo.stiffness_matrix),
+                          (True, solver.createMassMatrix, eo.mass_matrix)):
+        for coef in (a, None, 2.5):
+            A = fn(mesh, coef)
+            cf = np.broadcast_to(1.0 if coef is None else np.asarray(coef, float), (mesh.cellCount(),))
+            B = ofn(mesh, cf)
+            refs = [B] + ([s.fill_matrix(np.ascontiguousarray(cf), mass=mass)] if s is not None else [])
+            for R in refs:
+                np.testing.assert_array_equal(A.indptr, R.indptr)
+                np.testing.assert_array_equal(A.indices, R.indices)
+                err = np.abs(A.data - R.data).max() / np.abs(R.data).max()
+                print(dim, "mass" if mass else "stiffness", err)
+                assert err < 1e-13
+    with pytest.raises(Exception):
+        solver.createStiffnessMatrix(mesh, np.ones(mesh.cellCount() + 1))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_background_prolongation(dim):
+    """Background cells (marker -1) around a half-ellipse of parameter cells:
+    mapERTModel's prolongation sweeps (Mesh::prolongateEmptyCellsValues,
+    mesh.cpp:2247-2318) on the GPU against the oracle (pinned to the reference
+    build), then rhoa and J (no background columns) against the oracle."""
+    from pygimli_amd.ertModelling import ERTModelling
+    if dim == 2:
+        mesh, data, _ = workloads.small_2d(n_elec=6, seed=7)
+    else:
+        mesh, data, _ = workloads.small_3d()
+    cc = mesh.cellCenters()
+    sp = data.sensorPositions()
+    ctr = sp.mean(axis=0)
+    half = np.maximum(0.5 * (sp.max(axis=0) - sp.min(axis=0)), 1.0)
+    r2 = ((cc[:, 0] - ctr[0]) / (1.1 * half[0])) ** 2 + (cc[:, dim - 1] / (0.8 * half[0])) ** 2
+    if dim == 3:
+        r2 += ((cc[:, 1] - ctr[1]) / (1.1 * half[1])) ** 2
+    inside = r2 < 1.0
+    mk = -np.ones(mesh.cellCount(), np.int32)
+    mk[inside] = np.arange(inside.sum())
+    mesh.setCellMarkers(mk)
+    model = np.exp(np.random.default_rng(1).normal(3.0, 0.5, int(inside.sum())))
+    fo = eo.Forward(mesh, copy.deepcopy(data))
+    rho_o = eo.map_model(mesh, model)
+    rhoa_o = fo.response(model)
+    J_o = fo.jacobian(model)
+    fop = ERTModelling()
+    fop.setMesh(mesh)
+    fop.setData(copy.deepcopy(data))
+    fop.solverOptions.update(method="block64", rtol=1e-11, coarse=0)
+    rho = fop.mapERTModel(model).cpu().numpy()
+    err = np.abs(rho - rho_o).max() / rho_o.max()
+    print(dim, "prolongated rho rel", err)
+    assert err < 1e-14
+    rhoa = fop.response(model)
+    assert np.abs(rhoa - rhoa_o).max() / np.abs(rhoa_o).max() < 1e-9
+    fop.createJacobian(model)
+    J = fop.jacobian().cpu().numpy()
+    assert J.shape == J_o.shape
+    assert np.linalg.norm(J - J_o) / np.linalg.norm(J_o) < 1e-9

@@ -164,3 +164,23 @@ def test_large_survey_schemes():
                                                   z_h=1.0, z_growth=2.0)
     np.testing.assert_array_equal(ab, data2.abmn())
     np.testing.assert_array_equal(rho, rho2)
+
+
+def test_warm_start_checkpoint_round_trip(tmp_path):
+    """Trained warm-start parameters survive save/load bit for bit; foreign
+    files and other layer sizes are refused."""
+    from pygimli_amd import learned
+    p = learned.init_params(3)
+    path = tmp_path / "ws.npz"
+    learned.save_params(path, p, {"epochs": 20, "loss": 0.125})
+    q, meta = learned.load_params(path)
+    np.testing.assert_array_equal(p, q)
+    assert int(meta["epochs"]) == 20 and float(meta["loss"]) == 0.125
+    with pytest.raises(ValueError):
+        learned.save_params(tmp_path / "bad.npz", p[:-1])
+    np.savez(tmp_path / "other.npz", params=p)
+    with pytest.raises(ValueError):
+        learned.load_params(tmp_path / "other.npz")
+    np.savez(tmp_path / "sizes.npz", format=np.asarray(learned.FORMAT), sizes=np.asarray([32, 64, 1]), params=p)
+    with pytest.raises(ValueError):
+        learned.load_params(tmp_path / "sizes.npz")

@@ -230,6 +230,147 @@ def test_oracle_matches_live_reference_build():
     assert np.linalg.norm(J1 - J2) / np.linalg.norm(J1) < 1e-11
 
 
+@pytest.mark.parametrize("mi", [0, 1])
+@pytest.mark.parametrize("mass", [False, True])
+def test_fill_matrix_oracle_vs_reference_build(mi, mass):
+    """pg.solver.createStiffnessMatrix / createMassMatrix back end
+    (RSparseMatrix::fillStiffnessMatrix / fillMassMatrix, sparsematrix.h:1039-1065):
+    the oracle's P1 scatter against the reference build, identical pattern,
+    values to rounding; a = 1 gives the testFEM invariants (stiffness rows sum
+    to 0, mass entries sum to the volume)."""
+    if not ref.available():
+        pytest.skip("oracle/_ref not built")
+    mesh = _tiny_meshes()[mi] if mi == 0 else workloads.small_3d()[0]
+    rng = np.random.default_rng(11 + mi)
+    a = rng.uniform(0.1, 5.0, mesh.cellCount())
+    s = ref.Session(mesh)
+    A = s.fill_matrix(a, mass=mass)
+    B = (eo.mass_matrix if mass else eo.stiffness_matrix)(mesh, a)
+    np.testing.assert_array_equal(A.indptr, B.indptr)
+    np.testing.assert_array_equal(A.indices, B.indices)
+    assert np.abs(A.data - B.data).max() / np.abs(A.data).max() < 1e-14
+    one = s.fill_matrix(np.ones(mesh.cellCount()), mass=mass)
+    if mass:
+        assert abs(one.sum() - mesh.cellSizes().sum()) < 1e-12 * mesh.cellSizes().sum()
+    else:
+        assert np.abs(np.asarray(one.sum(axis=1))).max() < 1e-12 * np.abs(one.data).max()
+
+
+def _background_case(dim, seed=1):
+    """Parameter cells (one marker each) under the electrode spread, every
+    other cell background (marker -1), as the para-domain of ERTManager with a
+    boundary region (modellingbase.cpp:441-457)."""
+    if dim == 2:
+        mesh, data, _ = workloads.small_2d(n_elec=6, seed=7)
+    else:
+        mesh, data, _ = workloads.small_3d()
+    cc = mesh.cellCenters()
+    sp = data.sensorPositions()
+    # a half-ellipse(-oid) under the electrodes: its staircase edge puts
+    # parameter cells of different values across vertical and horizontal
+    # faces (weights ~1 and ~1e-6) of the same background cell
+    ctr = sp.mean(axis=0)
+    half = np.maximum(0.5 * (sp.max(axis=0) - sp.min(axis=0)), 1.0)
+    r2 = ((cc[:, 0] - ctr[0]) / (1.1 * half[0])) ** 2 + (cc[:, dim - 1] / (0.8 * half[0])) ** 2
+    if dim == 3:
+        r2 += ((cc[:, 1] - ctr[1]) / (1.1 * half[1])) ** 2
+    inside = r2 < 1.0
+    mk = -np.ones(mesh.cellCount(), np.int32)
+    mk[inside] = np.arange(inside.sum())
+    mesh.setCellMarkers(mk)
+    model = np.exp(np.random.default_rng(seed).normal(3.0, 0.5, int(inside.sum())))
+    return mesh, data, model
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_background_prolongation_vs_reference_build(dim):
+    """Background cells (marker -1) prolongated by Mesh::prolongateEmptyCellsValues
+    (mesh.cpp:2247-2318) inside createMappedModel (modellingbase.cpp:449-457):
+    oracle rhoa and J (background cells have no column) against the reference
+    build; unit weights instead of the horizontal face weights (a plausible
+    slip) move rhoa far beyond the tolerance."""
+    if not ref.available():
+        pytest.skip("oracle/_ref not built")
+    mesh, data, model = _background_case(dim)
+    rho = eo.map_model(mesh, model)
+    assert np.all(rho > 0) and np.any(mesh.cellMarkers() < 0)
+    s = ref.Session(mesh, copy.deepcopy(data))
+    fo = eo.Forward(mesh, copy.deepcopy(data))
+    r1, r2 = s.response(model), fo.response(model)
+    assert np.abs(r1 - r2).max() / np.abs(r1).max() < 1e-11
+    J1, J2 = s.jacobian(model), fo.jacobian(model)
+    assert J1.shape == (data.size(), len(model))
+    assert np.linalg.norm(J1 - J2) / np.linalg.norm(J1) < 1e-11
+    # planted: the same sweeps with unit weights
+    nb, _ = eo.cell_neighbours(mesh)
+    vals = np.where(mesh.cellMarkers() >= 0, model[np.maximum(mesh.cellMarkers(), 0)], 0.0)
+    while np.any(vals == 0.0):
+        new = vals.copy()
+        for i in np.nonzero(vals == 0.0)[0]:
+            ok = [o for o in nb[i] if o >= 0 and vals[o] != 0.0]
+            if ok:
+                new[i] = np.mean(vals[ok])
+        vals = new
+    assert np.abs(vals - rho).max() / rho.max() > 1e-3
+    fo2 = eo.Forward(mesh, copy.deepcopy(data))
+    bad = fo2.response(vals)
+    assert np.abs(bad - r1).max() / np.abs(r1).max() > 1e-6
+
+
+def _shared_marker_case(dim, seed=3):
+    """A mesh whose parameters are shared by groups of 4 (2D) or 8 (3D)
+    scattered cells (the H2-refined forward mesh of a parameter mesh,
+    pygimli/frameworks/modelling.py:586), and a model of length nParam."""
+    if dim == 2:
+        mesh, data, _ = workloads.small_2d(n_elec=6, seed=7)
+        grp = 4
+    else:
+        mesh, data, _ = workloads.small_3d()
+        grp = 8
+    rng = np.random.default_rng(seed)
+    mk = (rng.permutation(mesh.cellCount()) // grp).astype(np.int32)
+    mesh.setCellMarkers(mk)
+    model = np.exp(rng.normal(3.0, 0.5, int(mk.max()) + 1))
+    return mesh, data, model, mk
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_shared_marker_oracle_vs_reference_build(dim):
+    """Region parameters (several cells per marker): createMappedModel's
+    marker branch (modellingbase.cpp:417-437), the marker-column sums of calc2
+    (bertJacobian.cpp:191,380) and the k/m^2 scaling with len(model) = nParam
+    (dcfemmodelling.cpp:1375-1384), oracle against the reference build."""
+    if not ref.available():
+        pytest.skip("oracle/_ref not built")
+    mesh, data, model, mk = _shared_marker_case(dim)
+    s = ref.Session(mesh, copy.deepcopy(data))
+    fo = eo.Forward(mesh, copy.deepcopy(data))
+    r1, r2 = s.response(model), fo.response(model)
+    assert np.abs(r1 - r2).max() / np.abs(r1).max() < 1e-11
+    J1, J2 = s.jacobian(model), fo.jacobian(model)
+    assert J1.shape == (data.size(), len(model))
+    assert np.linalg.norm(J1 - J2) / np.linalg.norm(J1) < 1e-11
+    # chain rule through the marker map, on the reference build alone: with one
+    # marker per cell and rho_c = m_marker(c), the scaled cell columns
+    # k_i/rho_c^2 dU/drho_c summed over a marker's cells are its region column
+    mesh_c = copy.deepcopy(mesh)
+    mesh_c.setCellMarkers(np.arange(mesh.cellCount(), dtype=np.int32))
+    s_c = ref.Session(mesh_c, copy.deepcopy(data))
+    rho_c = model[mk]
+    np.testing.assert_allclose(s_c.response(rho_c), r1, rtol=1e-12)
+    Jc = s_c.jacobian(rho_c)
+    Jsum = np.zeros_like(J1)
+    for i in range(Jc.shape[0]):
+        Jsum[i] = np.bincount(mk, weights=Jc[i], minlength=len(model))
+    assert np.linalg.norm(Jsum - J1) / np.linalg.norm(J1) < 1e-12
+    # a cell-length model on the shared-marker mesh is copied (mapERTModel,
+    # dcfemmodelling.cpp:1211-1218) and J is left unscaled (len != nParam)
+    J3, J4 = s.jacobian(rho_c), fo.jacobian(rho_c)
+    assert np.linalg.norm(J3 - J4) / np.linalg.norm(J3) < 1e-11
+    kf = s.data_k()
+    np.testing.assert_allclose(J3 * kf[:, None] / (model * model)[None, :], J1, rtol=1e-10, atol=0)
+
+
 # ---------------------------------------------------------------- physics invariants
 def test_homogeneous_halfspace_3d_exact():
     """SR with homogeneous rho: the secondary RHS vanishes, u = rho * primary,
