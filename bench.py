@@ -34,14 +34,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline_worker(out_path, nthreads, sample_frac, pcg_iters, workload="ert3d_grid_grf"):
-    """Runs in a child process (started before the GPU work): the oracle's C
-    ports on the host cores, on a bounded sample of the same workload.
+def cpu_baseline_worker(out_path, nthreads, sample_frac, pcg_iters, workload="ert3d_grid_grf", ref_data=0):
+    """Runs in a child process (started before the GPU work): CPU work on the
+    host cores, on a bounded sample of the same workload.
     Forward = FP64 assembly of the system (oracle) + the plain FP64 Jacobi-PCG
-    port over all 64 sources, `pcg_iters` iterations timed (per-iteration cost
+    C port over all sources, `pcg_iters` iterations timed (per-iteration cost
     does not depend on the values), scaled by the caller to the iteration
-    count the same algorithm needs on this workload; Jacobian = the C
-    restatement of calc2 on a contiguous sample of cells, scaled to all cells.
+    count the same algorithm needs on this workload (the reference's own
+    direct solver cannot run here: CHOLMOD is absent).
+    Jacobian = with ref_data > 0 the reference's own createJacobian (calc2 on
+    its thread pool, oracle/_ref) for the first ref_data quadrupoles on the
+    whole mesh, scaled by D / ref_data (its cost is linear in the data count;
+    the potentials handed to it are the analytic primaries, which cost calc2
+    the same as any other values); otherwise the C restatement of calc2 on a
+    contiguous sample of cells, scaled to all cells.
     Threads: min(8, nCPU-2) (the reference's default thread count,
     modellingbase.cpp:77), the PCG split over right-hand sides."""
     from oracle import ert_oracle as eo
@@ -60,15 +66,33 @@ def cpu_baseline_worker(out_path, nthreads, sample_frac, pcg_iters, workload="er
     C = mesh.cellCount()
     c1 = max(1, int(C * sample_frac))
     U = fo.primaries()
-    # the sampled cells get their own columns 0..c1-1 (J of the sample only)
-    mk = np.full(C, -1, np.int32)
-    mk[:c1] = np.arange(c1)
-    t = time.time()
-    port.calc2(mesh, U, kv, wv, data.abmn(), c1, nthreads=nthreads, cell_range=(0, c1),
-               J=np.zeros((data.size(), c1)), markers=mk)
-    tj = (time.time() - t) * C / c1
+    res = {"ta": ta, "t_it": t_it, "it_sampled": it, "c1": c1, "C": C, "nrhs": B.shape[1], "D": data.size()}
+    if ref_data > 0:
+        from oracle import ref
+        if ref.available():
+            sub = workloads.make_data(data.sensorPositions(), data.abmn()[:ref_data])
+            sub.set("k", fo.geometric_factors()[:ref_data])
+            t = time.time()
+            sess = ref.Session(mesh, sub, nthreads=nthreads)
+            res["ref_setup_s"] = time.time() - t
+            sess.set_subpotentials(U)
+            t = time.time()
+            sess.jacobian_only(rho)
+            res["tj_sample"] = time.time() - t
+            res["tj"] = res["tj_sample"] * data.size() / ref_data
+            res["tj_kind"] = "reference"
+            res["ref_data"] = ref_data
+    if "tj" not in res:
+        # the sampled cells get their own columns 0..c1-1 (J of the sample only)
+        mk = np.full(C, -1, np.int32)
+        mk[:c1] = np.arange(c1)
+        t = time.time()
+        port.calc2(mesh, U, kv, wv, data.abmn(), c1, nthreads=nthreads, cell_range=(0, c1),
+                   J=np.zeros((data.size(), c1)), markers=mk)
+        res["tj"] = (time.time() - t) * C / c1
+        res["tj_kind"] = "port"
     with open(out_path, "w") as f:
-        json.dump({"ta": ta, "t_it": t_it, "it_sampled": it, "tj": tj, "c1": c1, "C": C, "nrhs": B.shape[1]}, f)
+        json.dump(res, f)
 
 
 def cpu_baseline_result(path, nthreads, pcg_iters_needed):
@@ -81,13 +105,19 @@ def cpu_baseline_result(path, nthreads, pcg_iters_needed):
         return {"error": "no plain-PCG iteration count to scale the CPU PCG sample with (run without --no-plain)",
                 "partial": r, "cores": nthreads}
     tfwd = r["ta"] + r["t_it"] * pcg_iters_needed
+    if r.get("tj_kind") == "reference":
+        jac = (f"Jacobian: the reference's own createJacobian (calc2, oracle/_ref, {nthreads} threads) on the whole "
+               f"mesh for the first {r['ref_data']} of {r['D']} quadrupoles = {r['tj_sample']:.2f} s, x "
+               f"{r['D'] / r['ref_data']:.0f} = {r['tj']:.1f} s")
+    else:
+        jac = f"Jacobian: C calc2 port on {r['c1']} of {r['C']} cells, scaled = {r['tj']:.1f} s"
     return {
         "value": 1.0 / (tfwd + r["tj"]), "unit": "evals/s", "cores": nthreads, "kind": "port",
         "sample": (f"forward: FP64 assembly {r['ta']:.1f} s + plain Jacobi-PCG C port over {r['nrhs']} sources, "
                    f"{r['it_sampled']} iterations timed ({r['t_it'] * 1e3:.0f} ms each) x {pcg_iters_needed} "
-                   f"iterations (the plain FP64 PCG's count on this workload) = {r['t_it'] * pcg_iters_needed:.0f} s; "
-                   f"Jacobian: C calc2 port on {r['c1']} of {r['C']} cells, scaled = {r['tj']:.1f} s; "
-                   f"{nthreads} threads"),
+                   f"iterations (the plain FP64 PCG's count on this workload) = {r['t_it'] * pcg_iters_needed:.0f} s "
+                   f"(the reference's direct solver cannot run: CHOLMOD is absent); {jac}; {nthreads} threads"),
+        "forward_s": tfwd, "jacobian_s": r["tj"], "jacobian_kind": r.get("tj_kind", "port"),
     }
 
 
@@ -614,7 +644,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--method", default="mixed")
-    ap.add_argument("--cpu-timeout", type=float, default=240.0)
+    ap.add_argument("--cpu-timeout", type=float, default=300.0)
     ap.add_argument("--coarse", type=int, default=1024, help="aggregates of the two-level preconditioner (0: off)")
     ap.add_argument("--learned-fields", type=int, default=8,
                     help="random fields the learned warm start is trained on before timing (0: off)")
@@ -810,7 +840,8 @@ def main():
         import multiprocessing as mp
         os.makedirs(os.path.dirname(cpu_path), exist_ok=True)
         ctx = mp.get_context("spawn")  # before anything touches the GPU
-        cpu_proc = ctx.Process(target=cpu_baseline_worker, args=(cpu_path, nth, 0.02, 30), daemon=True)
+        cpu_proc = ctx.Process(target=cpu_baseline_worker, args=(cpu_path, nth, 0.02, 30, "ert3d_grid_grf", 50),
+                               daemon=True)
         cpu_proc.start()
         log(f"[bench] cpu baseline started in pid {cpu_proc.pid} ({nth} threads)")
 
@@ -839,7 +870,7 @@ def main():
     L, h = fop._ctx.L, fop._ctx.h
     n, nb = mesh.nodeCount(), 1 * ((64 + 63) // 64)
 
-    def run(method, steps, warmup, coarse=0, learned=False, timed_kernel=1):
+    def run(method, steps, warmup, coarse=0, learned=False, timed_kernel=0):
         fop.solverOptions.update(method=method, rtol=args.rtol, coarse=coarse, learned=learned)
         for _ in range(warmup):
             fop.response_device(model)
@@ -865,10 +896,12 @@ def main():
         log(f"[bench] {method} coarse={coarse} learned={learned}: {res}")
         return res
 
-    plain = plain_block = None
+    plain = plain_block = block_coarse = None
     if not args.no_plain:
         plain = run("pcg64", 1, 0)
         plain_block = run("block64", 1, 0, coarse=0)
+        if args.coarse and args.method != "block64":
+            block_coarse = run("block64", 1, 0, coarse=args.coarse)
 
     # learned warm start: trained on the method's own solves of random
     # log-normal fields (outside the timed region)
@@ -889,8 +922,10 @@ def main():
                         "train_s": time.time() - t, "loss_first": losses[0], "loss_last": losses[-1]}
         log(f"[bench] learned warm start trained: {learned_info}")
 
+    # the timed steps run the product path: no per-kernel events, so the block
+    # solver replays its captured iteration graphs
     best = run(args.method, args.steps, args.warmup, args.coarse if args.method != "pcg64" else 0,
-               learned=use_learned, timed_kernel=3 if args.method != "pcg64" else 1)
+               learned=use_learned, timed_kernel=0)
     ablation = run(args.method, 1, 0, args.coarse, learned=False) if use_learned else None
 
     # per-kernel time per launch (one solve each) and the roofline of the
@@ -900,15 +935,26 @@ def main():
     vec = float(nb) * n * 64 * vt
     kernels = {1: ("k_spmm", None, None), 2: ("k_update", 6 * vec, 2 * 2 * 64 * 64 * n * nb),
                3: ("k_gram<Z>", 3 * vec, 2 * 3 * 64 * 64 * n * nb), 4: ("k_pupdate", 3 * vec, 2 * 2 * 64 * 64 * n * nb),
-               5: ("k_gram<PQR>", 3 * vec, 2 * 2 * 64 * 64 * n * nb)}
+               5: ("k_gram<PQR>", 3 * vec, 2 * 2 * 64 * 64 * n * nb),
+               6: ("k_small_alpha", None, None), 7: ("k_small_beta", None, None),
+               8: ("k_zform", 2 * vec + 12 * n * nb, None), 9: ("k_restrict", vec, None),
+               10: ("k_coarse_apply", None, None), 11: ("k_part_reduce", None, None)}
     mfma_peak = 157.3e12 if args.method == "mixed" else 78.6e12
+    # per-kernel breakdown: one instrumented evaluation per kernel (HIP events
+    # around every launch of that kernel on the solver stream; eager launches),
+    # separate from the timed steps
     breakdown = {}
     if args.method != "pcg64":
         for kid, (name, _, _) in kernels.items():
-            r = best if kid == 3 else run(args.method, 1, 0, args.coarse, learned=use_learned, timed_kernel=kid)
+            if kid in (9, 10) and not args.coarse:
+                continue
+            r = run(args.method, 1, 0, args.coarse, learned=use_learned, timed_kernel=kid)
             breakdown[name] = {"us_per_launch": r["k_ms"] * 1e3, "launches": r["k_launches"], "bytes": r["k_bytes"],
-                               "ms_per_step": r["k_ms"] * r["k_launches"] / (args.steps if kid == 3 else 1)}
-    dom = max(breakdown, key=lambda k: breakdown[k]["ms_per_step"]) if breakdown else None
+                               "ms_per_step": r["k_ms"] * r["k_launches"]}
+    # the dominant kernel among those with an algorithmic byte / flop count
+    costed = [k for k in breakdown if kernels[[i for i, v in kernels.items() if v[0] == k][0]][1] is not None
+              or k == "k_spmm"]
+    dom = max(costed, key=lambda k: breakdown[k]["ms_per_step"]) if costed else None
 
     def roof(name, us, nbytes, flops):
         t_h = nbytes / (HBM_PEAK_GBS * 1e9)
@@ -967,6 +1013,11 @@ def main():
     if learned_info:
         learned_info.update(iterations_without=ablation["iters"], ms_without=ablation["ms"])
         out["learned"] = learned_info
+    if block_coarse is not None:
+        out["fp64_block_pcg_coarse"] = {"value": 1e3 / block_coarse["ms"], "ms_per_step": block_coarse["ms"],
+                                        "iterations": block_coarse["iters"],
+                                        "potential_rel_err_fro": block_coarse["pot_err"]}
+        out["speedup_vs_fp64_block_pcg_coarse"] = block_coarse["ms"] / best["ms"]
     if plain is not None:
         for key, r in (("plain_fp64_pcg", plain), ("plain_fp64_block_pcg", plain_block)):
             out[key] = {"value": 1e3 / r["ms"], "ms_per_step": r["ms"], "iterations": r["iters"],

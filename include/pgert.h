@@ -286,6 +286,12 @@ int pgert_mlp_train_step(pgert_ctx * ctx, const float * feat, const float * targ
  *   PGERT_TIME_GRAM_Z  3  Z = M^-1 R and the Grams Q'Z, Z'Z, P'Z (block methods)
  *   PGERT_TIME_PUPDATE 4  P = Z T + P beta T (block methods)
  *   PGERT_TIME_GRAM_PQ 5  the Grams P'Q, P'R (block methods)
+ *   PGERT_TIME_SMALL_ALPHA 6   alpha = (P'Q)^-1 P'R, 64 x 64 (block methods)
+ *   PGERT_TIME_SMALL_BETA  7   beta, chol(W'W)^-T, 64 x 64 (block methods)
+ *   PGERT_TIME_ZFORM       8   Z = D^-1 R + coarse prolongation (block methods)
+ *   PGERT_TIME_RESTRICT    9   aggregate sums of R (block methods, coarse on)
+ *   PGERT_TIME_COARSE     10   E^-1 y (block methods, coarse on)
+ *   PGERT_TIME_REDUCE     11   fixed-order sum of the partial Grams (block methods)
  * reset: zero the accumulators.  Returns the accumulated milliseconds, number
  * of timed launches and the compulsory HBM bytes of one launch (matrix values
  * and indices once, every vector the kernel reads or writes once, all blocks
@@ -295,6 +301,12 @@ int pgert_mlp_train_step(pgert_ctx * ctx, const float * feat, const float * targ
 #define PGERT_TIME_GRAM_Z 3
 #define PGERT_TIME_PUPDATE 4
 #define PGERT_TIME_GRAM_PQ 5
+#define PGERT_TIME_SMALL_ALPHA 6
+#define PGERT_TIME_SMALL_BETA 7
+#define PGERT_TIME_ZFORM 8
+#define PGERT_TIME_RESTRICT 9
+#define PGERT_TIME_COARSE 10
+#define PGERT_TIME_REDUCE 11
 int pgert_kernel_timing(pgert_ctx * ctx, int enable, int reset, double * total_ms, long * launches,
                         double * bytes_per_launch);
 

@@ -52,6 +52,7 @@ def lib():
         L.ref_session_subpots.argtypes = [ctypes.c_void_p, _D]
         L.ref_session_data_k.argtypes = [ctypes.c_void_p, _D]
         L.ref_session_jacobian.argtypes = [ctypes.c_void_p, _D, ctypes.c_int, _D, _I, _I]
+        L.ref_session_set_subpots.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D]
         L.ref_assemble.argtypes = [ctypes.c_void_p, _D, ctypes.c_double, _D, _I, _I, _I, _D]
         L.ref_fill_matrix.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _I, _I, _I, _D]
         L.ref_mixed_bc.argtypes = [ctypes.c_void_p, _D, ctypes.c_double, _D]
@@ -225,6 +226,19 @@ class Session:
         out = np.empty((r.value, c.value))
         lib().ref_session_subpots(self.h, out.ctypes.data_as(_D))
         return out
+
+    def set_subpotentials(self, U):
+        """Hand the core per-k potentials [nK * nE, n] (rows source + nE * kIdx)
+        for createJacobian, as a previous response() would have left them."""
+        U = np.ascontiguousarray(U, np.float64)
+        _check(lib().ref_session_set_subpots(self.h, U.shape[0], U.shape[1], U.ctypes.data_as(_D)))
+
+    def jacobian_only(self, model):
+        """createJacobian(model) without copying J out; returns its shape."""
+        model, pm = _d(model)
+        r = ctypes.c_int(); c = ctypes.c_int()
+        _check(lib().ref_session_jacobian(self.h, pm, len(model), None, ctypes.byref(r), ctypes.byref(c)))
+        return r.value, c.value
 
     def jacobian(self, model):
         model, pm = _d(model)

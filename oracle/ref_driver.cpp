@@ -446,6 +446,20 @@ void ref_session_subpots(void * h, double * out) {
         for (Index j = 0; j < c; ++j) out[i * c + j] = s->subpots[i][j];
 }
 
+// Potentials for createJacobian without a response(): the per-k unweighted
+// matrix [rows = source + nE*kIdx][nodes] handed to the reference core
+// (DCMultiElectrodeModelling::collectSubPotentials, dcfemmodelling.h:220);
+// prepareJacobianT_ (dcfemmodelling.cpp:1245-1309) then uses it as is.
+int ref_session_set_subpots(void * h, int rows, int cols, const double * U) {
+    Session * s = (Session *)h;
+    return guarded([&] {
+        s->subpots = RMatrix(rows, cols);
+        for (int i = 0; i < rows; ++i)
+            for (int j = 0; j < cols; ++j) s->subpots[i][j] = U[(size_t)i * cols + j];
+        s->fop->collectSubPotentials(s->subpots);
+    });
+}
+
 // Data-container k after response() (the reference fills missing k analytically).
 void ref_session_data_k(void * h, double * out) {
     Session * s = (Session *)h;

@@ -1021,7 +1021,7 @@ int pgert_kernel_timing(pgert_ctx * c, int enable, int reset, double * total_ms,
             c->kTotalMs = 0.0;
             c->kLaunches = 0;
         }
-        PG_REQUIRE(enable >= 0 && enable <= PGERT_TIME_GRAM_PQ, "pgert_kernel_timing: unknown kernel");
+        PG_REQUIRE(enable >= 0 && enable <= PGERT_TIME_REDUCE, "pgert_kernel_timing: unknown kernel");
         c->kTiming = enable != 0;
         c->kSelect = enable;
         if (total_ms) *total_ms = c->kTotalMs;

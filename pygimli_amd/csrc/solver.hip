@@ -281,9 +281,13 @@ double residual_check(Ctx & c) {
     std::vector<double> h((size_t)nK * nE * 2);
     PG_CHECK(hipMemcpyAsync(h.data(), c.scal.p, h.size() * 8, hipMemcpyDeviceToHost, c.stream));
     PG_CHECK(hipStreamSynchronize(c.stream));
+    // a non-finite residual or right-hand-side norm (NaN from a bad initial
+    // guess, overflow) counts as a failed solve: +inf, never <= rtol
     double worst = 0;
-    for (size_t t = 0; t < (size_t)nK * nE; ++t)
+    for (size_t t = 0; t < (size_t)nK * nE; ++t) {
+        if (!std::isfinite(h[2 * t]) || !std::isfinite(h[2 * t + 1])) return HUGE_VAL;
         if (h[2 * t + 1] > 0) worst = std::max(worst, std::sqrt(h[2 * t] / h[2 * t + 1]));
+    }
     return worst;
 }
 

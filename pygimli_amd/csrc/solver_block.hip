@@ -107,11 +107,28 @@ template <typename VT> __device__ inline int drow(int lane, int r) {
 // memory, which makes every wavefront wait for all of its outstanding loads
 // (vmcnt(0)) and would drain a software-pipelined load ring at each step.
 template <typename VT> __device__ inline V4<VT> ld4(const VT * p) { return *reinterpret_cast<const V4<VT> *>(p); }
+// v, or zero when !keep, by a bit mask (a select of a whole vector would
+// become a branch)
+__device__ inline f32x4 zero_unless(bool keep, f32x4 v) {
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+    const unsigned m = keep ? 0xffffffffu : 0u;
+    return __builtin_bit_cast(f32x4, __builtin_bit_cast(u32x4, v) & m);
+}
+__device__ inline f64x4 zero_unless(bool keep, f64x4 v) {
+    typedef unsigned long long u64x4 __attribute__((ext_vector_type(4)));
+    const unsigned long long m = keep ? ~0ull : 0ull;
+    return __builtin_bit_cast(f64x4, __builtin_bit_cast(u64x4, v) & m);
+}
 template <typename VT> __device__ inline void st4(VT * p, V4<VT> v) { *reinterpret_cast<V4<VT> *>(p) = v; }
 
 // ---------------------------------------------------------------- 1. Q = A P
 // a wavefront takes 4 rows at once (16 lanes per row, 4 columns per lane as
-// one vector load) with 4 nonzeros of each row in flight.
+// one vector load).  The 16 lanes of a row load up to 16 of its column
+// indices and values side by side (coalesced) into the wavefront's LDS
+// slice; every lane then reads them back as broadcasts and issues the 16
+// gathers of P at once, so a row costs three dependent memory latencies
+// (row pointer, indices, gathers) instead of two per 4 nonzeros.  Slots past
+// the row's end read row 0 with weight 0 (no branch around a load).
 // XCD-aware rows: workgroups are dispatched round-robin over the 8 XCDs
 // (workgroup id % 8), so XCD x is given the contiguous rows [x n/8, (x+1) n/8)
 // and the block rows it gathers (neighbours within a few mesh planes) stay in
@@ -125,43 +142,92 @@ __device__ inline void xcd_rows(int n, int & lo, int & hi, int & first, int & st
     first = lo + wi * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
     stride = nw * 4;
 }
+constexpr int SPMM_W = 16;  // nonzeros of a row held in the wavefront's LDS slice per round
 template <typename VT, typename AT>
 __global__ void __launch_bounds__(256) k_spmm(const BlockDesc * __restrict__ blocks, const VT * __restrict__ P,
                                               VT * __restrict__ Q) {
     using V = V4<VT>;
     using VA = V4<AT>;
+    __shared__ int sIdx[4][4][SPMM_W];    // [wave][row of the wave][slot]
+    __shared__ AT sVal[4][4][SPMM_W];
     const int b = blockIdx.y;
     const BlockDesc bd = blocks[b];
     if (!bd.active) return;
     const int n = bd.n;
     const auto rowptr = glob(bd.rowptr);
     const auto colidx = glob(bd.colidx);
-    const int lane = threadIdx.x % 64;
-    const VT * Pb = P + bd.off + 4 * (lane % 16);
-    VT * Qb = Q + bd.off + 4 * (lane % 16);
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, rg = lane / 16, j = lane % 16;
+    const VT * Pb = P + bd.off + 4 * j;
+    VT * Qb = Q + bd.off + 4 * j;
     static_assert(sizeof(AT) == 8, "the matrix stays FP64");
     const auto A = glob(bd.valsD);
     const int x = blockIdx.x % NXCD, nw = gridDim.x / NXCD;
     const int lo = (int)((long)n * x / NXCD), hi = (int)((long)n * (x + 1) / NXCD);
-    const int g0 = (blockIdx.x / NXCD) * 4 + threadIdx.x / 64;
-    for (int r0 = lo + 4 * g0; r0 < hi; r0 += 16 * nw) {
-        const int row = r0 + lane / 16;
-        if (row >= hi) continue;
+    if (lo >= hi) return;
+    const int stride = 16 * nw;
+    int * myIdx = sIdx[wave][rg];
+    AT * myVal = sVal[wave][rg];
+    // Software pipeline over the wavefront's rounds of 4 rows: the row
+    // pointers are loaded two rounds ahead and the indices / values one round
+    // ahead, so they are in flight while this round's gathers are summed.
+    // Loads of rounds or rows past hi read row hi - 1 and keep nothing.
+    auto rp_load = [&](int r, int & p0, int & p1) {
+        const int rc = min(r + rg, hi - 1);
+        p0 = rowptr[rc];
+        p1 = rowptr[rc + 1];
+    };
+    auto ci_load = [&](int r, int p0, int p1, int & cv, AT & av) {
+        const int q = p0 + j;
+        const bool in = r + rg < hi && q < p1;
+        const int qc = in ? q : p0;  // p0: a valid slot of a non-empty row
+        const int c = colidx[qc];
+        const AT v = A[qc];
+        cv = in ? c : 0;
+        av = in ? v : (AT)0;
+    };
+    int r0 = lo + 4 * ((blockIdx.x / NXCD) * 4 + wave);
+    int p0c, p1c, p0n, p1n, cvc;
+    AT avc;
+    rp_load(r0, p0c, p1c);
+    ci_load(r0, p0c, p1c, cvc, avc);
+    rp_load(r0 + stride, p0n, p1n);
+    for (; r0 < hi; r0 += stride) {
+        const int row = r0 + rg;
+        myIdx[j] = cvc;
+        myVal[j] = avc;
+        __builtin_amdgcn_wave_barrier();
+        int ci[SPMM_W];
+#pragma unroll
+        for (int u = 0; u < SPMM_W; ++u) ci[u] = myIdx[u];
+        V xv[SPMM_W];
+#pragma unroll
+        for (int u = 0; u < SPMM_W; ++u) xv[u] = ld4(Pb + (size_t)ci[u] * S);
+        // next round's indices and values, the row pointers of the one after
+        int cvn, p0nn, p1nn;
+        AT avn;
+        ci_load(r0 + stride, p0n, p1n, cvn, avn);
+        rp_load(r0 + 2 * stride, p0nn, p1nn);
+        // every gather and prefetch is issued before the first product waits
+        __builtin_amdgcn_sched_barrier(0);
         VA acc0 = VA{0, 0, 0, 0}, acc1 = acc0;
-        int q = rowptr[row];
-        const int q1 = rowptr[row + 1];
-        for (; q + 4 <= q1; q += 4) {
-            const int c0 = colidx[q], c1 = colidx[q + 1], c2 = colidx[q + 2], c3 = colidx[q + 3];
-            const AT a0 = A[q], a1 = A[q + 1], a2 = A[q + 2], a3 = A[q + 3];
-            const VA x0 = __builtin_convertvector(ld4(Pb + (size_t)c0 * S), VA);
-            const VA x1 = __builtin_convertvector(ld4(Pb + (size_t)c1 * S), VA);
-            const VA x2 = __builtin_convertvector(ld4(Pb + (size_t)c2 * S), VA);
-            const VA x3 = __builtin_convertvector(ld4(Pb + (size_t)c3 * S), VA);
-            acc0 += a0 * x0 + a1 * x1;
-            acc1 += a2 * x2 + a3 * x3;
+#pragma unroll
+        for (int u = 0; u < SPMM_W; u += 2) {
+            acc0 += myVal[u] * __builtin_convertvector(xv[u], VA);
+            acc1 += myVal[u + 1] * __builtin_convertvector(xv[u + 1], VA);
         }
-        for (; q < q1; ++q) acc0 += A[q] * __builtin_convertvector(ld4(Pb + (size_t)colidx[q] * S), VA);
-        st4(Qb + (size_t)row * S, __builtin_convertvector(acc0 + acc1, V));
+        // nonzeros past the first SPMM_W of a row (none in the structured
+        // tetrahedral meshes; a plain loop for general meshes)
+        const int q0 = row < hi ? p0c : 0, q1 = row < hi ? p1c : 0;
+        for (int q = q0 + SPMM_W; q < q1; ++q)
+            acc0 += A[q] * __builtin_convertvector(ld4(Pb + (size_t)colidx[q] * S), VA);
+        if (row < hi) st4(Qb + (size_t)row * S, __builtin_convertvector(acc0 + acc1, V));
+        __builtin_amdgcn_wave_barrier();
+        p0c = p0n;
+        p1c = p1n;
+        p0n = p0nn;
+        p1n = p1nn;
+        cvc = cvn;
+        avc = avn;
     }
 }
 
@@ -325,13 +391,17 @@ __global__ void __launch_bounds__(256) k_zform(const BlockDesc * __restrict__ bl
 // the same layout as the loads (one vector store), in FP64 columns strided
 // by 4 (four scalar accesses).
 // Pre-swizzled LDS operand: op[((4 u + e) * 4 + tb) * 64 + lane].
+// Needs blockDim.x == 256: thread t stages entries t + 256 i, i < 16, and
+// issues its 16 loads before the first LDS store (a loop with a store per
+// load would wait for each load in turn).
 template <typename VT>
 __device__ inline void stage_op(VT * op, const double * M) {
-    for (int t = threadIdx.x; t < 16 * 4 * 64; t += blockDim.x) {
-        const int step = t / 256, tb = (t / 64) % 4, ln = t % 64;
-        const int u = step / 4, e = step % 4;
-        op[t] = (VT)M[(16 * u + 4 * (ln / 16) + e) * S + 16 * tb + ln % 16];
-    }
+    const int tb = threadIdx.x / 64, ln = threadIdx.x % 64;
+    double v[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = M[(16 * (i / 4) + 4 * (ln / 16) + i % 4) * S + 16 * tb + ln % 16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) op[i * 256 + threadIdx.x] = (VT)v[i];
 }
 
 template <typename VT>
@@ -527,44 +597,78 @@ __global__ void __launch_bounds__(256) k_restrict(const BlockDesc * __restrict__
     double s = 0.0;
     const auto aggPtr = glob(bd.aggPtr);
     const auto aggNodes = glob(bd.aggNodes);
-    for (int q = aggPtr[I] + wave; q < aggPtr[I + 1]; q += 4) s += (double)Rb[(size_t)aggNodes[q] * S];
+    const int q0 = aggPtr[I], q1 = aggPtr[I + 1];
+    // wavefront w takes the nodes q0 + 64 (4 c + w) + l: lane l loads one node
+    // index, then the wavefront's rows are loaded 16 at a time (indices
+    // broadcast from their lanes), all 16 loads in flight
+    for (int c0 = q0 + 64 * wave; c0 < q1; c0 += 256) {
+        const int cnt = min(64, q1 - c0);
+        const int myNode = lane < cnt ? aggNodes[c0 + lane] : 0;
+        for (int j0 = 0; j0 < cnt; j0 += 16) {
+            // lanes past cnt hold node 0: their loads stay in bounds and are
+            // not summed (no branch around a load, so all 16 stay in flight)
+            VT v[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) v[u] = Rb[(size_t)__builtin_amdgcn_readlane(myNode, j0 + u) * S];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) s += j0 + u < cnt ? (double)v[u] : 0.0;
+        }
+    }
     cn[wave][lane] = s;
     __syncthreads();
     if (wave == 0) y[bd.coff + (size_t)I * S + lane] = cn[0][lane] + cn[1][lane] + cn[2][lane] + cn[3][lane];
 }
 
 // cc[b] = Einv_k y[b] (FP64 MFMA): a workgroup computes one 16 x 16 tile
-// (rows 16 blockIdx.x, columns 16 * tile), its 4 wavefronts split the m-long
-// contraction and are summed through LDS
-__global__ void __launch_bounds__(256) k_coarse_apply(const BlockDesc * __restrict__ blocks,
-                                                      const double * __restrict__ y, double * __restrict__ cc) {
-    __shared__ double red[4][64][4];
+// (rows 16 blockIdx.x, columns 16 * tile); its 16 wavefronts split the
+// m-long contraction (each issues all loads of its share, up to 16 k-steps,
+// before its MFMAs) and are summed through LDS in a fixed order
+constexpr int CA_WAVES = 16;
+__global__ void __launch_bounds__(64 * CA_WAVES) k_coarse_apply(const BlockDesc * __restrict__ blocks,
+                                                                const double * __restrict__ y,
+                                                                double * __restrict__ cc) {
+    __shared__ double red[CA_WAVES][64][4];
     const int b = blockIdx.z;
     const BlockDesc bd = blocks[b];
     const int m = bd.m;
     const int I0 = 16 * blockIdx.x, c0 = 16 * blockIdx.y;
     if (!bd.active || I0 >= m) return;
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    const double * E = bd.Einv;
+    const auto E = glob(bd.Einv);
     const double * yb = y + bd.coff;
     const int i = I0 + lane % 16;
     f64x4 acc = {0, 0, 0, 0};
     const int nk = (m + 3) / 4;
-    const int k0 = nk * wave / 4, k1 = nk * (wave + 1) / 4;
-    for (int ks = k0; ks < k1; ++ks) {
+    const int k0 = nk * wave / CA_WAVES, k1 = nk * (wave + 1) / CA_WAVES;
+    auto a_at = [&](int ks) {
         const int J = 4 * ks + lane / 16;
-        const double a = (i < m && J < m) ? E[(size_t)i * m + J] : 0.0;
-        const double bv = J < m ? yb[(size_t)J * S + c0 + lane % 16] : 0.0;
-        acc = mfma4(a, bv, acc);
+        return (i < m && J < m) ? E[(size_t)i * m + J] : 0.0;
+    };
+    auto b_at = [&](int ks) {
+        const int J = 4 * ks + lane / 16;
+        return J < m ? yb[(size_t)J * S + c0 + lane % 16] : 0.0;
+    };
+    int ks = k0;
+    for (; ks + 16 <= k1; ks += 16) {
+        double a[16], bv[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            a[u] = a_at(ks + u);
+            bv[u] = b_at(ks + u);
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) acc = mfma4(a[u], bv[u], acc);
     }
+    for (; ks < k1; ++ks) acc = mfma4(a_at(ks), b_at(ks), acc);
     for (int jj = 0; jj < 4; ++jj) red[wave][lane][jj] = acc[jj];
     __syncthreads();
     if (wave == 0)
         for (int jj = 0; jj < 4; ++jj) {
             const int row = I0 + drow<double>(lane, jj);
-            if (row < m)
-                cc[bd.coff + (size_t)row * S + c0 + lane % 16] =
-                    red[0][lane][jj] + red[1][lane][jj] + red[2][lane][jj] + red[3][lane][jj];
+            double v = 0.0;
+#pragma unroll
+            for (int w = 0; w < CA_WAVES; ++w) v += red[w][lane][jj];
+            if (row < m) cc[bd.coff + (size_t)row * S + c0 + lane % 16] = v;
         }
 }
 
@@ -583,7 +687,19 @@ __global__ void __launch_bounds__(256) k_part_reduce(const BlockDesc * __restric
     if (t < count) {
         const int g0 = G * gc / 8, g1 = G * (gc + 1) / 8;
         const double * p = part + (size_t)b * G * count + t;
-        for (int g = g0; g < g1; ++g) s += p[(size_t)g * count];
+        int g = g0;
+        // 16 loads in flight, summed in a fixed order
+        for (; g + 16 <= g1; g += 16) {
+            double v[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) v[u] = p[(size_t)(g + u) * count];
+#pragma unroll
+            for (int w = 8; w > 0; w >>= 1)
+#pragma unroll
+                for (int u = 0; u < w; ++u) v[u] += v[u + w];
+            s += v[0];
+        }
+        for (; g < g1; ++g) s += p[(size_t)g * count];
     }
     acc[gc][o] = s;
     __syncthreads();
@@ -619,6 +735,15 @@ __device__ void mm64(const double * A, const double * B, double * C, bool transA
     __syncthreads();
 }
 
+// v[q] of four registers (static indices: value selects, no branches)
+__device__ inline double sel4(int q, double a, double b, double c, double d) {
+    double r = a;
+    r = q == 1 ? b : r;
+    r = q == 2 ? c : r;
+    r = q == 3 ? d : r;
+    return r;
+}
+
 // Li = L^-1 for G = L L' (lower Cholesky factor) with dependent directions
 // dropped: forward elimination of [G | I] whose right half becomes the
 // inverse unit-lower factor, rows then scaled by the pivots^-1/2.  Each thread
@@ -638,46 +763,61 @@ __device__ void inv_chol(const double * G, double * Li, int s) {
             li[i][c] = (i0 + i == c0 + c && i0 + i < s) ? 1.0 : 0.0;
         }
     if (threadIdx.x < S) d0[threadIdx.x] = threadIdx.x < s ? G[threadIdx.x * LD + threadIdx.x] : 0.0;
+    __syncthreads();
+    // Each step is one chain: publish, LDS-only barrier, one batch of
+    // unconditional LDS reads, selects and FMAs.  Register indices stay
+    // static (a lane-dependent index into a register array becomes a loop
+    // over the distinct lane values); the conditionals are value selects, so
+    // the reads of a step are not serialised behind branches.
     for (int j = 0; j < s; ++j) {
         const int buf = j & 1;
-        // static register indices only (a lane-dependent index into a
-        // register array becomes a loop over the distinct lane values)
+        const int qc = j - c0, qr = j - i0;
+        double pc[4], pr[4], pl[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            if (c0 + q == j)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) colb[buf][i0 + i] = g[i][q];
-            if (i0 + q == j)
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    rowb[buf][c0 + c] = g[q][c];
-                    lrow[buf][c0 + c] = li[q][c];
-                }
+        for (int i = 0; i < 4; ++i) {
+            pc[i] = sel4(qc, g[i][0], g[i][1], g[i][2], g[i][3]);
+            pr[i] = sel4(qr, g[0][i], g[1][i], g[2][i], g[3][i]);
+            pl[i] = sel4(qr, li[0][i], li[1][i], li[2][i], li[3][i]);
         }
-        __syncthreads();
-        const double d = rowb[buf][j];
-        const bool keep = d0[j] > 0.0 && d > 1e-12 * d0[j];
+        if ((unsigned)qc < 4u)
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-            if (i0 + i == j) piv[i] = keep ? d : 0.0;
-        if (keep) {
-            const double inv = 1.0 / d;
-            double f[4], rb[4], lr[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) f[i] = i0 + i > j ? colb[buf][i0 + i] * inv : 0.0;
+            for (int i = 0; i < 4; ++i) colb[buf][i0 + i] = pc[i];
+        if ((unsigned)qr < 4u)
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
-                rb[c] = c0 + c > j ? rowb[buf][c0 + c] : 0.0;
-                lr[c] = c0 + c <= j ? lrow[buf][c0 + c] : 0.0;
+                rowb[buf][c0 + c] = pr[c];
+                lrow[buf][c0 + c] = pl[c];
             }
+        lds_barrier();
+        const double d = rowb[buf][j], dj = d0[j];
+        double cb[4], rb[4], lr[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < 4; ++i) cb[i] = colb[buf][i0 + i];
 #pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    g[i][c] -= f[i] * rb[c];
-                    li[i][c] -= f[i] * lr[c];
-                }
+        for (int c = 0; c < 4; ++c) {
+            rb[c] = rowb[buf][c0 + c];
+            lr[c] = lrow[buf][c0 + c];
         }
+        const bool keep = dj > 0.0 && d > 1e-12 * dj;
+        const double rcp = 1.0 / d;
+        const double inv = keep ? rcp : 0.0;  // a dropped pivot eliminates nothing
+#pragma unroll
+        for (int i = 0; i < 4; ++i) piv[i] = i0 + i == j ? (keep ? d : 0.0) : piv[i];
+        double f[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) f[i] = i0 + i > j ? cb[i] * inv : 0.0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            rb[c] = c0 + c > j ? rb[c] : 0.0;
+            lr[c] = c0 + c <= j ? lr[c] : 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                g[i][c] -= f[i] * rb[c];
+                li[i][c] -= f[i] * lr[c];
+            }
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -688,12 +828,23 @@ __device__ void inv_chol(const double * G, double * Li, int s) {
     __syncthreads();
 }
 
-// global -> LDS copy of one s x s matrix (zero padded); no barrier: callers
-// issue a group of loads, then synchronise once
-__device__ void load_small(double * M, const double * src, int s) {
-    for (int t = threadIdx.x; t < SS; t += blockDim.x) {
-        const int i = t / S, j = t % S;
-        M[i * LD + j] = (i < s && j < s) ? src[t] : 0.0;
+// global -> LDS copy of s x s matrices (zero padded), split into a fetch of
+// N matrices into registers and the LDS stores, so that every load of a
+// thread is in flight before its first store (a loop with a store per load
+// waits for each load in turn).  Needs blockDim.x == 256; no barrier:
+// callers synchronise once.
+template <int N>
+__device__ inline void fetch_small(double (&v)[N][16], const double * const (&src)[N]) {
+#pragma unroll
+    for (int q = 0; q < N; ++q)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) v[q][r] = src[q][threadIdx.x + 256 * r];
+}
+__device__ inline void put_small(double * M, const double (&v)[16], int s) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int t = threadIdx.x + 256 * r, i = t / S, j = t % S;
+        M[i * LD + j] = (i < s && j < s) ? v[r] : 0.0;
     }
 }
 
@@ -709,8 +860,10 @@ __global__ void __launch_bounds__(256) k_small_alpha(const BlockDesc * __restric
     const int s = blocks[blockIdx.x].s;
     double * sm = small + (size_t)blockIdx.x * NSL * SS;
     double *M0 = lds, *M1 = lds + SMAT, *M2 = lds + 2 * SMAT, *M3 = lds + 3 * SMAT;
-    load_small(M0, sm + SL_G1 * SS, s);
-    load_small(M3, sm + SL_G2 * SS, s);
+    double v[2][16];
+    fetch_small<2>(v, {sm + SL_G1 * SS, sm + SL_G2 * SS});
+    put_small(M0, v[0], s);
+    put_small(M3, v[1], s);
     __syncthreads();
     inv_chol(M0, M1, s);
     mm64(M1, M1, M2, true, 1.0);   // G1^-1 = Li' Li
@@ -727,13 +880,16 @@ __global__ void __launch_bounds__(256) k_small_beta(const BlockDesc * __restrict
     const int s = blocks[blockIdx.x].s;
     double * sm = small + (size_t)blockIdx.x * NSL * SS;
     double *M0 = lds, *M1 = lds + SMAT, *Bt = lds + 2 * SMAT, *W = lds + 3 * SMAT;
-    load_small(W, sm + SL_ZZ * SS, s);
     if (!init) {
-        load_small(M0, sm + SL_G1 * SS, s);  // G1^-1
-        load_small(M1, sm + SL_QZ * SS, s);  // Q'Z
+        // W = Z'Z, M0 = G1^-1, M1 = Q'Z; P'Z waits in registers for M0
+        double v[4][16];
+        fetch_small<4>(v, {sm + SL_ZZ * SS, sm + SL_G1 * SS, sm + SL_QZ * SS, sm + SL_PZ * SS});
+        put_small(W, v[0], s);
+        put_small(M0, v[1], s);
+        put_small(M1, v[2], s);
         __syncthreads();
         mm64(M0, M1, Bt, false, -1.0);       // beta = -G1^-1 Q'Z
-        load_small(M0, sm + SL_PZ * SS, s);  // P'Z
+        put_small(M0, v[3], s);              // P'Z
         __syncthreads();
         mm64(M0, Bt, M1, true, 1.0);         // (P'Z)' beta
         for (int t = threadIdx.x; t < SS; t += blockDim.x) {
@@ -745,6 +901,9 @@ __global__ void __launch_bounds__(256) k_small_beta(const BlockDesc * __restrict
         for (int t = threadIdx.x; t < SS; t += blockDim.x) W[(t / S) * LD + t % S] += M1[(t / S) * LD + t % S];
         __syncthreads();
     } else {
+        double v[1][16];
+        fetch_small<1>(v, {sm + SL_ZZ * SS});
+        put_small(W, v[0], s);
         for (int t = threadIdx.x; t < SS; t += blockDim.x) Bt[(t / S) * LD + t % S] = 0.0;
         __syncthreads();
     }
@@ -803,7 +962,15 @@ __global__ void k_norm_reduce(int G, const double * __restrict__ part, double * 
     const int b = blockIdx.x;
     const int t = threadIdx.x;  // 0 .. 2S-1
     double s = 0.0;
-    for (int g = 0; g < G; ++g) s += part[((size_t)b * G + g) * 2 * S + t];
+    int g = 0;
+    for (; g + 16 <= G; g += 16) {
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = part[((size_t)b * G + g + u) * 2 * S + t];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) s += v[u];
+    }
+    for (; g < G; ++g) s += part[((size_t)b * G + g) * 2 * S + t];
     out[(size_t)b * 2 * S + t] = s;
 }
 
@@ -998,11 +1165,15 @@ void solve_blocks(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std
         PG_CHECK(hipGetLastError());
         PG_CHECK(hipStreamSynchronize(s));  // the host scale vectors are reused
     };
-    // worst relative residual of block b from the norms just read
+    // worst relative residual of block b from the norms just read; +inf when
+    // a norm is not finite (NaN from a bad initial guess, overflow): such a
+    // block is given up (deactivated unconverged, the final residual check
+    // reports the failure)
     auto block_rel = [&](int b) {
         double w = 0.0;
         for (int j = 0; j < pl.h[b].s; ++j) {
             const double rr = hn[(size_t)b * 2 * S + j], bb = hn[(size_t)b * 2 * S + S + j];
+            if (!std::isfinite(rr) || !std::isfinite(bb)) return HUGE_VAL;
             if (bb > 0) w = std::max(w, std::sqrt(rr / bb));
         }
         return w;
@@ -1027,6 +1198,8 @@ void solve_blocks(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std
             case PGERT_TIME_GRAM_Z: c0.kBytesPerLaunch = 3.0 * vec; break;
             case PGERT_TIME_PUPDATE: c0.kBytesPerLaunch = 3.0 * vec; break;
             case PGERT_TIME_GRAM_PQ: c0.kBytesPerLaunch = 3.0 * vec; break;
+            case PGERT_TIME_ZFORM: c0.kBytesPerLaunch = 2.0 * vec + 12.0 * (double)nmax; break;
+            case PGERT_TIME_RESTRICT: c0.kBytesPerLaunch = vec; break;
             default: break;
             }
         }
@@ -1055,24 +1228,33 @@ void solve_blocks(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std
         double lastRel = 0.0;
         for (int b = 0; b < nb; ++b) {
             const double r = block_rel(b);
-            if (r <= o.rtol) pl.h[b].active = 0;
-            lastRel = std::max(lastRel, r);
+            if (r <= o.rtol || r == HUGE_VAL) pl.h[b].active = 0;
+            else lastRel = std::max(lastRel, r);
         }
         PG_CHECK(hipMemcpyAsync(pl.d.p, pl.h.data(), pl.h.size() * sizeof(BlockDesc), hipMemcpyHostToDevice, s));
         int lastReplace = 0;
         const dim3 gG(G, nb), gU(GU, nb), gS(GS, nb), gZ(GZ, nb);
         auto z_grams = [&]() {
             if (anyCoarse) {
+                tb(PGERT_TIME_RESTRICT);
                 hipLaunchKernelGGL(k_restrict<VT>, dim3(mmax, nb), blk, 0, s, pl.d.p, Rw.p, c0.ycoarse.p);
-                hipLaunchKernelGGL(k_coarse_apply, dim3(div_up(mmax, 16), 4, nb), blk, 0, s, pl.d.p, c0.ycoarse.p,
+                te(PGERT_TIME_RESTRICT);
+                tb(PGERT_TIME_COARSE);
+                hipLaunchKernelGGL(k_coarse_apply, dim3(div_up(mmax, 16), 4, nb), dim3(64 * CA_WAVES), 0, s, pl.d.p,
+                                   c0.ycoarse.p,
                                    c0.ccoarse.p);
+                te(PGERT_TIME_COARSE);
             }
+            tb(PGERT_TIME_ZFORM);
             hipLaunchKernelGGL(k_zform<VT>, gZ, blk, 0, s, pl.d.p, Rw.p, ccp, Zw.p);
+            te(PGERT_TIME_ZFORM);
             tb(PGERT_TIME_GRAM_Z);
             hipLaunchKernelGGL((k_gram<VT, MODE_Z>), gG, blk, 0, s, pl.d.p, Pw.p, Qw.p, Zw.p, c0.part.p);
             te(PGERT_TIME_GRAM_Z);
+            tb(PGERT_TIME_REDUCE);
             hipLaunchKernelGGL(k_part_reduce, dim3(3 * SS / 32, nb), blk, 0, s, pl.d.p, G, 3 * SS, c0.part.p,
                                c0.small.p, SL_QZ * SS);
+            te(PGERT_TIME_REDUCE);
         };
         // one iteration; rr: reduce the recursive residual norms (last of a chunk)
         auto iteration = [&](bool rr) {
@@ -1082,9 +1264,13 @@ void solve_blocks(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std
             tb(PGERT_TIME_GRAM_PQ);
             hipLaunchKernelGGL((k_gram<VT, MODE_PQR>), gG, blk, 0, s, pl.d.p, Pw.p, Qw.p, Rw.p, c0.part.p);
             te(PGERT_TIME_GRAM_PQ);
+            tb(PGERT_TIME_REDUCE);
             hipLaunchKernelGGL(k_part_reduce, dim3(2 * SS / 32, nb), blk, 0, s, pl.d.p, G, 2 * SS, c0.part.p,
                                c0.small.p, SL_G1 * SS);
+            te(PGERT_TIME_REDUCE);
+            tb(PGERT_TIME_SMALL_ALPHA);
             hipLaunchKernelGGL(k_small_alpha, dim3(nb), blk, SMALL_LDS, s, pl.d.p, c0.small.p);
+            te(PGERT_TIME_SMALL_ALPHA);
             tb(PGERT_TIME_UPDATE);
             hipLaunchKernelGGL(k_update<VT>, gU, blk, 0, s, pl.d.p, c0.small.p, Pw.p, Qw.p, Xp, Rw.p, c0.part.p);
             te(PGERT_TIME_UPDATE);
@@ -1092,7 +1278,9 @@ void solve_blocks(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std
                 hipLaunchKernelGGL(k_part_reduce, dim3(S / 32, nb), blk, 0, s, pl.d.p, GU, S, c0.part.p, c0.small.p,
                                    SL_RR * SS);
             z_grams();
+            tb(PGERT_TIME_SMALL_BETA);
             hipLaunchKernelGGL(k_small_beta, dim3(nb), blk, SMALL_LDS, s, pl.d.p, c0.small.p, 0);
+            te(PGERT_TIME_SMALL_BETA);
             tb(PGERT_TIME_PUPDATE);
             hipLaunchKernelGGL(k_pupdate<VT>, gU, blk, 0, s, pl.d.p, c0.small.p, Zw.p, Pw.p);
             te(PGERT_TIME_PUPDATE);
@@ -1145,8 +1333,9 @@ void solve_blocks(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std
                 if (!pl.h[b].active) continue;
                 double wb = 0.0;
                 for (int j = 0; j < pl.h[b].s; ++j) {
-                    const double bb = bnorm2[b * S + j];
-                    if (bb > 0) wb = std::max(wb, std::sqrt(hs[(size_t)b * S + j] / bb) * hscale[b * S + j]);
+                    const double bb = bnorm2[b * S + j], rr = hs[(size_t)b * S + j];
+                    if (!std::isfinite(rr)) wb = -1.0;  // check the true residual now
+                    if (bb > 0 && wb >= 0.0) wb = std::max(wb, std::sqrt(rr / bb) * hscale[b * S + j]);
                 }
                 worst = std::max(worst, wb);
                 if (wb <= o.rtol) anyAtTarget = true;
@@ -1167,7 +1356,7 @@ void solve_blocks(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std
                 for (int b = 0; b < nb; ++b) {
                     if (!pl.h[b].active) continue;
                     const double tb2 = block_rel(b);
-                    if (tb2 <= o.rtol) {
+                    if (tb2 <= o.rtol || tb2 == HUGE_VAL) {
                         pl.h[b].active = 0;
                         changed = true;
                     } else {
