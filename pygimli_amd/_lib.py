@@ -61,6 +61,10 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"libpgert.so not built ({LIB_PATH}); run python -m pygimli_amd.build")
+    # torch's HIP runtime first: the library then binds to the runtime torch
+    # loaded (loaded the other way round, torch finds no device)
+    import torch
+    torch.cuda.is_available()
     L = ctypes.CDLL(LIB_PATH)
     vp = ctypes.c_void_p
     L.pgert_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int]
