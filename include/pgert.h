@@ -253,6 +253,20 @@ int pgert_cgls(pgert_ctx * ctx, const void * J, int dtype, int n_data, int n_mod
                const double * td, double lambda, const double * roughness, int max_iter, double tol,
                int * iterations, double * norm_r2);
 
+/* pgert_cgls without any device allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * semantics and results as pgert_cgls; its work vectors live in a fixed
+ * 64 MiB device array per GPU (held until the process ends), so the problem
+ * must fit 8 (5 n_model + 3 n_data + 4 max(n_const, 1)) bytes, each vector
+ * rounded up to 256 bytes, plus 4 KiB, within 64 MiB; a larger one fails
+ * with a message before anything runs.  Calls on one GPU share that array:
+ * calls that overlap on one GPU (other threads or streams) race. */
+int pgert_cgls_noalloc(pgert_ctx * ctx, const void * J, int dtype, int n_data, int n_model,
+                       const double * d_weight, const double * b, double * x, const double * wc,
+                       const double * wm, const double * tm, const double * td, double lambda,
+                       const double * roughness, int max_iter, double tol, int * iterations, double * norm_r2);
+
 /* Timing of the last pgert_forward / pgert_jacobian phases in milliseconds
  * (HIP events on the context stream): [0] assembly, [1] primary + SR rhs,
  * [2] solve, [3] response, [4] jacobian. */

@@ -30,7 +30,7 @@ EXPORTS = (
     "pgert_potentials", "pgert_potentials_node_major", "pgert_jacobian", "pgert_last_timings",
     "pgert_kernel_timing", "pgert_mlp_param_count", "pgert_mlp_set_params", "pgert_mlp_get_params",
     "pgert_mlp_collect", "pgert_mlp_predict", "pgert_mlp_train_step", "pgert_set_coarse",
-    "pgert_batch_forward", "pgert_batch_jacobian", "pgert_set_constraints", "pgert_cgls",
+    "pgert_batch_forward", "pgert_batch_jacobian", "pgert_set_constraints", "pgert_cgls", "pgert_cgls_noalloc",
     "pgert_forward_complex", "pgert_jacobian_complex", "pgert_forward_total", "pgert_set_bypass",
 )
 
@@ -91,6 +91,7 @@ def load():
     L.pgert_set_constraints.argtypes = [vp, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p, _c_double_p]
     L.pgert_cgls.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp, vp, vp,
                              ctypes.c_double, vp, ctypes.c_int, ctypes.c_double, _c_int_p, _c_double_p]
+    L.pgert_cgls_noalloc.argtypes = L.pgert_cgls.argtypes
     L.pgert_set_bypass.argtypes = [vp, ctypes.c_int, _c_int_p, _c_int_p, _c_double_p]
     L.pgert_forward_total.argtypes = [vp, vp, ctypes.POINTER(SolveOpts), vp, ctypes.POINTER(SolveStats)]
     L.pgert_forward_complex.argtypes = [vp, vp, vp, ctypes.POINTER(SolveOpts), vp, vp, ctypes.POINTER(SolveStats)]

@@ -81,6 +81,7 @@ int pgert_create(pgert_ctx ** out, int device) {
     c->device = device;
     int rc = guarded(c, [&] {
         for (auto & e : c->ev) PG_CHECK(hipEventCreate(&e));
+        cgls_scratch_init(device);
     });
     if (rc) {
         delete c;
@@ -965,6 +966,22 @@ int pgert_cgls(pgert_ctx * c, const void * J, int dtype, int n_data, int n_model
         PG_REQUIRE(max_iter >= 0, "pgert_cgls: bad iteration limit");
         const int it = cgls(*c, J, dtype, n_data, n_model, d_weight, b, x, wc, wm, tm, td, lambda, roughness,
                             max_iter, tol);
+        if (iterations) *iterations = it;
+        if (norm_r2) *norm_r2 = c->cglsNormR2;
+    });
+}
+
+int pgert_cgls_noalloc(pgert_ctx * c, const void * J, int dtype, int n_data, int n_model, const double * d_weight,
+                       const double * b, double * x, const double * wc, const double * wm, const double * tm,
+                       const double * td, double lambda, const double * roughness, int max_iter, double tol,
+                       int * iterations, double * norm_r2) {
+    return guarded(c, [&] {
+        PG_REQUIRE(J && (dtype == 0 || dtype == 1) && n_data > 0 && n_model > 0, "pgert_cgls_noalloc: bad Jacobian");
+        PG_REQUIRE(d_weight && b && x && wm && tm && td, "pgert_cgls_noalloc: null vector");
+        PG_REQUIRE(c->nConst == 0 || (wc && roughness), "pgert_cgls_noalloc: constraint weights / roughness missing");
+        PG_REQUIRE(max_iter >= 0, "pgert_cgls_noalloc: bad iteration limit");
+        const int it = cgls_noalloc(*c, J, dtype, n_data, n_model, d_weight, b, x, wc, wm, tm, td, lambda, roughness,
+                                    max_iter, tol);
         if (iterations) *iterations = it;
         if (norm_r2) *norm_r2 = c->cglsNormR2;
     });

@@ -12,6 +12,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <string>
 
 namespace pgert {
 
@@ -166,12 +168,66 @@ __global__ void __launch_bounds__(64) k_dot_sum(int np, const double * __restric
     if (threadIdx.x == 0) *out = s;
 }
 
+// work vectors of one CGLS run (device pointers)
 struct Work {
-    DBuf<double> u, q, z, zd, r, p, cdx, wcwc, wcr, g, ct, cv, part, dots;
-    std::vector<double> h;
+    double *u, *q, *z, *zd, *r, *p, *cdx, *wcwc, *wcr, *g, *ct, *cv, *part, *dots;
 };
 
+// carve the work vectors for D data, P parameters and nC constraints out of
+// one buffer of carved_doubles() doubles, each vector starting on a 256-byte
+// boundary (the buffer is 256-byte aligned)
+Work carve(double * base, int D, int P, int nC) {
+    const size_t c = (size_t)std::max(nC, 1);
+    size_t off = 0;
+    auto take = [&](size_t n) {
+        double * q = base + off;
+        off += (n + 31) / 32 * 32;
+        return q;
+    };
+    Work w;
+    w.u = take(P);
+    w.r = take(P);
+    w.p = take(P);
+    w.cdx = take(P);
+    w.ct = take(P);
+    w.q = take(D);
+    w.z = take(D);
+    w.zd = take(D);
+    w.wcwc = take(c);
+    w.wcr = take(c);
+    w.g = take(c);
+    w.cv = take(c);
+    w.part = take(2 * NRED);
+    w.dots = take(2);
+    return w;
+}
+size_t carved_doubles(int D, int P, int nC) {
+    const size_t c = (size_t)std::max(nC, 1);
+    auto r = [](size_t n) { return (n + 31) / 32 * 32; };
+    return 5 * r(P) + 3 * r(D) + 4 * r(c) + r(2 * NRED) + r(2);
+}
+
+// Workspace of pgert_cgls_noalloc: one array per device, held until the
+// process ends.  In host code `scratch` only names the symbol; its device
+// address comes from hipGetSymbolAddress, once per device, with that device
+// current.
+constexpr size_t SCRATCH_DOUBLES = size_t(8) << 20;  // 64 MiB
+static __device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+std::mutex scratchMu;
+double * scratchPtr[256] = {};
+
 }  // namespace
+
+void cgls_scratch_init(int device) {
+    PG_REQUIRE(device >= 0 && device < 256, "device index out of range");
+    std::lock_guard<std::mutex> lk(scratchMu);
+    if (scratchPtr[device]) return;
+    void * p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    PG_REQUIRE(e == hipSuccess && p, std::string("workspace of the allocation-free CGLS: hipGetSymbolAddress: ") +
+                                         hipGetErrorString(e));
+    scratchPtr[device] = (double *)p;
+}
 
 void set_constraints(Ctx & c, int nConst, int nModel, const int * rp, const int * ci, const double * va) {
     PG_REQUIRE(nConst >= 0 && nModel > 0 && rp && (rp[nConst] == 0 || (ci && va)), "pgert_set_constraints: bad CSR");
@@ -200,27 +256,13 @@ void set_constraints(Ctx & c, int nConst, int nModel, const int * rp, const int 
     PG_CHECK(hipStreamSynchronize(c.stream));
 }
 
+// the CGLS iteration on the work vectors w (no allocation in here)
 template <typename JT>
-static int run_cgls(Ctx & c, const JT * J, int D, int P, const double * dW, const double * b, double * x,
-                    const double * wc, const double * wm, const double * tm, const double * td, double lambda,
-                    const double * rough, int maxIter, double tol) {
+static int run_cgls(Ctx & c, const Work & w, const JT * J, int D, int P, const double * dW, const double * b,
+                    double * x, const double * wc, const double * wm, const double * tm, const double * td,
+                    double lambda, const double * rough, int maxIter, double tol) {
     hipStream_t s = c.stream;
     const int nC = c.nConst;
-    Work w;
-    w.u.alloc(P);
-    w.r.alloc(P);
-    w.p.alloc(P);
-    w.cdx.alloc(P);
-    w.ct.alloc(P);
-    w.q.alloc(D);
-    w.z.alloc(D);
-    w.zd.alloc(D);
-    w.wcwc.alloc(std::max(nC, 1));
-    w.wcr.alloc(std::max(nC, 1));
-    w.g.alloc(std::max(nC, 1));
-    w.cv.alloc(std::max(nC, 1));
-    w.part.alloc(2 * NRED);
-    w.dots.alloc(2);
     const dim3 blk(256);
     auto G = [](long n) { return dim3((unsigned)std::max(1L, (n + 255) / 256)); };
     const int gjv = std::min((D + JR - 1) / JR, 4096);
@@ -235,80 +277,80 @@ static int run_cgls(Ctx & c, const JT * J, int D, int P, const double * dW, cons
     // C' (wc^2 C (wm x)) into w.ct (zero without constraints)
     auto ctc = [&](const double * xv) {
         if (nC == 0) {
-            PG_CHECK(hipMemsetAsync(w.ct.p, 0, w.ct.bytes(), s));
+            PG_CHECK(hipMemsetAsync(w.ct, 0, (size_t)P * sizeof(double), s));
             return;
         }
-        hipLaunchKernelGGL(k_csr, G(nC), blk, 0, s, nC, crp, cci, cva, xv, wm, w.wcwc.p, 0, w.g.p);
-        hipLaunchKernelGGL(k_csr, G(P), blk, 0, s, P, trp, tci, tva, w.g.p, (const double *)nullptr,
-                           (const double *)nullptr, 0, w.ct.p);
+        hipLaunchKernelGGL(k_csr, G(nC), blk, 0, s, nC, crp, cci, cva, xv, wm, w.wcwc, 0, w.g);
+        hipLaunchKernelGGL(k_csr, G(P), blk, 0, s, P, trp, tci, tva, w.g, (const double *)nullptr,
+                           (const double *)nullptr, 0, w.ct);
     };
     auto dot = [&](const double * a, const double * bb, long n, int which) {
-        hipLaunchKernelGGL(k_dot, dim3(NRED), blk, 0, s, n, a, bb, w.part.p + which * NRED);
-        hipLaunchKernelGGL(k_dot_sum, dim3(1), dim3(64), 0, s, NRED, w.part.p + which * NRED, w.dots.p + which);
+        hipLaunchKernelGGL(k_dot, dim3(NRED), blk, 0, s, n, a, bb, w.part + which * NRED);
+        hipLaunchKernelGGL(k_dot_sum, dim3(1), dim3(64), 0, s, NRED, w.part + which * NRED, w.dots + which);
     };
     auto read_dots = [&](double * h2) {
-        PG_CHECK(hipMemcpyAsync(h2, w.dots.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+        PG_CHECK(hipMemcpyAsync(h2, w.dots, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
         PG_CHECK(hipStreamSynchronize(s));
     };
     if (nC > 0) {
-        hipLaunchKernelGGL(k_sq, G(nC), blk, 0, s, nC, wc, (const double *)nullptr, w.wcwc.p);
-        hipLaunchKernelGGL(k_sq, G(nC), blk, 0, s, nC, wc, rough, w.wcr.p);
+        hipLaunchKernelGGL(k_sq, G(nC), blk, 0, s, nC, wc, (const double *)nullptr, w.wcwc);
+        hipLaunchKernelGGL(k_sq, G(nC), blk, 0, s, nC, wc, rough, w.wcr);
         // cdx = lambda wm C'(wc rough)                                   (solver.cpp:62)
-        hipLaunchKernelGGL(k_csr, G(P), blk, 0, s, P, trp, tci, tva, w.wcr.p, (const double *)nullptr,
-                           (const double *)nullptr, 0, w.ct.p);
-        hipLaunchKernelGGL(k_lscale, G(P), blk, 0, s, (long)P, lambda, w.ct.p, wm, w.cdx.p);
+        hipLaunchKernelGGL(k_csr, G(P), blk, 0, s, P, trp, tci, tva, w.wcr, (const double *)nullptr,
+                           (const double *)nullptr, 0, w.ct);
+        hipLaunchKernelGGL(k_lscale, G(P), blk, 0, s, (long)P, lambda, w.ct, wm, w.cdx);
     } else {
-        PG_CHECK(hipMemsetAsync(w.cdx.p, 0, w.cdx.bytes(), s));
+        PG_CHECK(hipMemsetAsync(w.cdx, 0, (size_t)P * sizeof(double), s));
     }
     // z = (b - S (x / tm) td) dW                                          (:63)
-    hipLaunchKernelGGL(k_div, G(P), blk, 0, s, (long)P, x, tm, w.u.p);
-    launch_jv(w.u.p, nullptr, nullptr, w.q.p);
-    hipLaunchKernelGGL(k_resid0, G(D), blk, 0, s, D, b, w.q.p, td, dW, w.z.p);
+    hipLaunchKernelGGL(k_div, G(P), blk, 0, s, (long)P, x, tm, w.u);
+    launch_jv(w.u, nullptr, nullptr, w.q);
+    hipLaunchKernelGGL(k_resid0, G(D), blk, 0, s, D, b, w.q, td, dW, w.z);
     double h2[2];
     if (tol < 0.0) {
         // r0 = S'(b dW dW td) / tm - cdx, accuracy = max(TOLERANCE, 1e-8 r0.r0)   (:67, :98)
-        hipLaunchKernelGGL(k_mul3, G(D), blk, 0, s, (long)D, b, dW, dW, w.zd.p);
-        hipLaunchKernelGGL(k_mul3, G(D), blk, 0, s, (long)D, w.zd.p, td, (const double *)nullptr, w.zd.p);
-        launch_jtv(w.zd.p, nullptr, w.cdx.p, w.r.p);
-        dot(w.r.p, w.r.p, P, 0);
+        hipLaunchKernelGGL(k_mul3, G(D), blk, 0, s, (long)D, b, dW, dW, w.zd);
+        hipLaunchKernelGGL(k_mul3, G(D), blk, 0, s, (long)D, w.zd, td, (const double *)nullptr, w.zd);
+        launch_jtv(w.zd, nullptr, w.cdx, w.r);
+        dot(w.r, w.r, P, 0);
         read_dots(h2);
         tol = std::max(1e-12, 1e-8 * h2[0]);
     }
     // p = S'(z dW td) / tm - lambda wm C'(wc^2 C (wm x)) - cdx; r = p    (:64-66, :99)
-    hipLaunchKernelGGL(k_mul3, G(D), blk, 0, s, (long)D, w.z.p, dW, td, w.zd.p);
+    hipLaunchKernelGGL(k_mul3, G(D), blk, 0, s, (long)D, w.z, dW, td, w.zd);
     ctc(x);
-    launch_jtv(w.zd.p, nC ? w.ct.p : nullptr, w.cdx.p, w.p.p);
-    dot(w.p.p, w.p.p, P, 0);
+    launch_jtv(w.zd, nC ? w.ct : nullptr, w.cdx, w.p);
+    dot(w.p, w.p, P, 0);
     read_dots(h2);
     double normR2 = h2[0];
     int count = 0;
     while (count < maxIter && normR2 > tol) {
         ++count;
         // q = S (p / tm) dW td                                            (:123-124)
-        hipLaunchKernelGGL(k_div, G(P), blk, 0, s, (long)P, w.p.p, tm, w.u.p);
-        launch_jv(w.u.p, dW, td, w.q.p);
-        dot(w.q.p, w.q.p, D, 0);
+        hipLaunchKernelGGL(k_div, G(P), blk, 0, s, (long)P, w.p, tm, w.u);
+        launch_jv(w.u, dW, td, w.q);
+        dot(w.q, w.q, D, 0);
         // wcp = wc round(C (p wm), 1e-10)                                 (:131-138)
         if (nC > 0) {
-            hipLaunchKernelGGL(k_csr, G(nC), blk, 0, s, nC, crp, cci, cva, w.p.p, wm, wc, 1, w.cv.p);
-            dot(w.cv.p, w.cv.p, nC, 1);
+            hipLaunchKernelGGL(k_csr, G(nC), blk, 0, s, nC, crp, cci, cva, w.p, wm, wc, 1, w.cv);
+            dot(w.cv, w.cv, nC, 1);
         } else {
-            PG_CHECK(hipMemsetAsync(w.dots.p + 1, 0, sizeof(double), s));
+            PG_CHECK(hipMemsetAsync(w.dots + 1, 0, sizeof(double), s));
         }
         read_dots(h2);
         const double aQ = h2[0] + lambda * h2[1];                    // (:159)
         const double alpha = normR2 / aQ;                            // (:165)
-        hipLaunchKernelGGL(k_axpy, G(P), blk, 0, s, (long)P, alpha, w.p.p, x);       // (:168)
-        hipLaunchKernelGGL(k_axpy, G(D), blk, 0, s, (long)D, -alpha, w.q.p, w.z.p);  // (:175)
+        hipLaunchKernelGGL(k_axpy, G(P), blk, 0, s, (long)P, alpha, w.p, x);       // (:168)
+        hipLaunchKernelGGL(k_axpy, G(D), blk, 0, s, (long)D, -alpha, w.q, w.z);  // (:175)
         // r = S'(z dW td) / tm - lambda wm C'(wc^2 C (wm x)) - cdx       (:179-185)
-        hipLaunchKernelGGL(k_mul3, G(D), blk, 0, s, (long)D, w.z.p, dW, td, w.zd.p);
+        hipLaunchKernelGGL(k_mul3, G(D), blk, 0, s, (long)D, w.z, dW, td, w.zd);
         ctc(x);
-        launch_jtv(w.zd.p, nC ? w.ct.p : nullptr, w.cdx.p, w.r.p);
-        dot(w.r.p, w.r.p, P, 0);
+        launch_jtv(w.zd, nC ? w.ct : nullptr, w.cdx, w.r);
+        dot(w.r, w.r, P, 0);
         read_dots(h2);
         const double old = normR2;
         normR2 = h2[0];                                              // (:190)
-        hipLaunchKernelGGL(k_xpby, G(P), blk, 0, s, (long)P, w.r.p, normR2 / old, w.p.p);  // (:191-194)
+        hipLaunchKernelGGL(k_xpby, G(P), blk, 0, s, (long)P, w.r, normR2 / old, w.p);  // (:191-194)
     }
     PG_CHECK(hipGetLastError());
     PG_CHECK(hipStreamSynchronize(s));
@@ -320,9 +362,35 @@ int cgls(Ctx & c, const void * J, int dtype, int D, int P, const double * dW, co
          const double * wc, const double * wm, const double * tm, const double * td, double lambda,
          const double * rough, int maxIter, double tol) {
     PG_REQUIRE(c.nConst == 0 || c.cCols == P, "pgert_cgls: constraints have another model size");
+    DBuf<double> buf;
+    buf.alloc(carved_doubles(D, P, c.nConst));
+    const Work w = carve(buf.p, D, P, c.nConst);
     if (dtype == 0)
-        return run_cgls(c, (const double *)J, D, P, dW, b, x, wc, wm, tm, td, lambda, rough, maxIter, tol);
-    return run_cgls(c, (const float *)J, D, P, dW, b, x, wc, wm, tm, td, lambda, rough, maxIter, tol);
+        return run_cgls(c, w, (const double *)J, D, P, dW, b, x, wc, wm, tm, td, lambda, rough, maxIter, tol);
+    return run_cgls(c, w, (const float *)J, D, P, dW, b, x, wc, wm, tm, td, lambda, rough, maxIter, tol);
+}
+
+// pgert_cgls with its work vectors in the device's scratch array: no
+// allocation.  Calls on one device share the array (they must not overlap).
+int cgls_noalloc(Ctx & c, const void * J, int dtype, int D, int P, const double * dW, const double * b, double * x,
+                 const double * wc, const double * wm, const double * tm, const double * td, double lambda,
+                 const double * rough, int maxIter, double tol) {
+    PG_REQUIRE(c.nConst == 0 || c.cCols == P, "pgert_cgls_noalloc: constraints have another model size");
+    const size_t need = carved_doubles(D, P, c.nConst);
+    PG_REQUIRE(need <= SCRATCH_DOUBLES,
+               "pgert_cgls_noalloc: the problem needs " + std::to_string(need * 8) + " bytes of workspace (5 n_model + "
+               "3 n_data + 4 n_constraints doubles), more than the " + std::to_string(SCRATCH_DOUBLES * 8) +
+               "-byte scratch array; use pgert_cgls");
+    double * base;
+    {
+        std::lock_guard<std::mutex> lk(scratchMu);
+        base = scratchPtr[c.device];
+    }
+    PG_REQUIRE(base, "pgert_cgls_noalloc: no workspace address for this device (context not created by pgert_create)");
+    const Work w = carve(base, D, P, c.nConst);
+    if (dtype == 0)
+        return run_cgls(c, w, (const double *)J, D, P, dW, b, x, wc, wm, tm, td, lambda, rough, maxIter, tol);
+    return run_cgls(c, w, (const float *)J, D, P, dW, b, x, wc, wm, tm, td, lambda, rough, maxIter, tol);
 }
 
 }  // namespace pgert

@@ -274,6 +274,12 @@ void set_constraints(Ctx & c, int nConst, int nModel, const int * rowptr, const 
 int cgls(Ctx & c, const void * J, int dtype, int nData, int nModel, const double * dW, const double * b, double * x,
          const double * wc, const double * wm, const double * tm, const double * td, double lambda,
          const double * roughness, int maxIter, double tol);
+int cgls_noalloc(Ctx & c, const void * J, int dtype, int nData, int nModel, const double * dW, const double * b,
+                 double * x, const double * wc, const double * wm, const double * tm, const double * td, double lambda,
+                 const double * roughness, int maxIter, double tol);
+// device address of the allocation-free CGLS's workspace, looked up once per
+// device (the device must be current)
+void cgls_scratch_init(int device);
 
 // solvers (solver.hip); return per-solve statistics
 void solve_pcg_fp64(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats & st);
