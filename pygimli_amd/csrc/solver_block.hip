@@ -234,15 +234,16 @@ __global__ void __launch_bounds__(256) k_spmm(const BlockDesc * __restrict__ blo
 // ---------------------------------------------------------------- Gram passes
 // MODE_PQR: vectors (P, Q, R) -> G1 = P'Q, G2 = P'R.
 // MODE_Z:   vectors (P, Q, Z) -> Q'Z, Z'Z, P'Z.
-// A workgroup takes a contiguous range of 4-row steps; the rows of one step
-// are 1 KB (FP32) per vector and contiguous in the block layout.  Wavefronts
-// 0, 1, 2 each fetch one vector (one 16-byte load per lane and step) and keep
-// DEPTH steps in flight in registers (a ring with static indices: the inner
-// loop over the ring is unrolled); the current step is staged through
-// double-buffered LDS, one LDS-only barrier per step.  Every wavefront then
-// holds the operands of all 4 rows (lane l: row l/16, columns 4(l%16)..+3)
-// and runs the MFMAs of its own column tiles tb = wave: tile (ta, tb) holds
-// Gram entries (4i + ta, 4j + tb), result register jj of lane l entry
+// A workgroup of 8 wavefronts (two per SIMD) takes a contiguous range of
+// 4-row steps; the rows of one step are 1 KB (FP32) per vector and
+// contiguous in the block layout.  The steps go in groups of GS: every
+// thread loads 3 of the group's 3 x GS x 64 16-byte pieces, keeps two groups
+// in flight in registers and stages the current one through double-buffered
+// LDS (one LDS-only barrier per group).  Every wavefront then holds the
+// operands of the group's rows (lane l: row l/16 of a step, columns
+// 4(l%16)..+3) and runs the MFMAs of its tiles: column tile tb = wave % 4,
+// row tiles ta = 2 (wave / 4) + {0, 1}; tile (ta, tb) holds Gram entries
+// (4i + ta, 4j + tb), result register jj of lane l entry
 // (4 drow(l, jj) + ta, 4 (l%16) + tb).
 // FP32 inputs: the matrix cores accumulate FP32 over FLUSH steps (16 rows),
 // then the tile is added into FP64 registers -- the Grams keep FP64
@@ -252,109 +253,148 @@ __global__ void __launch_bounds__(256) k_spmm(const BlockDesc * __restrict__ blo
 constexpr int MODE_PQR = 0;
 constexpr int MODE_Z = 1;
 constexpr int FLUSH = 4;
+constexpr int GRAM_THREADS = 512;
 
 template <typename VT, int MODE>
-__global__ void __launch_bounds__(256) k_gram(const BlockDesc * __restrict__ blocks, const VT * __restrict__ V0,
-                                              const VT * __restrict__ V1, const VT * __restrict__ V2,
-                                              double * __restrict__ part) {
+__global__ void __launch_bounds__(GRAM_THREADS) k_gram(const BlockDesc * __restrict__ blocks,
+                                                       const VT * __restrict__ V0, const VT * __restrict__ V1,
+                                                       const VT * __restrict__ V2, double * __restrict__ part) {
     constexpr int NG = MODE == MODE_PQR ? 2 : 3;
     constexpr bool F32 = sizeof(VT) == 4;
-    constexpr int DEPTH = F32 ? (MODE == MODE_Z ? 12 : 16) : 8;  // steps in flight per wavefront
     using V = V4<VT>;
-    constexpr int GS = 4;          // 4-row steps per LDS exchange
+    constexpr int GS = 8;                                 // 4-row steps per LDS exchange
+    constexpr int NL = 3 * GS * 64 / GRAM_THREADS;       // pieces loaded per thread and group
+    static_assert(NL * GRAM_THREADS == 3 * GS * 64, "loads per group split evenly");
+    static_assert(GS % FLUSH == 0, "flushes at group boundaries");
     __shared__ V sv[2][GS][3][64];  // [buffer][step of the group][vector][lane]
     const int b = blockIdx.y;
     if (!blocks[b].active) return;
     const int n = blocks[b].n;
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int tb = wave % 4, ta0 = 2 * (wave / 4);
     const size_t vb = blocks[b].off;
     const int nst = (n + 3) / 4;
     const int st0 = (int)((long)nst * blockIdx.x / gridDim.x), st1 = (int)((long)nst * (blockIdx.x + 1) / gridDim.x);
-    const VT * src = (wave == 0 ? V0 : wave == 1 ? V1 : V2) + vb + 4 * lane;
-    auto fetch = [&](int st) -> V {
-        V v = V{0, 0, 0, 0};
-        if (wave < 3 && st < st1 && 4 * st + lane / 16 < n) v = ld4(src + (size_t)st * 4 * S);
-        return v;
+    // piece i of this thread: item = i * GRAM_THREADS + threadIdx.x -> (step
+    // ph[i] of the group, vector, lane); source pointers and LDS slots are
+    // fixed per thread
+    const VT * psrc[NL];
+    int ph[NL], pslot[NL];
+    const int ln = threadIdx.x % 64;
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+        const int item = i * GRAM_THREADS + threadIdx.x;
+        const int vec = (item / 64) % 3;
+        ph[i] = item / 192;
+        pslot[i] = (ph[i] * 3 + vec) * 64 + ln;
+        psrc[i] = (vec == 0 ? V0 : vec == 1 ? V1 : V2) + vb + 4 * (ln % 16);
+    }
+    // Loads without branches (a branch around a load makes the compiler wait
+    // for every outstanding load at the next use): rows past the range read a
+    // valid row, and stage() zeroes them.
+    auto fetch = [&](int s0, V (&r)[NL]) {
+#pragma unroll
+        for (int i = 0; i < NL; ++i) {
+            const int row = max(0, min(4 * min(s0 + ph[i], st1 - 1) + ln / 16, n - 1));
+            r[i] = ld4(psrc[i] + (size_t)row * S);
+        }
+    };
+    auto stage = [&](int buf, int s0, const V (&r)[NL]) {
+#pragma unroll
+        for (int i = 0; i < NL; ++i) {
+            const int st = s0 + ph[i];
+            (&sv[buf][0][0][0])[pslot[i]] = zero_unless(st < st1 && 4 * st + ln / 16 < n, r[i]);
+        }
     };
 
-    V acc[NG][4];
-    f64x4 acc64[F32 ? NG : 1][4];
+    V acc[NG][2];
+    f64x4 acc64[F32 ? NG : 1][2];
 #pragma unroll
     for (int g = 0; g < NG; ++g)
 #pragma unroll
-        for (int ta = 0; ta < 4; ++ta) {
-            acc[g][ta] = V{0, 0, 0, 0};
-            if constexpr (F32) acc64[g][ta] = f64x4{0, 0, 0, 0};
+        for (int t = 0; t < 2; ++t) {
+            acc[g][t] = V{0, 0, 0, 0};
+            if constexpr (F32) acc64[g][t] = f64x4{0, 0, 0, 0};
         }
     auto flush = [&]() {
         if constexpr (F32) {
 #pragma unroll
             for (int g = 0; g < NG; ++g)
 #pragma unroll
-                for (int ta = 0; ta < 4; ++ta) {
+                for (int t = 0; t < 2; ++t) {
 #pragma unroll
-                    for (int jj = 0; jj < 4; ++jj) acc64[g][ta][jj] += (double)acc[g][ta][jj];
-                    acc[g][ta] = V{0, 0, 0, 0};
+                    for (int jj = 0; jj < 4; ++jj) acc64[g][t][jj] += (double)acc[g][t][jj];
+                    acc[g][t] = V{0, 0, 0, 0};
                 }
         }
     };
 
-    V ring[DEPTH];
+    auto compute = [&](int buf) {
 #pragma unroll
-    for (int i = 0; i < DEPTH; ++i) ring[i] = fetch(st0 + i);
-    // GS 4-row steps per LDS exchange (one barrier per group)
-    static_assert(DEPTH % GS == 0 && FLUSH % GS == 0, "steps are exchanged in groups");
-    for (int s0 = st0; s0 < st1; s0 += DEPTH) {
-#pragma unroll
-        for (int i = 0; i < DEPTH; i += GS) {
-            const int buf = (i / GS) & 1;
-            if (wave < 3)
-#pragma unroll
-                for (int h = 0; h < GS; ++h) sv[buf][h][wave][lane] = ring[i + h];
-#pragma unroll
-            for (int h = 0; h < GS; ++h) ring[i + h] = fetch(s0 + i + h + DEPTH);
-            lds_barrier();
-#pragma unroll
-            for (int h = 0; h < GS; ++h) {
-                const V x0 = sv[buf][h][0][lane], x1 = sv[buf][h][1][lane], x2 = sv[buf][h][2][lane];
-                V a[NG], bo[NG];
-                if constexpr (MODE == MODE_PQR) {
-                    a[0] = x0;  // P'Q
-                    a[1] = x0;  // P'R
-                    bo[0] = x1;
-                    bo[1] = x2;
-                } else {
-                    a[0] = x1;  // Q'Z
-                    a[1] = x2;  // Z'Z
-                    a[2] = x0;  // P'Z
-                    bo[0] = x2;
-                    bo[1] = x2;
-                    bo[2] = x2;
-                }
-#pragma unroll
-                for (int g = 0; g < NG; ++g) {
-                    const VT bw = wave == 0 ? bo[g][0] : wave == 1 ? bo[g][1] : wave == 2 ? bo[g][2] : bo[g][3];
-#pragma unroll
-                    for (int ta = 0; ta < 4; ++ta) acc[g][ta] = mfma4(a[g][ta], bw, acc[g][ta]);
-                }
+        for (int h = 0; h < GS; ++h) {
+            const V x0 = sv[buf][h][0][lane], x1 = sv[buf][h][1][lane], x2 = sv[buf][h][2][lane];
+            V av[NG], bo[NG];
+            if constexpr (MODE == MODE_PQR) {
+                av[0] = x0;  // P'Q
+                av[1] = x0;  // P'R
+                bo[0] = x1;
+                bo[1] = x2;
+            } else {
+                av[0] = x1;  // Q'Z
+                av[1] = x2;  // Z'Z
+                av[2] = x0;  // P'Z
+                bo[0] = x2;
+                bo[1] = x2;
+                bo[2] = x2;
             }
-            if (((i + GS - 1) % FLUSH) == FLUSH - 1) flush();
+#pragma unroll
+            for (int g = 0; g < NG; ++g) {
+                const VT bw = tb == 0 ? bo[g][0] : tb == 1 ? bo[g][1] : tb == 2 ? bo[g][2] : bo[g][3];
+                const VT a0 = ta0 == 0 ? av[g][0] : av[g][2];
+                const VT a1 = ta0 == 0 ? av[g][1] : av[g][3];
+                acc[g][0] = mfma4(a0, bw, acc[g][0]);
+                acc[g][1] = mfma4(a1, bw, acc[g][1]);
+            }
+            if ((h % FLUSH) == FLUSH - 1) flush();
         }
+    };
+    // two groups in flight in two register sets used in turn (no register
+    // copies, which would wait for the younger loads too)
+    V ra[NL], rb[NL];
+    fetch(st0, ra);
+    __builtin_amdgcn_sched_barrier(0);  // ra's loads are issued before rb's
+    fetch(st0 + GS, rb);
+    for (int s0 = st0; s0 < st1; s0 += 2 * GS) {
+        // (the scheduling barriers keep each group's loads ahead of the
+        // compute phase that hides them)
+        stage(0, s0, ra);
+        fetch(s0 + 2 * GS, ra);
+        __builtin_amdgcn_sched_barrier(0);
+        lds_barrier();
+        compute(0);
+        // no early exit between the halves (a branch here would let the
+        // compiler sink the loads above into the path that uses them); a
+        // group past the range is staged as zeros and adds nothing
+        stage(1, s0 + GS, rb);
+        fetch(s0 + 3 * GS, rb);
+        __builtin_amdgcn_sched_barrier(0);
+        lds_barrier();
+        compute(1);
     }
     flush();
     double * out = part + ((size_t)b * gridDim.x + blockIdx.x) * NG * SS;
 #pragma unroll
     for (int g = 0; g < NG; ++g)
 #pragma unroll
-        for (int ta = 0; ta < 4; ++ta)
+        for (int t = 0; t < 2; ++t)
 #pragma unroll
             for (int jj = 0; jj < 4; ++jj) {
-                const int ai = 4 * drow<VT>(lane, jj) + ta, bj = 4 * (lane % 16) + wave;
+                const int ai = 4 * drow<VT>(lane, jj) + ta0 + t, bj = 4 * (lane % 16) + tb;
                 double v;
                 if constexpr (F32)
-                    v = acc64[g][ta][jj];
+                    v = acc64[g][t][jj];
                 else
-                    v = acc[g][ta][jj];
+                    v = acc[g][t][jj];
                 out[(size_t)g * SS + ai * S + bj] = v;
             }
 }
@@ -620,10 +660,10 @@ __global__ void __launch_bounds__(256) k_restrict(const BlockDesc * __restrict__
 }
 
 // cc[b] = Einv_k y[b] (FP64 MFMA): a workgroup computes one 16 x 16 tile
-// (rows 16 blockIdx.x, columns 16 * tile); its 16 wavefronts split the
-// m-long contraction (each issues all loads of its share, up to 16 k-steps,
-// before its MFMAs) and are summed through LDS in a fixed order
-constexpr int CA_WAVES = 16;
+// (rows 16 blockIdx.x, columns 16 * tile); its 4 wavefronts split the
+// m-long contraction (16 k-steps of loads in flight before their MFMAs) and
+// are summed through LDS in a fixed order
+constexpr int CA_WAVES = 4;
 __global__ void __launch_bounds__(64 * CA_WAVES) k_coarse_apply(const BlockDesc * __restrict__ blocks,
                                                                 const double * __restrict__ y,
                                                                 double * __restrict__ cc) {
@@ -735,15 +775,6 @@ __device__ void mm64(const double * A, const double * B, double * C, bool transA
     __syncthreads();
 }
 
-// v[q] of four registers (static indices: value selects, no branches)
-__device__ inline double sel4(int q, double a, double b, double c, double d) {
-    double r = a;
-    r = q == 1 ? b : r;
-    r = q == 2 ? c : r;
-    r = q == 3 ? d : r;
-    return r;
-}
-
 // Li = L^-1 for G = L L' (lower Cholesky factor) with dependent directions
 // dropped: forward elimination of [G | I] whose right half becomes the
 // inverse unit-lower factor, rows then scaled by the pivots^-1/2.  Each thread
@@ -764,29 +795,25 @@ __device__ void inv_chol(const double * G, double * Li, int s) {
         }
     if (threadIdx.x < S) d0[threadIdx.x] = threadIdx.x < s ? G[threadIdx.x * LD + threadIdx.x] : 0.0;
     __syncthreads();
-    // Each step is one chain: publish, LDS-only barrier, one batch of
-    // unconditional LDS reads, selects and FMAs.  Register indices stay
-    // static (a lane-dependent index into a register array becomes a loop
-    // over the distinct lane values); the conditionals are value selects, so
-    // the reads of a step are not serialised behind branches.
-    for (int j = 0; j < s; ++j) {
+    // Each step is one chain: publish, LDS-only barrier, one batch of LDS
+    // reads, FMAs.  The steps go in fours so that the register holding
+    // column / row j of a tile has a static index (j % 4); the owner threads
+    // publish under a predicate, all threads read unconditionally.  Only the
+    // row factors f are masked: updates of already eliminated columns of G
+    // (c <= j) are never read again, and row j of L^-1 is zero beyond its
+    // diagonal, so the row and Li updates need no masks.
+    auto step = [&](auto qtag, int j0) {
+        constexpr int Q = decltype(qtag)::value;
+        const int j = j0 + Q;
         const int buf = j & 1;
-        const int qc = j - c0, qr = j - i0;
-        double pc[4], pr[4], pl[4];
+        if (c0 == j0)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            pc[i] = sel4(qc, g[i][0], g[i][1], g[i][2], g[i][3]);
-            pr[i] = sel4(qr, g[0][i], g[1][i], g[2][i], g[3][i]);
-            pl[i] = sel4(qr, li[0][i], li[1][i], li[2][i], li[3][i]);
-        }
-        if ((unsigned)qc < 4u)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) colb[buf][i0 + i] = pc[i];
-        if ((unsigned)qr < 4u)
+            for (int i = 0; i < 4; ++i) colb[buf][i0 + i] = g[i][Q];
+        if (i0 == j0)
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
-                rowb[buf][c0 + c] = pr[c];
-                lrow[buf][c0 + c] = pl[c];
+                rowb[buf][c0 + c] = g[Q][c];
+                lrow[buf][c0 + c] = li[Q][c];
             }
         lds_barrier();
         const double d = rowb[buf][j], dj = d0[j];
@@ -801,16 +828,10 @@ __device__ void inv_chol(const double * G, double * Li, int s) {
         const bool keep = dj > 0.0 && d > 1e-12 * dj;
         const double rcp = 1.0 / d;
         const double inv = keep ? rcp : 0.0;  // a dropped pivot eliminates nothing
-#pragma unroll
-        for (int i = 0; i < 4; ++i) piv[i] = i0 + i == j ? (keep ? d : 0.0) : piv[i];
+        if (i0 == j0) piv[Q] = keep ? d : 0.0;
         double f[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) f[i] = i0 + i > j ? cb[i] * inv : 0.0;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            rb[c] = c0 + c > j ? rb[c] : 0.0;
-            lr[c] = c0 + c <= j ? lr[c] : 0.0;
-        }
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -818,6 +839,12 @@ __device__ void inv_chol(const double * G, double * Li, int s) {
                 g[i][c] -= f[i] * rb[c];
                 li[i][c] -= f[i] * lr[c];
             }
+    };
+    for (int j0 = 0; j0 < s; j0 += 4) {
+        step(std::integral_constant<int, 0>{}, j0);
+        if (j0 + 1 < s) step(std::integral_constant<int, 1>{}, j0);
+        if (j0 + 2 < s) step(std::integral_constant<int, 2>{}, j0);
+        if (j0 + 3 < s) step(std::integral_constant<int, 3>{}, j0);
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -1249,7 +1276,8 @@ void solve_blocks(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std
             hipLaunchKernelGGL(k_zform<VT>, gZ, blk, 0, s, pl.d.p, Rw.p, ccp, Zw.p);
             te(PGERT_TIME_ZFORM);
             tb(PGERT_TIME_GRAM_Z);
-            hipLaunchKernelGGL((k_gram<VT, MODE_Z>), gG, blk, 0, s, pl.d.p, Pw.p, Qw.p, Zw.p, c0.part.p);
+            hipLaunchKernelGGL((k_gram<VT, MODE_Z>), gG, dim3(GRAM_THREADS), 0, s, pl.d.p, Pw.p, Qw.p, Zw.p,
+                               c0.part.p);
             te(PGERT_TIME_GRAM_Z);
             tb(PGERT_TIME_REDUCE);
             hipLaunchKernelGGL(k_part_reduce, dim3(3 * SS / 32, nb), blk, 0, s, pl.d.p, G, 3 * SS, c0.part.p,
@@ -1262,7 +1290,8 @@ void solve_blocks(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std
             hipLaunchKernelGGL((k_spmm<VT, AT>), gS, blk, 0, s, pl.d.p, Pw.p, Qw.p);
             te(PGERT_TIME_SPMM);
             tb(PGERT_TIME_GRAM_PQ);
-            hipLaunchKernelGGL((k_gram<VT, MODE_PQR>), gG, blk, 0, s, pl.d.p, Pw.p, Qw.p, Rw.p, c0.part.p);
+            hipLaunchKernelGGL((k_gram<VT, MODE_PQR>), gG, dim3(GRAM_THREADS), 0, s, pl.d.p, Pw.p, Qw.p, Rw.p,
+                               c0.part.p);
             te(PGERT_TIME_GRAM_PQ);
             tb(PGERT_TIME_REDUCE);
             hipLaunchKernelGGL(k_part_reduce, dim3(2 * SS / 32, nb), blk, 0, s, pl.d.p, G, 2 * SS, c0.part.p,
