@@ -97,6 +97,17 @@ int pgert_set_mesh(pgert_ctx * ctx, int dim, int n_nodes, const double * xyz,
 int pgert_set_electrodes(pgert_ctx * ctx, int n_elec, const int * elec_node,
                          const double * source_center, double surface_z);
 
+/* Electrodes that may lie off the nodes (searchElectrodes_' fallback,
+ * dcfemmodelling.cpp:979-985): elec_node[e] >= 0 as in pgert_set_electrodes;
+ * elec_node[e] = -1 a free electrode in cell elec_cell[e] at elec_pos[e]
+ * (HOST [n_elec][3], mesh coordinates), modelled as the reference's
+ * ElectrodeShapeEntity (electrode.cpp:198-287): shape-function point source
+ * on the cell's nodes, potential interpolated there, rho_s = the cell's
+ * resistivity, primary potential from its position (singular value only at
+ * a cell node within 1e-4). */
+int pgert_set_electrodes_free(pgert_ctx * ctx, int n_elec, const int * elec_node, const int * elec_cell,
+                              const double * elec_pos, const double * source_center, double surface_z);
+
 /* Wavenumbers and weights (setkValues / setWeights, dcfemmodelling.h:239-243;
  * 3D: k = {0}, w = {1}). */
 int pgert_set_wavenumbers(pgert_ctx * ctx, int nk, const double * k, const double * w);
@@ -252,6 +263,31 @@ int pgert_cgls(pgert_ctx * ctx, const void * J, int dtype, int n_data, int n_mod
                const double * b, double * x, const double * wc, const double * wm, const double * tm,
                const double * td, double lambda, const double * roughness, int max_iter, double tol,
                int * iterations, double * norm_r2);
+
+/* Logarithmic transforms of the Gauss-Newton loop, elementwise on DEVICE
+ * vectors of length n: TransLogLU(lower, upper) (core/src/trans.h:292-322,
+ * trans.cpp:24-63), with upper = 0 TransLog(lower) (trans.h:241-287):
+ *   op 0: out = trans(a)      1: invTrans(a)      2: deriv(a)
+ *   op 3: out = invTrans(trans(a) + tau b)         (Trans::update, trans.h:76-79)
+ *   op 4: out = |a b deriv(a)|, b relative errors  (Trans::error, trans.h:88-93)
+ *   op 5: out = trans(a) - trans(b)
+ * b: DEVICE [n] for ops 3-5, else ignored. */
+int pgert_trans(pgert_ctx * ctx, int op, int n, const double * a, const double * b, double tau, double lower,
+                double upper, double * out);
+
+/* The objective RInversion::linesearch scans (core/src/inversion.h:615-640),
+ * at n_tau step lengths taus[k] (HOST): with the data transform log and the
+ * model transform TransLogLU(lower, upper),
+ *   phiD[k] = | (tD(data) - tD(tD.update(resp, taus[k] dd))) / derr |^2   (getPhiD, inversion.cpp:84-98)
+ *   phiM[k] = | cw (C (tM(tM.update(model, taus[k] dm)) mw)) |^2           (getPhiM, roughness, inversion.h:554-563)
+ * and for taus[k] = 0 resp and model as they are (getPhi()).  C is the
+ * matrix of pgert_set_constraints.  DEVICE: model, dm, mw [n_model], cw
+ * [n_const], data, resp, dd, derr [n_data].  HOST outputs phiD, phiM [n_tau].
+ * Fixed-order sums. */
+int pgert_gn_phi(pgert_ctx * ctx, int n_tau, const double * taus, int n_model, const double * model,
+                 const double * dm, const double * mw, const double * cw, double lower, double upper, int n_data,
+                 const double * data, const double * resp, const double * dd, const double * derr, double * phiD,
+                 double * phiM);
 
 /* pgert_cgls without any device allocation: no counterpart in the reference.
  * For callers that cannot allocate device memory (code under HIP graph

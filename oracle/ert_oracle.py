@@ -226,6 +226,24 @@ def geometric_factors(sensors, abmn, dim):
 
 
 # --------------------------------------------------------------------------- mesh-side set-up
+def find_cell(mesh, p, tol=1e-12):
+    """The cell holding position p (Mesh::findCell, mesh.cpp) and the P1 shape
+    functions there (barycentric coordinates, MeshEntity::N(rst(p)));
+    (-1, None) outside the mesh.  A position on a face shared by two cells
+    goes to the lower cell index."""
+    dim = mesh.dim()
+    P = mesh.nodes[mesh.cells][:, :, :dim]
+    x = np.asarray(p, np.float64)[:dim]
+    T = np.transpose(P[:, 1:, :] - P[:, :1, :], (0, 2, 1))      # (C, d, d): columns = edges
+    lam = np.linalg.solve(T, (x - P[:, 0, :])[..., None])[..., 0]
+    N = np.concatenate([1.0 - lam.sum(-1, keepdims=True), lam], axis=1)
+    inside = np.nonzero(np.all(N >= -tol, axis=1))[0]
+    if len(inside) == 0:
+        return -1, None
+    c = int(inside[0])
+    return c, N[c]
+
+
 class Setup:
     """Mesh/data dependent state of DCMultiElectrodeModelling
     (updateMeshDependency_ dcfemmodelling.cpp:711-773, searchElectrodes_
@@ -266,6 +284,10 @@ class Setup:
         cand = list(mesh.findNodesIdxByMarker(-99))
         nodes = mesh.nodes
         self.elecNode = np.empty(len(ePos), np.int64)
+        # free electrodes (no bypass group, no marker -99 node within 0.01):
+        # ElectrodeShapeEntity of the cell holding the position (:979-985)
+        self.elecCell = np.full(len(ePos), -1, np.int64)
+        self.elecShape = np.zeros((len(ePos), dim + 1))
         for i, p in enumerate(ePos):
             hit = None
             for m in free:
@@ -281,36 +303,82 @@ class Setup:
                     hit = j
                     break
             if hit is None:
-                raise NotImplementedError("electrode not on a marker -99 node")
+                c, N = find_cell(mesh, p)
+                if c < 0:
+                    raise RuntimeError("There is a requested electrode that does not match the given mesh.")
+                self.elecNode[i] = -1
+                self.elecCell[i] = c
+                self.elecShape[i] = N
+                continue
             self.elecNode[i] = cand.pop(hit)
-        self.elecPos = nodes[self.elecNode]
+        self.elecPos = np.where((self.elecNode >= 0)[:, None], nodes[np.maximum(self.elecNode, 0)], ePos)
         self.sourceCenter = self.elecPos.sum(0) / len(self.elecPos)
-        # cells around each electrode node, nearest neighbour distance
+        # cells around each electrode node, nearest neighbour distance; a free
+        # electrode within 1e-4 of a node of its cell has its singular value
+        # at that node (ElectrodeShapeEntity::setSingValue, electrode.cpp:207-249)
         self.nodeCells = []
-        self.minRadius = np.empty(len(self.elecNode))
+        self.singNode = np.full(len(self.elecNode), -1, np.int64)
+        self.minRadius = np.zeros(len(self.elecNode))
         for i, nid in enumerate(self.elecNode):
+            if nid < 0:
+                self.nodeCells.append(np.array([self.elecCell[i]]))
+                cn = mesh.cells[self.elecCell[i]]
+                d = np.sqrt(((nodes[cn] - self.elecPos[i]) ** 2).sum(-1))
+                close = np.nonzero(d < 1e-4)[0]
+                if len(close) == 0:
+                    continue
+                nid = int(cn[close[-1]])
+            self.singNode[i] = nid
             cs = np.nonzero((mesh.cells == nid).any(axis=1))[0]
-            self.nodeCells.append(cs)
+            if self.elecNode[i] >= 0:
+                self.nodeCells.append(cs)
             nb = np.unique(mesh.cells[cs].ravel())
             nb = nb[nb != nid]
             self.minRadius[i] = np.sqrt(((nodes[nb] - nodes[nid]) ** 2).sum(-1)).min()
 
     def geom_mean_rho(self, i, rho):
-        """ElectrodeShapeNode::geomMeanCellAttributes (electrode.cpp:102-127)."""
+        """ElectrodeShapeNode::geomMeanCellAttributes (electrode.cpp:102-127);
+        a free electrode: the attribute of its cell
+        (ElectrodeShapeEntity::geomMeanCellAttributes, electrode.cpp:252-266)."""
+        if self.elecNode[i] < 0:
+            return float(rho[self.elecCell[i]])
         v = rho[self.nodeCells[i]]
         return float(np.exp(np.log(v).sum() / len(v)))
 
     def primary(self, i, k):
         """exactDCSolution(mesh, elec, k, surfaceZ, setSingValue=true)
-        (bertMisc.cpp:230-257) + ElectrodeShapeNode::setSingValue with
-        scale 0 (electrode.cpp:154-189)."""
+        (bertMisc.cpp:230-257) + ElectrodeShapeNode::setSingValue /
+        ElectrodeShapeEntity::setSingValue with scale 0 (electrode.cpp:154-249)."""
         u = exact_dc(self.mesh.nodes, self.elecPos[i], k, self.surfaceZ, 0.0)
+        sn = self.singNode[i]
+        if sn < 0:
+            return u
         r = self.minRadius[i]
         if k > 0.0:
-            u[self.elecNode[i]] = bessel_k0(np.array([r / 6.0 * k]))[0] / PI
+            u[sn] = bessel_k0(np.array([r / 6.0 * k]))[0] / PI
         else:
-            u[self.elecNode[i]] = 1.0 / (2.0 * PI * r / 2.0)
+            u[sn] = 1.0 / (2.0 * PI * r / 2.0)
         return u
+
+    def source_rhs(self, i, n):
+        """Point-source right-hand side of electrode i
+        (ElectrodeShapeNode::assembleRHS, electrode.cpp:133-151;
+        ElectrodeShapeEntity::assembleRHS: shape functions at the position,
+        electrode.cpp:274-287)."""
+        rhs = np.zeros(n)
+        if self.elecNode[i] >= 0:
+            rhs[self.elecNode[i]] = 1.0
+        else:
+            rhs[self.mesh.cells[self.elecCell[i]]] = self.elecShape[i]
+        return rhs
+
+    def pot(self, i, sol):
+        """Potential of the fields `sol` [.., n] at electrode i
+        (ElectrodeShapeNode::pot, electrode.cpp:129-131;
+        ElectrodeShapeEntity::pot -> MeshEntity::pot, meshentities.cpp:347-349)."""
+        if self.elecNode[i] >= 0:
+            return sol[..., self.elecNode[i]]
+        return sol[..., self.mesh.cells[self.elecCell[i]]] @ self.elecShape[i]
 
 
 # --------------------------------------------------------------------------- a4/a5
@@ -616,8 +684,7 @@ class Forward:
                 S = S.tocsr()
             lu = spla.splu(S.tocsc())
             for i in range(self.nE):
-                rhs = np.zeros(n, dt)
-                rhs[self.setup.elecNode[i]] = 1.0
+                rhs = self.setup.source_rhs(i, n).astype(dt)
                 U[i + ki * self.nE] = lu.solve(rhs)
         return U
 
@@ -629,7 +696,7 @@ class Forward:
 
     def electrode_map(self, sol):
         """DataMap::collect (datamap.cpp:57-98): map[source][electrode]."""
-        return sol[:, self.setup.elecNode]
+        return np.stack([self.setup.pot(f, sol) for f in range(self.nE)], axis=1)
 
     def geometric_factors(self):
         return geometric_factors(self.data.sensorPositions(), self.data.abmn(), self.mesh.dim())

@@ -53,6 +53,9 @@ def lib():
         L.ref_session_data_k.argtypes = [ctypes.c_void_p, _D]
         L.ref_session_jacobian.argtypes = [ctypes.c_void_p, _D, ctypes.c_int, _D, _I, _I]
         L.ref_session_set_subpots.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D]
+        L.ref_session_invert.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, ctypes.c_int, _D, ctypes.c_double,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_int, _I, _I, _D, _D, _D, _D, _I]
         L.ref_assemble.argtypes = [ctypes.c_void_p, _D, ctypes.c_double, _D, _I, _I, _I, _D]
         L.ref_fill_matrix.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _I, _I, _I, _D]
         L.ref_mixed_bc.argtypes = [ctypes.c_void_p, _D, ctypes.c_double, _D]
@@ -226,6 +229,22 @@ class Session:
         out = np.empty((r.value, c.value))
         lib().ref_session_subpots(self.h, out.ctypes.data_as(_D))
         return out
+
+    def invert(self, data, err, start, C, lam=20.0, max_iter=3, line_search=True, max_cgls=200, cgls_tol=-1.0,
+               m_lower=0.0, m_upper=0.0):
+        """The reference's RInversion::run on this operator (see ref_driver.cpp):
+        returns (model, response, chi2, iterations)."""
+        import scipy.sparse as sp
+        C = sp.csr_matrix(C)
+        d, pd = _d(data); e, pe = _d(err); m0, pm = _d(start)
+        rp, prp = _i(C.indptr); ci, pci = _i(C.indices); cv, pcv = _d(C.data)
+        out_m = np.empty(len(m0)); out_r = np.empty(len(d)); chi = np.empty(1)
+        it = ctypes.c_int()
+        _check(lib().ref_session_invert(self.h, len(d), pd, pe, len(m0), pm, lam, max_iter, 1 if line_search else 0,
+                                        max_cgls, cgls_tol, m_lower, m_upper, C.shape[0], prp, pci, pcv,
+                                        out_m.ctypes.data_as(_D), out_r.ctypes.data_as(_D),
+                                        chi.ctypes.data_as(_D), ctypes.byref(it)))
+        return out_m, out_r, float(chi[0]), it.value
 
     def set_subpotentials(self, U):
         """Hand the core per-k potentials [nK * nE, n] (rows source + nE * kIdx)

@@ -24,6 +24,8 @@
 #include "bert/dcfemmodelling.h"
 #include "bert/bertDataContainer.h"
 #include "bert/bertMisc.h"
+#include "inversion.h"
+#include "trans.h"
 
 #include <algorithm>
 #include <cmath>
@@ -457,6 +459,51 @@ int ref_session_set_subpots(void * h, int rows, int cols, const double * U) {
         for (int i = 0; i < rows; ++i)
             for (int j = 0; j < cols; ++j) s->subpots[i][j] = U[(size_t)i * cols + j];
         s->fop->collectSubPotentials(s->subpots);
+    });
+}
+
+// RInversion::run (inversion.cpp:135-246) on the session's operator: data
+// `dvals` with relative errors `err`, log data / log model transforms
+// (TransLogLU with the given bounds; upper 0 = TransLog), the constraint
+// matrix given as CSR (installed with ModellingBase::setConstraints after
+// the inversion is built, so checkConstraints keeps it), constraint and model
+// weights 1, fixed lambda, maxIter iterations without the chi^2 = 1 stop and
+// the 2 % abort, line search on or off.  Returns the final model and
+// response and per-iteration chi^2 (phiD / nData) in chi2[0..maxIter].
+int ref_session_invert(void * h, int nd, const double * dvals, const double * err, int nm, const double * start,
+                       double lambda, int maxIter, int lineSearch, int maxCGLS, double cglsTol, double mLower,
+                       double mUpper,
+                       int nC, const int * crp, const int * cci, const double * cval, double * outModel,
+                       double * outResp, double * chi2, int * iters) {
+    Session * s = (Session *)h;
+    return guarded([&] {
+        RVector d(nd), e(nd), m0(nm);
+        for (int i = 0; i < nd; ++i) { d[i] = dvals[i]; e[i] = err[i]; }
+        for (int i = 0; i < nm; ++i) m0[i] = start[i];
+        TransLogLU<RVector> tD(0.0, 0.0), tM(mLower, mUpper);
+        RInversion inv(d, *s->fop, tD, tM, false, false);
+        RSparseMapMatrix C(nC, nm);
+        for (int r = 0; r < nC; ++r)
+            for (int q = crp[r]; q < crp[r + 1]; ++q) C.setVal(r, cci[q], cval[q]);
+        s->fop->setConstraints(&C);
+        inv.setRelativeError(e);
+        inv.setLambda(lambda);
+        inv.setMaxIter(maxIter);
+        inv.setLineSearch(lineSearch != 0);
+        inv.setMaxCGLSIter(maxCGLS);
+        inv.setCGLSTolerance(cglsTol);
+        inv.stopAtChi1(false);
+        inv.setDeltaPhiAbortPercent(-1e300);
+        inv.setModel(m0);
+        inv.setCWeight(RVector(nC, 1.0));
+        inv.setMWeight(RVector(nm, 1.0));
+        const RVector & m = inv.run();
+        for (int i = 0; i < nm; ++i) outModel[i] = m[i];
+        const RVector & r = inv.response();
+        for (int i = 0; i < nd; ++i) outResp[i] = r[i];
+        *iters = inv.iter();
+        if (chi2) chi2[0] = inv.chi2();
+        s->fop->setConstraints(nullptr);
     });
 }
 

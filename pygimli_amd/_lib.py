@@ -32,6 +32,7 @@ EXPORTS = (
     "pgert_mlp_collect", "pgert_mlp_predict", "pgert_mlp_train_step", "pgert_set_coarse",
     "pgert_batch_forward", "pgert_batch_jacobian", "pgert_set_constraints", "pgert_cgls", "pgert_cgls_noalloc",
     "pgert_forward_complex", "pgert_jacobian_complex", "pgert_forward_total", "pgert_set_bypass",
+    "pgert_trans", "pgert_gn_phi", "pgert_set_electrodes_free",
 )
 
 
@@ -76,6 +77,8 @@ def load():
     L.pgert_set_mesh.argtypes = [vp, ctypes.c_int, ctypes.c_int, _c_double_p, ctypes.c_int, _c_int_p,
                                  _c_int_p, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p]
     L.pgert_set_electrodes.argtypes = [vp, ctypes.c_int, _c_int_p, _c_double_p, ctypes.c_double]
+    L.pgert_set_electrodes_free.argtypes = [vp, ctypes.c_int, _c_int_p, _c_int_p, _c_double_p, _c_double_p,
+                                            ctypes.c_double]
     L.pgert_set_wavenumbers.argtypes = [vp, ctypes.c_int, _c_double_p, _c_double_p]
     L.pgert_set_data.argtypes = [vp, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_int_p, _c_double_p]
     L.pgert_info.argtypes = [vp, _c_int_p, _c_int_p]
@@ -92,6 +95,10 @@ def load():
     L.pgert_cgls.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp, vp, vp,
                              ctypes.c_double, vp, ctypes.c_int, ctypes.c_double, _c_int_p, _c_double_p]
     L.pgert_cgls_noalloc.argtypes = L.pgert_cgls.argtypes
+    L.pgert_trans.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_double, ctypes.c_double,
+                              ctypes.c_double, vp]
+    L.pgert_gn_phi.argtypes = [vp, ctypes.c_int, _c_double_p, ctypes.c_int, vp, vp, vp, vp, ctypes.c_double,
+                               ctypes.c_double, ctypes.c_int, vp, vp, vp, vp, _c_double_p, _c_double_p]
     L.pgert_set_bypass.argtypes = [vp, ctypes.c_int, _c_int_p, _c_int_p, _c_double_p]
     L.pgert_forward_total.argtypes = [vp, vp, ctypes.POINTER(SolveOpts), vp, ctypes.POINTER(SolveStats)]
     L.pgert_forward_complex.argtypes = [vp, vp, vp, ctypes.POINTER(SolveOpts), vp, vp, ctypes.POINTER(SolveStats)]

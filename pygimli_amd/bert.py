@@ -134,12 +134,27 @@ def geometric_factors(sensors, abmn, dim=3):
         return 1.0 / (G(a, m) - G(b, m) - G(a, n) + G(b, n))
 
 
+def _find_cell(mesh, p, tol=1e-12):
+    """Index of the cell holding position p (Mesh::findCell), -1 outside the
+    mesh; on a face shared by two cells the lower cell index."""
+    dim = mesh.dim()
+    X = mesh.positions()[np.asarray(mesh.cells)][:, :, :dim]
+    E = np.transpose(X[:, 1:, :] - X[:, :1, :], (0, 2, 1))
+    b = np.asarray(p, np.float64)[:dim] - X[:, 0, :]
+    lam = np.linalg.solve(E, b[..., None])[..., 0]
+    inside = np.all(lam >= -tol, axis=1) & (lam.sum(1) <= 1.0 + tol)
+    hit = np.nonzero(inside)[0]
+    return int(hit[0]) if len(hit) else -1
+
+
 class MeshAnalysis:
     """Boundary/topography analysis and electrode search
     (updateMeshDependency_ dcfemmodelling.cpp:711-773, searchElectrodes_ :790-1073)
-    for node electrodes (marker -99) and bypass electrodes: nodes sharing a
+    for node electrodes (marker -99), bypass electrodes (nodes sharing a
     marker <= -10000 form one electrode at the group's first node, joined
-    node to node by bypass resistors (:815-826, :886-897)."""
+    node to node by bypass resistors, :815-826, :886-897) and free electrodes
+    (a sensor matching neither lies in a cell: elecNode -1, elecCell, the
+    position in elecPos, :979-985)."""
 
     def __init__(self, mesh, sensors):
         dim = mesh.dim()
@@ -176,6 +191,8 @@ class MeshAnalysis:
         cand = list(mesh.findNodesIdxByMarker(-99))
         nodes = mesh.positions()
         self.elecNode = np.empty(len(pos), np.int32)
+        self.elecCell = np.full(len(pos), -1, np.int32)
+        self.freePos = np.zeros((len(pos), 3))
         for i, p in enumerate(pos):
             hit = None
             for m in free:
@@ -193,11 +210,18 @@ class MeshAnalysis:
                 if len(close):
                     hit = int(close[0])
             if hit is None:
-                raise NotImplementedError(
-                    f"sensor {i} at {p} matches no electrode node (marker -99); "
-                    "free (in-cell) electrodes are not supported")
+                # a free electrode in the cell holding the position
+                # (ElectrodeShapeEntity, searchElectrodes_ :979-985)
+                c = _find_cell(mesh, p)
+                if c < 0:
+                    raise RuntimeError(f"sensor {i} at {p}: there is a requested electrode that does not match "
+                                       "the given mesh")
+                self.elecNode[i] = -1
+                self.elecCell[i] = c
+                self.freePos[i] = p
+                continue
             self.elecNode[i] = cand.pop(hit)
-        self.elecPos = nodes[self.elecNode]
+        self.elecPos = np.where((self.elecNode >= 0)[:, None], nodes[np.maximum(self.elecNode, 0)], self.freePos)
         self.sourceCenter = self.elecPos.sum(0) / len(self.elecPos)
 
     def bypass_pairs(self, resistances=None):

@@ -284,81 +284,158 @@ int pgert_set_mesh(pgert_ctx * c, int dim, int n_nodes, const double * xyz, int 
     });
 }
 
+// Electrodes on nodes (elec_node[e] >= 0) or free in a cell (elec_node[e] = -1,
+// elec_cell[e] the cell, elec_pos the position): searchElectrodes_
+// (dcfemmodelling.cpp:790-1073) with ElectrodeShapeNode / ElectrodeShapeEntity.
+static void set_electrodes(pgert_ctx * c, int n_elec, const int * elec_node, const int * elec_cell,
+                           const double * elec_pos, const double * source_center, double surface_z) {
+    PG_REQUIRE(c->n > 0, "pgert_set_electrodes: no mesh");
+    PG_REQUIRE(n_elec > 0 && elec_node && source_center, "pgert_set_electrodes: bad arguments");
+    PG_REQUIRE(n_elec <= 256, "pgert_set_electrodes: at most 256 current electrodes per context");
+    std::vector<double> hx((size_t)c->n * 3);
+    PG_CHECK(hipMemcpy(hx.data(), c->xyz.p, hx.size() * 8, hipMemcpyDeviceToHost));
+    std::vector<int> hc((size_t)c->C * c->nv);
+    PG_CHECK(hipMemcpy(hc.data(), c->cells.p, hc.size() * 4, hipMemcpyDeviceToHost));
+    const int dim = c->dim, nv = c->nv;
+    c->nE = n_elec;
+    c->hElecNode.assign(elec_node, elec_node + n_elec);
+    auto dist = [&](const double * a, const double * b) {
+        double d2 = 0;
+        for (int q = 0; q < 3; ++q) d2 += (a[q] - b[q]) * (a[q] - b[q]);
+        return std::sqrt(d2);
+    };
+    // the singular node of every electrode: its node, or for a free electrode
+    // the last node of its cell within 1e-4 (ElectrodeShapeEntity::setSingValue,
+    // electrode.cpp:207-227); shape nodes and weights; electrode positions
+    std::vector<int> sing(n_elec, -1), shn((size_t)n_elec * SHAPE_NODES, -1);
+    std::vector<double> shw((size_t)n_elec * SHAPE_NODES, 0.0), pos((size_t)3 * n_elec);
+    for (int e = 0; e < n_elec; ++e) {
+        if (elec_node[e] >= 0) {
+            PG_REQUIRE(elec_node[e] < c->n, "pgert_set_electrodes: node out of range");
+            sing[e] = elec_node[e];
+            shn[(size_t)SHAPE_NODES * e] = elec_node[e];
+            shw[(size_t)SHAPE_NODES * e] = 1.0;
+            for (int q = 0; q < 3; ++q) pos[3 * e + q] = hx[3 * (size_t)elec_node[e] + q];
+            continue;
+        }
+        PG_REQUIRE(elec_cell && elec_pos, "pgert_set_electrodes: free electrode without cell and position");
+        const int cc = elec_cell[e];
+        PG_REQUIRE(cc >= 0 && cc < c->C, "pgert_set_electrodes: free electrode cell out of range");
+        const double * p = elec_pos + 3 * (size_t)e;
+        for (int q = 0; q < 3; ++q) pos[3 * e + q] = p[q];
+        // shape functions at p: barycentric coordinates of the simplex
+        // (MeshEntity::N(shape().rst(p))), by Cramer's rule on the edge matrix
+        const int * cn = hc.data() + (size_t)cc * nv;
+        const double * x0 = hx.data() + 3 * (size_t)cn[0];
+        double T[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, r[3] = {0, 0, 0}, lam[3] = {0, 0, 0};
+        for (int q = 0; q < dim; ++q) {
+            for (int j = 0; j < dim; ++j) T[q][j] = hx[3 * (size_t)cn[j + 1] + q] - x0[q];
+            r[q] = p[q] - x0[q];
+        }
+        if (dim == 2) {
+            const double det = T[0][0] * T[1][1] - T[0][1] * T[1][0];
+            PG_REQUIRE(std::fabs(det) > 0.0, "pgert_set_electrodes: degenerate cell");
+            lam[0] = (r[0] * T[1][1] - T[0][1] * r[1]) / det;
+            lam[1] = (T[0][0] * r[1] - r[0] * T[1][0]) / det;
+        } else {
+            auto det3 = [](const double (&M)[3][3]) {
+                return M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) -
+                       M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+                       M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
+            };
+            const double det = det3(T);
+            PG_REQUIRE(std::fabs(det) > 0.0, "pgert_set_electrodes: degenerate cell");
+            for (int j = 0; j < 3; ++j) {
+                double M[3][3];
+                for (int q = 0; q < 3; ++q)
+                    for (int l = 0; l < 3; ++l) M[q][l] = l == j ? r[q] : T[q][l];
+                lam[j] = det3(M) / det;
+            }
+        }
+        double l0 = 1.0;
+        for (int j = 0; j < dim; ++j) l0 -= lam[j];
+        for (int j = 0; j < nv; ++j) {
+            shn[(size_t)SHAPE_NODES * e + j] = cn[j];
+            shw[(size_t)SHAPE_NODES * e + j] = j == 0 ? l0 : lam[j - 1];
+            if (dist(hx.data() + 3 * (size_t)cn[j], p) < 1e-4) sing[e] = cn[j];
+        }
+    }
+    // cells around each singular node (rho_s for node electrodes, minimum
+    // radius for every singular node); a free electrode's rho_s cell is its own
+    std::vector<int> headE(c->n, -1), nextE(n_elec, -1);
+    for (int e = n_elec - 1; e >= 0; --e)
+        if (sing[e] >= 0) {
+            nextE[e] = headE[sing[e]];
+            headE[sing[e]] = e;
+        }
+    std::vector<std::vector<int>> cl(n_elec);
+    for (int cc = 0; cc < c->C; ++cc)
+        for (int i = 0; i < nv; ++i)
+            for (int e = headE[hc[(size_t)cc * nv + i]]; e >= 0; e = nextE[e]) cl[e].push_back(cc);
+    std::vector<int> ptr(1, 0), lst;
+    std::vector<double> rmin(n_elec, 0.0);
+    for (int e = 0; e < n_elec; ++e) {
+        if (sing[e] >= 0) {
+            PG_REQUIRE(!cl[e].empty(), "pgert_set_electrodes: electrode node has no cell");
+            // nearest neighbour distance (ElectrodeShapeNode::setSingValue, electrode.cpp:157-171)
+            double rr = DBL_MAX;
+            const int v = sing[e];
+            for (int cc : cl[e])
+                for (int i = 0; i < nv; ++i) {
+                    const int u = hc[(size_t)cc * nv + i];
+                    if (u != v) rr = std::min(rr, dist(hx.data() + 3 * (size_t)u, hx.data() + 3 * (size_t)v));
+                }
+            rmin[e] = rr;
+        }
+        if (elec_node[e] >= 0)
+            lst.insert(lst.end(), cl[e].begin(), cl[e].end());
+        else
+            lst.push_back(elec_cell[e]);
+        ptr.push_back((int)lst.size());
+    }
+    hipStream_t s = c->stream;
+    c->elecNode.upload(c->hElecNode, s);
+    c->elecPos.upload(pos, s);
+    c->minRadius.upload(rmin, s);
+    c->singNode.upload(sing, s);
+    c->shapeNode.upload(shn, s);
+    c->shapeW.upload(shw, s);
+    c->ecPtr.upload(ptr, s);
+    c->ecCell.upload(lst, s);
+    for (int q = 0; q < 3; ++q) c->srcCenter[q] = source_center[q];
+    c->surfaceZ = surface_z;
+    // feature length scale: 4 x median nearest-neighbour electrode distance
+    std::vector<double> nn;
+    for (int e = 0; e < n_elec; ++e) {
+        double best = DBL_MAX;
+        for (int f = 0; f < n_elec; ++f) {
+            if (f == e) continue;
+            const double d = dist(&pos[3 * e], &pos[3 * f]);
+            if (d > 0) best = std::min(best, d);
+        }
+        if (best < DBL_MAX) nn.push_back(best);
+    }
+    if (!nn.empty()) {
+        std::nth_element(nn.begin(), nn.begin() + nn.size() / 2, nn.end());
+        c->featureScale = (float)(4.0 * nn[nn.size() / 2]);
+    }
+    c->s1Valid = c->primValid = c->haveU = c->haveUc = c->srState = false;
+}
+
 int pgert_set_electrodes(pgert_ctx * c, int n_elec, const int * elec_node,
                          const double * source_center, double surface_z) {
     return guarded(c, [&] {
-        PG_REQUIRE(c->n > 0, "pgert_set_electrodes: no mesh");
-        PG_REQUIRE(n_elec > 0 && elec_node && source_center, "pgert_set_electrodes: bad arguments");
-        PG_REQUIRE(n_elec <= 256, "pgert_set_electrodes: at most 256 current electrodes per context");
-        std::vector<double> hx((size_t)c->n * 3);
-        PG_CHECK(hipMemcpy(hx.data(), c->xyz.p, hx.size() * 8, hipMemcpyDeviceToHost));
-        std::vector<int> hc((size_t)c->C * c->nv);
-        PG_CHECK(hipMemcpy(hc.data(), c->cells.p, hc.size() * 4, hipMemcpyDeviceToHost));
-        c->nE = n_elec;
-        c->hElecNode.assign(elec_node, elec_node + n_elec);
-        // per-node linked list of the electrodes sitting on it
-        std::vector<int> headE(c->n, -1), nextE(n_elec, -1);
-        for (int e = n_elec - 1; e >= 0; --e) {
-            PG_REQUIRE(elec_node[e] >= 0 && elec_node[e] < c->n, "pgert_set_electrodes: node out of range");
-            nextE[e] = headE[elec_node[e]];
-            headE[elec_node[e]] = e;
-        }
-        std::vector<std::vector<int>> cl(n_elec);
-        for (int cc = 0; cc < c->C; ++cc)
-            for (int i = 0; i < c->nv; ++i)
-                for (int e = headE[hc[(size_t)cc * c->nv + i]]; e >= 0; e = nextE[e])
-                    cl[e].push_back(cc);
-        std::vector<int> ptr(1, 0), lst;
-        std::vector<double> pos, rmin;
-        for (int e = 0; e < n_elec; ++e) {
-            PG_REQUIRE(!cl[e].empty(), "pgert_set_electrodes: electrode node has no cell");
-            lst.insert(lst.end(), cl[e].begin(), cl[e].end());
-            ptr.push_back((int)lst.size());
-            int v = elec_node[e];
-            for (int q = 0; q < 3; ++q) pos.push_back(hx[3 * (size_t)v + q]);
-            // nearest neighbour distance (ElectrodeShapeNode::setSingValue, electrode.cpp:157-171)
-            double r = DBL_MAX;
-            for (int cc : cl[e])
-                for (int i = 0; i < c->nv; ++i) {
-                    int u = hc[(size_t)cc * c->nv + i];
-                    if (u == v) continue;
-                    double d2 = 0;
-                    for (int q = 0; q < 3; ++q) {
-                        double d = hx[3 * (size_t)u + q] - hx[3 * (size_t)v + q];
-                        d2 += d * d;
-                    }
-                    r = std::min(r, std::sqrt(d2));
-                }
-            rmin.push_back(r);
-        }
-        hipStream_t s = c->stream;
-        c->elecNode.upload(c->hElecNode, s);
-        c->elecPos.upload(pos, s);
-        c->minRadius.upload(rmin, s);
-        c->ecPtr.upload(ptr, s);
-        c->ecCell.upload(lst, s);
-        for (int q = 0; q < 3; ++q) c->srcCenter[q] = source_center[q];
-        c->surfaceZ = surface_z;
-        // feature length scale: 4 x median nearest-neighbour electrode distance
-        std::vector<double> nn;
-        for (int e = 0; e < n_elec; ++e) {
-            double best = DBL_MAX;
-            for (int f = 0; f < n_elec; ++f) {
-                if (f == e) continue;
-                double d2 = 0;
-                for (int q = 0; q < 3; ++q) {
-                    double d = pos[3 * e + q] - pos[3 * f + q];
-                    d2 += d * d;
-                }
-                if (d2 > 0) best = std::min(best, std::sqrt(d2));
-            }
-            if (best < DBL_MAX) nn.push_back(best);
-        }
-        if (!nn.empty()) {
-            std::nth_element(nn.begin(), nn.begin() + nn.size() / 2, nn.end());
-            c->featureScale = (float)(4.0 * nn[nn.size() / 2]);
-        }
-        c->s1Valid = c->primValid = c->haveU = c->haveUc = c->srState = false;
+        for (int e = 0; elec_node && e < n_elec; ++e)
+            PG_REQUIRE(elec_node[e] >= 0, "pgert_set_electrodes: negative node (free electrodes: pgert_set_electrodes_free)");
+        set_electrodes(c, n_elec, elec_node, nullptr, nullptr, source_center, surface_z);
+        sync(*c);
+    });
+}
+
+int pgert_set_electrodes_free(pgert_ctx * c, int n_elec, const int * elec_node, const int * elec_cell,
+                              const double * elec_pos, const double * source_center, double surface_z) {
+    return guarded(c, [&] {
+        set_electrodes(c, n_elec, elec_node, elec_cell, elec_pos, source_center, surface_z);
         sync(*c);
     });
 }
@@ -984,6 +1061,19 @@ int pgert_cgls_noalloc(pgert_ctx * c, const void * J, int dtype, int n_data, int
                                     max_iter, tol);
         if (iterations) *iterations = it;
         if (norm_r2) *norm_r2 = c->cglsNormR2;
+    });
+}
+
+int pgert_trans(pgert_ctx * c, int op, int n, const double * a, const double * b, double tau, double lower,
+                double upper, double * out) {
+    return guarded(c, [&] { gn_trans(*c, op, n, a, b, tau, lower, upper, out); });
+}
+
+int pgert_gn_phi(pgert_ctx * c, int n_tau, const double * taus, int n_model, const double * model, const double * dm,
+                 const double * mw, const double * cw, double lower, double upper, int n_data, const double * data,
+                 const double * resp, const double * dd, const double * derr, double * phiD, double * phiM) {
+    return guarded(c, [&] {
+        gn_phi(*c, n_tau, taus, n_model, model, dm, mw, cw, lower, upper, n_data, data, resp, dd, derr, phiD, phiM);
     });
 }
 

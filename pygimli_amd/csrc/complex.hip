@@ -91,15 +91,20 @@ __global__ void k_assemble_boundary_c(int nbs, const int * __restrict__ bslot, c
     vim[bslot[t]] += ai;
 }
 
-// unit point source of electrode e at its node, every wavenumber
-// (ElectrodeShapeNode::assembleRHS, core/src/bert/electrode.cpp:133-151)
-__global__ void k_point_rhs(long total, int n, int nE, const int * __restrict__ elecNode,
-                            double * __restrict__ B) {
+// unit point source of electrode e, every wavenumber: 1 at its node
+// (ElectrodeShapeNode::assembleRHS, core/src/bert/electrode.cpp:133-151) or
+// the shape functions at its position on its cell's nodes for a free one
+// (ElectrodeShapeEntity::assembleRHS, :274-287)
+__global__ void k_point_rhs(long total, int n, int nE, const int * __restrict__ shapeNode,
+                            const double * __restrict__ shapeW, double * __restrict__ B) {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
     const int e = (int)(t % nE);
     const int node = (int)((t / nE) % n);
-    B[t] = node == elecNode[e] ? 1.0 : 0.0;
+    double v = 0.0;
+    for (int j = 0; j < SHAPE_NODES; ++j)
+        if (shapeNode[SHAPE_NODES * e + j] == node) v = shapeW[SHAPE_NODES * e + j];
+    B[t] = v;
 }
 
 // out[k] = base[k] + sign * A_k X[k] for node-major blocks [k][node][e]
@@ -191,15 +196,21 @@ __global__ void k_response_c(int D, int nE, const int * __restrict__ abmn, const
     out_im[i] = u[1] * kfac[i];
 }
 
-// map[e][f] = sum_k w_k U[k][node_f][e]
+// map[e][f] = sum_j N_fj sum_k w_k U[k][node_fj][e] (as k_electrode_map)
 __global__ void k_electrode_map_c(int n, int nE, int nK, const double * __restrict__ U,
-                                  const double * __restrict__ w, const int * __restrict__ elecNode,
-                                  double * __restrict__ emap) {
+                                  const double * __restrict__ w, const int * __restrict__ shapeNode,
+                                  const double * __restrict__ shapeW, double * __restrict__ emap) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nE * nE) return;
     int e = t / nE, f = t % nE;
     double s = 0.0;
-    for (int kk = 0; kk < nK; ++kk) s += w[kk] * U[((size_t)kk * n + elecNode[f]) * nE + e];
+    for (int j = 0; j < SHAPE_NODES; ++j) {
+        const int v = shapeNode[SHAPE_NODES * f + j];
+        if (v < 0) break;
+        double u = 0.0;
+        for (int kk = 0; kk < nK; ++kk) u += w[kk] * U[((size_t)kk * n + v) * nE + e];
+        s += u * shapeW[SHAPE_NODES * f + j];
+    }
     emap[t] = s;
 }
 
@@ -266,7 +277,7 @@ void launch_assemble_complex(Ctx & c, const double * rre, const double * rim, do
 void launch_point_rhs(Ctx & c, double * B) {
     const long total = (long)c.nK * c.n * c.nE;
     hipLaunchKernelGGL(k_point_rhs, dim3(div_up(total, 256)), dim3(256), 0, c.stream, total, c.n, c.nE,
-                       c.elecNode.p, B);
+                       c.shapeNode.p, c.shapeW.p, B);
     PG_CHECK(hipGetLastError());
 }
 
@@ -293,10 +304,10 @@ void launch_response_complex(Ctx & c, const double * Ure, const double * Uim, do
     c.emap.alloc((size_t)2 * c.nE * c.nE);
     const int ne2 = c.nE * c.nE;
     hipLaunchKernelGGL(k_electrode_map_c, dim3(div_up(ne2, 256)), dim3(256), 0, c.stream, c.n, c.nE, c.nK, Ure,
-                       c.dw.p, c.elecNode.p, c.emap.p);
+                       c.dw.p, c.shapeNode.p, c.shapeW.p, c.emap.p);
     PG_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_electrode_map_c, dim3(div_up(ne2, 256)), dim3(256), 0, c.stream, c.n, c.nE, c.nK, Uim,
-                       c.dw.p, c.elecNode.p, c.emap.p + ne2);
+                       c.dw.p, c.shapeNode.p, c.shapeW.p, c.emap.p + ne2);
     PG_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_response_c, dim3(div_up(c.D, 256)), dim3(256), 0, c.stream, c.D, c.nE, c.abmn.p,
                        c.kfac.p, c.emap.p, c.emap.p + ne2, out_re, out_im);

@@ -25,6 +25,8 @@
 
 namespace pgert {
 
+constexpr int SHAPE_NODES = 4;  // nodes of a P1 simplex (an electrode's shape nodes)
+
 struct MlpState;
 
 // Owned device allocation.
@@ -112,11 +114,14 @@ struct Ctx {
 
     // ---- electrodes
     int nE = 0;
-    std::vector<int> hElecNode;
+    std::vector<int> hElecNode;  // -1: free electrode
     DBuf<int> elecNode;
     DBuf<double> elecPos;    // nE * 3
     DBuf<double> minRadius;  // nE
-    DBuf<int> ecPtr, ecCell; // cells around each electrode node
+    DBuf<int> singNode;      // nE: node with the singular primary value, -1: none
+    DBuf<int> shapeNode;     // nE * SHAPE_NODES: the electrode's nodes (-1 padded)
+    DBuf<double> shapeW;     // nE * SHAPE_NODES: their shape-function weights
+    DBuf<int> ecPtr, ecCell; // cells whose attributes give rho_s (around the node / the free electrode's cell)
     double srcCenter[3] = {0, 0, 0};
     double surfaceZ = 0;
     float featureScale = 1.0f;  // length scale of the warm-start features
@@ -277,6 +282,12 @@ int cgls(Ctx & c, const void * J, int dtype, int nData, int nModel, const double
 int cgls_noalloc(Ctx & c, const void * J, int dtype, int nData, int nModel, const double * dW, const double * b,
                  double * x, const double * wc, const double * wm, const double * tm, const double * td, double lambda,
                  const double * roughness, int maxIter, double tol);
+// Gauss-Newton loop: logarithmic transforms and the line-search objective (gn.hip)
+void gn_trans(Ctx & c, int op, int n, const double * a, const double * b, double tau, double lower, double upper,
+              double * out);
+void gn_phi(Ctx & c, int nTau, const double * taus, int nModel, const double * m, const double * dm, const double * mw,
+            const double * cw, double lower, double upper, int nData, const double * d, const double * r,
+            const double * dd, const double * e, double * phiD, double * phiM);
 // device address of the allocation-free CGLS's workspace, looked up once per
 // device (the device must be current)
 void cgls_scratch_init(int device);

@@ -37,8 +37,10 @@ __device__ double exact_dc(const double * v, const double * src, double k, doubl
     return (bessel_k0(r * k) + bessel_k0(rm * k)) / (2.0 * PI);
 }
 
+// singNode[e]: the node that takes the singular value (the electrode node; a
+// free electrode's cell node within 1e-4 of its position, else -1)
 __global__ void k_primary(int n, int nE, int nK, const double * __restrict__ xyz,
-                          const double * __restrict__ elecPos, const int * __restrict__ elecNode,
+                          const double * __restrict__ elecPos, const int * __restrict__ singNode,
                           const double * __restrict__ minRadius, const double * __restrict__ kv,
                           double surfaceZ, double * __restrict__ prim) {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -50,8 +52,9 @@ __global__ void k_primary(int n, int nE, int nK, const double * __restrict__ xyz
     int kk = (int)(kn / n);
     double k = kv[kk];
     double u;
-    if (node == elecNode[e]) {
-        // ElectrodeShapeNode::setSingValue, scale 0 (electrode.cpp:154-189)
+    if (node == singNode[e]) {
+        // ElectrodeShapeNode / ElectrodeShapeEntity::setSingValue, scale 0
+        // (electrode.cpp:154-189, :207-249)
         double rmin = minRadius[e];
         u = k > 0.0 ? bessel_k0(rmin / 6.0 * k) / PI : 1.0 / (2.0 * PI * rmin / 2.0);
     } else {
@@ -135,15 +138,25 @@ __global__ void k_total_potential(long total, int nE, const double * __restrict_
 }
 
 // DataMap::collect (datamap.cpp:57-98) of the k-weighted sum
-// (dcfemmodelling.cpp:1707-1729): map[e][f] = sum_k w_k U[k][node_f][e].
+// (dcfemmodelling.cpp:1707-1729): map[e][f] = potential of source e at
+// electrode f, sum_j N_fj sum_k w_k U[k][node_fj][e] over the electrode's
+// shape nodes (one node with N = 1 for a node electrode, ElectrodeShapeNode::pot,
+// electrode.cpp:129-131; the cell's nodes and shape functions at the position
+// for a free one, MeshEntity::pot, meshentities.cpp:347-349).
 __global__ void k_electrode_map(int n, int nE, int nK, const double * __restrict__ U,
-                                const double * __restrict__ w, const int * __restrict__ elecNode,
-                                double * __restrict__ emap) {
+                                const double * __restrict__ w, const int * __restrict__ shapeNode,
+                                const double * __restrict__ shapeW, double * __restrict__ emap) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nE * nE) return;
     int e = t / nE, f = t % nE;
     double s = 0.0;
-    for (int kk = 0; kk < nK; ++kk) s += w[kk] * U[((size_t)kk * n + elecNode[f]) * nE + e];
+    for (int j = 0; j < SHAPE_NODES; ++j) {
+        const int v = shapeNode[SHAPE_NODES * f + j];
+        if (v < 0) break;
+        double u = 0.0;
+        for (int kk = 0; kk < nK; ++kk) u += w[kk] * U[((size_t)kk * n + v) * nE + e];
+        s += u * shapeW[SHAPE_NODES * f + j];
+    }
     emap[t] = s;
 }
 
@@ -188,7 +201,7 @@ __global__ void k_transpose_pots(int n, int nE, int nK, const double * __restric
 void launch_primary(Ctx & c) {
     long total = (long)c.nK * c.n * c.nE;
     hipLaunchKernelGGL(k_primary, dim3(div_up(total, 256)), dim3(256), 0, c.stream, c.n, c.nE, c.nK,
-                       c.xyz.p, c.elecPos.p, c.elecNode.p, c.minRadius.p, c.dk.p, c.surfaceZ, c.prim.p);
+                       c.xyz.p, c.elecPos.p, c.singNode.p, c.minRadius.p, c.dk.p, c.surfaceZ, c.prim.p);
     PG_CHECK(hipGetLastError());
 }
 
@@ -214,7 +227,7 @@ void launch_total_potential(Ctx & c) {
 void launch_response(Ctx & c, double * rhoa) {
     c.emap.alloc((size_t)c.nE * c.nE);
     hipLaunchKernelGGL(k_electrode_map, dim3(div_up((long)c.nE * c.nE, 256)), dim3(256), 0, c.stream, c.n,
-                       c.nE, c.nK, c.U.p, c.dw.p, c.elecNode.p, c.emap.p);
+                       c.nE, c.nK, c.U.p, c.dw.p, c.shapeNode.p, c.shapeW.p, c.emap.p);
     PG_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_response, dim3(div_up(c.D, 256)), dim3(256), 0, c.stream, c.D, c.nE, c.abmn.p,
                        c.kfac.p, c.emap.p, rhoa);

@@ -177,7 +177,13 @@ class ERTModelling:
         en, pen = _host_i(an.elecNode)
         sc, psc = _host_d(an.sourceCenter)
         ctx = self._ctx
-        ctx.check(ctx.L.pgert_set_electrodes(ctx.h, len(en), pen, psc, an.surfaceZ), "setElectrodes")
+        if np.any(an.elecNode < 0):
+            ec, pec = _host_i(an.elecCell)
+            ep, pep = _host_d(an.elecPos)
+            ctx.check(ctx.L.pgert_set_electrodes_free(ctx.h, len(en), pen, pec, pep, psc, an.surfaceZ),
+                      "setElectrodes")
+        else:
+            ctx.check(ctx.L.pgert_set_electrodes(ctx.h, len(en), pen, psc, an.surfaceZ), "setElectrodes")
         ba, bb, bg = an.bypass_pairs(self._bypassR)
         ba, pba = _host_i(ba)
         bb, pbb = _host_i(bb)

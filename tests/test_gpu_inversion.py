@@ -112,3 +112,100 @@ def test_cgls_on_jacobian_with_smoothness():
     assert it5 == 5 and err5 < 1e-6
     # (measured: objective 61835 -> 75.4 on the GPU, 75.1 in the oracle)
     assert it == ito == 25 and fg < 2e-3 * f0 and abs(fg - fo) <= 2e-2 * fo
+
+
+@pytest.mark.parametrize("name", ["small_2d", "small_3d"])
+def test_inversion_loop_vs_reference(name):
+    """Three Gauss-Newton iterations with line search (pygimli_amd.inversion.
+    Inversion: forward, Jacobian, CGLS, transforms and the line-search
+    objective on the GPU) against the reference's own RInversion (oracle/_ref;
+    the oracle's restatement where the reference build is absent) on the same
+    data, errors, constraints and start model, the CGLS run to convergence:
+    same step lengths, models within 1e-6, chi^2 within 1e-6."""
+    import torch
+    from oracle import ert_oracle as eo
+    from oracle import inversion as oi
+    from oracle import ref
+    from pygimli_amd.ertModelling import ERTModelling
+    from pygimli_amd.inversion import Inversion, smoothness_constraints
+    from pygimli_amd import workloads
+    mesh, data, rho = getattr(workloads, name)()
+    rng = np.random.default_rng(0)
+    fo = eo.Forward(mesh, copy.deepcopy(data))
+    dobs = fo.response(rho) * (1 + 0.02 * rng.standard_normal(data.size()))
+    C = smoothness_constraints(mesh)
+    start = np.full(mesh.cellCount(), float(np.median(dobs)))
+    err = np.full(len(dobs), 0.02)
+    kw = dict(lam=20.0, max_iter=3, line_search=True, max_cgls=3000, cgls_tol=1e-26)
+    if ref.available():
+        s = ref.Session(mesh, copy.deepcopy(data), nthreads=1)
+        s.response(rho)
+        m_r, r_r, chi2_r, _ = s.invert(dobs, err, start, C, **kw)
+        taus_r = None
+    else:
+        inv_o = oi.Inversion(fo, dobs, err, C, stop_at_chi1=False, dphi_abort_percent=-1e300, **kw)
+        m_r = inv_o.run(start)
+        chi2_r, taus_r = inv_o.chi2_history[-1], inv_o.tau_history
+    fop = ERTModelling()
+    fop.setMesh(mesh)
+    fop.setData(copy.deepcopy(data))
+    fop.solverOptions.update(method="block64", rtol=1e-12, coarse=0)
+    inv = Inversion(fop, dobs, err, C, lam=20.0, maxIter=3, lineSearch=True, maxCGLSIter=3000, CGLStol=1e-26,
+                    stopAtChi1=False, dPhiAbortPercent=-1e300)
+    m = inv.run(start).cpu().numpy()
+    print(name, "chi2", inv.chi2History, "tau", inv.tauHistory, "ref chi2", chi2_r, inv.timings)
+    assert inv.iter == 3
+    if taus_r is not None:
+        np.testing.assert_allclose(inv.tauHistory, taus_r)
+    assert np.abs(m - m_r).max() / np.abs(m_r).max() < 1e-6
+    assert abs(inv.chi2History[-1] - chi2_r) < 1e-6 * chi2_r
+    assert inv.chi2History[-1] < inv.chi2History[0]
+
+
+def test_transforms_and_objective_vs_oracle():
+    """pgert_trans (all six operations, with and without bounds) and
+    pgert_gn_phi at several step lengths against the oracle's TransLogLU and
+    RInversion objective."""
+    import torch
+    from oracle import inversion as oi
+    from pygimli_amd.inversion import Inversion
+    from pygimli_amd.ertModelling import ERTModelling
+    from pygimli_amd import workloads
+    from pygimli_amd.inversion import smoothness_constraints
+    rng = np.random.default_rng(1)
+    a = rng.uniform(0.5, 90.0, 1000)
+    b = rng.normal(0.0, 0.3, 1000)
+    mesh, data, rho = workloads.small_2d()
+    fop = ERTModelling()
+    fop.setMesh(mesh)
+    fop.setData(copy.deepcopy(data))
+    C = smoothness_constraints(mesh)
+    D, P = data.size(), mesh.cellCount()
+    dobs = rng.uniform(10.0, 200.0, D)
+    err = np.full(D, 0.03)
+    for lo, up in ((0.0, 0.0), (0.1, 100.0)):
+        inv = Inversion(fop, dobs, err, C, modelLower=lo, modelUpper=up)
+        t = oi.TransLogLU(lo, up)
+        ad, bd = torch.as_tensor(a, device=fop.device), torch.as_tensor(b, device=fop.device)
+        for op, ref in ((0, t.trans(a)), (1, t.invTrans(b)), (2, t.deriv(a)), (3, t.invTrans(t.trans(a) + 0.7 * b)),
+                        (4, np.abs(a * np.abs(b) * t.deriv(a))), (5, t.trans(a) - t.trans(np.abs(b) + 1.0))):
+            bb = bd if op in (3,) else torch.abs(bd) if op == 4 else torch.abs(bd) + 1.0
+            x = ad if op != 1 else bd
+            got = inv._trans(op, x, bb, 0.7).cpu().numpy()
+            np.testing.assert_allclose(got, ref, rtol=1e-13, err_msg=f"op {op} bounds {lo} {up}")
+        # objective along a step
+        m = torch.as_tensor(rng.uniform(20.0, 80.0, P), device=fop.device)
+        dm = torch.as_tensor(rng.normal(0.0, 0.2, P), device=fop.device)
+        r = torch.as_tensor(dobs * rng.uniform(0.8, 1.2, D), device=fop.device)
+        dd = torch.as_tensor(rng.normal(0.0, 0.1, D), device=fop.device)
+        fz = torch.as_tensor(dobs, device=fop.device)
+        inv._derr = inv._trans(4, fz, inv.error, data=True)
+        taus = [0.0, 0.01, 0.37, 1.0]
+        pd, pm = inv._phi(taus, m, dm, r, dd)
+        o = oi.Inversion(None, dobs, err, C, m_lower=lo, m_upper=up)
+        mh, dmh, rh, ddh = (v.cpu().numpy() for v in (m, dm, r, dd))
+        for k, tau in enumerate(taus):
+            am = mh if tau == 0 else o.tM.update(mh, dmh * tau)
+            ar = rh if tau == 0 else o.tD.update(rh, ddh * tau)
+            np.testing.assert_allclose(pd[k], o.phiD(ar), rtol=1e-12)
+            np.testing.assert_allclose(pm[k], o.phiM(am), rtol=1e-12)

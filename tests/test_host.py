@@ -72,17 +72,28 @@ def test_electrode_search_and_wavenumbers_match_oracle(name):
 
 def test_electrode_search_2d_swap_and_errors():
     """2D sensors given as (x, 0, z) are read as (x, z) (dcfemmodelling.cpp:859-867);
-    a sensor off every electrode node raises."""
+    a sensor off every electrode node is a free electrode in the cell holding
+    it (same cell as the oracle's search); a sensor outside the mesh raises."""
     mesh, data, _ = workloads.small_2d()
     s = data.sensorPositions().copy()
     s2 = np.stack([s[:, 0], np.zeros(len(s)), s[:, 1]], 1)
     an = bert.MeshAnalysis(mesh, s2)
     an0 = bert.MeshAnalysis(mesh, s)
     np.testing.assert_array_equal(an.elecNode, an0.elecNode)
-    bad = s.copy()
-    bad[0, 0] += 0.3
-    with pytest.raises(NotImplementedError):
-        bert.MeshAnalysis(mesh, bad)
+    free = s.copy()
+    free[0, 0] += 0.3
+    free[3, 1] -= 0.2
+    an = bert.MeshAnalysis(mesh, free)
+    st = eo.Setup(mesh, free)
+    np.testing.assert_array_equal(an.elecNode, st.elecNode)
+    assert an.elecNode[0] == -1 and an.elecNode[3] == -1
+    np.testing.assert_array_equal(an.elecCell[[0, 3]], st.elecCell[[0, 3]])
+    np.testing.assert_allclose(an.elecPos, st.elecPos, rtol=0, atol=0)
+    np.testing.assert_allclose(an.sourceCenter, st.sourceCenter, rtol=1e-15)
+    out = s.copy()
+    out[0, 1] += 1.0   # above the surface
+    with pytest.raises(RuntimeError):
+        bert.MeshAnalysis(mesh, out)
 
 
 @pytest.mark.parametrize("name", ["small_2d", "small_3d"])

@@ -604,6 +604,30 @@ def test_bypass_oracle_vs_reference_build(tmp_path, monkeypatch):
     assert np.abs(eo.response_total(f1, rho) - r).max() > 1e-3 * np.abs(r).max()
 
 
+@pytest.mark.parametrize("name", ["small_2d", "small_3d"])
+@pytest.mark.parametrize("sr", [True, False])
+def test_free_electrodes_oracle_vs_reference_build(name, sr):
+    """Free electrodes (ElectrodeShapeEntity, electrode.cpp:198-287): the
+    oracle against the reference build with and without singularity
+    removal; moving the sensors changes the result."""
+    if not ref.available():
+        pytest.skip("oracle/_ref not built")
+    mesh, data, rho = workloads.small_free(name)
+    s = ref.Session(mesh, copy.deepcopy(data), nthreads=1, sr=sr)
+    r = s.response(rho)
+    J = s.jacobian(rho)
+    f = eo.Forward(mesh, copy.deepcopy(data), total_field=not sr)
+    assert np.all(f.setup.elecNode < 0)
+    o = f.response(rho) if sr else eo.response_total(f, rho)
+    oJ = f.jacobian(rho)
+    assert np.abs(o - r).max() <= 1e-11 * np.abs(r).max()
+    assert np.linalg.norm(oJ - J) <= 1e-11 * np.linalg.norm(J)
+    m0, d0, _ = getattr(workloads, name)()
+    f0 = eo.Forward(m0, copy.deepcopy(d0), total_field=not sr)
+    o0 = f0.response(rho) if sr else eo.response_total(f0, rho)
+    assert np.abs(o0 - r).max() > 1e-4 * np.abs(r).max()
+
+
 def test_topography_total_field_oracle_vs_reference_build():
     """Topography in the total-field operator (no primary potentials needed):
     oracle against the reference build on a mesh with a surface bump."""
@@ -622,3 +646,60 @@ def test_topography_total_field_oracle_vs_reference_build():
     assert np.linalg.norm(oJ - J) <= 1e-12 * np.linalg.norm(J)
     with pytest.raises(NotImplementedError):
         eo.Forward(mesh, copy.deepcopy(data), k, w)
+
+
+@pytest.mark.parametrize("name", ["small_2d", "small_3d"])
+@pytest.mark.parametrize("line_search", [True, False])
+def test_inversion_loop_vs_reference_build(name, line_search):
+    """The oracle's Gauss-Newton loop (oracle/inversion.py: RInversion::run /
+    oneStep / linesearch with log transforms) against the reference's own
+    RInversion (oracle/_ref) on the same operator, data, errors, constraints
+    and start model, three iterations: with the CGLS run to convergence
+    (so its stopping point does not depend on rounding) the models agree to
+    1e-9 and the responses exactly; the line search picks the same steps."""
+    if not ref.available():
+        pytest.skip("oracle/_ref not built")
+    from oracle import inversion as oi
+    from pygimli_amd.inversion import smoothness_constraints
+    mesh, data, rho = getattr(workloads, name)()
+    s = ref.Session(mesh, copy.deepcopy(data), nthreads=1)
+    rng = np.random.default_rng(0)
+    dobs = s.response(rho) * (1 + 0.02 * rng.standard_normal(data.size()))
+    C = smoothness_constraints(mesh)
+    start = np.full(mesh.cellCount(), float(np.median(dobs)))
+    err = np.full(len(dobs), 0.02)
+    m_r, r_r, chi2_r, it_r = s.invert(dobs, err, start, C, lam=20.0, max_iter=3, line_search=line_search,
+                                      max_cgls=3000, cgls_tol=1e-26)
+    fo = eo.Forward(mesh, copy.deepcopy(data))
+    fo.data.set("k", s.data_k())
+    inv = oi.Inversion(fo, dobs, err, C, lam=20.0, max_iter=3, line_search=line_search, max_cgls=3000,
+                       cgls_tol=1e-26, stop_at_chi1=False, dphi_abort_percent=-1e300)
+    m_o = inv.run(start)
+    assert it_r == inv.iter == 3
+    assert np.abs(m_o - m_r).max() / np.abs(m_r).max() < 1e-9
+    assert np.abs(inv.response - r_r).max() / np.abs(r_r).max() < 1e-12
+    assert abs(inv.chi2_history[-1] - chi2_r) < 1e-9 * chi2_r
+    assert inv.chi2_history[-1] < inv.chi2_history[0]
+
+
+def test_transforms_closed_form():
+    """TransLogLU / TransLog of the oracle: trans, invTrans and deriv by the
+    closed forms (log barrier with bounds), update as a round trip, error
+    as |a e deriv(a)|, and the bound capping."""
+    from oracle import inversion as oi
+    a = np.array([0.5, 2.0, 30.0, 99.0])
+    t = oi.TransLogLU(0.0, 0.0)
+    np.testing.assert_allclose(t.trans(a), np.log(a), rtol=1e-15)
+    np.testing.assert_allclose(t.deriv(a), 1.0 / a, rtol=1e-15)
+    np.testing.assert_allclose(t.update(a, np.log(2.0)), 2.0 * a, rtol=1e-14)
+    np.testing.assert_allclose(t.error(a, 0.03), 0.03, rtol=1e-15)
+    t = oi.TransLogLU(0.1, 100.0)
+    f = np.log(a - 0.1) - np.log(100.0 - a)
+    np.testing.assert_allclose(t.trans(a), f, rtol=1e-15)
+    np.testing.assert_allclose(t.invTrans(f), a, rtol=1e-12)
+    np.testing.assert_allclose(t.deriv(a), 1.0 / (a - 0.1) + 1.0 / (100.0 - a), rtol=1e-15)
+    # finite-difference check of deriv
+    h = 1e-6
+    np.testing.assert_allclose((t.trans(a + h) - t.trans(a - h)) / (2 * h), t.deriv(a), rtol=1e-6)
+    # values beyond the bounds are capped at (1 +- 1e-8) of the bound
+    assert np.isfinite(t.trans(np.array([0.0, 1000.0]))).all()
