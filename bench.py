@@ -713,6 +713,9 @@ def main():
     ap.add_argument("--method", default="mixed")
     ap.add_argument("--cpu-timeout", type=float, default=300.0)
     ap.add_argument("--coarse", type=int, default=1024, help="aggregates of the two-level preconditioner (0: off)")
+    ap.add_argument("--smooth", type=int, default=2,
+                    help="two-level cycle: 2 = symmetric multiplicative with degree-2 Chebyshev smoothing, "
+                         "0 = additive (Jacobi + coarse correction)")
     ap.add_argument("--learned-fields", type=int, default=8,
                     help="random fields the learned warm start is trained on before timing (0: off)")
     ap.add_argument("--batched", type=int, default=0,
@@ -944,8 +947,9 @@ def main():
     L, h = fop._ctx.L, fop._ctx.h
     n, nb = mesh.nodeCount(), 1 * ((64 + 63) // 64)
 
-    def run(method, steps, warmup, coarse=0, learned=False, timed_kernel=0):
-        fop.solverOptions.update(method=method, rtol=args.rtol, coarse=coarse, learned=learned)
+    def run(method, steps, warmup, coarse=0, learned=False, timed_kernel=0, smooth=None):
+        fop.solverOptions.update(method=method, rtol=args.rtol, coarse=coarse, learned=learned,
+                                 smooth=args.smooth if smooth is None else smooth)
         for _ in range(warmup):
             fop.response_device(model)
             fop.createJacobian(model)
@@ -987,7 +991,7 @@ def main():
         fields = [10.0 ** workloads.gaussian_random_field(cc, 2.0, 0.3 + 0.1 * (i % 6), 2.0 + 3.0 * (i % 7), 100 + i,
                                                            (-30, -30, -40), (30, 30, 0))
                   for i in range(args.learned_fields)]
-        fop.solverOptions.update(method=args.method, rtol=args.rtol, coarse=args.coarse)
+        fop.solverOptions.update(method=args.method, rtol=args.rtol, coarse=args.coarse, smooth=args.smooth)
         lw = LearnedWarmStart(fop, seed=0)
         t = time.time()
         losses = lw.train([torch.as_tensor(f, device=dev) for f in fields], samples_per_model=65536, epochs=20,
@@ -1001,6 +1005,9 @@ def main():
     best = run(args.method, args.steps, args.warmup, args.coarse if args.method != "pcg64" else 0,
                learned=use_learned, timed_kernel=0)
     ablation = run(args.method, 1, 0, args.coarse, learned=False) if use_learned else None
+    # the same with the additive two-level preconditioner (the cycle's gain)
+    additive = (run(args.method, 1, 0, args.coarse, learned=use_learned, smooth=0)
+                if args.coarse and args.smooth and args.method != "pcg64" else None)
 
     # per-kernel time per launch (one solve each) and the roofline of the
     # dominant one: algorithmic bytes = each vector read/written once, flops =
@@ -1013,6 +1020,13 @@ def main():
                6: ("k_small_alpha", None, None), 7: ("k_small_beta", None, None),
                8: ("k_zform", 2 * vec + 12 * n * nb, None), 9: ("k_restrict", vec, None),
                10: ("k_coarse_apply", None, None), 11: ("k_part_reduce", None, None)}
+    if args.smooth and args.coarse:
+        # the smoothed cycle: its sparse passes' bytes come from the library
+        # (matrix + gathers + row vectors); Z is formed by the post-smoothing pass
+        kernels[8] = ("k_smooth<SZ>", None, None)
+        kernels[12] = ("k_smooth<S0,S1>", None, None)
+        kernels[13] = ("k_coarse_resid", None, None)
+    lib_bytes = {1, 8, 12, 13} if args.smooth and args.coarse else {1}
     mfma_peak = 157.3e12 if args.method == "mixed" else 78.6e12
     # per-kernel breakdown: one instrumented evaluation per kernel (HIP events
     # around every launch of that kernel on the solver stream; eager launches),
@@ -1027,11 +1041,10 @@ def main():
             # the streaming kernels' from the vector count above, none for the
             # latency-bound small kernels
             breakdown[name] = {"us_per_launch": r["k_ms"] * 1e3, "launches": r["k_launches"],
-                               "bytes": r["k_bytes"] if kid == 1 else kb,
+                               "bytes": r["k_bytes"] if kid in lib_bytes else kb,
                                "ms_per_step": r["k_ms"] * r["k_launches"]}
     # the dominant kernel among those with an algorithmic byte / flop count
-    costed = [k for k in breakdown if kernels[[i for i, v in kernels.items() if v[0] == k][0]][1] is not None
-              or k == "k_spmm"]
+    costed = [k for k in breakdown if breakdown[k]["bytes"]]
     dom = max(costed, key=lambda k: breakdown[k]["ms_per_step"]) if costed else None
 
     def roof(name, us, nbytes, flops):
@@ -1073,6 +1086,8 @@ def main():
             "workload": "3D ERT: 201898-node / 1150488-tet graded mesh, 64 electrodes (8x8 @ 2 m), "
                         "2000 random quadrupoles, log10-rho GRF (2.0, 0.5, 5 m); J 2000 x 1150488 FP32",
             "method": args.method, "rtol": args.rtol, "coarse_aggregates": args.coarse,
+            "preconditioner": ("two-level multiplicative, degree-2 Chebyshev smoothing" if args.smooth
+                               else "two-level additive (Jacobi + coarse)") if args.coarse else "Jacobi",
             "learned_warm_start": use_learned, "nodes": n, "cells": mesh.cellCount(),
             "electrodes": 64, "data": data.size(), "rhs": 64,
         },
@@ -1091,6 +1106,9 @@ def main():
     if learned_info:
         learned_info.update(iterations_without=ablation["iters"], ms_without=ablation["ms"])
         out["learned"] = learned_info
+    if additive is not None:
+        out["additive_two_level"] = {"value": 1e3 / additive["ms"], "ms_per_step": additive["ms"],
+                                     "iterations": additive["iters"], "potential_rel_err_fro": additive["pot_err"]}
     if block_coarse is not None:
         out["fp64_block_pcg_coarse"] = {"value": 1e3 / block_coarse["ms"], "ms_per_step": block_coarse["ms"],
                                         "iterations": block_coarse["iters"],

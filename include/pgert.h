@@ -59,6 +59,10 @@ typedef struct pgert_solve_opts {
                                   guess (when it started from a learned or given guess), then with
                                   FP64 block-PCG (when it was mixed), and the call fails if that
                                   also misses; 1: return with stats.converged = 0 instead */
+    int smooth;                /* block methods with coarse = 1: 0 = additive two-level preconditioner
+                                  (Jacobi + coarse correction); 2 = symmetric multiplicative two-level
+                                  cycle with degree-2 Chebyshev smoothing (three sparse products more
+                                  per iteration, several times fewer iterations) */
 } pgert_solve_opts;
 
 typedef struct pgert_solve_stats {
@@ -342,6 +346,9 @@ int pgert_mlp_train_step(pgert_ctx * ctx, const float * feat, const float * targ
  *   PGERT_TIME_RESTRICT    9   aggregate sums of R (block methods, coarse on)
  *   PGERT_TIME_COARSE     10   E^-1 y (block methods, coarse on)
  *   PGERT_TIME_REDUCE     11   fixed-order sum of the partial Grams (block methods)
+ *   PGERT_TIME_SMOOTH     12   the two Chebyshev pre-smoothing products (smooth = 2)
+ *   PGERT_TIME_COARSE_RESID 13 r1 - (A Z) c (smooth = 2); with smooth = 2, PGERT_TIME_ZFORM
+ *                              times the post-smoothing product that forms Z
  * reset: zero the accumulators.  Returns the accumulated milliseconds, number
  * of timed launches and the compulsory HBM bytes of one launch (matrix values
  * and indices once, every vector the kernel reads or writes once, all blocks
@@ -357,6 +364,8 @@ int pgert_mlp_train_step(pgert_ctx * ctx, const float * feat, const float * targ
 #define PGERT_TIME_RESTRICT 9
 #define PGERT_TIME_COARSE 10
 #define PGERT_TIME_REDUCE 11
+#define PGERT_TIME_SMOOTH 12
+#define PGERT_TIME_COARSE_RESID 13
 int pgert_kernel_timing(pgert_ctx * ctx, int enable, int reset, double * total_ms, long * launches,
                         double * bytes_per_launch);
 

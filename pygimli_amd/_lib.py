@@ -40,7 +40,7 @@ class SolveOpts(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("rtol", ctypes.c_double), ("max_iter", ctypes.c_int),
                 ("check_every", ctypes.c_int), ("inner_rtol", ctypes.c_double),
                 ("warm_start", ctypes.c_void_p), ("learned_warm_start", ctypes.c_int), ("coarse", ctypes.c_int),
-                ("no_graph", ctypes.c_int), ("report_only", ctypes.c_int)]
+                ("no_graph", ctypes.c_int), ("report_only", ctypes.c_int), ("smooth", ctypes.c_int)]
 
 
 class SolveStats(ctypes.Structure):

@@ -3,6 +3,8 @@
 // and electrode tables.
 #include "internal.h"
 
+#include <cstdio>
+
 #include <algorithm>
 #include <array>
 #include <cfloat>
@@ -538,7 +540,30 @@ int pgert_set_coarse(pgert_ctx * c, int n_agg) {
                 cij.push_back((int)(key[q].first % m));
             }
         }
+        // A Z: per row the distinct aggregates of its columns, and for each
+        // such entry the row's slots that sum into it (in slot order)
+        std::vector<int> azp(1, 0), aza, azsp(1, 0), azs;
+        azs.reserve(c->nnz);
+        std::vector<std::pair<int, int>> rowKeys;
+        for (int r = 0; r < n; ++r) {
+            rowKeys.clear();
+            for (int q = c->hRowptr[r]; q < c->hRowptr[r + 1]; ++q) rowKeys.push_back({agg[c->hColidx[q]], q});
+            std::sort(rowKeys.begin(), rowKeys.end());
+            for (size_t u = 0; u < rowKeys.size(); ++u) {
+                azs.push_back(rowKeys[u].second);
+                if (u + 1 == rowKeys.size() || rowKeys[u + 1].first != rowKeys[u].first) {
+                    aza.push_back(rowKeys[u].first);
+                    azsp.push_back((int)azs.size());
+                }
+            }
+            azp.push_back((int)aza.size());
+        }
         hipStream_t s = c->stream;
+        c->azPtr.upload(azp, s);
+        c->azAgg.upload(aza, s);
+        c->azSlotPtr.upload(azsp, s);
+        c->azSlot.upload(azs, s);
+        c->nAZ = (int)aza.size();
         c->agg.upload(agg, s);
         c->aggPtr.upload(aptr, s);
         c->aggNodes.upload(anodes, s);
@@ -606,6 +631,7 @@ pgert_solve_opts resolve_opts(const pgert_solve_opts * opts) {
         o.inner_rtol = 1e-3;
     }
     PG_REQUIRE(o.rtol > 0 && o.max_iter > 0, "pgert_forward: bad solver options");
+    PG_REQUIRE(o.smooth == 0 || o.smooth == 2, "pgert_forward: smooth must be 0 or 2");
     if (o.check_every <= 0) o.check_every = 8;
     return o;
 }
@@ -717,9 +743,13 @@ void solve_with_fallback(Ctx & c, const pgert_solve_opts & o, pgert_solve_stats 
         f.method = PGERT_BLOCK_FP64;
         again(2);
     }
-    PG_REQUIRE(st.converged, std::string(who) + ": the solve did not reach rtol " + std::to_string(o.rtol) +
-                                 " within max_iter " + std::to_string(o.max_iter) +
-                                 " iterations (max relative residual " + std::to_string(st.max_rel_residual) + ")");
+    if (!st.converged) {
+        char msg[256];
+        std::snprintf(msg, sizeof msg,
+                      "%s: the solve did not reach rtol %.3e within max_iter %d iterations (max relative residual %.3e)",
+                      who, o.rtol, o.max_iter, st.max_rel_residual);
+        PG_REQUIRE(false, std::string(msg));
+    }
 }
 
 int pgert_forward(pgert_ctx * c, const double * rho, const pgert_solve_opts * opts, double * rhoa,
@@ -1128,7 +1158,7 @@ int pgert_kernel_timing(pgert_ctx * c, int enable, int reset, double * total_ms,
             c->kTotalMs = 0.0;
             c->kLaunches = 0;
         }
-        PG_REQUIRE(enable >= 0 && enable <= PGERT_TIME_REDUCE, "pgert_kernel_timing: unknown kernel");
+        PG_REQUIRE(enable >= 0 && enable <= PGERT_TIME_COARSE_RESID, "pgert_kernel_timing: unknown kernel");
         c->kTiming = enable != 0;
         c->kSelect = enable;
         if (total_ms) *total_ms = c->kTotalMs;

@@ -135,6 +135,13 @@ struct Ctx {
     DBuf<int> cePtr, ceSlot, ceIJ;
     DBuf<double> Einv;      // nK * m * m
     DBuf<double> ycoarse, ccoarse;  // per block m * 64
+    // multiplicative two-level variant (Chebyshev smoothing): A Z per row
+    // (the aggregates of the row's columns and the summed values), A D^-1,
+    // and the Gershgorin bound of D^-1 A, per wavenumber
+    int nAZ = 0;
+    DBuf<int> azPtr, azAgg, azSlotPtr, azSlot;
+    DBuf<double> azVals, valsAD, gershPart;
+    std::vector<double> hGersh;
 
     // ---- wavenumbers
     int nK = 0;
@@ -307,6 +314,7 @@ void solve_band(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std::
 
 // two-level preconditioner (coarse.hip)
 void coarse_prepare(Ctx & c);
+void smooth_prepare(Ctx & c);
 
 // learned warm start (mlp.hip)
 int mlp_param_count();

@@ -8,7 +8,9 @@
 //   1. Q = A P                                              (k_spmm)
 //   2. G1 = P'Q, G2 = P'R;  alpha = G1^-1 G2                (k_gram<PQR>, k_small_alpha)
 //   3. X += P alpha;  R -= Q alpha;  |R|^2 per column       (k_update)
-//   4. Z = M^-1 R;  Q'Z, Z'Z, P'Z                  (k_zform, k_gram<Z>; M = Jacobi [+ coarse])
+//   4. Z = M^-1 R;  Q'Z, Z'Z, P'Z                  (k_gram<Z>; M = Jacobi [+ additive coarse]
+//      correction, k_zform], or the multiplicative two-level cycle with
+//      Chebyshev smoothing [k_smooth, k_restrict, k_coarse_apply, k_coarse_resid])
 //   5. beta = -G1^-1 Q'Z;  W = Z + P beta;  W'W from the Grams (P'P = I);
 //      T = chol(W'W)^-T;  P = Z T + P (beta T)              (k_small_beta, k_pupdate)
 // Dependent directions are dropped in both factorisations (a pivot below
@@ -40,6 +42,7 @@
 #include <algorithm>
 #include <cmath>
 #include <type_traits>
+#include <utility>
 
 namespace pgert {
 
@@ -83,6 +86,12 @@ struct BlockDesc {
     const double * Einv;   // E_k^-1, m x m
     const double * Bsrc;   // right-hand sides of wavenumber k, node-major [node][nE]
     double * Xdst;         // solution of wavenumber k, node-major [node][nE]
+    // multiplicative two-level preconditioner with Chebyshev smoothing
+    const double * valsAD;  // A_k D^-1
+    const int * azPtr;      // A_k Z: per row the aggregates of its columns ...
+    const int * azAgg;
+    const double * azVals;  // ... and the summed values
+    double ch[4];           // Chebyshev coefficients c0, c1, c2, cz (see k_smooth)
 };
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -228,6 +237,166 @@ __global__ void __launch_bounds__(256) k_spmm(const BlockDesc * __restrict__ blo
         p1n = p1nn;
         cvc = cvn;
         avc = avn;
+    }
+}
+
+// ---------------------------------------------------------------- smoothed two-level preconditioner
+// z = M^-1 r with M^-1 the symmetric multiplicative two-level cycle: a
+// degree-2 Chebyshev smoother S on D^-1 A over [b/11, b] (b the Gershgorin
+// bound of D^-1 A), the coarse correction on the aggregates, the same
+// smoother again:
+//   z1 = S r,  r1 = r - A z1,  c = E^-1 Z' r1,  r2 = r1 - (A Z) c,
+//   z  = z1 + Z c + S r2.
+// With theta = (b + a)/2, delta = (b - a)/2, sigma = theta/delta, rho0 =
+// 1/sigma, rho1 = 1/(2 sigma - rho0): S r = D^-1 (cz r + c2 s0) where
+// s0 = r - A D^-1 r c0, and r - A S r = s0 - A D^-1 (c1 r + c2 s0), with
+// c0 = 1/theta, c1 = rho1 rho0/theta, c2 = 2 rho1/delta, cz = (1 + rho1 rho0)/theta.
+// M^-1 is symmetric (S is a polynomial in D^-1 A times D^-1) and positive
+// definite (the smoother contracts in the A-norm on [0, b]).
+// The three sparse passes share k_spmm's gather scheme, on AD = A D^-1:
+//   MODE_S0: out = s0 = U - c0 AD U (U = r);  out2 = w = c1 U + c2 s0
+//   MODE_S1: out = r1 = V - AD U               (U = w, V = s0)
+//   MODE_SZ: s = U - c0 AD U (U = r2);  out = Z = D^-1 (cz (Rr + U) + c2 (V + s)) + cc[agg]
+//            (Rr = r, V = s0)
+// (each pass gathers one vector: r1 = s0 - A D^-1 (c1 r + c2 s0) uses the
+// combination w that the first pass writes row by row)
+constexpr int MODE_S0 = 0;
+constexpr int MODE_S1 = 1;
+constexpr int MODE_SZ = 2;
+template <typename VT, int MODE>
+__global__ void __launch_bounds__(256) k_smooth(const BlockDesc * __restrict__ blocks, const VT * __restrict__ U,
+                                                const VT * __restrict__ V, const VT * __restrict__ Rr,
+                                                const double * __restrict__ cc, VT * __restrict__ out,
+                                                VT * __restrict__ out2) {
+    using Vv = V4<VT>;
+    using VA = f64x4;
+    __shared__ int sIdx[4][4][SPMM_W];
+    __shared__ double sVal[4][4][SPMM_W];
+    const int b = blockIdx.y;
+    const BlockDesc bd = blocks[b];
+    if (!bd.active) return;
+    const int n = bd.n;
+    const auto rowptr = glob(bd.rowptr);
+    const auto colidx = glob(bd.colidx);
+    const auto A = glob(bd.valsAD);
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, rg = lane / 16, j = lane % 16;
+    const VT * Ub = U + bd.off + 4 * j;
+    const VT * Vb = V + bd.off + 4 * j;
+    VT * Ob = out + bd.off + 4 * j;
+    const int x = blockIdx.x % NXCD, nw = gridDim.x / NXCD;
+    const int lo = (int)((long)n * x / NXCD), hi = (int)((long)n * (x + 1) / NXCD);
+    if (lo >= hi) return;
+    const int stride = 16 * nw;
+    int * myIdx = sIdx[wave][rg];
+    double * myVal = sVal[wave][rg];
+    auto rp_load = [&](int r, int & p0, int & p1) {
+        const int rc = min(r + rg, hi - 1);
+        p0 = rowptr[rc];
+        p1 = rowptr[rc + 1];
+    };
+    auto ci_load = [&](int r, int p0, int p1, int & cv, double & av) {
+        const int q = p0 + j;
+        const bool in = r + rg < hi && q < p1;
+        const int qc = in ? q : p0;
+        const int c = colidx[qc];
+        const double v = A[qc];
+        cv = in ? c : 0;
+        av = in ? v : 0.0;
+    };
+    int r0 = lo + 4 * ((blockIdx.x / NXCD) * 4 + wave);
+    int p0c, p1c, p0n, p1n, cvc;
+    double avc;
+    rp_load(r0, p0c, p1c);
+    ci_load(r0, p0c, p1c, cvc, avc);
+    rp_load(r0 + stride, p0n, p1n);
+    for (; r0 < hi; r0 += stride) {
+        const int row = r0 + rg;
+        const int rowc = min(row, hi - 1);
+        myIdx[j] = cvc;
+        myVal[j] = avc;
+        __builtin_amdgcn_wave_barrier();
+        int ci[SPMM_W];
+#pragma unroll
+        for (int u = 0; u < SPMM_W; ++u) ci[u] = myIdx[u];
+        Vv xu[SPMM_W];
+#pragma unroll
+        for (int u = 0; u < SPMM_W; ++u) xu[u] = ld4(Ub + (size_t)ci[u] * S);
+        // this row's own values (the self term, and for MODE_SZ r, s0, D^-1, cc)
+        const Vv self = ld4((MODE == MODE_S1 ? Vb : Ub) + (size_t)rowc * S);
+        Vv rOwn = Vv{0, 0, 0, 0}, sOwn = Vv{0, 0, 0, 0};
+        double dOwn = 0.0;
+        VA cOwn = VA{0, 0, 0, 0};
+        if constexpr (MODE == MODE_SZ) {
+            rOwn = ld4(Rr + bd.off + 4 * j + (size_t)rowc * S);
+            sOwn = ld4(Vb + (size_t)rowc * S);
+            dOwn = glob(bd.dinv)[rowc];
+            cOwn = ld4(cc + bd.coff + (size_t)glob(bd.agg)[rowc] * S + 4 * j);
+        }
+        int cvn, p0nn, p1nn;
+        double avn;
+        ci_load(r0 + stride, p0n, p1n, cvn, avn);
+        rp_load(r0 + 2 * stride, p0nn, p1nn);
+        __builtin_amdgcn_sched_barrier(0);
+        VA au0 = VA{0, 0, 0, 0}, au1 = au0;
+#pragma unroll
+        for (int u = 0; u < SPMM_W; u += 2) {
+            au0 += myVal[u] * __builtin_convertvector(xu[u], VA);
+            au1 += myVal[u + 1] * __builtin_convertvector(xu[u + 1], VA);
+        }
+        const int q0 = row < hi ? p0c : 0, q1 = row < hi ? p1c : 0;
+        for (int q = q0 + SPMM_W; q < q1; ++q)
+            au0 += A[q] * __builtin_convertvector(ld4(Ub + (size_t)colidx[q] * S), VA);
+        const VA sf = __builtin_convertvector(self, VA);
+        VA res;
+        if constexpr (MODE == MODE_S0) {
+            res = sf - bd.ch[0] * (au0 + au1);
+            if (row < hi) st4(out2 + bd.off + 4 * j + (size_t)row * S,
+                              __builtin_convertvector(bd.ch[1] * sf + bd.ch[2] * res, Vv));
+        } else if constexpr (MODE == MODE_S1) {
+            res = sf - (au0 + au1);
+        } else {
+            const VA sp = sf - bd.ch[0] * (au0 + au1);
+            res = dOwn * (bd.ch[3] * (__builtin_convertvector(rOwn, VA) + sf) +
+                          bd.ch[2] * (__builtin_convertvector(sOwn, VA) + sp)) + cOwn;
+        }
+        if (row < hi) st4(Ob + (size_t)row * S, __builtin_convertvector(res, Vv));
+        __builtin_amdgcn_wave_barrier();
+        p0c = p0n;
+        p1c = p1n;
+        p0n = p0nn;
+        p1n = p1nn;
+        cvc = cvn;
+        avc = avn;
+    }
+}
+
+// r2 = r1 - (A Z) c in place; one thread per 4 columns of a row.  Most rows
+// have one or two entries (their neighbours lie in one or two aggregates):
+// the first two entries' indices and coarse rows are loaded together
+// (a row with one entry repeats it with weight 0), further entries in a loop.
+template <typename VT>
+__global__ void __launch_bounds__(256) k_coarse_resid(const BlockDesc * __restrict__ blocks,
+                                                      const double * __restrict__ cc, VT * __restrict__ R12) {
+    const int b = blockIdx.y;
+    const BlockDesc bd = blocks[b];
+    if (!bd.active) return;
+    const auto azPtr = glob(bd.azPtr);
+    const auto azAgg = glob(bd.azAgg);
+    const auto azVals = glob(bd.azVals);
+    const double * cb = cc + bd.coff;
+    const long total = (long)bd.n * (S / 4);
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+        const int row = (int)(t / (S / 4)), c4 = 4 * (int)(t % (S / 4));
+        const size_t o = bd.off + (size_t)row * S + c4;
+        const V4<VT> r = ld4(R12 + o);
+        const int p0 = azPtr[row], p1 = azPtr[row + 1];
+        const int q1 = min(p0 + 1, p1 - 1);
+        const int a0 = azAgg[p0], a1 = azAgg[q1];
+        const double v0 = azVals[p0], w1 = azVals[q1];
+        const double v1 = q1 > p0 ? w1 : 0.0;
+        f64x4 acc = v0 * ld4(cb + (size_t)a0 * S + c4) + v1 * ld4(cb + (size_t)a1 * S + c4);
+        for (int q = p0 + 2; q < p1; ++q) acc += azVals[q] * ld4(cb + (size_t)azAgg[q] * S + c4);
+        st4(R12 + o, __builtin_convertvector(__builtin_convertvector(r, f64x4) - acc, V4<VT>));
     }
 }
 
@@ -855,6 +1024,155 @@ __device__ void inv_chol(const double * G, double * Li, int s) {
     __syncthreads();
 }
 
+// The same result as inv_chol, blocked in 16 x 16 tiles so that the 64
+// sequential pivot steps need no workgroup barrier: for each diagonal tile K
+//   (a) every wavefront eliminates the tile in registers (16 pivot steps;
+//       row j reaches the other lanes of its 16-lane row through DPP
+//       row_share, column j through one permute), giving V_K = U_KK^-1 and the
+//       pivots D_K (dependent pivots dropped as in inv_chol);
+//   (b) panel Y_IK = G_IK V_K' and U_IK = Y_IK D_K^+ (I > K, one wavefront each);
+//   (c) trailing update G_IJ -= U_IK Y_JK' (K < J <= I);
+// then X = U^-1 blockwise (X_KK = V_K, X_IJ = -V_I sum_K U_IK X_KJ, three
+// rounds) and Li = D^-1/2 X (zero rows for dropped pivots).  All products are
+// 16 x 16 x 16 FP64 MFMA tiles.  G is overwritten (U in its strictly lower
+// tiles); Li holds Y as scratch until X replaces it.
+// f(integral_constant<J>) for J = 0, 1, ... (every index a compile-time constant)
+template <typename F, int... J> __device__ inline void unrolled(F && f, std::integer_sequence<int, J...>) {
+    (f(std::integral_constant<int, J>{}), ...);
+}
+__device__ inline double readlane_d(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane), hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+// value of lane 16 (l / 16) + J (within the lane's DPP row of 16)
+template <int J> __device__ inline double row_share_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x150 + J, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x150 + J, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+// acc + op(A) op(B) for 16 x 16 LDS tiles (leading dimension LD), FP64 MFMA
+// layout: lane l supplies A[l%16][k] and B[k][l%16], k = 4 ks + l/16
+__device__ inline f64x4 tile_mm(const double * A, bool tA, const double * B, bool tB, f64x4 acc, double sgn) {
+    const int l = threadIdx.x % 64, i = l % 16, kq = l / 16;
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+        const int k = 4 * ks + kq;
+        const double a = tA ? A[k * LD + i] : A[i * LD + k];
+        const double b = tB ? B[i * LD + k] : B[k * LD + i];
+        acc = mfma4(sgn * a, b, acc);
+    }
+    return acc;
+}
+__device__ inline f64x4 tile_load(const double * C) {
+    const int l = threadIdx.x % 64;
+    f64x4 v;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) v[r] = C[drow<double>(l, r) * LD + l % 16];
+    return v;
+}
+__device__ inline void tile_store(double * C, f64x4 v) {
+    const int l = threadIdx.x % 64;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) C[drow<double>(l, r) * LD + l % 16] = v[r];
+}
+
+__device__ void inv_chol_blk(double * G, double * Li, int s) {
+    __shared__ double d0[S], piv[S];
+    const int wave = threadIdx.x / 64, l = threadIdx.x % 64, r = l % 16, qg = l / 16;
+    if (threadIdx.x < S) d0[threadIdx.x] = threadIdx.x < s ? G[threadIdx.x * LD + threadIdx.x] : 0.0;
+    lds_barrier();
+    auto T = [&](double * M, int I, int J) { return M + 16 * I * LD + 16 * J; };
+    for (int K = 0; K < 4; ++K) {
+        // (a) the diagonal tile, redundantly in every wavefront
+        double g[4], v[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            g[e] = T(G, K, K)[r * LD + 4 * qg + e];
+            v[e] = r == 4 * qg + e ? 1.0 : 0.0;
+        }
+        auto step = [&](auto jtag) {
+            constexpr int j = decltype(jtag)::value;
+            const double d = readlane_d(g[j % 4], j + 16 * (j / 4));
+            const double gc = __shfl(g[j % 4], r + 16 * (j / 4));
+            double gr[4], vr[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                gr[e] = row_share_d<j>(g[e]);
+                vr[e] = row_share_d<j>(v[e]);
+            }
+            const double dj = d0[16 * K + j];
+            const bool keep = dj > 0.0 && d > 1e-12 * dj;
+            const double rcp = 1.0 / d;
+            const double inv = keep ? rcp : 0.0;
+            if (l == 0 && wave == 0) piv[16 * K + j] = keep ? d : 0.0;
+            const double f = r > j ? gc * inv : 0.0;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                g[e] -= f * gr[e];
+                v[e] -= f * vr[e];
+            }
+        };
+        unrolled(step, std::make_integer_sequence<int, 16>{});
+        if (wave == 0)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) T(Li, K, K)[r * LD + 4 * qg + e] = v[e];
+        lds_barrier();
+        if (K == 3) break;
+        // (b) panels
+        const int I = K + 1 + wave;
+        if (I < 4) {
+            const f64x4 y = tile_mm(T(G, I, K), false, T(Li, K, K), true, f64x4{0, 0, 0, 0}, 1.0);
+            tile_store(T(Li, I, K), y);
+            const double ip = piv[16 * K + l % 16] > 0.0 ? 1.0 / piv[16 * K + l % 16] : 0.0;
+            tile_store(T(G, I, K), y * ip);
+        }
+        lds_barrier();
+        // (c) trailing tiles (I, J), K < J <= I
+        for (int p = wave;; p += 4) {
+            int I2 = -1, J2 = -1, cnt = 0;
+            for (int a = K + 1; a < 4 && I2 < 0; ++a)
+                for (int b2 = K + 1; b2 <= a; ++b2)
+                    if (cnt++ == p) {
+                        I2 = a;
+                        J2 = b2;
+                        break;
+                    }
+            if (I2 < 0) break;
+            f64x4 acc = tile_load(T(G, I2, J2));
+            acc = tile_mm(T(G, I2, K), false, T(Li, J2, K), true, acc, -1.0);
+            tile_store(T(G, I2, J2), acc);
+        }
+        lds_barrier();
+    }
+    // X = U^-1 into Li's lower tiles (X_KK = V_K already there)
+    for (int dist = 1; dist < 4; ++dist) {
+        const int J = wave, I = J + dist;
+        f64x4 t = {0, 0, 0, 0};
+        if (I < 4) {
+            for (int K = J; K < I; ++K) t = tile_mm(T(G, I, K), false, T(Li, K, J), false, t, 1.0);
+        }
+        lds_barrier();  // every read of this round's inputs is done before X_IJ overwrites Y_IJ
+        if (I < 4) {
+            // X_IJ = -V_I t; t's result registers are the B operands of the k-steps
+            const double * Vi = T(Li, I, I);
+            f64x4 x = {0, 0, 0, 0};
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) x = mfma4(-Vi[(l % 16) * LD + 4 * ks + l / 16], t[ks], x);
+            tile_store(T(Li, I, J), x);
+        }
+        lds_barrier();
+    }
+    // rows scaled by the pivots^-1/2, tiles above the diagonal zero
+    for (int t2 = threadIdx.x; t2 < SS; t2 += blockDim.x) {
+        const int i = t2 / S, c = t2 % S;
+        const double sc = piv[i] > 0.0 ? 1.0 / sqrt(piv[i]) : 0.0;
+        Li[i * LD + c] = c / 16 > i / 16 ? 0.0 : Li[i * LD + c] * sc;
+    }
+    __syncthreads();
+}
+
 // global -> LDS copy of s x s matrices (zero padded), split into a fetch of
 // N matrices into registers and the LDS stores, so that every load of a
 // thread is in flight before its first store (a loop with a store per load
@@ -1073,6 +1391,11 @@ void solve_blocks(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std
             mp[p] = c.nAgg;
         }
     }
+    // the smoothed multiplicative cycle needs the coarse space of every problem
+    bool smooth = o.smooth > 0;
+    for (int p = 0; p < np; ++p) smooth = smooth && mp[p] > 0;
+    if (smooth)
+        for (int p = 0; p < np; ++p) smooth_prepare(*cs[p]);
     PG_CHECK(hipGetLastError());
     // the solver's own stream, after everything queued so far on the caller's
     if (!c0.solveStream) {
@@ -1114,6 +1437,20 @@ void solve_blocks(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std
                 bd.Einv = mp[p] ? c.Einv.p + (size_t)k * mp[p] * mp[p] : nullptr;
                 bd.Bsrc = c.B.p + (size_t)k * c.n * c.nE;
                 bd.Xdst = c.X.p + (size_t)k * c.n * c.nE;
+                if (smooth) {
+                    bd.valsAD = c.valsAD.p + (size_t)k * c.nnz;
+                    bd.azPtr = c.azPtr.p;
+                    bd.azAgg = c.azAgg.p;
+                    bd.azVals = c.azVals.p + (size_t)k * c.nAZ;
+                    // smoothing interval [b / 11, b], b the Gershgorin bound
+                    const double hiB = c.hGersh[k], loB = hiB / 11.0;
+                    const double th = 0.5 * (hiB + loB), dl = 0.5 * (hiB - loB);
+                    const double sg = th / dl, r0 = 1.0 / sg, r1 = 1.0 / (2.0 * sg - r0);
+                    bd.ch[0] = 1.0 / th;
+                    bd.ch[1] = r1 * r0 / th;
+                    bd.ch[2] = 2.0 * r1 / dl;
+                    bd.ch[3] = (1.0 + r1 * r0) / th;
+                }
                 pl.h.push_back(bd);
                 nmax = std::max(nmax, c.n);
                 mmax = std::max(mmax, mp[p]);
@@ -1219,14 +1556,25 @@ void solve_blocks(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std
             double mat = 0.0;
             for (int p = 0; p < np; ++p)
                 mat += (double)cs[p]->nK * ((double)cs[p]->nnz * (sizeof(AT) + 4) + 4.0 * (cs[p]->n + 1));
+            c0.kBytesPerLaunch = 0.0;  // latency-bound kernels: no algorithmic byte count
             switch (c0.kSelect) {
             case PGERT_TIME_SPMM: c0.kBytesPerLaunch = mat + 2.0 * vec; break;
             case PGERT_TIME_UPDATE: c0.kBytesPerLaunch = 6.0 * vec; break;
             case PGERT_TIME_GRAM_Z: c0.kBytesPerLaunch = 3.0 * vec; break;
             case PGERT_TIME_PUPDATE: c0.kBytesPerLaunch = 3.0 * vec; break;
             case PGERT_TIME_GRAM_PQ: c0.kBytesPerLaunch = 3.0 * vec; break;
-            case PGERT_TIME_ZFORM: c0.kBytesPerLaunch = 2.0 * vec + 12.0 * (double)nmax; break;
+            case PGERT_TIME_ZFORM:
+                // smoothing: the post-smoothing product gathers r2 and reads r, s0 of its row
+                c0.kBytesPerLaunch = smooth ? mat + 4.0 * vec + 12.0 * (double)nmax : 2.0 * vec + 12.0 * (double)nmax;
+                break;
             case PGERT_TIME_RESTRICT: c0.kBytesPerLaunch = vec; break;
+            case PGERT_TIME_SMOOTH: c0.kBytesPerLaunch = mat + 3.0 * vec; break;  // gather, row, 1-2 writes
+            case PGERT_TIME_COARSE_RESID: {
+                double az = 0.0;
+                for (int p = 0; p < np; ++p) az += (double)cs[p]->nK * (12.0 * cs[p]->nAZ + 4.0 * (cs[p]->n + 1));
+                c0.kBytesPerLaunch = 2.0 * vec + az;
+                break;
+            }
             default: break;
             }
         }
@@ -1236,8 +1584,13 @@ void solve_blocks(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std
         auto te = [&](int id) {
             if (c0.kSelect == id) c0.kev_end();
         };
-        DBuf<VT> Xw, Rw, Pw, Qw, Zw;
+        DBuf<VT> Xw, Rw, Pw, Qw, Zw, S0w, R1w, Ww;
         Zw.alloc(tot);
+        if (smooth) {
+            S0w.alloc(tot);
+            R1w.alloc(tot);
+            Ww.alloc(tot);
+        }
         Rw.alloc(tot);
         Pw.alloc(tot);
         Qw.alloc(tot);
@@ -1262,6 +1615,30 @@ void solve_blocks(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std
         int lastReplace = 0;
         const dim3 gG(G, nb), gU(GU, nb), gS(GS, nb), gZ(GZ, nb);
         auto z_grams = [&]() {
+            if (smooth) {
+                tb(PGERT_TIME_SMOOTH);
+                hipLaunchKernelGGL((k_smooth<VT, MODE_S0>), gS, blk, 0, s, pl.d.p, Rw.p, Rw.p, Rw.p, ccp, S0w.p,
+                                   Ww.p);
+                te(PGERT_TIME_SMOOTH);
+                tb(PGERT_TIME_SMOOTH);
+                hipLaunchKernelGGL((k_smooth<VT, MODE_S1>), gS, blk, 0, s, pl.d.p, Ww.p, S0w.p, Rw.p, ccp, R1w.p,
+                                   R1w.p);
+                te(PGERT_TIME_SMOOTH);
+                tb(PGERT_TIME_RESTRICT);
+                hipLaunchKernelGGL(k_restrict<VT>, dim3(mmax, nb), blk, 0, s, pl.d.p, R1w.p, c0.ycoarse.p);
+                te(PGERT_TIME_RESTRICT);
+                tb(PGERT_TIME_COARSE);
+                hipLaunchKernelGGL(k_coarse_apply, dim3(div_up(mmax, 16), 4, nb), dim3(64 * CA_WAVES), 0, s, pl.d.p,
+                                   c0.ycoarse.p, c0.ccoarse.p);
+                te(PGERT_TIME_COARSE);
+                tb(PGERT_TIME_COARSE_RESID);
+                hipLaunchKernelGGL(k_coarse_resid<VT>, gZ, blk, 0, s, pl.d.p, ccp, R1w.p);
+                te(PGERT_TIME_COARSE_RESID);
+                tb(PGERT_TIME_ZFORM);
+                hipLaunchKernelGGL((k_smooth<VT, MODE_SZ>), gS, blk, 0, s, pl.d.p, R1w.p, S0w.p, Rw.p, ccp, Zw.p,
+                                   Zw.p);
+                te(PGERT_TIME_ZFORM);
+            } else {
             if (anyCoarse) {
                 tb(PGERT_TIME_RESTRICT);
                 hipLaunchKernelGGL(k_restrict<VT>, dim3(mmax, nb), blk, 0, s, pl.d.p, Rw.p, c0.ycoarse.p);
@@ -1275,6 +1652,7 @@ void solve_blocks(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std
             tb(PGERT_TIME_ZFORM);
             hipLaunchKernelGGL(k_zform<VT>, gZ, blk, 0, s, pl.d.p, Rw.p, ccp, Zw.p);
             te(PGERT_TIME_ZFORM);
+            }
             tb(PGERT_TIME_GRAM_Z);
             hipLaunchKernelGGL((k_gram<VT, MODE_Z>), gG, dim3(GRAM_THREADS), 0, s, pl.d.p, Pw.p, Qw.p, Zw.p,
                                c0.part.p);

@@ -73,7 +73,8 @@ class ERTModelling:
         # start, then in FP64, and raises if that also misses; True: lastStats
         # carries converged = 0 instead
         self.solverOptions = dict(method="mixed", rtol=1e-6, max_iter=20000, check_every=8,
-                                  inner_rtol=1e-3, learned=False, coarse=0, graph=True, report_only=False)
+                                  inner_rtol=1e-3, learned=False, coarse=0, smooth=2, graph=True,
+                                  report_only=False)
         self._coarseSet = None
         self.lastStats = None
         self.warmStart = None
@@ -279,6 +280,7 @@ class ERTModelling:
             self._ctx.check(self._ctx.L.pgert_set_coarse(self._ctx.h, nagg), "set_coarse")
             self._coarseSet = nagg
         opts.coarse = 1 if nagg > 0 else 0
+        opts.smooth = int(o.get("smooth", 0) or 0) if nagg > 0 else 0
         opts.no_graph = 0 if o.get("graph", True) else 1
         opts.report_only = 1 if o.get("report_only", False) else 0
         if self.warmStart is not None:

@@ -96,16 +96,26 @@ def test_bypass_vs_oracle_and_reference(tmp_path, monkeypatch):
         s = ref.Session(mesh, copy.deepcopy(data), k, w, sr=False)
         rr = s.response(rho)
         assert np.abs(r - rr).max() <= 1e-9 * np.abs(rr).max()
-    # bypass electrodes need the electrode at the group's first node
-    with pytest.raises(Exception):
-        from pygimli_amd.mesh import Mesh
-        nm = mesh.nodeMarkers().copy()
-        g = np.nonzero(nm == -10000)[0]
-        nm[g[0]] = 0
-        bad = Mesh(2, mesh.nodes, mesh.cells, mesh.cellMarkers(), mesh.bounds, mesh.boundaryMarkers(), nm)
-        fop = ERTModelling(sr=False)
-        fop.setMesh(bad)
-        fop.setData(copy.deepcopy(data))
+    # a sensor whose bypass group lost its first node matches neither a group
+    # nor a node electrode: like the reference (searchElectrodes_,
+    # dcfemmodelling.cpp:979-985) it becomes a free electrode in a cell at
+    # the same position, which the oracle models the same way
+    from pygimli_amd.mesh import Mesh
+    nm = mesh.nodeMarkers().copy()
+    g = np.nonzero(nm == -10000)[0]
+    nm[g[0]] = 0
+    bad = Mesh(2, mesh.nodes, mesh.cells, mesh.cellMarkers(), mesh.bounds, mesh.boundaryMarkers(), nm)
+    fop = ERTModelling(sr=False)
+    fop.setBypassResistances(R)
+    fop.setMesh(bad)
+    fop.setData(copy.deepcopy(data))
+    fop.solverOptions.update(method="band", rtol=1e-10, coarse=0)
+    rb = fop.response(rho)
+    fb = eo.Forward(bad, copy.deepcopy(data), k, w, total_field=True)
+    fb.bypassR = R
+    assert (fb.setup.elecNode < 0).sum() == 1
+    ob = eo.response_total(fb, rho)
+    assert np.abs(rb - ob).max() <= 1e-9 * np.abs(ob).max()
 
 
 def test_topography_total_field():
