@@ -1014,7 +1014,7 @@ def main():
     # the MFMA products' 2 m n k
     vt = 4 if args.method == "mixed" else 8
     vec = float(nb) * n * 64 * vt
-    kernels = {1: ("k_spmm", None, None), 2: ("k_update", 6 * vec, 2 * 2 * 64 * 64 * n * nb),
+    kernels = {1: ("k_spmm", None, None), 2: ("k_update", 3 * vec, 2 * 64 * 64 * n * nb),
                3: ("k_gram<Z>", 3 * vec, 2 * 3 * 64 * 64 * n * nb), 4: ("k_pupdate", 3 * vec, 2 * 2 * 64 * 64 * n * nb),
                5: ("k_gram<PQR>", 3 * vec, 2 * 2 * 64 * 64 * n * nb),
                6: ("k_small_alpha", None, None), 7: ("k_small_beta", None, None),
@@ -1130,6 +1130,16 @@ def main():
         timed_out = cpu_proc.is_alive()
         if timed_out:
             cpu_proc.terminate()
+        elif not os.path.exists(cpu_path):
+            # the child ended without a result (exit code in the log): once more,
+            # now that the GPU work is done
+            log(f"[bench] cpu baseline child ended with exit code {cpu_proc.exitcode} and no result; running it again")
+            cpu_proc = mp.get_context("spawn").Process(target=cpu_baseline_worker,
+                                                       args=(cpu_path, nth, 0.02, 30, "ert3d_grid_grf", 50), daemon=True)
+            cpu_proc.start()
+            cpu_proc.join(args.cpu_timeout)
+            if cpu_proc.is_alive():
+                cpu_proc.terminate()
         out["cpu_baseline"] = cpu_baseline_result(cpu_path, nth, plain["iters"] if plain else None)
     print(json.dumps(out))
 

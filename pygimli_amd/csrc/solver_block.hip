@@ -7,7 +7,8 @@
 // (s <= 64 columns) and takes an s x s step:
 //   1. Q = A P                                              (k_spmm)
 //   2. G1 = P'Q, G2 = P'R;  alpha = G1^-1 G2                (k_gram<PQR>, k_small_alpha)
-//   3. X += P alpha;  R -= Q alpha;  |R|^2 per column       (k_update)
+//   3. R -= Q alpha;  |R|^2 per column                      (k_update; X += P alpha
+//      runs beside step 5's 64 x 64 work, in the same launch)
 //   4. Z = M^-1 R;  Q'Z, Z'Z, P'Z                  (k_gram<Z>; M = Jacobi [+ additive coarse]
 //      correction, k_zform], or the multiplicative two-level cycle with
 //      Chebyshev smoothing [k_smooth, k_restrict, k_coarse_apply, k_coarse_resid])
@@ -372,8 +373,8 @@ __global__ void __launch_bounds__(256) k_smooth(const BlockDesc * __restrict__ b
 
 // r2 = r1 - (A Z) c in place; one thread per 4 columns of a row.  Most rows
 // have one or two entries (their neighbours lie in one or two aggregates):
-// the first two entries' indices and coarse rows are loaded together
-// (a row with one entry repeats it with weight 0), further entries in a loop.
+// the first two entries' indices and coarse rows are loaded together (a row
+// with one entry repeats it with weight 0), further entries in a loop.
 template <typename VT>
 __global__ void __launch_bounds__(256) k_coarse_resid(const BlockDesc * __restrict__ blocks,
                                                       const double * __restrict__ cc, VT * __restrict__ R12) {
@@ -623,12 +624,12 @@ __device__ inline void strip_mm(const VT * __restrict__ op, const V4<VT> (&x)[4]
             for (int tb = 0; tb < 4; ++tb) c[tb] = mfma4(op[((4 * u + e) * 4 + tb) * 64 + lane], x[u][e], c[tb]);
 }
 
-// 3. X += P alpha, R -= Q alpha; partial |R|^2 per column -> part[(blk * G + g) * S]
+// 3. R -= Q alpha; partial |R|^2 per column -> part[(blk * G + g) * S]
+// (X += P alpha runs later, beside the beta step: x_update)
 template <typename VT>
 __global__ void __launch_bounds__(256) k_update(const BlockDesc * __restrict__ blocks,
-                                                const double * __restrict__ small, const VT * __restrict__ P,
-                                                const VT * __restrict__ Q, VT * __restrict__ X, VT * __restrict__ R,
-                                                double * __restrict__ part) {
+                                                const double * __restrict__ small, const VT * __restrict__ Q,
+                                                VT * __restrict__ R, double * __restrict__ part) {
     using V = V4<VT>;
     __shared__ VT op[16 * 4 * 64];
     __shared__ double cn[4][S];
@@ -644,71 +645,54 @@ __global__ void __launch_bounds__(256) k_update(const BlockDesc * __restrict__ b
     double rr[4][4] = {};
     const int nstrip = (n + 15) / 16;
     const int stride = gridDim.x * 4;
-    // software pipeline: the next strip's P and Q are in flight while the
-    // matrix cores work on the current one
-    auto fetch = [&](int si, V (&pp)[4], V (&qq)[4]) {
+    // software pipeline: the next strip's Q is in flight while the matrix
+    // cores work on the current one
+    auto fetch = [&](int si, V (&qq)[4]) {
         const int row = 16 * si + lane % 16;
         const bool ok = si < nstrip && row < n;
         const size_t o = vb + (size_t)row * S + cq;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            pp[u] = ok ? ld4(P + o + 16 * u) : V{0, 0, 0, 0};
-            qq[u] = ok ? ld4(Q + o + 16 * u) : V{0, 0, 0, 0};
-        }
+        for (int u = 0; u < 4; ++u) qq[u] = ok ? ld4(Q + o + 16 * u) : V{0, 0, 0, 0};
     };
     int sidx = blockIdx.x * 4 + wave;
-    V p[4], q[4];
-    fetch(sidx, p, q);
+    V q[4];
+    fetch(sidx, q);
     for (; sidx < nstrip; sidx += stride) {
         const int row = 16 * sidx + lane % 16;
         const bool ok = row < n;
         const size_t o = vb + (size_t)row * S + cq, ob = vb + (size_t)row * S;
-        // this strip's X and R in the result layout (independent of the products)
-        V xv[4], rv[4];
+        // this strip's R in the result layout (independent of the product)
+        V rv[4];
 #pragma unroll
         for (int tb = 0; tb < 4; ++tb) {
             if constexpr (sizeof(VT) == 4) {
-                xv[tb] = ok ? ld4(X + o + 16 * tb) : V{0, 0, 0, 0};
                 rv[tb] = ok ? ld4(R + o + 16 * tb) : V{0, 0, 0, 0};
             } else {
 #pragma unroll
-                for (int jj = 0; jj < 4; ++jj) {
-                    const size_t oc = ob + 16 * tb + drow<VT>(lane, jj);
-                    xv[tb][jj] = ok ? X[oc] : 0.0;
-                    rv[tb][jj] = ok ? R[oc] : 0.0;
-                }
+                for (int jj = 0; jj < 4; ++jj) rv[tb][jj] = ok ? R[ob + 16 * tb + drow<VT>(lane, jj)] : 0.0;
             }
         }
-        V pn[4], qn[4], cx[4], cr[4];
-        fetch(sidx + stride, pn, qn);
+        V qn[4], cr[4];
+        fetch(sidx + stride, qn);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) cx[u] = cr[u] = V{0, 0, 0, 0};
-        strip_mm(op, p, cx, lane);
+        for (int u = 0; u < 4; ++u) cr[u] = V{0, 0, 0, 0};
         strip_mm(op, q, cr, lane);
         if (ok) {
 #pragma unroll
             for (int tb = 0; tb < 4; ++tb) {
-                const V xn = xv[tb] + cx[tb], rn = rv[tb] - cr[tb];
+                const V rn = rv[tb] - cr[tb];
 #pragma unroll
                 for (int jj = 0; jj < 4; ++jj) rr[tb][jj] += (double)rn[jj] * (double)rn[jj];
                 if constexpr (sizeof(VT) == 4) {
-                    st4(X + o + 16 * tb, xn);
                     st4(R + o + 16 * tb, rn);
                 } else {
 #pragma unroll
-                    for (int jj = 0; jj < 4; ++jj) {
-                        const size_t oc = ob + 16 * tb + drow<VT>(lane, jj);
-                        X[oc] = xn[jj];
-                        R[oc] = rn[jj];
-                    }
+                    for (int jj = 0; jj < 4; ++jj) R[ob + 16 * tb + drow<VT>(lane, jj)] = rn[jj];
                 }
             }
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            p[u] = pn[u];
-            q[u] = qn[u];
-        }
+        for (int u = 0; u < 4; ++u) q[u] = qn[u];
     }
     // sum over the 16 rows of a lane group, then over the wavefronts
 #pragma unroll
@@ -725,6 +709,66 @@ __global__ void __launch_bounds__(256) k_update(const BlockDesc * __restrict__ b
     __syncthreads();
     if (wave == 0)
         part[((size_t)b * gridDim.x + blockIdx.x) * S + lane] = cn[0][lane] + cn[1][lane] + cn[2][lane] + cn[3][lane];
+}
+
+// X += P alpha over the strips g, g + ng, ... (workgroup g of ng); op: LDS
+// space for the staged alpha (16 * 4 * 64 VT)
+template <typename VT>
+__device__ void x_update(const BlockDesc & bd, const double * alpha, const VT * __restrict__ P, VT * __restrict__ X,
+                         VT * op, int g, int ng) {
+    using V = V4<VT>;
+    stage_op(op, alpha);
+    __syncthreads();
+    const int n = bd.n;
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const size_t vb = bd.off;
+    const int cq = 4 * (lane / 16);
+    const int nstrip = (n + 15) / 16;
+    const int stride = ng * 4;
+    auto fetch = [&](int si, V (&pp)[4]) {
+        const int row = 16 * si + lane % 16;
+        const bool ok = si < nstrip && row < n;
+        const size_t o = vb + (size_t)row * S + cq;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) pp[u] = ok ? ld4(P + o + 16 * u) : V{0, 0, 0, 0};
+    };
+    int sidx = g * 4 + wave;
+    V p[4];
+    fetch(sidx, p);
+    for (; sidx < nstrip; sidx += stride) {
+        const int row = 16 * sidx + lane % 16;
+        const bool ok = row < n;
+        const size_t o = vb + (size_t)row * S + cq, ob = vb + (size_t)row * S;
+        V xv[4];
+#pragma unroll
+        for (int tb = 0; tb < 4; ++tb) {
+            if constexpr (sizeof(VT) == 4) {
+                xv[tb] = ok ? ld4(X + o + 16 * tb) : V{0, 0, 0, 0};
+            } else {
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj) xv[tb][jj] = ok ? X[ob + 16 * tb + drow<VT>(lane, jj)] : 0.0;
+            }
+        }
+        V pn[4], cx[4];
+        fetch(sidx + stride, pn);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) cx[u] = V{0, 0, 0, 0};
+        strip_mm(op, p, cx, lane);
+        if (ok) {
+#pragma unroll
+            for (int tb = 0; tb < 4; ++tb) {
+                const V xn = xv[tb] + cx[tb];
+                if constexpr (sizeof(VT) == 4) {
+                    st4(X + o + 16 * tb, xn);
+                } else {
+#pragma unroll
+                    for (int jj = 0; jj < 4; ++jj) X[ob + 16 * tb + drow<VT>(lane, jj)] = xn[jj];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) p[u] = pn[u];
+    }
 }
 
 // 5. P = Z T + P M2
@@ -1218,12 +1262,24 @@ __global__ void __launch_bounds__(256) k_small_alpha(const BlockDesc * __restric
 }
 
 // init = 1: first step of a block (no previous direction): beta = 0, W = Z.
+// Workgroup 0 of each block (blockIdx.y) takes the beta step; workgroups
+// 1 .. gridDim.x - 1 meanwhile add the previous direction's step into the
+// solution, X += P alpha (independent of beta; it must precede the P update,
+// the next launch), so that streaming work fills the GPU while the single
+// 64 x 64 workgroup runs.
+template <typename VT>
 __global__ void __launch_bounds__(256) k_small_beta(const BlockDesc * __restrict__ blocks, double * __restrict__ small,
-                                                    int init) {
+                                                    int init, const VT * __restrict__ P, VT * __restrict__ X) {
     extern __shared__ double lds[];
-    if (!blocks[blockIdx.x].active) return;
-    const int s = blocks[blockIdx.x].s;
-    double * sm = small + (size_t)blockIdx.x * NSL * SS;
+    const int b = blockIdx.y;
+    if (!blocks[b].active) return;
+    if (blockIdx.x > 0) {
+        x_update<VT>(blocks[b], small + ((size_t)b * NSL + SL_G2) * SS, P, X, reinterpret_cast<VT *>(lds),
+                     blockIdx.x - 1, gridDim.x - 1);
+        return;
+    }
+    const int s = blocks[b].s;
+    double * sm = small + (size_t)b * NSL * SS;
     double *M0 = lds, *M1 = lds + SMAT, *Bt = lds + 2 * SMAT, *W = lds + 3 * SMAT;
     if (!init) {
         // W = Z'Z, M0 = G1^-1, M1 = Q'Z; P'Z waits in registers for M0
@@ -1463,7 +1519,10 @@ void solve_blocks(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std
     // grids: the Gram passes keep ~2048 workgroups (partials of 2-3 64 x 64
     // FP64 matrices each); the streaming kernels ~64k workgroups overall
     const int G = wg_per_block(nmax, 16, nb, 2048, 256);
-    const int GU = wg_per_block(nmax, 64, nb, 4096, 1024);
+    // streaming updates: 256 workgroups (one per CU) with a long strip loop
+    // each beat 1024-4096 short-lived ones (measured on the 202k-node block:
+    // update 60 vs 70-79 us, P update 46 vs 47-53 us)
+    const int GU = wg_per_block(nmax, 64, nb, 4096, 256);
     const int GS = NXCD * wg_per_block(nmax, 16 * NXCD, nb, 65536 / NXCD, 256);
     const int GR = NXCD * wg_per_block(nmax, 4 * NXCD, nb, 16384 / NXCD, 128);
     const int GZ = wg_per_block(nmax * (S / 4), 256, nb, 65536, 1 << 20);
@@ -1482,7 +1541,9 @@ void solve_blocks(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std
     const double * ccp = anyCoarse ? c0.ccoarse.p : nullptr;
     PG_CHECK(hipFuncSetAttribute((const void *)k_small_alpha, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)SMALL_LDS));
-    PG_CHECK(hipFuncSetAttribute((const void *)k_small_beta, hipFuncAttributeMaxDynamicSharedMemorySize,
+    PG_CHECK(hipFuncSetAttribute((const void *)k_small_beta<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)SMALL_LDS));
+    PG_CHECK(hipFuncSetAttribute((const void *)k_small_beta<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)SMALL_LDS));
 
     // right-hand sides and the FP64 solution in block layout
@@ -1559,7 +1620,7 @@ void solve_blocks(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std
             c0.kBytesPerLaunch = 0.0;  // latency-bound kernels: no algorithmic byte count
             switch (c0.kSelect) {
             case PGERT_TIME_SPMM: c0.kBytesPerLaunch = mat + 2.0 * vec; break;
-            case PGERT_TIME_UPDATE: c0.kBytesPerLaunch = 6.0 * vec; break;
+            case PGERT_TIME_UPDATE: c0.kBytesPerLaunch = 3.0 * vec; break;
             case PGERT_TIME_GRAM_Z: c0.kBytesPerLaunch = 3.0 * vec; break;
             case PGERT_TIME_PUPDATE: c0.kBytesPerLaunch = 3.0 * vec; break;
             case PGERT_TIME_GRAM_PQ: c0.kBytesPerLaunch = 3.0 * vec; break;
@@ -1679,14 +1740,15 @@ void solve_blocks(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std
             hipLaunchKernelGGL(k_small_alpha, dim3(nb), blk, SMALL_LDS, s, pl.d.p, c0.small.p);
             te(PGERT_TIME_SMALL_ALPHA);
             tb(PGERT_TIME_UPDATE);
-            hipLaunchKernelGGL(k_update<VT>, gU, blk, 0, s, pl.d.p, c0.small.p, Pw.p, Qw.p, Xp, Rw.p, c0.part.p);
+            hipLaunchKernelGGL(k_update<VT>, gU, blk, 0, s, pl.d.p, c0.small.p, Qw.p, Rw.p, c0.part.p);
             te(PGERT_TIME_UPDATE);
             if (rr)
                 hipLaunchKernelGGL(k_part_reduce, dim3(S / 32, nb), blk, 0, s, pl.d.p, GU, S, c0.part.p, c0.small.p,
                                    SL_RR * SS);
             z_grams();
             tb(PGERT_TIME_SMALL_BETA);
-            hipLaunchKernelGGL(k_small_beta, dim3(nb), blk, SMALL_LDS, s, pl.d.p, c0.small.p, 0);
+            hipLaunchKernelGGL(k_small_beta<VT>, dim3(1 + GU, nb), blk, SMALL_LDS, s, pl.d.p, c0.small.p, 0, Pw.p,
+                               Xp);
             te(PGERT_TIME_SMALL_BETA);
             tb(PGERT_TIME_PUPDATE);
             hipLaunchKernelGGL(k_pupdate<VT>, gU, blk, 0, s, pl.d.p, c0.small.p, Zw.p, Pw.p);
@@ -1696,7 +1758,8 @@ void solve_blocks(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std
             // P = 0, Q = 0: Z = M^-1 R and Z'Z only; then P = Z chol(Z'Z)^-T
             PG_CHECK(hipMemsetAsync(c0.small.p, 0, c0.small.bytes(), s));
             z_grams();
-            hipLaunchKernelGGL(k_small_beta, dim3(nb), blk, SMALL_LDS, s, pl.d.p, c0.small.p, 1);
+            hipLaunchKernelGGL(k_small_beta<VT>, dim3(1, nb), blk, SMALL_LDS, s, pl.d.p, c0.small.p, 1,
+                               (const VT *)nullptr, (VT *)nullptr);
             hipLaunchKernelGGL(k_pupdate<VT>, gU, blk, 0, s, pl.d.p, c0.small.p, Zw.p, Pw.p);
             PG_CHECK(hipGetLastError());
         };
