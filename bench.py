@@ -50,6 +50,8 @@ def cpu_baseline_worker(out_path, nthreads, sample_frac, pcg_iters, workload="er
     contiguous sample of cells, scaled to all cells.
     Threads: min(8, nCPU-2) (the reference's default thread count,
     modellingbase.cpp:77), the PCG split over right-hand sides."""
+    import faulthandler
+    faulthandler.enable()  # a crash in native code prints the Python stack to stderr
     from oracle import ert_oracle as eo
     from oracle import port
     from pygimli_amd import workloads

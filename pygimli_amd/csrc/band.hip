@@ -519,6 +519,13 @@ void solve_band(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std::
             maxnnz = std::max(maxnnz, c->nnz);
             h.push_back(d);
         }
+    // longest work first: workgroups are dispatched in index order, so the
+    // largest factorisations (n (b + 1)^2) start in the first round instead of
+    // trailing behind many small ones (the storage offsets above are fixed per
+    // descriptor, so the order is free)
+    std::stable_sort(h.begin(), h.end(), [](const BandDesc & a, const BandDesc & b2) {
+        return (double)a.n * (a.b + 1) * (a.b + 1) > (double)b2.n * (b2.b + 1) * (b2.b + 1);
+    });
     DBuf<unsigned char> & ddb = c0.bandDesc;
     ddb.reserve(h.size() * sizeof(BandDesc));
     PG_CHECK(hipMemcpyAsync(ddb.p, h.data(), h.size() * sizeof(BandDesc), hipMemcpyHostToDevice, s));

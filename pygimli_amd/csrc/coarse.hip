@@ -22,14 +22,17 @@ namespace {
 
 constexpr int GJB = 32;  // Gauss-Jordan block size
 
-__global__ void k_coarse_assemble(int nce, int m, const int * __restrict__ cptr, const int * __restrict__ cslot,
-                                  const int * __restrict__ cij, const double * __restrict__ vals,
-                                  double * __restrict__ E) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+// one wavefront per coarse entry: lane l sums slots l, l + 64, ... of the
+// entry, then a fixed butterfly over the lanes (deterministic)
+__global__ void __launch_bounds__(256) k_coarse_assemble(int nce, int m, const int * __restrict__ cptr,
+                                                         const int * __restrict__ cslot, const int * __restrict__ cij,
+                                                         const double * __restrict__ vals, double * __restrict__ E) {
+    const int t = blockIdx.x * 4 + threadIdx.x / 64, l = threadIdx.x % 64;
     if (t >= nce) return;
     double s = 0.0;
-    for (int q = cptr[t]; q < cptr[t + 1]; ++q) s += vals[cslot[q]];
-    E[(size_t)cij[2 * t] * m + cij[2 * t + 1]] = s;
+    for (int q = cptr[t] + l; q < cptr[t + 1]; q += 64) s += vals[cslot[q]];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (l == 0) E[(size_t)cij[2 * t] * m + cij[2 * t + 1]] = s;
 }
 
 // (a) Pinv = inverse of the GJB x GJB diagonal block K (one workgroup, LDS
@@ -197,7 +200,7 @@ void coarse_prepare(Ctx & c) {
     for (int kk = 0; kk < c.nK; ++kk) {
         double * M = c.Einv.p + (size_t)kk * m * m;
         PG_CHECK(hipMemsetAsync(M, 0, (size_t)m * m * 8, c.stream));
-        hipLaunchKernelGGL(k_coarse_assemble, dim3(div_up(c.nCE, 256)), dim3(256), 0, c.stream, c.nCE, m, c.cePtr.p,
+        hipLaunchKernelGGL(k_coarse_assemble, dim3(div_up(c.nCE, 4)), dim3(256), 0, c.stream, c.nCE, m, c.cePtr.p,
                            c.ceSlot.p, c.ceIJ.p, c.valsA.p + (size_t)kk * c.nnz, M);
         for (int K0 = 0; K0 < m; K0 += GJB) {
             hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(256), 0, c.stream, m, K0, M, Pinv.p);
