@@ -59,36 +59,40 @@ def bessel_i1(x):
 
 
 def bessel_k0(x):
-    x = np.asarray(x, np.float64)
-    out = np.empty_like(x)
-    s = x <= 2.0
-    xs = x[s]
-    y = xs * xs / 4.0
-    out[s] = (-np.log(xs / 2.0) * bessel_i0(xs)) + (-0.57721566 + y * (0.42278420 + y * (
-        0.23069756 + y * (0.3488590e-1 + y * (0.262698e-2 + y * (0.10750e-3 + y * 0.74e-5))))))
-    xl = x[~s]
-    y = 2.0 / xl
-    out[~s] = (np.exp(-xl) / np.sqrt(xl)) * (1.25331414 + y * (-0.7832358e-1 + y * (
-        0.2189568e-1 + y * (-0.1062446e-1 + y * (0.587872e-2 + y * (
-            -0.251540e-2 + y * 0.53208e-3))))))
-    return out
+    # x = 0 (a point on the source): log(0) and 1/0 give the limits -inf / +inf
+    with np.errstate(divide="ignore", invalid="ignore"):
+        x = np.asarray(x, np.float64)
+        out = np.empty_like(x)
+        s = x <= 2.0
+        xs = x[s]
+        y = xs * xs / 4.0
+        out[s] = (-np.log(xs / 2.0) * bessel_i0(xs)) + (-0.57721566 + y * (0.42278420 + y * (
+            0.23069756 + y * (0.3488590e-1 + y * (0.262698e-2 + y * (0.10750e-3 + y * 0.74e-5))))))
+        xl = x[~s]
+        y = 2.0 / xl
+        out[~s] = (np.exp(-xl) / np.sqrt(xl)) * (1.25331414 + y * (-0.7832358e-1 + y * (
+            0.2189568e-1 + y * (-0.1062446e-1 + y * (0.587872e-2 + y * (
+                -0.251540e-2 + y * 0.53208e-3))))))
+        return out
 
 
 def bessel_k1(x):
-    x = np.asarray(x, np.float64)
-    out = np.empty_like(x)
-    s = x <= 2.0
-    xs = x[s]
-    y = xs * xs / 4.0
-    out[s] = (np.log(xs / 2.0) * bessel_i1(xs)) + (1.0 / xs) * (1.0 + y * (0.15443144 + y * (
-        -0.67278579 + y * (-0.18156897 + y * (-0.1919402e-1 + y * (
-            -0.110404e-2 + y * (-0.4686e-4)))))))
-    xl = x[~s]
-    y = 2.0 / xl
-    out[~s] = (np.exp(-xl) / np.sqrt(xl)) * (1.25331414 + y * (0.23498619 + y * (
-        -0.3655620e-1 + y * (0.1504268e-1 + y * (-0.780353e-2 + y * (
-            0.325614e-2 + y * (-0.68245e-3)))))))
-    return out
+    # x = 0 (a point on the source): log(0) and 1/0 give the limits -inf / +inf
+    with np.errstate(divide="ignore", invalid="ignore"):
+        x = np.asarray(x, np.float64)
+        out = np.empty_like(x)
+        s = x <= 2.0
+        xs = x[s]
+        y = xs * xs / 4.0
+        out[s] = (np.log(xs / 2.0) * bessel_i1(xs)) + (1.0 / xs) * (1.0 + y * (0.15443144 + y * (
+            -0.67278579 + y * (-0.18156897 + y * (-0.1919402e-1 + y * (
+                -0.110404e-2 + y * (-0.4686e-4)))))))
+        xl = x[~s]
+        y = 2.0 / xl
+        out[~s] = (np.exp(-xl) / np.sqrt(xl)) * (1.25331414 + y * (0.23498619 + y * (
+            -0.3655620e-1 + y * (0.1504268e-1 + y * (-0.780353e-2 + y * (
+                0.325614e-2 + y * (-0.68245e-3)))))))
+        return out
 
 
 # --------------------------------------------------------------------------- a1
@@ -435,7 +439,10 @@ def mixed_bc_alpha(mesh, source, k):
         return (rma * rma * rn / ra + ra * ra * rmn / rma) / (rma * ra * (ra + rma))
     k0a = bessel_k0(ra * k)
     k0m = bessel_k0(rma * k)
-    out = k * (rn / ra * bessel_k1(ra * k) + rmn / rma * bessel_k1(rma * k)) / (k0a + k0m)
+    # faces far from the source at large k: both K0 underflow to 0 and the
+    # quotient is 0/0; the reference sets those to 0 (the mask below)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        out = k * (rn / ra * bessel_k1(ra * k) + rmn / rma * bessel_k1(rma * k)) / (k0a + k0m)
     return np.where((np.abs(k0a) < TOLERANCE) | (np.abs(k0m) < TOLERANCE), 0.0, out)
 
 

@@ -39,10 +39,12 @@ def _fop(name, method, rtol=1e-10):
     fop = ERTModelling()
     fop.setMesh(mesh)
     fop.setData(copy.deepcopy(data))
-    coarse = 0
-    if method.endswith("+coarse"):
+    coarse, smooth = 0, 2
+    if method.endswith("+coarse"):      # multiplicative two-level cycle (the default with a coarse space)
         method, coarse = method.split("+")[0], 64
-    fop.solverOptions.update(method=method, rtol=rtol, coarse=coarse)
+    elif method.endswith("+additive"):  # additive two-level preconditioner
+        method, coarse, smooth = method.split("+")[0], 64, 0
+    fop.solverOptions.update(method=method, rtol=rtol, coarse=coarse, smooth=smooth)
     return fop, mesh, data, rho
 
 
@@ -61,7 +63,8 @@ def test_system_matrix(name, oracle_results):
 
 
 @pytest.mark.parametrize("name", CASES)
-@pytest.mark.parametrize("method", ["pcg64", "block64", "mixed", "block64+coarse", "mixed+coarse"])
+@pytest.mark.parametrize("method", ["pcg64", "block64", "mixed", "block64+coarse", "mixed+coarse", "block64+additive",
+                                    "mixed+additive"])
 def test_forward_and_jacobian(name, method, oracle_results):
     fo, rhoa_o, J_o = oracle_results[name]
     fop, mesh, data, rho = _fop(name, method)
@@ -80,6 +83,30 @@ def test_forward_and_jacobian(name, method, oracle_results):
     jerr = np.linalg.norm(J - J_o) / np.linalg.norm(J_o)
     print("J rel fro", jerr)
     assert jerr < 1e-5
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("method", ["block64", "mixed"])
+def test_two_level_cycle_cuts_iterations(name, method, oracle_results):
+    """The multiplicative two-level cycle with Chebyshev smoothing (smooth=2)
+    and the additive preconditioner (smooth=0) reach the same potentials (the
+    oracle's, to the solve tolerance), the cycle in fewer iterations; a solve
+    with too few iterations for the cycle reports non-convergence."""
+    fo = oracle_results[name][0]
+    its = {}
+    for smooth in (0, 2):
+        fop, mesh, data, rho = _fop(name, method + "+coarse", rtol=1e-10)
+        fop.solverOptions.update(smooth=smooth)
+        fop.response(rho)
+        its[smooth] = fop.lastStats["iterations"]
+        U = fop.subPotentials().cpu().numpy()
+        err = np.linalg.norm(U - fo.subpots) / np.linalg.norm(fo.subpots)
+        print(name, method, "smooth", smooth, fop.lastStats, "potential rel", err)
+        assert fop.lastStats["converged"] == 1 and err < 1e-6
+    assert its[2] <= 0.6 * its[0]
+    fop.solverOptions.update(smooth=2, max_iter=2, report_only=True)
+    fop.response(rho)
+    assert fop.lastStats["converged"] == 0
 
 
 @pytest.mark.parametrize("name", CASES)
