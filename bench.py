@@ -72,6 +72,15 @@ def cpu_baseline_worker(out_path, nthreads, sample_frac, pcg_iters, workload="er
     if ref_data > 0:
         from oracle import ref
         if ref.available():
+            # the reference fills its shape-function caches lazily
+            # (ShapeFunctionCache, shape.cpp:253-317) with no lock, so its
+            # threaded createJacobian can crash when the threads race on the
+            # first fill: fill them first on a small mesh of the same cell type,
+            # single-threaded
+            ms, ds, rs = (workloads.small_3d if mesh.dim() == 3 else workloads.small_2d)()
+            warm = ref.Session(ms, ds, nthreads=1)
+            warm.response(rs)
+            warm.jacobian_only(rs)
             sub = workloads.make_data(data.sensorPositions(), data.abmn()[:ref_data])
             sub.set("k", fo.geometric_factors()[:ref_data])
             t = time.time()

@@ -1322,35 +1322,77 @@ __global__ void __launch_bounds__(256) k_small_beta(const BlockDesc * __restrict
 // per column, stored as VT (the recurrence restarts from a residual of unit
 // scale, so its FP32 products neither underflow nor overflow however small
 // the true residual of a block is).
+// In k_spmm's scheme: a wavefront takes 4 rows (16 lanes x 4 FP64 columns
+// per row), the row's indices and values go through the wavefront's LDS
+// slice and its 16 gathers of X rows are issued together; rows past hi read
+// row hi - 1 and keep nothing.
 template <typename VT>
 __global__ void __launch_bounds__(256) k_bresid(const BlockDesc * __restrict__ blocks, const double * __restrict__ X,
                                                 const double * __restrict__ B, VT * __restrict__ R,
                                                 double * __restrict__ part, const double * __restrict__ rinv) {
+    __shared__ int sIdx[4][4][SPMM_W];
+    __shared__ double sVal[4][4][SPMM_W];
     __shared__ double cn[4][2][S];
     const int b = blockIdx.y;
     const BlockDesc bd = blocks[b];
-    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, rg = lane / 16, j = lane % 16;
     const int n = bd.n;
-    const size_t vb = bd.off + lane;
     const auto A = glob(bd.valsD);
     const auto rowptr = glob(bd.rowptr);
     const auto colidx = glob(bd.colidx);
-    int lo, hi, first, stride;
-    xcd_rows(n, lo, hi, first, stride);
-    double rr = 0.0, bb = 0.0;
-    for (int row = first; row < hi; row += stride) {
-        double y = 0.0;
-        for (int t = rowptr[row]; t < rowptr[row + 1]; ++t) y += A[t] * X[vb + (size_t)colidx[t] * S];
-        const size_t o = vb + (size_t)row * S;
-        const double bv = B[o];
-        const double r = bv - y;
-        if (rinv) R[o] = (VT)(r * rinv[(size_t)b * S + lane]);
-        rr += r * r;
-        bb += bv * bv;
+    const double * Xb = X + bd.off + 4 * j;
+    const double * Bb = B + bd.off + 4 * j;
+    VT * Rb = R + bd.off + 4 * j;
+    const int x = blockIdx.x % NXCD, nw = gridDim.x / NXCD;
+    const int lo = (int)((long)n * x / NXCD), hi = (int)((long)n * (x + 1) / NXCD);
+    const int stride = 16 * nw;
+    int * myIdx = sIdx[wave][rg];
+    double * myVal = sVal[wave][rg];
+    f64x4 sc = {1.0, 1.0, 1.0, 1.0};
+    if (rinv) sc = ld4(rinv + (size_t)b * S + 4 * j);
+    f64x4 rr = {0, 0, 0, 0}, bb = {0, 0, 0, 0};
+    for (int r0 = lo + 4 * ((blockIdx.x / NXCD) * 4 + wave); r0 < hi; r0 += stride) {
+        const int row = r0 + rg, rowc = min(row, hi - 1);
+        const int p0 = rowptr[rowc], p1 = rowptr[rowc + 1];
+        const f64x4 bv = ld4(Bb + (size_t)rowc * S);
+        f64x4 y = {0, 0, 0, 0};
+        for (int q0 = p0; q0 < p1; q0 += SPMM_W) {
+            const int q = q0 + j;
+            const bool in = q < p1;
+            const int qc = in ? q : p0;
+            const int c = colidx[qc];
+            const double v = A[qc];
+            myIdx[j] = in ? c : 0;
+            myVal[j] = in ? v : 0.0;
+            __builtin_amdgcn_wave_barrier();
+            f64x4 xv[SPMM_W];
+#pragma unroll
+            for (int u = 0; u < SPMM_W; ++u) xv[u] = ld4(Xb + (size_t)myIdx[u] * S);
+#pragma unroll
+            for (int u = 0; u < SPMM_W; ++u) y += myVal[u] * xv[u];
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (row < hi) {
+            const f64x4 r = bv - y;
+            if (rinv) st4(Rb + (size_t)row * S, __builtin_convertvector(r * sc, V4<VT>));
+            rr += r * r;
+            bb += bv * bv;
+        }
     }
     if (rinv) return;
-    cn[wave][0][lane] = rr;
-    cn[wave][1][lane] = bb;
+    // sum over the 4 row groups of the wavefront, then over the wavefronts
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        double a = rr[e], c2 = bb[e];
+        a += __shfl_xor(a, 16);
+        a += __shfl_xor(a, 32);
+        c2 += __shfl_xor(c2, 16);
+        c2 += __shfl_xor(c2, 32);
+        if (rg == 0) {
+            cn[wave][0][4 * j + e] = a;
+            cn[wave][1][4 * j + e] = c2;
+        }
+    }
     __syncthreads();
     if (wave == 0) {
         double * out = part + ((size_t)b * gridDim.x + blockIdx.x) * 2 * S;
@@ -1812,7 +1854,9 @@ void solve_blocks(const std::vector<Ctx *> & cs, const pgert_solve_opts & o, std
             }
             // Residual replacement: the true FP64 residual b - A x decides which
             // blocks are done.  FP32 recursions are replaced when they have
-            // gained a decade, reached the target or run 256 iterations; FP64
+            // gained a decade, reached the target or run 256 iterations (every
+            // two decades instead: 216 iterations instead of 168 on the 3D
+            // headline, the restarts keep the FP32 recursion on course); FP64
             // ones when a block's recursive residual reaches the target.
             const bool replaceNow =
                 fp32 ? (worst < 0.1 * lastRel || anyAtTarget || it - lastReplace >= 256) : anyAtTarget;
