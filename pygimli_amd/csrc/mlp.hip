@@ -83,7 +83,7 @@ __device__ void features(const FeatArgs & a, int node, int e, float * f) {
     dm[up] = (float)(x[up] - (2.0 * zs - s[up]));
     float rm = sqrtf(dm[0] * dm[0] + dm[1] * dm[1] + dm[2] * dm[2]);
     float p = (float)a.prim[(size_t)node * a.nE + e];
-    float lrs = (float)log10(a.rhoS[e]);
+    float lrs = log10f((float)a.rhoS[e]);
     float l0 = a.lr0[node], l1 = a.s1[node], l2 = a.s2[node], l3 = a.s3[node];
     float pn = 6.2831853f * L * p;
     float ex = expf(-r / L), rr = r / (r + L);
@@ -260,6 +260,9 @@ __device__ float mlp_tile(const MlpL & w, unsigned short * x0, unsigned short * 
 
 // Inference over every (node, source) sample of one wavenumber: features on
 // the fly, MLP, and X0 = t_hat rho_s p written in the solver's node-major layout.
+// A wavefront takes 64 samples at a time: every lane computes one sample's
+// features (kept in registers), then the four 16-sample tiles go through the
+// MLP, the lanes of tile t handing their features over through LDS.
 __global__ void __launch_bounds__(64 * MLP_WAVES) k_mlp_warmstart(long nsamp, FeatArgs a, MlpW w,
                                                                   double * __restrict__ X0) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -267,28 +270,32 @@ __global__ void __launch_bounds__(64 * MLP_WAVES) k_mlp_warmstart(long nsamp, Fe
     __shared__ __attribute__((aligned(16))) unsigned short hA[MLP_WAVES][16 * NH], hB[MLP_WAVES][16 * NH];
     const MlpL wl = stage_weights(w, smem);
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    const long ntile = (nsamp + 15) / 16;
-    for (long tile = (long)blockIdx.x * MLP_WAVES + wave; tile < ntile; tile += (long)gridDim.x * MLP_WAVES) {
-        const long s0 = tile * 16;
-        if (lane < 16) {
-            float f[NF];
-            long s = s0 + lane;
+    const long ngroup = (nsamp + 63) / 64;
+    for (long grp = (long)blockIdx.x * MLP_WAVES + wave; grp < ngroup; grp += (long)gridDim.x * MLP_WAVES) {
+        const long s0 = grp * 64;
+        float f[NF];
+        {
+            const long s = s0 + lane;
             if (s < nsamp) features(a, (int)(s / a.nE), (int)(s % a.nE), f);
             else
                 for (int q = 0; q < NF; ++q) f[q] = 0.f;
-            for (int q = 0; q < NF; ++q) xs[wave][lane * NF + q] = f2bf(f[q]);
         }
-        __builtin_amdgcn_wave_barrier();
-        float y = mlp_tile(wl, xs[wave], hA[wave], hB[wave], lane, nullptr, nullptr, nullptr, nullptr, nullptr,
-                           nullptr);
-        if (lane < 16) {
-            long s = s0 + lane;
-            if (s < nsamp) {
-                int e = (int)(s % a.nE);
-                X0[s] = (double)y * a.rhoS[e] * a.prim[s];
+        for (int t = 0; t < 4; ++t) {
+            if (lane / 16 == t)
+#pragma unroll
+                for (int q = 0; q < NF; ++q) xs[wave][(lane % 16) * NF + q] = f2bf(f[q]);
+            __builtin_amdgcn_wave_barrier();
+            const float y = mlp_tile(wl, xs[wave], hA[wave], hB[wave], lane, nullptr, nullptr, nullptr,
+                                     nullptr, nullptr, nullptr);
+            if (lane < 16) {
+                const long s = s0 + 16 * t + lane;
+                if (s < nsamp) {
+                    const int e = (int)(s % a.nE);
+                    X0[s] = (double)y * a.rhoS[e] * a.prim[s];
+                }
             }
+            __builtin_amdgcn_wave_barrier();
         }
-        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -639,7 +646,7 @@ void mlp_warm_start(Ctx & c) {
     for (int kk = 0; kk < c.nK; ++kk) {
         FeatArgs a = feat_args(c, kk);
         long nsamp = (long)c.n * c.nE;
-        hipLaunchKernelGGL(k_mlp_warmstart, dim3(std::max(1L, std::min(512L, (nsamp + 127) / 128))),
+        hipLaunchKernelGGL(k_mlp_warmstart, dim3(std::max(1L, std::min(512L, (nsamp + 511) / 512))),
                            dim3(64 * MLP_WAVES), MLP_W_LDS, c.stream, nsamp, a, weights(m),
                            c.X.p + (size_t)kk * c.n * c.nE);
         PG_CHECK(hipGetLastError());
