@@ -93,6 +93,18 @@ int pgert_set_mesh(pgert_ctx * ctx, int dim, int n_nodes, const double * xyz,
                    int n_bounds, const int * bounds, const int * bound_marker,
                    const int * bound_cell);
 
+/* pgert_set_mesh whose matrix pattern also couples the n_couple node pairs
+ * (couple_a[i], couple_b[i]) that may share no cell: bypass resistors between
+ * distant nodes, which the reference adds to a map matrix and converts back,
+ * widening the pattern (assembleStiffnessMatrixDCFEMByPass_,
+ * dcfemmodelling.cpp:695-707).  The coupled slots hold zero after assembly
+ * until pgert_set_bypass puts a conductance there.  HOST arrays. */
+int pgert_set_mesh_coupled(pgert_ctx * ctx, int dim, int n_nodes, const double * xyz,
+                           int n_cells, const int * cells, const int * cell_marker,
+                           int n_bounds, const int * bounds, const int * bound_marker,
+                           const int * bound_cell, int n_couple, const int * couple_a,
+                           const int * couple_b);
+
 /* Electrodes after DCMultiElectrodeModelling::searchElectrodes_
  * (dcfemmodelling.cpp:790-1073): node id of every current electrode (pole
  * sources, createCurrentPattern :1482-1525), the mean electrode position used
@@ -199,8 +211,8 @@ int pgert_forward_total(pgert_ctx * ctx, const double * rho, const pgert_solve_o
  * (assembleStiffnessMatrixDCFEMByPass_, dcfemmodelling.cpp:601-707, applied in
  * calculateK_ :1833 only): conductance[i] = 1/R between nodes a[i] and b[i],
  * added to every wavenumber's matrix by pgert_forward_total.  The pairs must
- * share a cell (matrix pattern of the mesh).  n_pairs = 0 removes them. HOST
- * arrays. */
+ * share a cell or have been coupled by pgert_set_mesh_coupled.  n_pairs = 0
+ * removes them. HOST arrays. */
 int pgert_set_bypass(pgert_ctx * ctx, int n_pairs, const int * a, const int * b, const double * conductance);
 
 /* Complex resistivity, DCMultiElectrodeModelling with setComplex(true)

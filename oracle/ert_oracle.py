@@ -616,6 +616,29 @@ def prolongate_empty_cells(mesh, vals):
 
 
 # --------------------------------------------------------------------------- a7/a9/a10
+def bypass_map_rows(text, elecNode, nodeMarkers):
+    """Resistors of a bypass.map file (assembleStiffnessMatrixDCFEMByPass_,
+    dcfemmodelling.cpp:608-667) as (node, node, resistance) in file order:
+    "e1 e2 R" joins the nodes of electrodes e1, e2 (from 1; rows with e1 < 1
+    or e1 >= nE are skipped, the reference's bound check); "marker R" chains
+    the nodes carrying the marker in index order; '#' starts a comment;
+    integers as atoi, resistances as atof rounded to float (toInt / toFloat,
+    gimli.h:477-478); other rows are skipped."""
+    import re
+    out = []
+    nodeMarkers = np.asarray(nodeMarkers)
+    for line in text.splitlines():
+        tok = line.split("#")[0].split()
+        ints = [int(m.group()) if (m := re.match(r"[+-]?[0-9]+", t)) else 0 for t in tok]
+        res = float(np.float32(float(re.match(r"[+-]?[0-9.]+([eE][+-]?[0-9]+)?", tok[-1]).group()))) if tok else 0.0
+        if len(tok) == 3 and 1 <= ints[0] < len(elecNode):
+            out.append((int(elecNode[ints[0] - 1]), int(elecNode[ints[1] - 1]), res))
+        elif len(tok) == 2:
+            ids = np.flatnonzero(nodeMarkers == ints[0])
+            out += [(int(i), int(j), res) for i, j in zip(ids[:-1], ids[1:])]
+    return out
+
+
 class Forward:
     """DCSRMultiElectrodeModelling restated: response() and createJacobian()."""
 
@@ -636,6 +659,7 @@ class Forward:
         self.subpots = None
         self.prim = None
         self.bypassR = {}      # node marker -> bypass resistance (electrodeBypassResistances.map)
+        self.bypassMap = []    # (node, node, resistance) rows of a bypass.map file (bypass_map_rows)
 
     def primaries(self):
         """checkPrimpotentials_ (dcfemmodelling.cpp:2086-2110): rows
@@ -677,17 +701,20 @@ class Forward:
         U = np.empty((len(self.k) * self.nE, n), dt)
         for ki, k in enumerate(self.k):
             S = assemble(self.mesh, rho, k, self.setup.sourceCenter, self.geo)
-            if dt == np.float64 and self.setup.bypass:
-                # assembleStiffnessMatrixDCFEMByPass_ (:671-707): 1e-6 Ohm between
+            if dt == np.float64 and (self.setup.bypass or self.bypassMap):
+                # assembleStiffnessMatrixDCFEMByPass_ (:601-707): the bypass.map
+                # rows, then 1e-6 Ohm (or the group's resistance) between
                 # consecutive nodes of a group (real systems only, :597-599)
-                S = S.tolil()
+                pairs = list(self.bypassMap)
                 for m, ids in self.setup.bypass.items():
-                    for a1, a2 in zip(ids[:-1], ids[1:]):
-                        g = 1.0 / self.bypassR.get(m, 1e-6)
-                        S[a1, a1] += g
-                        S[a2, a2] += g
-                        S[a1, a2] -= g
-                        S[a2, a1] -= g
+                    pairs += [(a1, a2, self.bypassR.get(m, 1e-6)) for a1, a2 in zip(ids[:-1], ids[1:])]
+                S = S.tolil()
+                for a1, a2, r in pairs:
+                    g = 1.0 / r
+                    S[a1, a1] += g
+                    S[a2, a2] += g
+                    S[a1, a2] -= g
+                    S[a2, a1] -= g
                 S = S.tocsr()
             lu = spla.splu(S.tocsc())
             for i in range(self.nE):

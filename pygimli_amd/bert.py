@@ -7,6 +7,8 @@ DCMultiElectrodeModelling (core/src/bert/dcfemmodelling.cpp:711-1073).
 from __future__ import annotations
 
 import math
+import re
+import warnings
 
 import numpy as np
 
@@ -237,3 +239,124 @@ class MeshAnalysis:
                 b.append(ids[i + 1])
                 g.append(1.0 / r)
         return np.array(a, np.int32), np.array(b, np.int32), np.array(g, np.float64)
+
+
+# ---------------------------------------------------------------- map files
+def _map_rows(path):
+    """Rows of a whitespace-separated text file with '#' comments, empty rows
+    skipped (getNonEmptyRow / getRowSubstrings, core/src/gimli.cpp:245-259)."""
+    with open(path) as f:
+        for line in f:
+            row = line.split("#", 1)[0].split()
+            if row:
+                yield row
+
+
+def _atoi(tok):
+    """std::atoi: the leading integer of the token, 0 if there is none
+    (toInt, core/src/gimli.h:477)."""
+    m = re.match(r"[+-]?\d+", tok)
+    return int(m.group(0)) if m else 0
+
+
+def _atof32(tok):
+    """std::atof rounded to single precision (toFloat, core/src/gimli.h:478)."""
+    m = re.match(r"[+-]?(\d+\.?\d*|\.\d+)([eE][+-]?\d+)?", tok)
+    return float(np.float32(float(m.group(0)))) if m else 0.0
+
+
+def load_float_map(path):
+    """loadFloatMap (core/src/gimli.cpp:312-334): rows "key value", both read
+    as single-precision floats; a file with no two-column row before another
+    row is an error."""
+    out = {}
+    for row in _map_rows(path):
+        if len(row) == 2:
+            out[_atof32(row[0])] = _atof32(row[1])
+        elif not out:
+            raise ValueError(f"{path}: no proper format found for map <float, float> ({len(row)} columns)")
+    return out
+
+
+def bypass_marker_resistances(path):
+    """electrodeBypassResistances.map as marker -> Ohm: the reference looks a
+    group up by float(marker) (dcfemmodelling.cpp:672-691), so only integral
+    keys can match a marker."""
+    return {int(k): v for k, v in load_float_map(path).items() if float(k).is_integer()}
+
+
+def read_bypass_map(path, elecNode, nodeMarkers):
+    """The resistor rows of a bypass.map file
+    (assembleStiffnessMatrixDCFEMByPass_, dcfemmodelling.cpp:608-667), in file
+    order, as node pairs and resistances (Ohm, single precision as the
+    reference reads them):
+
+    * "e1 e2 R": a resistor between the nodes of electrodes e1 and e2
+      (counted from 1).  Like the reference, e1 must lie in 1 .. nE - 1 (its
+      bound check is e1 < nE, so the last electrode is refused too) and rows
+      failing that are skipped with a warning; the reference checks e1 again
+      in place of e2 and reads past its electrode list for a bad e2, which
+      raises here, as does an electrode without a node (a free electrode).
+    * "marker R": resistors chaining the nodes with that node marker in index
+      order (Mesh::findNodesIdxByMarker); a marker no node has is skipped with
+      a warning.
+    * rows of any other length are skipped with a warning.
+
+    A pair of one node with itself adds nothing and is dropped; a resistance
+    of zero or not finite raises."""
+    elecNode = np.asarray(elecNode)
+    nodeMarkers = np.asarray(nodeMarkers)
+    nE = len(elecNode)
+    a, b, r = [], [], []
+
+    def add(n1, n2, res, row):
+        if not (np.isfinite(res) and res != 0.0):
+            raise ValueError(f"{path}: row {' '.join(row)}: resistance must be finite and non-zero")
+        if n1 != n2:
+            a.append(int(n1))
+            b.append(int(n2))
+            r.append(res)
+
+    for row in _map_rows(path):
+        if len(row) == 3:
+            e1, e2, res = _atoi(row[0]), _atoi(row[1]), _atof32(row[2])
+            if e1 < 1:
+                warnings.warn(f"{path}: bypass electrode unknown: {e1}, electrode indices count from 1; ignoring")
+                continue
+            if e1 >= nE:
+                warnings.warn(f"{path}: bypass electrode unknown {e1} (e-size = {nE}); ignoring")
+                continue
+            if not 1 <= e2 <= nE:
+                raise ValueError(f"{path}: row {' '.join(row)}: electrode {e2} is not one of 1 .. {nE}")
+            n1, n2 = int(elecNode[e1 - 1]), int(elecNode[e2 - 1])
+            if n1 < 0 or n2 < 0:
+                raise ValueError(f"{path}: row {' '.join(row)}: a free electrode (on no node) has no bypass node")
+            add(n1, n2, res, row)
+        elif len(row) == 2:
+            m, res = _atoi(row[0]), _atof32(row[1])
+            ids = np.nonzero(nodeMarkers == m)[0]
+            if len(ids) == 0:
+                warnings.warn(f"{path}: no node with marker {m}; ignoring")
+                continue
+            for i in range(len(ids) - 1):
+                add(ids[i], ids[i + 1], res, row)
+        else:
+            warnings.warn(f"{path}: wrong format ({len(row)} columns), expected \"electrode electrode resistance\" "
+                          "or \"node-marker resistance\"; ignoring")
+    return np.array(a, np.int32), np.array(b, np.int32), np.array(r, np.float64)
+
+
+def uncoupled_pairs(cells, a, b):
+    """The node pairs (a[i], b[i]) that share no cell: they need their own
+    matrix slots (pgert_set_mesh_coupled).  Returns a sorted list of
+    (min, max) tuples without repeats."""
+    if len(a) == 0:
+        return []
+    cells = np.asarray(cells)
+    u = np.unique(np.concatenate([a, b]))
+    sub = cells[np.isin(cells, u).any(1)]
+    out = set()
+    for x, y in zip(np.asarray(a).tolist(), np.asarray(b).tolist()):
+        if not ((sub == x).any(1) & (sub == y).any(1)).any():
+            out.add((min(x, y), max(x, y)))
+    return sorted(out)

@@ -33,11 +33,18 @@ template <class F> int guarded(pgert_ctx * c, F f) {
 }
 
 // Sorted neighbour lists of every node (SparseMatrix::buildSparsityPattern,
-// sparsematrix.h:966-1032: all node pairs sharing a cell, sorted per row).
-void build_pattern(Ctx & c, const int * cells) {
+// sparsematrix.h:966-1032: all node pairs sharing a cell, sorted per row),
+// plus the n_couple node pairs (ca[i], cb[i]) in both directions: couplings
+// that no cell carries, such as bypass resistors between distant nodes (the
+// reference inserts them into a map matrix, dcfemmodelling.cpp:695-707).
+void build_pattern(Ctx & c, const int * cells, int n_couple, const int * ca, const int * cb) {
     const int n = c.n, C = c.C, nv = c.nv;
     std::vector<int> cnt(n + 1, 0);
     for (long e = 0; e < (long)C * nv; ++e) cnt[cells[e] + 1] += nv;
+    for (int i = 0; i < n_couple; ++i) {
+        cnt[ca[i] + 1]++;
+        cnt[cb[i] + 1]++;
+    }
     for (int i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
     std::vector<int> cand(cnt[n]);
     std::vector<int> fill(cnt.begin(), cnt.end() - 1);
@@ -45,6 +52,10 @@ void build_pattern(Ctx & c, const int * cells) {
         const int * v = cells + (long)cc * nv;
         for (int i = 0; i < nv; ++i)
             for (int j = 0; j < nv; ++j) cand[fill[v[i]]++] = v[j];
+    }
+    for (int i = 0; i < n_couple; ++i) {
+        cand[fill[ca[i]]++] = cb[i];
+        cand[fill[cb[i]]++] = ca[i];
     }
     c.hRowptr.assign(n + 1, 0);
     c.hColidx.clear();
@@ -115,12 +126,26 @@ int pgert_set_stream(pgert_ctx * c, void * stream) {
 int pgert_set_mesh(pgert_ctx * c, int dim, int n_nodes, const double * xyz, int n_cells,
                    const int * cells, const int * cell_marker, int n_bounds, const int * bounds,
                    const int * bound_marker, const int * bound_cell) {
+    return pgert_set_mesh_coupled(c, dim, n_nodes, xyz, n_cells, cells, cell_marker, n_bounds, bounds,
+                                  bound_marker, bound_cell, 0, nullptr, nullptr);
+}
+
+int pgert_set_mesh_coupled(pgert_ctx * c, int dim, int n_nodes, const double * xyz, int n_cells,
+                           const int * cells, const int * cell_marker, int n_bounds, const int * bounds,
+                           const int * bound_marker, const int * bound_cell, int n_couple,
+                           const int * couple_a, const int * couple_b) {
     return guarded(c, [&] {
         PG_REQUIRE(dim == 2 || dim == 3, "pgert_set_mesh: dim must be 2 or 3");
         PG_REQUIRE(n_nodes > 0 && n_cells > 0, "pgert_set_mesh: empty mesh");
         PG_REQUIRE(xyz && cells && cell_marker, "pgert_set_mesh: null array");
         PG_REQUIRE(n_bounds == 0 || (bounds && bound_marker && bound_cell),
                    "pgert_set_mesh: null boundary array");
+        PG_REQUIRE(n_couple >= 0 && (n_couple == 0 || (couple_a && couple_b)),
+                   "pgert_set_mesh_coupled: null coupling array");
+        for (int i = 0; i < n_couple; ++i)
+            PG_REQUIRE(couple_a[i] >= 0 && couple_a[i] < n_nodes && couple_b[i] >= 0 && couple_b[i] < n_nodes &&
+                           couple_a[i] != couple_b[i],
+                       "pgert_set_mesh_coupled: bad node pair " + std::to_string(i));
         c->dim = dim;
         c->nv = dim + 1;
         c->n = n_nodes;
@@ -165,7 +190,7 @@ int pgert_set_mesh(pgert_ctx * c, int dim, int n_nodes, const double * xyz, int 
         build_prolongation(*c, xyz, cells, cell_marker);
 
         // pattern + contributions sorted by slot (counting sort, cell order kept)
-        build_pattern(*c, cells);
+        build_pattern(*c, cells, n_couple, couple_a, couple_b);
         const int nv = c->nv;
         std::vector<int> slotOf((size_t)n_cells * nv * nv);
         std::vector<int> cptr(c->nnz + 1, 0);
@@ -911,8 +936,8 @@ int pgert_set_bypass(pgert_ctx * c, int n_pairs, const int * a, const int * b, c
                        "pgert_set_bypass: bad node pair " + std::to_string(i));
             const int sab = find_slot(C, a[i], b[i]), sba = find_slot(C, b[i], a[i]);
             PG_REQUIRE(sab >= 0 && sba >= 0, "pgert_set_bypass: nodes " + std::to_string(a[i]) + " and " +
-                                                 std::to_string(b[i]) + " share no cell (only bypass resistors "
-                                                 "along mesh edges are supported)");
+                                                 std::to_string(b[i]) + " share no cell; pass the pair to "
+                                                 "pgert_set_mesh_coupled");
             const double g = conductance[i];
             slot.insert(slot.end(), {find_slot(C, a[i], a[i]), find_slot(C, b[i], b[i]), sab, sba});
             val.insert(val.end(), {g, g, -g, -g});

@@ -9,12 +9,14 @@ tensors; all arithmetic runs in the library's kernels.
 from __future__ import annotations
 
 import ctypes
+import os
 
 import numpy as np
 import torch
 
 from . import _lib
-from .bert import MeshAnalysis, geometric_factors, kwave_list
+from .bert import (MeshAnalysis, bypass_marker_resistances, geometric_factors, kwave_list, read_bypass_map,
+                   uncoupled_pairs)
 from .data import DataContainerERT
 
 _D = ctypes.POINTER(ctypes.c_double)
@@ -56,6 +58,13 @@ class ERTModelling:
         self.verbose = verbose
         self._sr = bool(sr)
         self._bypassR = None
+        # files the total-field operator reads from the working directory at
+        # every assembly, like the reference (byPassFile_, dcfemmodelling.cpp:565;
+        # :672-674); read here whenever they change
+        self.byPassFile = "bypass.map"
+        self.bypassResistanceFile = "electrodeBypassResistances.map"
+        self._bypassSig = None
+        self._couplings = ()
         self._complex = False
         self._Jc = None
         self.device = torch.device("cuda", device)
@@ -99,8 +108,9 @@ class ERTModelling:
 
     def setBypassResistances(self, resistances):
         """Resistance (Ohm) of the bypass resistors per node marker <= -10000
-        (the reference reads electrodeBypassResistances.map; default 1e-6).
-        Bypass resistors act in the total-field operator (sr=False), as in the
+        (default 1e-6), in place of the electrodeBypassResistances.map file
+        that is read otherwise.  Bypass resistors, these and the rows of a
+        bypass.map file, act in the total-field operator (sr=False), as in the
         reference."""
         self._bypassR = {int(k): float(v) for k, v in dict(resistances).items()}
         if self._mesh is not None and self._data is not None:
@@ -112,6 +122,14 @@ class ERTModelling:
     def setMesh(self, mesh, ignoreRegionManager=False):
         """ModellingBase::setMesh (core/src/modellingbase.cpp:179-211)."""
         self._mesh = mesh
+        self._upload_mesh(())
+        if self._data is not None:
+            self._setup_electrodes()
+
+    def _upload_mesh(self, couplings):
+        """Mesh to the library; `couplings`: node pairs sharing no cell that
+        need matrix slots (bypass resistors between distant nodes)."""
+        mesh = self._mesh
         ctx = self._ctx
         xyz, pxyz = _host_d(mesh.positions())
         cells, pc = _host_i(mesh.cells)
@@ -120,17 +138,25 @@ class ERTModelling:
         bm, pbm = _host_i(mesh.boundaryMarkers())
         lc, plc = _host_i(mesh.boundaryLeftCell())
         self._set_stream()
-        ctx.check(ctx.L.pgert_set_mesh(ctx.h, mesh.dim(), mesh.nodeCount(), pxyz, mesh.cellCount(), pc, pcm,
-                                       mesh.boundaryCount(), pb, pbm, plc), "setMesh")
+        if couplings:
+            ca, pca = _host_i([p[0] for p in couplings])
+            cb, pcb = _host_i([p[1] for p in couplings])
+            ctx.check(ctx.L.pgert_set_mesh_coupled(ctx.h, mesh.dim(), mesh.nodeCount(), pxyz, mesh.cellCount(), pc,
+                                                   pcm, mesh.boundaryCount(), pb, pbm, plc, len(ca), pca, pcb),
+                      "setMesh")
+        else:
+            ctx.check(ctx.L.pgert_set_mesh(ctx.h, mesh.dim(), mesh.nodeCount(), pxyz, mesh.cellCount(), pc, pcm,
+                                           mesh.boundaryCount(), pb, pbm, plc), "setMesh")
+        self._couplings = tuple(couplings)
         nnz = ctypes.c_int()
         npar = ctypes.c_int()
         ctx.L.pgert_info(ctx.h, ctypes.byref(nnz), ctypes.byref(npar))
         self._nnz = nnz.value
         self._nParam = npar.value
         self._coarseSet = None
+        self._analysis = None
+        self._model = None
         self._dirty = True
-        if self._data is not None:
-            self._setup_electrodes()
 
     def mesh(self):
         return self._mesh
@@ -174,6 +200,18 @@ class ERTModelling:
             raise NotImplementedError("topography with singularity removal needs numerical primary potentials, "
                                       "which this path does not provide; the total-field operator "
                                       "ERTModelling(sr=False) models topography")
+        # bypass resistors: bypass.map rows, then the node groups
+        # (assembleStiffnessMatrixDCFEMByPass_, dcfemmodelling.cpp:601-707);
+        # total-field operator only
+        self._bypassSig = self._bypass_file_sig()
+        ba, bb, bg = an.bypass_pairs(self._bypass_resistances())
+        if not self._sr and os.path.isfile(self.byPassFile):
+            fa, fb, fr = read_bypass_map(self.byPassFile, an.elecNode, mesh.nodeMarkers())
+            ba, bb, bg = np.concatenate([fa, ba]), np.concatenate([fb, bb]), np.concatenate([1.0 / fr, bg])
+        if not self._sr:
+            need = uncoupled_pairs(mesh.cells, ba, bb)
+            if not set(need) <= set(self._couplings):
+                self._upload_mesh(tuple(sorted(set(need) | set(self._couplings))))
         self._analysis = an
         en, pen = _host_i(an.elecNode)
         sc, psc = _host_d(an.sourceCenter)
@@ -185,7 +223,6 @@ class ERTModelling:
                       "setElectrodes")
         else:
             ctx.check(ctx.L.pgert_set_electrodes(ctx.h, len(en), pen, psc, an.surfaceZ), "setElectrodes")
-        ba, bb, bg = an.bypass_pairs(self._bypassR)
         ba, pba = _host_i(ba)
         bb, pbb = _host_i(bb)
         bg, pbg = _host_d(bg)
@@ -194,9 +231,33 @@ class ERTModelling:
             self._kv, self._wv = kwave_list(mesh.dim(), an.elecPos)
         self._dirty = True
 
+    def _bypass_file_sig(self):
+        """State of the two bypass files the total-field operator reads."""
+        if self._sr:
+            return None
+        sig = []
+        for f in (self.byPassFile, self.bypassResistanceFile):
+            try:
+                st = os.stat(f)
+                sig.append((os.path.abspath(f), st.st_mtime_ns, st.st_size))
+            except OSError:
+                sig.append(None)
+        return tuple(sig)
+
+    def _bypass_resistances(self):
+        if self._bypassR is not None:
+            return self._bypassR
+        if not self._sr and os.path.isfile(self.bypassResistanceFile):
+            return bypass_marker_resistances(self.bypassResistanceFile)
+        return None
+
     def _sync_setup(self):
         if self._mesh is None or self._data is None:
             raise RuntimeError("mesh and data must be set")
+        if self._analysis is not None and self._bypass_file_sig() != self._bypassSig:
+            self._analysis = None
+            self._model = None
+            self._dirty = True
         if not self._dirty:
             return
         if self._analysis is None:
@@ -377,6 +438,7 @@ class ERTModelling:
         ertModelling.py:221-238 -> DCMultiElectrodeModelling::createJacobian,
         dcfemmodelling.cpp:1446-1476).  Reuses the potentials of the last
         response() when it was computed for the same model."""
+        self._sync_setup()
         t = self._model_tensor(mod)
         if self._model is None or self._model.shape != t.shape or not torch.equal(self._model, t):
             self.response_device(t)

@@ -118,6 +118,58 @@ def test_bypass_vs_oracle_and_reference(tmp_path, monkeypatch):
     assert np.abs(rb - ob).max() <= 1e-9 * np.abs(ob).max()
 
 
+def test_bypass_map_vs_oracle_and_reference(tmp_path, monkeypatch):
+    """bypass.map in the working directory (dcfemmodelling.cpp:565, :608-667):
+    electrode pairs and a node-marker chain whose nodes share no cell, so
+    the matrix pattern gains slots for them (pgert_set_mesh_coupled).  Against
+    the oracle given the same rows (rhoa and J 1e-9, band and block64), and
+    against the reference build reading the same file.  A changed file is
+    read again at the next response."""
+    from oracle import ref
+    from pygimli_amd.ertModelling import ERTModelling
+    mesh, data, rho, text = workloads.small_2d_bypass_map()
+    monkeypatch.chdir(tmp_path)
+    (tmp_path / "bypass.map").write_text(text)
+    R = {-10000: 50.0, -10001: 100.0}
+    out = {}
+    for method in ("band", "block64"):
+        fop = ERTModelling(sr=False)
+        fop.setBypassResistances(R)
+        fop.setMesh(mesh)
+        nnz0 = fop._nnz
+        with pytest.warns(UserWarning):
+            fop.setData(copy.deepcopy(data))
+        assert fop._nnz == nnz0 + 8          # four uncoupled pairs, both directions
+        fop.solverOptions.update(method=method, rtol=1e-12, coarse=0)
+        r = fop.response(rho)
+        fop.createJacobian(rho)
+        out[method] = (r, fop.jacobian().cpu().numpy(), fop.kValues(), fop.weights(), fop)
+    r, J, k, w, fop = out["band"]
+    f = eo.Forward(mesh, copy.deepcopy(data), k, w)
+    f.bypassR = R
+    f.bypassMap = eo.bypass_map_rows(text, f.setup.elecNode, mesh.nodeMarkers())
+    o = eo.response_total(f, rho)
+    oJ = f.jacobian(rho)
+    for method, (rm, Jm, *_) in out.items():
+        e1 = np.abs(rm - o).max() / np.abs(o).max()
+        e2 = np.linalg.norm(Jm - oJ) / np.linalg.norm(oJ)
+        print("bypass.map", method, e1, e2)
+        assert e1 < 1e-9 and e2 < 1e-9
+    if ref.available():
+        (tmp_path / "electrodeBypassResistances.map").write_text("-10000 100\n-10001 200\n")
+        s = ref.Session(mesh, copy.deepcopy(data), k, w, sr=False)
+        rr = s.response(rho)
+        print("bypass.map vs reference build", np.abs(r - rr).max() / np.abs(rr).max())
+        assert np.abs(r - rr).max() <= 1e-9 * np.abs(rr).max()
+    # the file changes: the next response reads it again
+    (tmp_path / "bypass.map").write_text("1 6 150\n")
+    r2 = fop.response(rho)
+    f.bypassMap = [f.bypassMap[0]]
+    o2 = eo.response_total(f, rho)
+    assert np.abs(r2 - o2).max() <= 1e-9 * np.abs(o2).max()
+    assert np.abs(r2 - r).max() > 1e-3 * np.abs(r).max()
+
+
 def test_topography_total_field():
     """Topography with the total-field operator against the oracle and the
     reference build; the SR operator and missing geometric factors raise."""

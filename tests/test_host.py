@@ -96,6 +96,44 @@ def test_electrode_search_2d_swap_and_errors():
         bert.MeshAnalysis(mesh, out)
 
 
+def test_bypass_map_files(tmp_path):
+    """bypass.map rows as the reference reads them (dcfemmodelling.cpp:608-667):
+    the product's parser gives the oracle's rows (which the reference build
+    pins), warns for each row the reference skips, raises where the reference
+    would read past its electrode list or use a free electrode's missing
+    node, and finds the pairs that need matrix slots of their own.
+    electrodeBypassResistances.map is read as loadFloatMap does."""
+    mesh, data, _, text = workloads.small_2d_bypass_map()
+    an = bert.MeshAnalysis(mesh, data.sensorPositions())
+    f = tmp_path / "bypass.map"
+    f.write_text(text)
+    with pytest.warns(UserWarning) as rec:
+        a, b, r = bert.read_bypass_map(str(f), an.elecNode, mesh.nodeMarkers())
+    assert len(rec) == 4
+    rows = eo.bypass_map_rows(text, an.elecNode, mesh.nodeMarkers())
+    assert list(zip(a.tolist(), b.tolist(), r.tolist())) == [tuple(x) for x in rows]
+    # electrode 4 is the bypass electrode: its node is the group's first node
+    assert a[1] == an.bypassGroups[-10000][0] and r[1] == float(np.float32(80.3))
+    assert bert.uncoupled_pairs(mesh.cells, a, b) == sorted({(min(x, y), max(x, y)) for x, y in zip(a, b)})
+    ga, gb, _ = an.bypass_pairs()
+    assert bert.uncoupled_pairs(mesh.cells, ga, gb) == []
+    for bad in ("1 9 10\n", "2 0 10\n", "1 2 0\n"):
+        f.write_text(bad)
+        with pytest.raises(ValueError):
+            bert.read_bypass_map(str(f), an.elecNode, mesh.nodeMarkers())
+    free = an.elecNode.copy()
+    free[1] = -1
+    f.write_text("1 2 10\n")
+    with pytest.raises(ValueError):
+        bert.read_bypass_map(str(f), free, mesh.nodeMarkers())
+    m = tmp_path / "electrodeBypassResistances.map"
+    m.write_text("# marker resistance\n-10000 100.1\n-10001 2e2\n-10000.5 7\n")
+    assert bert.bypass_marker_resistances(str(m)) == {-10000: float(np.float32(100.1)), -10001: 200.0}
+    m.write_text("1 2 3\n")
+    with pytest.raises(ValueError):
+        bert.load_float_map(str(m))
+
+
 @pytest.mark.parametrize("name", ["small_2d", "small_3d"])
 def test_meshes_are_conforming(name):
     mesh, _, _ = getattr(workloads, name)()

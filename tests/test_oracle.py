@@ -604,6 +604,37 @@ def test_bypass_oracle_vs_reference_build(tmp_path, monkeypatch):
     assert np.abs(eo.response_total(f1, rho) - r).max() > 1e-3 * np.abs(r).max()
 
 
+def test_bypass_map_oracle_vs_reference_build(tmp_path, monkeypatch):
+    """bypass.map rows (assembleStiffnessMatrixDCFEMByPass_,
+    dcfemmodelling.cpp:608-667): two electrode pairs that share no cell, one
+    from a bypass electrode, a node-marker chain whose nodes share no cell,
+    and the rows the reference skips.  The reference build reads the file
+    from the working directory at every assembly; the oracle, given the rows
+    it parses from the same text, matches it to rounding.  Without the file
+    the result differs."""
+    if not ref.available():
+        pytest.skip("oracle/_ref not built")
+    mesh, data, rho, text = workloads.small_2d_bypass_map()
+    (tmp_path / "electrodeBypassResistances.map").write_text("-10000 100\n-10001 200\n")
+    monkeypatch.chdir(tmp_path)
+    s0 = ref.Session(mesh, copy.deepcopy(data), sr=False)
+    k, w = s0.kw()
+    r0 = s0.response(rho)
+    (tmp_path / "bypass.map").write_text(text)
+    s = ref.Session(mesh, copy.deepcopy(data), sr=False)
+    r = s.response(rho)
+    J = s.jacobian(rho)
+    f = eo.Forward(mesh, copy.deepcopy(data), k, w)
+    f.bypassR = {-10000: 50.0, -10001: 100.0}
+    f.bypassMap = eo.bypass_map_rows(text, f.setup.elecNode, mesh.nodeMarkers())
+    assert len(f.bypassMap) == 4 and f.bypassMap[1][2] == float(np.float32(80.3))
+    o = eo.response_total(f, rho)
+    oJ = f.jacobian(rho)
+    assert np.abs(o - r).max() <= 1e-12 * np.abs(r).max()
+    assert np.linalg.norm(oJ - J) <= 1e-12 * np.linalg.norm(J)
+    assert np.abs(r - r0).max() > 1e-3 * np.abs(r).max()
+
+
 @pytest.mark.parametrize("name", ["small_2d", "small_3d"])
 @pytest.mark.parametrize("sr", [True, False])
 def test_free_electrodes_oracle_vs_reference_build(name, sr):
