@@ -84,7 +84,10 @@ int  pgert_set_stream(pgert_ctx * ctx, void * stream);
 /* Mesh (ModellingBase::setMesh, core/src/modellingbase.cpp:179-211).
  * dim 2 (triangles) or 3 (tetrahedra).  xyz: n x 3; cells: C x (dim+1);
  * cell_marker: C (Jacobian column = marker, bertJacobian.cpp:191,380);
- * bounds: F x dim node ids, bound_marker: F (-1 Neumann, -2 mixed);
+ * bounds: F x dim node ids, bound_marker: F (-1 Neumann, -2 mixed, -3
+ * homogeneous Dirichlet: the rows and columns of its nodes are cleared and
+ * the diagonal set to one in every assembled matrix,
+ * dcfemmodelling.cpp:280-298; -4 is refused, as the reference throws);
  * bound_cell: F, the cell adjacent to each boundary (Boundary::leftCell).
  * Builds the CSR pattern once (SparseMatrix::buildSparsityPattern,
  * sparsematrix.h:966-1032) and the element geometry on the device. */

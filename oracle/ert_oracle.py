@@ -500,10 +500,21 @@ def assemble(mesh, rho, k, source, geo=None):
     # one COO -> CSR conversion keeps explicit zeros, i.e. the full pattern
     A = sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
                       shape=(n, n)).tocsr()
-    if np.any(np.isin(bm, (-3, -4))):
-        raise NotImplementedError("Dirichlet boundaries are not part of this path")
+    if np.any(bm == -4):
+        raise NotImplementedError("inhomogeneous Dirichlet boundaries (THROW_TO_IMPL in the reference)")
     A.sum_duplicates()
     A.sort_indices()
+    # homogeneous Dirichlet faces (marker -3): rows and columns of their nodes
+    # cleared, diagonal one (:280-298, assembleStiffnessMatrixHomogenDirichletBC
+    # :141-153); pattern entries stay as explicit zeros
+    hd = np.unique(mesh.bounds[bm == -3])
+    if len(hd):
+        flag = np.zeros(n, bool)
+        flag[hd] = True
+        rows_of = np.repeat(np.arange(n), np.diff(A.indptr))
+        hit = flag[rows_of] | flag[A.indices]
+        A.data[hit] = 0.0
+        A.data[hit & (rows_of == A.indices)] = 1.0
     diag = A.diagonal().real
     if np.any(diag < TOLERANCE):
         raise RuntimeError("zero diagonal (all cells around a node have rho = 0)")

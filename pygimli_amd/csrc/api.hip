@@ -230,14 +230,51 @@ int pgert_set_mesh_coupled(pgert_ctx * c, int dim, int n_nodes, const double * x
         launch_contrib_geometry(*c, code.p);
         sync(*c);  // the code table is released at the end of this scope
 
-        // mixed boundaries (marker -2); Dirichlet markers are rejected
+        // homogeneous Dirichlet boundaries (marker -3): their nodes' rows and
+        // columns are cleared and the diagonal set to one after the boundary
+        // terms (dcfemBoundaryAssembleStiffnessMatrix, dcfemmodelling.cpp:280-298
+        // with assembleStiffnessMatrixHomogenDirichletBC :141-153); every
+        // slot so set is listed once with its value
+        std::vector<char> isDir(n_nodes, 0);
+        for (int f = 0; f < n_bounds; ++f) {
+            PG_REQUIRE(bound_marker[f] != -4, "pgert_set_mesh: inhomogeneous Dirichlet boundaries (marker -4) are "
+                                              "not supported (the reference does not implement them either)");
+            if (bound_marker[f] != -3) continue;
+            for (int a = 0; a < dim; ++a) {
+                const int v = bounds[(long)f * dim + a];
+                PG_REQUIRE(v >= 0 && v < n_nodes, "pgert_set_mesh: boundary node out of range");
+                isDir[v] = 1;
+            }
+        }
+        {
+            std::vector<std::pair<int, double>> ds;
+            for (int i = 0; i < n_nodes; ++i) {
+                if (!isDir[i]) continue;
+                for (int q = c->hRowptr[i]; q < c->hRowptr[i + 1]; ++q) {
+                    const int j = c->hColidx[q];
+                    ds.push_back({q, j == i ? 1.0 : 0.0});
+                    if (j != i && !isDir[j]) ds.push_back({find_slot(*c, j, i), 0.0});
+                }
+            }
+            std::sort(ds.begin(), ds.end());
+            std::vector<int> dslot(ds.size());
+            std::vector<double> dval(ds.size());
+            for (size_t q = 0; q < ds.size(); ++q) {
+                dslot[q] = ds[q].first;
+                dval[q] = ds[q].second;
+            }
+            c->nDirSlot = (int)ds.size();
+            c->dirSlot.upload(dslot, s);
+            c->dirVal.upload(dval, s);
+        }
+
+        // mixed boundaries (marker -2)
         std::vector<int> mixCell;
         std::vector<double> mixCtr, mixNrm;
         std::vector<std::array<int, 3>> bcon;  // slot, face, code (i*dim+j)
         std::vector<double> bcoef;
         for (int f = 0; f < n_bounds; ++f) {
             int mk = bound_marker[f];
-            PG_REQUIRE(mk != -3 && mk != -4, "pgert_set_mesh: Dirichlet boundaries are not supported");
             if (mk != -2) continue;
             const int * bn = bounds + (long)f * dim;
             PG_REQUIRE(bound_cell[f] >= 0 && bound_cell[f] < n_cells,

@@ -150,6 +150,14 @@ __global__ void k_fill_matrix(int nnz, const int * __restrict__ cptr, const int 
     vals[s] = acc;
 }
 
+// vals[slot[i]] = val[i] (val null: 0): homogeneous Dirichlet rows and columns
+__global__ void k_set_slots(int ns, const int * __restrict__ slot, const double * __restrict__ val,
+                            double * __restrict__ vals) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ns) return;
+    vals[slot[i]] = val ? val[i] : 0.0;
+}
+
 __global__ void k_map_model(int C, int nModel, const double * __restrict__ model,
                             const int * __restrict__ marker, double * __restrict__ rho) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -196,6 +204,14 @@ void launch_assemble(Ctx & c, const double * rho, double k, double * vals, bool 
                            c.mixNrm.p, r, c.srcCenter[0], c.srcCenter[1], c.srcCenter[2], k, vals);
         PG_CHECK(hipGetLastError());
     }
+    launch_dirichlet(c, vals, false);
+}
+
+void launch_dirichlet(Ctx & c, double * vals, bool imagPart) {
+    if (c.nDirSlot == 0) return;
+    hipLaunchKernelGGL(k_set_slots, dim3(div_up(c.nDirSlot, 256)), dim3(256), 0, c.stream, c.nDirSlot,
+                       c.dirSlot.p, imagPart ? nullptr : c.dirVal.p, vals);
+    PG_CHECK(hipGetLastError());
 }
 
 void launch_fill_matrix(Ctx & c, int mass, const double * a, double * vals) {

@@ -272,6 +272,8 @@ void launch_assemble_complex(Ctx & c, const double * rre, const double * rim, do
                            rim, c.srcCenter[0], c.srcCenter[1], c.srcCenter[2], k, vre, vim);
         PG_CHECK(hipGetLastError());
     }
+    launch_dirichlet(c, vre, false);
+    launch_dirichlet(c, vim, true);
 }
 
 void launch_point_rhs(Ctx & c, double * B) {

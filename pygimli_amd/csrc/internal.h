@@ -106,6 +106,9 @@ struct Ctx {
     DBuf<int> mixCell;     // adjacent cell per mixed face
     DBuf<double> mixCtr;   // face centre (3)
     DBuf<double> mixNrm;   // unit normal (3)
+    int nDirSlot = 0;
+    DBuf<int> dirSlot;     // slots of homogeneous Dirichlet rows / columns (unique)
+    DBuf<double> dirVal;   // their values: 1 on the diagonal, 0 elsewhere
     int nBSlot = 0;
     DBuf<int> bSlot;       // slots touched by mixed faces (unique)
     DBuf<int> bPtr;        // nBSlot + 1
@@ -251,6 +254,9 @@ struct Ctx {
 void launch_cell_geometry(Ctx & c);
 void launch_contrib_geometry(Ctx & c, const int * codes);
 void launch_assemble(Ctx & c, const double * rho, double k, double * vals, bool unitRho);
+// homogeneous Dirichlet rows / columns of an assembled matrix (imagPart: the
+// imaginary half of a complex one, whose diagonal entries there are 0)
+void launch_dirichlet(Ctx & c, double * vals, bool imagPart);
 void launch_fill_matrix(Ctx & c, int mass, const double * a, double * vals);
 void launch_map_model(Ctx & c, const double * model, int nModel, double * rho);
 void build_prolongation(Ctx & c, const double * xyz, const int * cells, const int * marker);

@@ -193,9 +193,9 @@ class ERTModelling:
     def _setup_electrodes(self):
         mesh, data = self._mesh, self._data
         an = MeshAnalysis(mesh, data.sensorPositions())
-        if not np.isin(mesh.boundaryMarkers(), (-2,)).any():
-            raise NotImplementedError("Neumann-only domains (no mixed boundary) need a calibration node, "
-                                      "which this path does not provide")
+        if an.neumannDomain:
+            raise NotImplementedError("3D Neumann-only domains (no mixed or Dirichlet boundary) need a calibration "
+                                      "node and a reference electrode, which this path does not provide")
         if an.topography and self._sr:
             raise NotImplementedError("topography with singularity removal needs numerical primary potentials, "
                                       "which this path does not provide; the total-field operator "

@@ -637,6 +637,59 @@ def test_bypass_map_oracle_vs_reference_build(tmp_path, monkeypatch):
 
 @pytest.mark.parametrize("name", ["small_2d", "small_3d"])
 @pytest.mark.parametrize("sr", [True, False])
+def test_dirichlet_boundary_oracle_vs_reference_build(name, sr):
+    """Homogeneous Dirichlet faces (marker -3, dcfemmodelling.cpp:280-298):
+    rows and columns of their nodes cleared, diagonal one, in both SR
+    matrices and the total-field one.  The oracle against the reference
+    build; the cleared rows fix the potentials there (SR: the unscaled
+    primary, rhs = S1 p / rs - S p with p scaled by rs; total field: zero);
+    the result differs from the mixed-boundary one."""
+    if not ref.available():
+        pytest.skip("oracle/_ref not built")
+    mesh, data, rho = workloads.small_dirichlet(name)
+    s = ref.Session(mesh, copy.deepcopy(data), nthreads=1, sr=sr)
+    k, w = s.kw()
+    r = s.response(rho)
+    J = s.jacobian(rho)
+    f = eo.Forward(mesh, copy.deepcopy(data), k, w, total_field=not sr)
+    o = f.response(rho) if sr else eo.response_total(f, rho)
+    oJ = f.jacobian(rho)
+    assert np.abs(o - r).max() <= 1e-12 * np.abs(r).max()
+    assert np.linalg.norm(oJ - J) <= 1e-12 * np.linalg.norm(J)
+    hd = np.unique(mesh.bounds[mesh.boundaryMarkers() == -3])
+    U = f.subpots
+    if sr:
+        P = f.primaries()
+        np.testing.assert_allclose(U[:, hd], P[:, hd], rtol=1e-13, atol=0)   # P (1 - rs) + P rs
+    else:
+        assert np.all(U[:, hd] == 0.0)
+    m0, d0, _ = {"small_2d": workloads.small_2d, "small_3d": workloads.small_3d}[name]()
+    f0 = eo.Forward(m0, copy.deepcopy(d0), k, w, total_field=not sr)
+    o0 = f0.response(rho) if sr else eo.response_total(f0, rho)
+    assert np.abs(o0 - r).max() > 1e-5 * np.abs(r).max()
+
+
+def test_neumann_2d_total_field_oracle_vs_reference_build():
+    """A 2.5D mesh with only homogeneous Neumann faces: the reference turns
+    the Neumann domain off in 2D (dcfemmodelling.cpp:758-764), so the total
+    field needs no calibration node; oracle = reference build."""
+    if not ref.available():
+        pytest.skip("oracle/_ref not built")
+    mesh, data, rho = workloads.small_2d_neumann()
+    s = ref.Session(mesh, copy.deepcopy(data), nthreads=1, sr=False)
+    k, w = s.kw()
+    r = s.response(rho)
+    J = s.jacobian(rho)
+    f = eo.Forward(mesh, copy.deepcopy(data), k, w, total_field=True)
+    assert f.setup.topography and not f.setup.neumannDomain
+    o = eo.response_total(f, rho)
+    oJ = f.jacobian(rho)
+    assert np.abs(o - r).max() <= 1e-12 * np.abs(r).max()
+    assert np.linalg.norm(oJ - J) <= 1e-12 * np.linalg.norm(J)
+
+
+@pytest.mark.parametrize("name", ["small_2d", "small_3d"])
+@pytest.mark.parametrize("sr", [True, False])
 def test_free_electrodes_oracle_vs_reference_build(name, sr):
     """Free electrodes (ElectrodeShapeEntity, electrode.cpp:198-287): the
     oracle against the reference build with and without singularity
