@@ -405,6 +405,137 @@ def element_matrices(mesh):
     return K, M, vol, grads
 
 
+# --------------------------------------------------------------- f4 topography
+# Numerical primary potentials of the SR operator with topography
+# (DCSRMultiElectrodeModelling::checkPrimpotentials_, dcfemmodelling.cpp:2009-2056):
+# the total-field problem with rho = 1 on the P2 mesh (Mesh::createP2,
+# mesh.cpp:1285-1330, quadratic triangles / tetrahedra), interpolated to the
+# mesh nodes (the vertices of the P2 mesh, where the P2 field is its nodal value).
+
+P2_EDGES = {1: [(0, 1)], 2: [(0, 1), (1, 2), (0, 2)], 3: [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]}
+
+
+def p2_basis(d):
+    """P2 Lagrange basis of a d-simplex as quadratic forms in the barycentric
+    coordinates, phi_i = lam^T Q_i lam: vertex a, lam_a (2 lam_a - 1) =
+    2 lam_a^2 - lam_a sum(lam); edge (p, q), 4 lam_p lam_q.  Nodes: the d + 1
+    vertices, then the edges in P2_EDGES order."""
+    nb = d + 1
+    Q = []
+    for a in range(nb):
+        q = np.zeros((nb, nb))
+        q[a, :] -= 0.5
+        q[:, a] -= 0.5
+        q[a, a] = 1.0
+        Q.append(q)
+    for p, r in P2_EDGES[d]:
+        q = np.zeros((nb, nb))
+        q[p, r] = q[r, p] = 2.0
+        Q.append(q)
+    return np.array(Q)
+
+
+def _simplex_moment(d, idx):
+    """Integral of prod(lam[idx]) over a d-simplex divided by its measure:
+    d! prod(alpha!) / (d + |alpha|)!."""
+    from math import factorial
+    alpha = np.bincount(np.asarray(idx), minlength=d + 1)
+    num = factorial(d)
+    for a in alpha:
+        num *= factorial(int(a))
+    return num / factorial(d + len(idx))
+
+
+def p2_tables(d):
+    """Per-measure P2 element tables of a d-simplex: mass M[i, j] (exact, the
+    reference's order-4 rules are exact for it, elementmatrix.cpp:761-782) and
+    stiffness weights W[i, j, a, c] with K_ij = |T| sum_ac (grad lam_a .
+    grad lam_c) W[i, j, a, c] (exact; order-2 rules, :872-1060)."""
+    Q = p2_basis(d)
+    nb, nn = d + 1, len(Q)
+    I2 = np.array([[_simplex_moment(d, [b, e]) for e in range(nb)] for b in range(nb)])
+    I4 = np.zeros((nb,) * 4)
+    for a in range(nb):
+        for b in range(nb):
+            for c in range(nb):
+                for e in range(nb):
+                    I4[a, b, c, e] = _simplex_moment(d, [a, b, c, e])
+    M = np.einsum("iab,jce,abce->ij", Q, Q, I4)
+    W = 4.0 * np.einsum("iab,jce,be->ijac", Q, Q, I2)
+    return M, W
+
+
+def p2_mesh(mesh):
+    """Mesh::createP2: the vertices keep their ids, one node per edge at its
+    midpoint after them.  Returns nodes, cells (C x (d+1 + edges)) and
+    boundaries (F x (d + face edges)) in P2_EDGES order."""
+    n, d = mesh.nodeCount(), mesh.dim()
+    c = mesh.cells.astype(np.int64)
+    ce = np.sort(np.stack([c[:, [p, q]] for p, q in P2_EDGES[d]], 1), axis=2)
+    keys = ce[..., 0] * n + ce[..., 1]
+    uk, inv = np.unique(keys.ravel(), return_inverse=True)
+    a, b = uk // n, uk % n
+    nodes = np.vstack([mesh.nodes, 0.5 * (mesh.nodes[a] + mesh.nodes[b])])
+    cells = np.hstack([c, n + inv.reshape(keys.shape)])
+    bnd = mesh.bounds.astype(np.int64)
+    be = np.sort(np.stack([bnd[:, [p, q]] for p, q in P2_EDGES[d - 1]], 1), axis=2)
+    bk = be[..., 0] * n + be[..., 1]
+    bounds = np.hstack([bnd, n + np.searchsorted(uk, bk)])
+    return nodes, cells, bounds
+
+
+def p2_primaries(mesh, setup, kvalues):
+    """[nK * nE][n] primary potentials of unit point sources at the electrode
+    nodes, total field on the P2 mesh with rho = 1 and the mixed and
+    Dirichlet faces of the mesh (calculateK_ on the primary operator)."""
+    if np.any(setup.elecNode < 0):
+        raise NotImplementedError("numerical primaries: node electrodes only")
+    if setup.bypass:
+        raise NotImplementedError("numerical primaries with bypass electrodes")
+    d, n = mesh.dim(), mesh.nodeCount()
+    nodes, cells, bounds = p2_mesh(mesh)
+    N = len(nodes)
+    K1, _, vol, grads = element_matrices(mesh)
+    M2, W2 = p2_tables(d)
+    G = vol[:, None, None] * np.einsum("xad,xed->xae", grads, grads)    # |T| grad lam_a . grad lam_e
+    Ke = np.einsum("xae,ijae->xij", G, W2)
+    Me = vol[:, None, None] * M2[None]
+    nl = cells.shape[1]
+    rows = np.repeat(cells, nl, axis=1).ravel()
+    cols = np.tile(cells, (1, nl)).ravel()
+    bm = mesh.boundaryMarkers()
+    mix = np.nonzero(bm == -2)[0]
+    size, _, _ = boundary_geometry(mesh)
+    Mf, _ = p2_tables(d - 1)
+    fl = bounds.shape[1]
+    hd = np.unique(bounds[bm == -3])
+    P = np.empty((len(kvalues) * len(setup.elecNode), n))
+    for ki, k in enumerate(kvalues):
+        vals = [(Ke + (k * k) * Me).ravel() if k > 0.0 else Ke.ravel()]
+        r, cc = [rows], [cols]
+        if len(mix):
+            alpha = mixed_bc_alpha(mesh, setup.sourceCenter, k)
+            Be = (alpha[mix] * size[mix])[:, None, None] * Mf[None]
+            bb = bounds[mix]
+            r.append(np.repeat(bb, fl, axis=1).ravel())
+            cc.append(np.tile(bb, (1, fl)).ravel())
+            vals.append(Be.ravel())
+        A = sp.coo_matrix((np.concatenate(vals), (np.concatenate(r), np.concatenate(cc))), shape=(N, N)).tocsr()
+        A.sum_duplicates()
+        if len(hd):
+            A = A.tolil()
+            A[hd, :] = 0.0
+            A[:, hd] = 0.0
+            A[hd, hd] = 1.0
+            A = A.tocsr()
+        lu = spla.splu(A.tocsc())
+        for i, e in enumerate(setup.elecNode):
+            rhs = np.zeros(N)
+            rhs[e] = 1.0
+            P[i + ki * len(setup.elecNode)] = lu.solve(rhs)[:n]
+    return P
+
+
 def boundary_geometry(mesh):
     """Boundary measure, centre and unit normal (Boundary::norm)."""
     dim = mesh.dim()
@@ -659,8 +790,8 @@ class Forward:
         self.mesh = mesh
         self.data = data
         self.setup = Setup(mesh, data.sensorPositions())
-        if self.setup.topography and not total_field:
-            raise NotImplementedError("topography / Neumann-only domains are not part of this path")
+        if self.setup.neumannDomain:
+            raise NotImplementedError("3D Neumann-only domains are not part of this path")
         if kvalues is None:
             kvalues, weights = kwave_list(mesh.dim(), self.setup.elecPos)
         self.k = np.asarray(kvalues, np.float64)
@@ -674,7 +805,10 @@ class Forward:
 
     def primaries(self):
         """checkPrimpotentials_ (dcfemmodelling.cpp:2086-2110): rows
-        source + nE * kIdx."""
+        source + nE * kIdx; numerical P2 primaries with topography
+        (:2009-2056)."""
+        if self.prim is None and self.setup.topography:
+            self.prim = p2_primaries(self.mesh, self.setup, self.k)
         if self.prim is None:
             P = np.empty((len(self.k) * self.nE, self.mesh.nodeCount()))
             for ki, k in enumerate(self.k):

@@ -45,6 +45,7 @@ def lib():
                                            _I, _I, _I, _I, _D]
         L.ref_session_init_fop.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, ctypes.c_int]
         L.ref_session_nk.argtypes = [ctypes.c_void_p]
+        L.ref_session_primary_p2.argtypes = [ctypes.c_void_p, _D]
         L.ref_session_kw.argtypes = [ctypes.c_void_p, _D, _D]
         L.ref_session_geometric_factors.argtypes = [ctypes.c_void_p, _D]
         L.ref_session_response.argtypes = [ctypes.c_void_p, _D, ctypes.c_int, _D]
@@ -188,6 +189,15 @@ class Session:
         k = np.empty(nk); w = np.empty(nk)
         lib().ref_session_kw(self.h, k.ctypes.data_as(_D), w.ctypes.data_as(_D))
         return k, w
+
+    def primary_p2(self):
+        """Numerical primary potentials of the SR operator with topography
+        (checkPrimpotentials_, dcfemmodelling.cpp:2009-2056), installed in the
+        session; returns them, [nK * nE][n]."""
+        nk = lib().ref_session_nk(self.h)
+        out = np.empty((nk * self.data.sensorCount(), self.mesh.nodeCount()))
+        _check(lib().ref_session_primary_p2(self.h, out.ctypes.data_as(_D)))
+        return out
 
     def geometric_factors(self):
         out = np.empty(self.data.size())

@@ -26,6 +26,8 @@
 #include "bert/bertMisc.h"
 #include "inversion.h"
 #include "trans.h"
+#include "interpolate.h"
+#include "bert/datamap.h"
 
 #include <algorithm>
 #include <cmath>
@@ -210,7 +212,9 @@ struct Session {
     DataContainerERT data;
     DCMultiElectrodeModelling * fop = nullptr;
     EnvelopeCholesky solver;
+    EnvelopeCholesky solverP2;
     RMatrix subpots;
+    RMatrix prim;
     int dim = 2;
     Session(int d) : mesh(d) { dim = d; }
     ~Session() { delete fop; }
@@ -400,6 +404,37 @@ int ref_session_jacobian_complex(void * h, const double * re, const double * im,
                     out_re[i * J->cols() + j] = (*J)[i][j].real();
                     out_im[i * J->cols() + j] = (*J)[i][j].imag();
                 }
+    });
+}
+
+// Numerical primary potentials of the SR operator with topography, the steps of
+// DCSRMultiElectrodeModelling::checkPrimpotentials_ (dcfemmodelling.cpp:2009-2056)
+// run here because its inner DCMultiElectrodeModelling builds a LinSolver, for
+// which this image has no back end: the same P2 mesh (Mesh::createP2), cell
+// attributes 1, a total-field operator on the same data with this file's
+// solver plugged in, its per-k potentials (collectSubPotentials + calculate)
+// interpolated to the mesh nodes, installed with setPrimaryPotential.  Call
+// after ref_session_init_fop; out: [nK * nE][n] or null.
+int ref_session_primary_p2(void * h, double * out) {
+    Session * s = (Session *)h;
+    return guarded([&] {
+        DCSRMultiElectrodeModelling * sr = dynamic_cast<DCSRMultiElectrodeModelling *>(s->fop);
+        if (!sr) throw std::runtime_error("primary potentials need the SR operator");
+        Mesh p2(s->fop->mesh()->createP2());
+        p2.setCellAttributes(1.0);
+        DCMultiElectrodeModelling f(s->fop->dataContainer(), false);
+        f.setMesh(p2);
+        f.setSolver(&s->solverP2);
+        RMatrix pp;
+        f.collectSubPotentials(pp);
+        DataMap dm;
+        f.calculate(dm);
+        s->prim = RMatrix(pp.rows(), s->fop->mesh()->nodeCount());
+        interpolate(p2, pp, s->fop->mesh()->positions(), s->prim, false);
+        sr->setPrimaryPotential(s->prim);
+        if (out)
+            for (Index i = 0; i < s->prim.rows(); ++i)
+                for (Index j = 0; j < s->prim.cols(); ++j) out[i * s->prim.cols() + j] = s->prim[i][j];
     });
 }
 
