@@ -108,6 +108,24 @@ int pgert_set_mesh_coupled(pgert_ctx * ctx, int dim, int n_nodes, const double *
                            const int * bound_cell, int n_couple, const int * couple_a,
                            const int * couple_b);
 
+/* Quadratic (P2) mesh for the primary potentials of the SR operator with
+ * topography: Mesh::createP2 (mesh.cpp:1285-1330) of a linear mesh, on which
+ * the reference solves the total-field problem with rho = 1
+ * (DCSRMultiElectrodeModelling::checkPrimpotentials_, dcfemmodelling.cpp:
+ * 2009-2056).  cells: C x 6 (triangles) or C x 10 (tetrahedra), the vertices
+ * first, then the edge nodes (0,1), (1,2), (0,2) or (0,1), (0,2), (0,3),
+ * (1,2), (1,3), (2,3); bounds: F x 3 (edges: two vertices, the midpoint) or
+ * F x 6 (triangles as the cells); other arguments as pgert_set_mesh.  Element
+ * matrices are the exact P2 integrals (the reference's quadrature of orders
+ * 2 and 4 is exact for them, elementmatrix.cpp:761-782, :872-1060).  On such
+ * a context pgert_forward_total, pgert_potentials(_node_major), pgert_assemble
+ * and pgert_fill_matrix run; the SR forward, the Jacobians and free
+ * electrodes are refused. */
+int pgert_set_mesh_p2(pgert_ctx * ctx, int dim, int n_nodes, const double * xyz,
+                      int n_cells, const int * cells, const int * cell_marker,
+                      int n_bounds, const int * bounds, const int * bound_marker,
+                      const int * bound_cell);
+
 /* Electrodes after DCMultiElectrodeModelling::searchElectrodes_
  * (dcfemmodelling.cpp:790-1073): node id of every current electrode (pole
  * sources, createCurrentPattern :1482-1525), the mean electrode position used
@@ -209,6 +227,16 @@ int pgert_potentials(pgert_ctx * ctx, double * out);
  * a total-field guess). */
 int pgert_forward_total(pgert_ctx * ctx, const double * rho, const pgert_solve_opts * opts, double * rhoa,
                         pgert_solve_stats * stats);
+
+/* Numerical primary potentials for the SR operator, in place of the analytic
+ * half-space ones (DCSRMultiElectrodeModelling::setPrimaryPotential,
+ * dcfemmodelling.h:374; computed by checkPrimpotentials_,
+ * dcfemmodelling.cpp:2009-2056, when the mesh has topography): prim DEVICE
+ * [nK][n][nE] (the layout of pgert_potentials_node_major), unit-current
+ * potentials of every electrode and wavenumber at the mesh nodes.  They hold
+ * until the mesh, the electrodes or the wavenumbers change; pgert_forward
+ * then fails until they are set again. */
+int pgert_set_primary_potentials(pgert_ctx * ctx, const double * prim);
 
 /* Bypass resistors of the total-field operator
  * (assembleStiffnessMatrixDCFEMByPass_, dcfemmodelling.cpp:601-707, applied in

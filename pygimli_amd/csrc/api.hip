@@ -123,6 +123,11 @@ int pgert_set_stream(pgert_ctx * c, void * stream) {
     return guarded(c, [&] { c->stream = (hipStream_t)stream; });
 }
 
+static void mesh_setup(pgert_ctx * c, int elemOrder, int dim, int n_nodes, const double * xyz, int n_cells,
+                       const int * cells, const int * cell_marker, int n_bounds, const int * bounds,
+                       const int * bound_marker, const int * bound_cell, int n_couple, const int * couple_a,
+                       const int * couple_b);
+
 int pgert_set_mesh(pgert_ctx * c, int dim, int n_nodes, const double * xyz, int n_cells,
                    const int * cells, const int * cell_marker, int n_bounds, const int * bounds,
                    const int * bound_marker, const int * bound_cell) {
@@ -135,6 +140,26 @@ int pgert_set_mesh_coupled(pgert_ctx * c, int dim, int n_nodes, const double * x
                            const int * bound_marker, const int * bound_cell, int n_couple,
                            const int * couple_a, const int * couple_b) {
     return guarded(c, [&] {
+        mesh_setup(c, 1, dim, n_nodes, xyz, n_cells, cells, cell_marker, n_bounds, bounds, bound_marker, bound_cell,
+                   n_couple, couple_a, couple_b);
+    });
+}
+
+int pgert_set_mesh_p2(pgert_ctx * c, int dim, int n_nodes, const double * xyz, int n_cells, const int * cells,
+                      const int * cell_marker, int n_bounds, const int * bounds, const int * bound_marker,
+                      const int * bound_cell) {
+    return guarded(c, [&] {
+        mesh_setup(c, 2, dim, n_nodes, xyz, n_cells, cells, cell_marker, n_bounds, bounds, bound_marker, bound_cell,
+                   0, nullptr, nullptr);
+    });
+}
+
+// Mesh set-up of pgert_set_mesh (order 1) and pgert_set_mesh_p2 (order 2).
+static void mesh_setup(pgert_ctx * c, int elemOrder, int dim, int n_nodes, const double * xyz, int n_cells,
+                       const int * cells, const int * cell_marker, int n_bounds, const int * bounds,
+                       const int * bound_marker, const int * bound_cell, int n_couple, const int * couple_a,
+                       const int * couple_b) {
+    {
         PG_REQUIRE(dim == 2 || dim == 3, "pgert_set_mesh: dim must be 2 or 3");
         PG_REQUIRE(n_nodes > 0 && n_cells > 0, "pgert_set_mesh: empty mesh");
         PG_REQUIRE(xyz && cells && cell_marker, "pgert_set_mesh: null array");
@@ -147,7 +172,10 @@ int pgert_set_mesh_coupled(pgert_ctx * c, int dim, int n_nodes, const double * x
                            couple_a[i] != couple_b[i],
                        "pgert_set_mesh_coupled: bad node pair " + std::to_string(i));
         c->dim = dim;
-        c->nv = dim + 1;
+        c->order = elemOrder;
+        c->nvert = dim + 1;
+        c->nv = elemOrder == 1 ? dim + 1 : (dim == 2 ? 6 : 10);
+        const int nbf = elemOrder == 1 ? dim : (dim == 2 ? 3 : 6);   // nodes per boundary face
         c->n = n_nodes;
         c->C = n_cells;
         for (long e = 0; e < (long)n_cells * c->nv; ++e)
@@ -184,10 +212,15 @@ int pgert_set_mesh_coupled(pgert_ctx * c, int dim, int n_nodes, const double * x
         c->cells.upload(cells, (size_t)n_cells * c->nv, s);
         c->cellMarker.upload(cell_marker, n_cells, s);
         c->sortedCells.upload(order, s);
-        c->grads.alloc((size_t)n_cells * c->nv * 3);
+        c->grads.alloc((size_t)n_cells * c->nvert * 3);
         c->vol.alloc(n_cells);
         launch_cell_geometry(*c);
-        build_prolongation(*c, xyz, cells, cell_marker);
+        if (elemOrder == 1) {
+            build_prolongation(*c, xyz, cells, cell_marker);
+        } else {
+            c->prolLayerPtr.clear();
+            c->prolStatus = 0;
+        }
 
         // pattern + contributions sorted by slot (counting sort, cell order kept)
         build_pattern(*c, cells, n_couple, couple_a, couple_b);
@@ -227,7 +260,19 @@ int pgert_set_mesh_coupled(pgert_ctx * c, int dim, int n_nodes, const double * x
         code.upload(ccode, s);
         c->cK.alloc(ccell.size());
         c->cM.alloc(ccell.size());
-        launch_contrib_geometry(*c, code.p);
+        std::vector<double> faceMass;   // per-measure P2 mass of a boundary face
+        if (elemOrder == 1) {
+            launch_contrib_geometry(*c, code.p);
+        } else {
+            std::vector<double> M, W, Mf, Wf;
+            p2_tables(dim, M, W);
+            p2_tables(dim - 1, faceMass, Wf);
+            DBuf<double> dM, dW;
+            dM.upload(M, s);
+            dW.upload(W, s);
+            launch_contrib_geometry_p2(*c, code.p, dM.p, dW.p);
+            sync(*c);   // the tables are released at the end of this scope
+        }
         sync(*c);  // the code table is released at the end of this scope
 
         // homogeneous Dirichlet boundaries (marker -3): their nodes' rows and
@@ -240,8 +285,8 @@ int pgert_set_mesh_coupled(pgert_ctx * c, int dim, int n_nodes, const double * x
             PG_REQUIRE(bound_marker[f] != -4, "pgert_set_mesh: inhomogeneous Dirichlet boundaries (marker -4) are "
                                               "not supported (the reference does not implement them either)");
             if (bound_marker[f] != -3) continue;
-            for (int a = 0; a < dim; ++a) {
-                const int v = bounds[(long)f * dim + a];
+            for (int a = 0; a < nbf; ++a) {
+                const int v = bounds[(long)f * nbf + a];
                 PG_REQUIRE(v >= 0 && v < n_nodes, "pgert_set_mesh: boundary node out of range");
                 isDir[v] = 1;
             }
@@ -276,7 +321,7 @@ int pgert_set_mesh_coupled(pgert_ctx * c, int dim, int n_nodes, const double * x
         for (int f = 0; f < n_bounds; ++f) {
             int mk = bound_marker[f];
             if (mk != -2) continue;
-            const int * bn = bounds + (long)f * dim;
+            const int * bn = bounds + (long)f * nbf;
             PG_REQUIRE(bound_cell[f] >= 0 && bound_cell[f] < n_cells,
                        "pgert_set_mesh: no cell found for a mixed boundary");
             double p[3][3];
@@ -304,15 +349,18 @@ int pgert_set_mesh_coupled(pgert_ctx * c, int dim, int n_nodes, const double * x
             int fi = (int)mixCell.size();
             mixCell.push_back(bound_cell[f]);
             for (int q = 0; q < 3; ++q) { mixCtr.push_back(ctr[q]); mixNrm.push_back(nrm[q]); }
-            // boundary mass: edge len/6 (1 + d_ij), triangle area/12 (1 + d_ij)
+            // boundary mass: edge len/6 (1 + d_ij), triangle area/12 (1 + d_ij);
+            // P2 faces: the measure times the P2 mass table of the face
             double mb = size / (dim * (dim + 1));
-            for (int i = 0; i < dim; ++i)
-                for (int j = 0; j < dim; ++j) {
+            for (int i = 0; i < nbf; ++i) {
+                PG_REQUIRE(bn[i] >= 0 && bn[i] < n_nodes, "pgert_set_mesh: boundary node out of range");
+                for (int j = 0; j < nbf; ++j) {
                     int sl = find_slot(*c, bn[i], bn[j]);
                     PG_REQUIRE(sl >= 0, "pgert_set_mesh: boundary face is not a cell face");
-                    bcon.push_back({sl, fi, i * dim + j});
-                    bcoef.push_back(mb * (i == j ? 2.0 : 1.0));
+                    bcon.push_back({sl, fi, i * nbf + j});
+                    bcoef.push_back(elemOrder == 1 ? mb * (i == j ? 2.0 : 1.0) : size * faceMass[(size_t)i * nbf + j]);
                 }
+            }
         }
         c->nMixed = (int)mixCell.size();
         std::vector<int> idx(bcon.size());
@@ -344,8 +392,9 @@ int pgert_set_mesh_coupled(pgert_ctx * c, int dim, int n_nodes, const double * x
         c->nE = 0;
         c->nByp = 0;   // bypass slots index the previous pattern
         c->s1Valid = c->primValid = c->haveU = c->haveUc = c->srState = false;
+        c->primNumeric = false;
         sync(*c);
-    });
+    }
 }
 
 // Electrodes on nodes (elec_node[e] >= 0) or free in a cell (elec_node[e] = -1,
@@ -383,6 +432,7 @@ static void set_electrodes(pgert_ctx * c, int n_elec, const int * elec_node, con
             continue;
         }
         PG_REQUIRE(elec_cell && elec_pos, "pgert_set_electrodes: free electrode without cell and position");
+        PG_REQUIRE(c->order == 1, "pgert_set_electrodes: free electrodes need a linear mesh");
         const int cc = elec_cell[e];
         PG_REQUIRE(cc >= 0 && cc < c->C, "pgert_set_electrodes: free electrode cell out of range");
         const double * p = elec_pos + 3 * (size_t)e;
@@ -703,6 +753,7 @@ pgert_solve_opts resolve_opts(const pgert_solve_opts * opts) {
 void forward_prepare(Ctx & c, const double * rho, const pgert_solve_opts & o) {
     PG_REQUIRE(c.C > 0 && c.nE > 0 && c.nK > 0 && c.D > 0,
                "pgert_forward: mesh, electrodes, wavenumbers and data must be set");
+    PG_REQUIRE(c.order == 1, "pgert_forward: a P2 mesh serves the total-field primary problem only");
     PG_REQUIRE(rho, "pgert_forward: null pointer");
     hipStream_t s = c.stream;
     const size_t nk = c.nK, nn = c.n, ne = c.nE;
@@ -721,6 +772,8 @@ void forward_prepare(Ctx & c, const double * rho, const pgert_solve_opts & o) {
     PG_CHECK(hipEventRecord(c.ev[1], s));
     // primary potentials (once per electrode/wavenumber set) and SR right-hand sides
     if (!c.primValid) {
+        PG_REQUIRE(!c.primNumeric, "pgert_forward: the numerical primary potentials belong to another electrode or "
+                                   "wavenumber set; set them again (pgert_set_primary_potentials)");
         c.prim.alloc(nk * nn * ne);
         launch_primary(c);
         c.primValid = true;
@@ -961,6 +1014,22 @@ int pgert_forward_total(pgert_ctx * c, const double * rho, const pgert_solve_opt
     });
 }
 
+int pgert_set_primary_potentials(pgert_ctx * c, const double * prim) {
+    return guarded(c, [&] {
+        Ctx & C = *c;
+        PG_REQUIRE(C.order == 1 && C.n > 0 && C.nE > 0 && C.nK > 0,
+                   "pgert_set_primary_potentials: set a linear mesh, the electrodes and the wavenumbers first");
+        PG_REQUIRE(prim, "pgert_set_primary_potentials: null pointer");
+        const size_t total = (size_t)C.nK * C.n * C.nE;
+        C.prim.alloc(total);
+        PG_CHECK(hipMemcpyAsync(C.prim.p, prim, total * sizeof(double), hipMemcpyDeviceToDevice, C.stream));
+        C.primValid = true;
+        C.primNumeric = true;
+        C.srState = false;
+        sync(C);
+    });
+}
+
 int pgert_set_bypass(pgert_ctx * c, int n_pairs, const int * a, const int * b, const double * conductance) {
     return guarded(c, [&] {
         Ctx & C = *c;
@@ -1077,6 +1146,7 @@ int pgert_potentials_node_major(pgert_ctx * c, double * out) {
 int pgert_jacobian(pgert_ctx * c, const double * model, int n_model, void * J, int dtype) {
     return guarded(c, [&] {
         PG_REQUIRE(c->haveU, "pgert_jacobian: call pgert_forward first");
+        PG_REQUIRE(c->order == 1, "pgert_jacobian: linear meshes only");
         PG_REQUIRE(J && (dtype == 0 || dtype == 1), "pgert_jacobian: bad output");
         PG_REQUIRE(n_model >= 0 && (n_model == 0 || model), "pgert_jacobian: bad model");
         PG_CHECK(hipEventRecord(c->ev[4], c->stream));

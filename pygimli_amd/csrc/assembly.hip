@@ -17,7 +17,8 @@ namespace {
 
 constexpr double TOL = 1e-12;  // core/src/gimli.h:206
 
-__global__ void k_cell_geometry(int C, int dim, const double * __restrict__ xyz,
+// stride: nodes per cell in `cells` (the vertices come first)
+__global__ void k_cell_geometry(int C, int dim, int stride, const double * __restrict__ xyz,
                                 const int * __restrict__ cells, double * __restrict__ grads,
                                 double * __restrict__ vol) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -25,7 +26,7 @@ __global__ void k_cell_geometry(int C, int dim, const double * __restrict__ xyz,
     const int nv = dim + 1;
     double p[4][3];
     for (int v = 0; v < nv; ++v) {
-        int id = cells[(size_t)c * nv + v];
+        int id = cells[(size_t)c * stride + v];
         for (int q = 0; q < 3; ++q) p[v][q] = xyz[3 * (size_t)id + q];
     }
     double g[4][3] = {{0}};
@@ -82,6 +83,31 @@ __global__ void k_contrib_geometry(long nc, int nv, const int * __restrict__ cce
     double V = vol[c];
     cK[e] = V * (gi[0] * gj[0] + gi[1] * gj[1] + gi[2] * gj[2]);
     cM[e] = V / (nv * (nv + 1)) * (i == j ? 2.0 : 1.0);
+}
+
+// P2 element matrices per contribution (cell, i, j), exact for straight
+// simplices as the reference's quadrature of orders 2 (stiffness) and 4 (mass)
+// is (elementmatrix.cpp:761-782, :872-1060)
+__global__ void k_contrib_geometry_p2(long nc, int nv, int nvert, const int * __restrict__ ccell,
+                                      const int * __restrict__ code, const double * __restrict__ grads,
+                                      const double * __restrict__ vol, const double * __restrict__ M,
+                                      const double * __restrict__ W, double * __restrict__ cK,
+                                      double * __restrict__ cM) {
+    long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nc) return;
+    const int c = ccell[e];
+    const int ij = code[e];
+    const double V = vol[c];
+    const double * g = grads + (size_t)c * nvert * 3;
+    const double * w = W + (size_t)ij * nvert * nvert;
+    double k = 0.0;
+    for (int a = 0; a < nvert; ++a)
+        for (int b = 0; b < nvert; ++b) {
+            const double gab = g[3 * a] * g[3 * b] + g[3 * a + 1] * g[3 * b + 1] + g[3 * a + 2] * g[3 * b + 2];
+            k += gab * w[a * nvert + b];
+        }
+    cK[e] = V * k;
+    cM[e] = V * M[ij];
 }
 
 __global__ void k_assemble_domain(int nnz, const int * __restrict__ cptr, const int * __restrict__ ccell,
@@ -181,9 +207,76 @@ __global__ void k_diag_inv(int n, int nK, int nnz, const int * __restrict__ diag
 }  // namespace
 
 void launch_cell_geometry(Ctx & c) {
-    hipLaunchKernelGGL(k_cell_geometry, dim3(div_up(c.C, 256)), dim3(256), 0, c.stream, c.C, c.dim,
+    hipLaunchKernelGGL(k_cell_geometry, dim3(div_up(c.C, 256)), dim3(256), 0, c.stream, c.C, c.dim, c.nv,
                        c.xyz.p, c.cells.p, c.grads.p, c.vol.p);
     PG_CHECK(hipGetLastError());
+}
+
+void launch_contrib_geometry_p2(Ctx & c, const int * codes, const double * M, const double * W) {
+    long nc = (long)c.ccell.n;
+    hipLaunchKernelGGL(k_contrib_geometry_p2, dim3(div_up(nc, 256)), dim3(256), 0, c.stream, nc, c.nv, c.nvert,
+                       c.ccell.p, codes, c.grads.p, c.vol.p, M, W, c.cK.p, c.cM.p);
+    PG_CHECK(hipGetLastError());
+}
+
+// The P2 basis of a d-simplex as quadratic forms in the barycentric
+// coordinates, phi_i = lam^T Q_i lam: vertex a, 2 lam_a^2 - lam_a sum(lam);
+// edge (p, q), 4 lam_p lam_q; nodes: the vertices, then the edges (0,1),
+// (1,2), (0,2) of a triangle, (0,1), (0,2), (0,3), (1,2), (1,3), (2,3) of a
+// tetrahedron.  Integrals of lam monomials over the simplex divided by its
+// measure: d! prod(alpha!) / (d + |alpha|)!.
+void p2_tables(int d, std::vector<double> & M, std::vector<double> & W) {
+    static const int E1[1][2] = {{0, 1}};
+    static const int E2[3][2] = {{0, 1}, {1, 2}, {0, 2}};
+    static const int E3[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+    const int nb = d + 1, ne = d == 1 ? 1 : d == 2 ? 3 : 6, nn = nb + ne;
+    const int (*E)[2] = d == 1 ? E1 : d == 2 ? E2 : E3;
+    std::vector<double> Q((size_t)nn * nb * nb, 0.0);
+    auto q = [&](int i, int a, int b) -> double & { return Q[((size_t)i * nb + a) * nb + b]; };
+    for (int a = 0; a < nb; ++a) {
+        for (int b = 0; b < nb; ++b) {
+            q(a, a, b) -= 0.5;
+            q(a, b, a) -= 0.5;
+        }
+        q(a, a, a) = 1.0;
+    }
+    for (int t = 0; t < ne; ++t) q(nb + t, E[t][0], E[t][1]) = q(nb + t, E[t][1], E[t][0]) = 2.0;
+    auto fact = [](int m) {
+        double f = 1.0;
+        for (int i = 2; i <= m; ++i) f *= i;
+        return f;
+    };
+    auto moment = [&](std::initializer_list<int> idx) {
+        int alpha[4] = {0, 0, 0, 0};
+        for (int v : idx) alpha[v]++;
+        double num = fact(d);
+        for (int v = 0; v < nb; ++v) num *= fact(alpha[v]);
+        return num / fact(d + (int)idx.size());
+    };
+    M.assign((size_t)nn * nn, 0.0);
+    W.assign((size_t)nn * nn * nb * nb, 0.0);
+    for (int i = 0; i < nn; ++i)
+        for (int j = 0; j < nn; ++j) {
+            double m = 0.0;
+            for (int a = 0; a < nb; ++a)
+                for (int b = 0; b < nb; ++b)
+                    for (int cc = 0; cc < nb; ++cc)
+                        for (int e = 0; e < nb; ++e) {
+                            const double w = q(i, a, b) * q(j, cc, e);
+                            if (w != 0.0) m += w * moment({a, b, cc, e});
+                        }
+            M[(size_t)i * nn + j] = m;
+            // grad phi_i = sum_a 2 (Q_i lam)_a grad lam_a
+            for (int a = 0; a < nb; ++a)
+                for (int cc = 0; cc < nb; ++cc) {
+                    double w = 0.0;
+                    for (int b = 0; b < nb; ++b)
+                        for (int e = 0; e < nb; ++e)
+                            if (q(i, a, b) != 0.0 && q(j, cc, e) != 0.0)
+                                w += 4.0 * q(i, a, b) * q(j, cc, e) * moment({b, e});
+                    W[(((size_t)i * nn + j) * nb + a) * nb + cc] = w;
+                }
+        }
 }
 
 void launch_contrib_geometry(Ctx & c, const int * codes) {

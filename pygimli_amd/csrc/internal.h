@@ -70,12 +70,14 @@ struct Ctx {
 
     // ---- mesh
     int dim = 0, nv = 0, n = 0, C = 0;
+    int order = 1;               // 1: linear simplices; 2: quadratic (P2, primary potentials only)
+    int nvert = 0;               // vertices per cell, dim + 1
     int nParam = 0;
     bool uniqueMarkers = false;  // marker -> cell is a bijection over cells with marker >= 0
     DBuf<double> xyz;            // n * 3
-    DBuf<int> cells;             // C * nv
+    DBuf<int> cells;             // C * nv (P2: the vertices first, then the edge nodes)
     DBuf<int> cellMarker;        // C
-    DBuf<double> grads;          // C * nv * 3 (components >= dim are 0)
+    DBuf<double> grads;          // C * nvert * 3, barycentric gradients (components >= dim are 0)
     DBuf<double> vol;            // C
     DBuf<int> sortedCells;       // cells with marker >= 0, sorted by marker
     int nSorted = 0;
@@ -165,6 +167,7 @@ struct Ctx {
     DBuf<double> dinv;      // nK * n
     DBuf<double> prim;      // nK * n * nE   unit-resistivity primary potentials
     bool primValid = false;
+    bool primNumeric = false;    // primaries come from pgert_set_primary_potentials
     DBuf<double> U;         // nK * n * nE   total potentials (secondary + rho_s * primary)
     bool haveU = false;
     bool srState = false;   // X / prim / rhoS / rho hold a singularity-removal forward
@@ -253,6 +256,10 @@ struct Ctx {
 // ---- kernels / launchers (defined in the .hip files)
 void launch_cell_geometry(Ctx & c);
 void launch_contrib_geometry(Ctx & c, const int * codes);
+// per-measure P2 tables of a d-simplex (d = 1..3): mass M[i][j] and stiffness
+// weights W[i][j][a][e], K_ij = |T| sum_ae (grad lam_a . grad lam_e) W[i][j][a][e]
+void p2_tables(int d, std::vector<double> & M, std::vector<double> & W);
+void launch_contrib_geometry_p2(Ctx & c, const int * codes, const double * M, const double * W);
 void launch_assemble(Ctx & c, const double * rho, double k, double * vals, bool unitRho);
 // homogeneous Dirichlet rows / columns of an assembled matrix (imagPart: the
 // imaginary half of a complex one, whose diagonal entries there are 0)

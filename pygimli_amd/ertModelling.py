@@ -17,6 +17,7 @@ import torch
 from . import _lib
 from .bert import (MeshAnalysis, bypass_marker_resistances, geometric_factors, kwave_list, read_bypass_map,
                    uncoupled_pairs)
+from .mesh import p2_refinement
 from .data import DataContainerERT
 
 _D = ctypes.POINTER(ctypes.c_double)
@@ -65,6 +66,8 @@ class ERTModelling:
         self.bypassResistanceFile = "electrodeBypassResistances.map"
         self._bypassSig = None
         self._couplings = ()
+        self._prim = None          # numerical primary potentials (topography with SR), [nK][n][nE]
+        self._primKey = None
         self._complex = False
         self._Jc = None
         self.device = torch.device("cuda", device)
@@ -197,9 +200,12 @@ class ERTModelling:
             raise NotImplementedError("3D Neumann-only domains (no mixed or Dirichlet boundary) need a calibration "
                                       "node and a reference electrode, which this path does not provide")
         if an.topography and self._sr:
-            raise NotImplementedError("topography with singularity removal needs numerical primary potentials, "
-                                      "which this path does not provide; the total-field operator "
-                                      "ERTModelling(sr=False) models topography")
+            # numerical primaries from the P2 total-field problem; its
+            # operator would also carry free electrodes (P2 shape functions)
+            # and bypass resistors (on the P2 nodes), which are not built
+            if np.any(an.elecNode < 0) or an.bypassGroups or os.path.isfile(self.byPassFile):
+                raise NotImplementedError("singularity removal with topography: numerical primary potentials are "
+                                          "built for node electrodes without bypass resistors")
         # bypass resistors: bypass.map rows, then the node groups
         # (assembleStiffnessMatrixDCFEMByPass_, dcfemmodelling.cpp:601-707);
         # total-field operator only
@@ -270,7 +276,74 @@ class ERTModelling:
         ctx.check(ctx.L.pgert_set_wavenumbers(ctx.h, len(k), pk, pw), "setWavenumbers")
         self._ensure_k()
         self._upload_data()
+        if self._sr and self._analysis.topography:
+            self._set_numerical_primaries()
         self._dirty = False
+
+    def _set_numerical_primaries(self):
+        """DCSRMultiElectrodeModelling::checkPrimpotentials_ with topography
+        (dcfemmodelling.cpp:2009-2056): the total-field problem with rho = 1
+        on the P2 mesh (Mesh::createP2) for every electrode and wavenumber,
+        its values at the mesh nodes (the P2 mesh's vertices) as the primary
+        potentials.  Solved with this operator's solver options on a second
+        library context; kept while mesh, electrodes and wavenumbers stay.
+        The reference's inner operator takes its own default wavenumbers,
+        equal to these unless setkValues changed them."""
+        an = self._analysis
+        key = (id(self._mesh), self._mesh.nodeCount(), tuple(an.elecNode.tolist()), tuple(self._kv.tolist()),
+               tuple(an.sourceCenter.tolist()))
+        if self._prim is None or self._primKey != key:
+            self._prim = None
+            self._prim = self._p2_primaries()
+            self._primKey = key
+        self._ctx.check(self._ctx.L.pgert_set_primary_potentials(self._ctx.h, _dptr(self._prim)), "primaries")
+
+    def _p2_primaries(self):
+        mesh, an, d = self._mesh, self._analysis, self._data
+        nodes, cells, bounds, _ = p2_refinement(mesh)
+        ctx = _lib.Context(self._ctx.device)
+        try:
+            s = torch.cuda.current_stream(self.device).cuda_stream
+            ctx.check(ctx.L.pgert_set_stream(ctx.h, ctypes.c_void_p(s)), "set_stream")
+            xyz, pxyz = _host_d(nodes)
+            c2, pc = _host_i(cells)
+            cm, pcm = _host_i(np.zeros(mesh.cellCount()))
+            b2, pb = _host_i(bounds)
+            bm, pbm = _host_i(mesh.boundaryMarkers())
+            lc, plc = _host_i(mesh.boundaryLeftCell())
+            ctx.check(ctx.L.pgert_set_mesh_p2(ctx.h, mesh.dim(), len(nodes), pxyz, mesh.cellCount(), pc, pcm,
+                                              mesh.boundaryCount(), pb, pbm, plc), "P2 mesh")
+            en, pen = _host_i(an.elecNode)
+            sc, psc = _host_d(an.sourceCenter)
+            ctx.check(ctx.L.pgert_set_electrodes(ctx.h, len(en), pen, psc, an.surfaceZ), "P2 electrodes")
+            k, pk = _host_d(self._kv)
+            w, pw = _host_d(self._wv)
+            ctx.check(ctx.L.pgert_set_wavenumbers(ctx.h, len(k), pk, pw), "P2 wavenumbers")
+            a, pa = _host_i(d["a"])
+            bb, pbb = _host_i(d["b"])
+            m, pm = _host_i(d["m"])
+            nn, pn = _host_i(d["n"])
+            kf, pkf = _host_d(d["k"])
+            ctx.check(ctx.L.pgert_set_data(ctx.h, d.size(), pa, pbb, pm, pn, pkf), "P2 data")
+            o = self.solverOptions
+            opts = _lib.SolveOpts()
+            opts.method = _lib.METHODS[o["method"]] if isinstance(o["method"], str) else int(o["method"])
+            opts.rtol = float(o["rtol"])
+            opts.max_iter = int(o["max_iter"])
+            opts.check_every = int(o["check_every"])
+            opts.inner_rtol = float(o["inner_rtol"])
+            opts.no_graph = 0 if o.get("graph", True) else 1
+            rho = torch.ones(mesh.cellCount(), dtype=torch.float64, device=self.device)
+            rhoa = torch.empty(d.size(), dtype=torch.float64, device=self.device)
+            stats = _lib.SolveStats()
+            ctx.check(ctx.L.pgert_forward_total(ctx.h, _dptr(rho), ctypes.byref(opts), _dptr(rhoa),
+                                                ctypes.byref(stats)), "P2 primary potentials")
+            U = torch.empty((len(k), len(nodes), len(en)), dtype=torch.float64, device=self.device)
+            ctx.check(ctx.L.pgert_potentials_node_major(ctx.h, _dptr(U)), "P2 potentials")
+            self.primaryStats = stats.as_dict()
+            return U[:, :mesh.nodeCount(), :].contiguous()
+        finally:
+            ctx.close()
 
     def _ensure_k(self):
         """response() fills missing geometric factors analytically

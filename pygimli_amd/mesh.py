@@ -190,3 +190,39 @@ def _face_owner(cells, bounds, n):
     hit = fk[pos] == bk
     out = np.where(hit, owners[pos], -1).astype(np.int32)
     return out
+
+
+# local edges of the P2 cells and faces: triangle (0,1), (1,2), (0,2);
+# tetrahedron (0,1), (0,2), (0,3), (1,2), (1,3), (2,3); edge (0,1)
+P2_LOCAL_EDGES = {1: ((0, 1),), 2: ((0, 1), (1, 2), (0, 2)),
+                  3: ((0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3))}
+
+
+def p2_refinement(mesh):
+    """Mesh::createP2 (core/src/mesh.cpp:1285-1330): the mesh with one more
+    node at the midpoint of every edge (numbered after the vertices, which
+    keep their ids), as quadratic cells and faces in the local order of
+    P2_LOCAL_EDGES.  Returns nodes (N x 3), cells (C x 6 | 10), boundaries
+    (F x 3 | 6) and node markers (vertices keep theirs; a midpoint takes
+    markerT's rule, mesh.cpp:1291-1297)."""
+    n, d = mesh.nodeCount(), mesh.dim()
+    c = mesh.cells.astype(np.int64)
+    pairs = np.sort(np.stack([c[:, [a, b]] for a, b in P2_LOCAL_EDGES[d]], axis=1), axis=2)
+    key = pairs[..., 0] * n + pairs[..., 1]
+    edges, inv = np.unique(key.ravel(), return_inverse=True)
+    e0, e1 = edges // n, edges % n
+    nodes = np.vstack([mesh.nodes, 0.5 * (mesh.nodes[e0] + mesh.nodes[e1])])
+    cells = np.hstack([c, n + inv.reshape(key.shape)]).astype(np.int32)
+    b = mesh.bounds.astype(np.int64)
+    bpairs = np.sort(np.stack([b[:, [x, y]] for x, y in P2_LOCAL_EDGES[d - 1]], axis=1), axis=2)
+    bkey = bpairs[..., 0] * n + bpairs[..., 1]
+    pos = np.searchsorted(edges, bkey)
+    if len(bkey) and (np.any(pos >= len(edges)) or np.any(edges[np.minimum(pos, len(edges) - 1)] != bkey)):
+        raise ValueError("a boundary edge is no cell edge")
+    bounds = np.hstack([b, n + pos]).astype(np.int32)
+    nm = mesh.nodeMarkers()
+    m0, m1 = nm[e0], nm[e1]
+    mid = np.where((m0 == MARKER_NODE_ELECTRODE) & (m1 == MARKER_NODE_ELECTRODE), -1,
+                   np.where(m0 == MARKER_NODE_ELECTRODE, m1,
+                            np.where(m1 == MARKER_NODE_ELECTRODE, m0, np.where(m0 == m1, m0, 0))))
+    return nodes, cells, bounds, np.concatenate([nm, mid]).astype(np.int32)

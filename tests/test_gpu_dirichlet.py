@@ -95,8 +95,9 @@ def test_dirichlet_complex_vs_oracle():
 @pytest.mark.parametrize("method", ["band", "block64"])
 def test_neumann_2d_total_field(method):
     """Neumann faces only in 2.5D: the total field against the oracle and
-    the reference build; singularity removal raises (it would need numerical
-    primaries, as for topography).  Without a Robin or Dirichlet term the
+    the reference build; singularity removal too, with numerical primaries
+    (the reference counts such a mesh as topography).  Without a Robin or
+    Dirichlet term the
     matrix of the smallest wavenumber is held away from singular only by
     k^2 times the mass matrix, so its FP64 direct solve leaves a relative
     residual near 3e-10: the solves run to rtol 1e-9 and the comparison
@@ -121,10 +122,16 @@ def test_neumann_2d_total_field(method):
         rr = s.response(rho)
         print("Neumann 2D vs reference build", np.abs(r - rr).max() / np.abs(rr).max())
         assert np.abs(r - rr).max() <= 1e-7 * np.abs(rr).max()
-    with pytest.raises(NotImplementedError):
-        g = ERTModelling(sr=True)
-        g.setMesh(mesh)
-        g.setData(copy.deepcopy(data))
+    g = ERTModelling(sr=True)
+    g.setMesh(mesh)
+    g.setData(copy.deepcopy(data))
+    g.solverOptions.update(method=method, rtol=1e-9, coarse=0)
+    rs = g.response(rho)
+    fs = eo.Forward(mesh, copy.deepcopy(data), g.kValues(), g.weights())
+    os_ = fs.response(rho)
+    e3 = np.abs(rs - os_).max() / np.abs(os_).max()
+    print("Neumann 2D SR", method, e3)
+    assert e3 < 1e-7
 
 
 def test_neumann_3d_and_inhomogeneous_dirichlet_raise():

@@ -172,7 +172,8 @@ def test_bypass_map_vs_oracle_and_reference(tmp_path, monkeypatch):
 
 def test_topography_total_field():
     """Topography with the total-field operator against the oracle and the
-    reference build; the SR operator and missing geometric factors raise."""
+    reference build; SR with topography and a free electrode, and missing
+    geometric factors, raise."""
     from pygimli_amd.ertModelling import ERTModelling
     mesh, data, rho = workloads.small_2d_topo()
     fop = ERTModelling(sr=False)
@@ -195,8 +196,12 @@ def test_topography_total_field():
         assert np.abs(r - s.response(rho)).max() <= 1e-10 * np.abs(r).max()
     sr = ERTModelling()
     sr.setMesh(mesh)
+    free = copy.deepcopy(data)
+    pos = free.sensorPositions().copy()
+    pos[0, 0] += 0.25
+    free.setSensorPositions(pos)
     with pytest.raises(NotImplementedError):
-        sr.setData(copy.deepcopy(data))
+        sr.setData(free)
     nok = copy.deepcopy(data)
     nok.set("k", np.zeros(nok.size()))
     f2 = ERTModelling(sr=False)

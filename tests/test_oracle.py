@@ -739,6 +739,15 @@ def test_neumann_2d_total_field_oracle_vs_reference_build():
     oJ = f.jacobian(rho)
     assert np.abs(o - r).max() <= 1e-12 * np.abs(r).max()
     assert np.linalg.norm(oJ - J) <= 1e-12 * np.linalg.norm(J)
+    # singularity removal: numerical P2 primaries (a topography mesh to the
+    # reference); the smallest wavenumber's P2 matrix has a condition number
+    # near 1e9, so the primaries agree to 1e-8 and rhoa to rounding
+    s2 = ref.Session(mesh, copy.deepcopy(data), nthreads=1, sr=True)
+    P = s2.primary_p2()
+    r2 = s2.response(rho)
+    f2 = eo.Forward(mesh, copy.deepcopy(data), k, w)
+    assert np.abs(f2.primaries() - P).max() <= 1e-8 * np.abs(P).max()
+    assert np.abs(f2.response(rho) - r2).max() <= 1e-12 * np.abs(r2).max()
 
 
 @pytest.mark.parametrize("name", ["small_2d", "small_3d"])
