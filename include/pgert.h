@@ -228,6 +228,20 @@ int pgert_potentials(pgert_ctx * ctx, double * out);
 int pgert_forward_total(pgert_ctx * ctx, const double * rho, const pgert_solve_opts * opts, double * rhoa,
                         pgert_solve_stats * stats);
 
+/* Complete electrode model of the total-field operator
+ * (assembleCompleteElectrodeModel_, dcfemmodelling.cpp:315-410, applied in
+ * calculateK_ :1790-1829): n_terms constant entries vals[i] added at
+ * (rows[i], cols[i]) of every wavenumber's matrix before the bypass
+ * resistors.  The electrode unknowns are mesh nodes of no cell, appended
+ * after the cell nodes and coupled to their electrode's face nodes through
+ * pgert_set_mesh_coupled; the electrodes are set on those nodes
+ * (pgert_set_electrodes), so the unit current enters at the unknown and the
+ * electrode potential is its value.  Terms per electrode e with unknown v,
+ * contact impedance z and faces f: (i, v) and (v, i) -int_f N_i / z,
+ * (i, j) int_f N_i N_j / z, (v, v) area / z.  n_terms = 0 removes them.
+ * HOST arrays. */
+int pgert_set_cem(pgert_ctx * ctx, int n_terms, const int * rows, const int * cols, const double * vals);
+
 /* Numerical primary potentials for the SR operator, in place of the analytic
  * half-space ones (DCSRMultiElectrodeModelling::setPrimaryPotential,
  * dcfemmodelling.h:374; computed by checkPrimpotentials_,

@@ -288,6 +288,42 @@ class Setup:
         cand = list(mesh.findNodesIdxByMarker(-99))
         nodes = mesh.nodes
         self.elecNode = np.empty(len(ePos), np.int64)
+        # complete electrode model (:804-851, :868-884): boundary faces with
+        # marker <= -9999 grouped by id = -10000 - marker; a sensor takes the
+        # first remaining electrode (by id) within 2 sqrt(area) or with its
+        # index; every electrode then has an unknown after the n nodes
+        bm = mesh.boundaryMarkers()
+        cm = mesh.cellMarkers()
+        if np.any((cm <= -10000) & (cm > -1000000)):
+            raise NotImplementedError("electrode cells of the complete electrode model")
+        self.cem = {}
+        cemIds = sorted({int(-10000 - m) for m in bm[bm <= -9999]})
+        if cemIds:
+            if cemIds[0] < 0:
+                raise NotImplementedError("passive bodies of the complete electrode model")
+            size, _, _ = boundary_geometry(mesh)
+            domains = []
+            for eid in cemIds:
+                faces = np.nonzero(bm == -10000 - eid)[0]
+                un = np.unique(mesh.bounds[faces])
+                domains.append((eid, faces, nodes[un].mean(0), size[faces].sum()))
+            for i, p in enumerate(ePos):
+                for j, (eid, faces, pos, area) in enumerate(domains):
+                    if np.linalg.norm(p - pos) < np.sqrt(area) * 2.0 or eid == i:
+                        self.cem[i] = faces
+                        domains.pop(j)
+                        break
+            if len(self.cem) != len(ePos):
+                raise NotImplementedError("complete electrode model with node or free electrodes beside it")
+            self.elecNode[:] = mesh.nodeCount() + np.arange(len(ePos))
+            self.elecCell = np.full(len(ePos), -1, np.int64)
+            self.elecShape = np.zeros((len(ePos), dim + 1))
+            self.elecPos = ePos
+            self.sourceCenter = ePos.sum(0) / len(ePos)
+            self.nodeCells = [np.zeros(0, np.int64)] * len(ePos)
+            self.singNode = np.full(len(ePos), -1, np.int64)
+            self.minRadius = np.zeros(len(ePos))
+            return
         # free electrodes (no bypass group, no marker -99 node within 0.01):
         # ElectrodeShapeEntity of the cell holding the position (:979-985)
         self.elecCell = np.full(len(ePos), -1, np.int64)
@@ -802,6 +838,8 @@ class Forward:
         self.prim = None
         self.bypassR = {}      # node marker -> bypass resistance (electrodeBypassResistances.map)
         self.bypassMap = []    # (node, node, resistance) rows of a bypass.map file (bypass_map_rows)
+        self.contactImpedance = None    # per electrode, Ohm m^2 (setContactImpedances)
+        self.contactResistance = None   # per electrode, Ohm (contactResistance.map)
 
     def primaries(self):
         """checkPrimpotentials_ (dcfemmodelling.cpp:2086-2110): rows
@@ -843,9 +881,16 @@ class Forward:
         rho = np.asarray(rho)
         n = self.mesh.nodeCount()
         dt = np.complex128 if np.iscomplexobj(rho) else np.float64
+        cem = self.setup.cem
+        if cem:
+            if dt != np.float64:
+                raise NotImplementedError("the complete electrode model is real only (dcfemmodelling.cpp:412-417)")
+            n = n + len(self.setup.elecNode)     # the electrode unknowns after the nodes
         U = np.empty((len(self.k) * self.nE, n), dt)
         for ki, k in enumerate(self.k):
             S = assemble(self.mesh, rho, k, self.setup.sourceCenter, self.geo)
+            if cem:
+                S = self.cem_matrix(S)
             if dt == np.float64 and (self.setup.bypass or self.bypassMap):
                 # assembleStiffnessMatrixDCFEMByPass_ (:601-707): the bypass.map
                 # rows, then 1e-6 Ohm (or the group's resistance) between
@@ -866,6 +911,35 @@ class Forward:
                 rhs = self.setup.source_rhs(i, n).astype(dt)
                 U[i + ki * self.nE] = lu.solve(rhs)
         return U
+
+    def cem_matrix(self, S):
+        """assembleCompleteElectrodeModel_ (dcfemmodelling.cpp:315-410) without a
+        reference electrode: for electrode e (unknown v = n + e, contact
+        impedance z_e, faces f): S[i, v] = S[v, i] -= int_f N_i / z,
+        S[i, j] += int_f N_i N_j / z, S[v, v] = area / z; z = contactImpedance
+        (default 1 Ohm m^2), or contactResistance * area when one is given."""
+        n = self.mesh.nodeCount()
+        nE = len(self.setup.elecNode)
+        d = self.mesh.dim()
+        size, _, _ = boundary_geometry(self.mesh)
+        S = sp.bmat([[S, None], [None, sp.csr_matrix((nE, nE))]]).tolil()
+        for e, faces in self.setup.cem.items():
+            area = size[faces].sum()
+            z = self.contactImpedance[e] if self.contactImpedance is not None else 1.0
+            if self.contactResistance is not None:
+                z = self.contactResistance[e] * area
+            v = n + e
+            for f in faces:
+                nodes = self.mesh.bounds[f]
+                u = size[f] / d                              # int_f N_i (P1 face)
+                Mf = size[f] / (d * (d + 1)) * (np.ones((d, d)) + np.eye(d))
+                for a, na in enumerate(nodes):
+                    S[na, v] -= u / z
+                    S[v, na] -= u / z
+                    for b, nb in enumerate(nodes):
+                        S[na, nb] += Mf[a, b] / z
+            S[v, v] = area / z
+        return S.tocsr()
 
     def potentials(self, U=None):
         """Wavenumber sum (dcfemmodelling.cpp:1707-1729)."""

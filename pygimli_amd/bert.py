@@ -181,10 +181,13 @@ class MeshAnalysis:
             if zvar and ystill:
                 pos[:, [1, 2]] = pos[:, [2, 1]]
         cm = mesh.cellMarkers()
-        if np.any(bm <= MARKER_BOUND_ELECTRODE + 1) or np.any((cm <= MARKER_BOUND_ELECTRODE) &
-                                                               (cm > MARKER_FIXEDVALUE_REGION)):
-            raise NotImplementedError("complete electrode model (electrode boundaries / cells with marker "
-                                      "<= -10000) is not supported")
+        if np.any((cm <= MARKER_BOUND_ELECTRODE) & (cm > MARKER_FIXEDVALUE_REGION)):
+            raise NotImplementedError("complete electrode model: electrode cells (marker <= -10000) are not "
+                                      "supported, electrode faces are")
+        self.cemFaces = {}
+        if np.any(bm <= MARKER_BOUND_ELECTRODE + 1):
+            self._search_cem(mesh, pos)
+            return
         nmk = np.asarray(mesh.nodeMarkers())
         self.bypassGroups = {}             # marker -> node ids ascending, markers in order of appearance
         for i in np.nonzero(nmk <= MARKER_BOUND_ELECTRODE)[0]:
@@ -225,6 +228,83 @@ class MeshAnalysis:
             self.elecNode[i] = cand.pop(hit)
         self.elecPos = np.where((self.elecNode >= 0)[:, None], nodes[np.maximum(self.elecNode, 0)], self.freePos)
         self.sourceCenter = self.elecPos.sum(0) / len(self.elecPos)
+
+    def _search_cem(self, mesh, pos):
+        """Complete-electrode-model electrodes (searchElectrodes_,
+        dcfemmodelling.cpp:804-851, :868-884): boundary faces with marker
+        -10000 - id grouped per id (ElectrodeShapeDomain: position = mean of
+        the faces' nodes, size = their area); sensor i takes the first
+        remaining electrode, in id order, within 2 sqrt(size) of it or with id
+        i.  Every sensor must find one (the reference then gives node
+        electrodes unknowns that no current reaches); the electrode unknowns
+        follow the n mesh nodes: elecNode = n + i."""
+        if mesh.dim() != 3:
+            raise NotImplementedError("complete electrode model in 2.5D: the reference keeps only the last "
+                                      "wavenumber's electrode potentials (dcfemmodelling.cpp:1910-1913)")
+        bm = mesh.boundaryMarkers()
+        ids = sorted({int(MARKER_BOUND_ELECTRODE - m) for m in bm[bm <= MARKER_BOUND_ELECTRODE + 1]})
+        if ids[0] < 0:
+            raise NotImplementedError("complete electrode model: passive bodies (marker -9999) are not supported")
+        size = mesh.boundarySizes()
+        nodes = mesh.positions()
+        domains = []
+        for eid in ids:
+            faces = np.nonzero(bm == MARKER_BOUND_ELECTRODE - eid)[0]
+            centre = nodes[np.unique(mesh.bounds[faces])].mean(0)
+            domains.append((eid, faces, centre, float(size[faces].sum())))
+        for i, p in enumerate(pos):
+            for j, (eid, faces, centre, area) in enumerate(domains):
+                if np.sqrt(((p - centre) ** 2).sum()) < math.sqrt(area) * 2.0 or eid == i:
+                    self.cemFaces[i] = faces
+                    domains.pop(j)
+                    break
+        if len(self.cemFaces) != len(pos):
+            raise NotImplementedError("complete electrode model: every sensor needs an electrode face group "
+                                      "(node or free electrodes beside it are not supported)")
+        n = mesh.nodeCount()
+        self.bypassGroups = {}
+        self.elecNode = (n + np.arange(len(pos))).astype(np.int32)
+        self.elecCell = np.full(len(pos), -1, np.int32)
+        self.freePos = np.zeros((len(pos), 3))
+        self.elecPos = pos.copy()
+        self.sourceCenter = self.elecPos.sum(0) / len(self.elecPos)
+
+    def cem_terms(self, mesh, impedances=None, resistances=None):
+        """Matrix terms of the complete electrode model
+        (assembleCompleteElectrodeModel_, dcfemmodelling.cpp:315-410, no
+        reference electrode): electrode e with unknown v = n + e, faces f,
+        area A = sum |f| and contact impedance z (impedances[e], default
+        1 Ohm m^2; z = resistances[e] * A when contact resistances are given):
+        (i, v), (v, i): -|f| / d / z per face node i (int_f N_i); (i, j):
+        |f| (1 + d_ij) / (d (d + 1)) / z (int_f N_i N_j); (v, v): A / z.
+        Returns rows, cols, vals and the node pairs (v, i) that need slots."""
+        d = mesh.dim()
+        n = mesh.nodeCount()
+        size = mesh.boundarySizes()
+        rows, cols, vals, pairs = [], [], [], []
+        for e, faces in sorted(self.cemFaces.items()):
+            area = float(size[faces].sum())
+            z = 1.0 if impedances is None else float(impedances[e])
+            if resistances is not None:
+                z = float(resistances[e]) * area
+            v = n + e
+            for f in faces:
+                fn = mesh.bounds[f]
+                u = size[f] / d
+                for a in range(d):
+                    rows += [fn[a], v]
+                    cols += [v, fn[a]]
+                    vals += [-u / z, -u / z]
+                    pairs.append((v, int(fn[a])))
+                    for b in range(d):
+                        rows.append(fn[a])
+                        cols.append(fn[b])
+                        vals.append(size[f] / (d * (d + 1)) * (2.0 if a == b else 1.0) / z)
+            rows.append(v)
+            cols.append(v)
+            vals.append(area / z)
+        return (np.array(rows, np.int32), np.array(cols, np.int32), np.array(vals, np.float64),
+                sorted(set(pairs)))
 
     def bypass_pairs(self, resistances=None):
         """Node pairs (i, i+1) of every bypass group with their conductance
@@ -360,3 +440,26 @@ def uncoupled_pairs(cells, a, b):
         if not ((sub == x).any(1) & (sub == y).any(1)).any():
             out.add((min(x, y), max(x, y)))
     return sorted(out)
+
+
+def load_electrode_vector(path, n, default):
+    """checkIfMapFileExistAndLoadToVector (dcfemmodelling.cpp:114-137): a
+    loadFloatMap file of electrode index -> value into a vector of n
+    defaults; indices count from 1 unless the first key rounds to 0; key -1
+    is the last electrode; other indices out of range are skipped."""
+    v = np.full(n, float(default))
+    m = load_float_map(path)
+    if not m:
+        return v
+    from_one = int(np.rint(min(m))) != 0
+    for key in sorted(m):
+        if key == -1:
+            v[-1] = m[key]
+            continue
+        idx = int(np.rint(key))
+        if from_one:
+            if 0 < idx <= n:
+                v[idx - 1] = m[key]
+        elif 0 <= idx < n:
+            v[idx] = m[key]
+    return v

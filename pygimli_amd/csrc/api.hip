@@ -34,16 +34,18 @@ template <class F> int guarded(pgert_ctx * c, F f) {
 
 // Sorted neighbour lists of every node (SparseMatrix::buildSparsityPattern,
 // sparsematrix.h:966-1032: all node pairs sharing a cell, sorted per row),
-// plus the n_couple node pairs (ca[i], cb[i]) in both directions: couplings
-// that no cell carries, such as bypass resistors between distant nodes (the
-// reference inserts them into a map matrix, dcfemmodelling.cpp:695-707).
+// plus the n_couple node pairs (ca[i], cb[i]) in both directions and their
+// diagonals: couplings that no cell carries, such as bypass resistors between
+// distant nodes (the reference inserts them into a map matrix,
+// dcfemmodelling.cpp:695-707) or the electrode unknowns of the complete
+// electrode model (:315-410), which belong to no cell.
 void build_pattern(Ctx & c, const int * cells, int n_couple, const int * ca, const int * cb) {
     const int n = c.n, C = c.C, nv = c.nv;
     std::vector<int> cnt(n + 1, 0);
     for (long e = 0; e < (long)C * nv; ++e) cnt[cells[e] + 1] += nv;
     for (int i = 0; i < n_couple; ++i) {
-        cnt[ca[i] + 1]++;
-        cnt[cb[i] + 1]++;
+        cnt[ca[i] + 1] += 2;
+        cnt[cb[i] + 1] += 2;
     }
     for (int i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
     std::vector<int> cand(cnt[n]);
@@ -55,7 +57,9 @@ void build_pattern(Ctx & c, const int * cells, int n_couple, const int * ca, con
     }
     for (int i = 0; i < n_couple; ++i) {
         cand[fill[ca[i]]++] = cb[i];
+        cand[fill[ca[i]]++] = ca[i];
         cand[fill[cb[i]]++] = ca[i];
+        cand[fill[cb[i]]++] = cb[i];
     }
     c.hRowptr.assign(n + 1, 0);
     c.hColidx.clear();
@@ -390,7 +394,8 @@ static void mesh_setup(pgert_ctx * c, int elemOrder, int dim, int n_nodes, const
 
         c->hElecNode.clear();
         c->nE = 0;
-        c->nByp = 0;   // bypass slots index the previous pattern
+        c->nByp = 0;   // bypass and electrode-model slots index the previous pattern
+        c->nCem = 0;
         c->s1Valid = c->primValid = c->haveU = c->haveUc = c->srState = false;
         c->primNumeric = false;
         sync(*c);
@@ -411,6 +416,7 @@ static void set_electrodes(pgert_ctx * c, int n_elec, const int * elec_node, con
     PG_CHECK(hipMemcpy(hc.data(), c->cells.p, hc.size() * 4, hipMemcpyDeviceToHost));
     const int dim = c->dim, nv = c->nv;
     c->nE = n_elec;
+    c->virtualElec = false;
     c->hElecNode.assign(elec_node, elec_node + n_elec);
     auto dist = [&](const double * a, const double * b) {
         double d2 = 0;
@@ -489,8 +495,11 @@ static void set_electrodes(pgert_ctx * c, int n_elec, const int * elec_node, con
     std::vector<int> ptr(1, 0), lst;
     std::vector<double> rmin(n_elec, 0.0);
     for (int e = 0; e < n_elec; ++e) {
-        if (sing[e] >= 0) {
-            PG_REQUIRE(!cl[e].empty(), "pgert_set_electrodes: electrode node has no cell");
+        if (sing[e] >= 0 && cl[e].empty()) {
+            // a node of no cell: an electrode unknown of the complete electrode
+            // model, coupled to its faces only (total-field operator only)
+            c->virtualElec = true;
+        } else if (sing[e] >= 0) {
             // nearest neighbour distance (ElectrodeShapeNode::setSingValue, electrode.cpp:157-171)
             double rr = DBL_MAX;
             const int v = sing[e];
@@ -754,6 +763,8 @@ void forward_prepare(Ctx & c, const double * rho, const pgert_solve_opts & o) {
     PG_REQUIRE(c.C > 0 && c.nE > 0 && c.nK > 0 && c.D > 0,
                "pgert_forward: mesh, electrodes, wavenumbers and data must be set");
     PG_REQUIRE(c.order == 1, "pgert_forward: a P2 mesh serves the total-field primary problem only");
+    PG_REQUIRE(!c.virtualElec, "pgert_forward: electrodes on nodes of no cell (complete electrode model) need the "
+                               "total-field operator");
     PG_REQUIRE(rho, "pgert_forward: null pointer");
     hipStream_t s = c.stream;
     const size_t nk = c.nK, nn = c.n, ne = c.nE;
@@ -887,6 +898,9 @@ int pgert_forward_complex(pgert_ctx * c, const double * rho_re, const double * r
                           pgert_solve_stats * stats) {
     return guarded(c, [&] {
         PG_REQUIRE(rho_re && rho_im && resp_re && resp_im, "pgert_forward_complex: null pointer");
+        PG_REQUIRE(c->nCem == 0 && !c->virtualElec,
+                   "pgert_forward_complex: the complete electrode model is real only (as in the reference, "
+                   "dcfemmodelling.cpp:412-417)");
         Ctx & C = *c;
         PG_REQUIRE(C.C > 0 && C.nE > 0 && C.nK > 0 && C.D > 0,
                    "pgert_forward_complex: mesh, electrodes, wavenumbers and data must be set");
@@ -1011,6 +1025,26 @@ int pgert_forward_total(pgert_ctx * c, const double * rho, const pgert_solve_opt
         sync(C);
         read_timings(C);
         if (stats) *stats = st;
+    });
+}
+
+int pgert_set_cem(pgert_ctx * c, int n_terms, const int * rows, const int * cols, const double * vals) {
+    return guarded(c, [&] {
+        Ctx & C = *c;
+        PG_REQUIRE(C.n > 0, "pgert_set_cem: set the mesh first");
+        PG_REQUIRE(n_terms >= 0 && (n_terms == 0 || (rows && cols && vals)), "pgert_set_cem: bad arguments");
+        std::vector<int> slot(n_terms);
+        for (int i = 0; i < n_terms; ++i) {
+            PG_REQUIRE(rows[i] >= 0 && rows[i] < C.n && cols[i] >= 0 && cols[i] < C.n,
+                       "pgert_set_cem: bad entry " + std::to_string(i));
+            slot[i] = find_slot(C, rows[i], cols[i]);
+            PG_REQUIRE(slot[i] >= 0, "pgert_set_cem: entry (" + std::to_string(rows[i]) + ", " +
+                                         std::to_string(cols[i]) + ") is not in the matrix pattern");
+        }
+        C.nCem = n_terms;
+        C.cemSlot.upload(slot, C.stream);
+        C.cemVal.upload(vals, n_terms, C.stream);
+        sync(C);
     });
 }
 

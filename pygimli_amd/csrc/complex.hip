@@ -316,7 +316,14 @@ void launch_response_complex(Ctx & c, const double * Ure, const double * Uim, do
     PG_CHECK(hipGetLastError());
 }
 
+// the complete electrode model's terms, then the bypass resistors, as the
+// reference adds them (calculateK_, dcfemmodelling.cpp:1790-1833)
 void launch_bypass(Ctx & c, double * vals) {
+    if (c.nCem > 0) {
+        hipLaunchKernelGGL(k_bypass, dim3(div_up(c.nK, 64)), dim3(64), 0, c.stream, c.nK, c.nnz, c.nCem,
+                           c.cemSlot.p, c.cemVal.p, vals);
+        PG_CHECK(hipGetLastError());
+    }
     if (c.nByp == 0) return;
     hipLaunchKernelGGL(k_bypass, dim3(div_up(c.nK, 64)), dim3(64), 0, c.stream, c.nK, c.nnz, 4 * c.nByp,
                        c.bypSlot.p, c.bypVal.p, vals);

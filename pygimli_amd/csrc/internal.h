@@ -168,6 +168,7 @@ struct Ctx {
     DBuf<double> prim;      // nK * n * nE   unit-resistivity primary potentials
     bool primValid = false;
     bool primNumeric = false;    // primaries come from pgert_set_primary_potentials
+    bool virtualElec = false;    // an electrode sits on a node of no cell (complete electrode model)
     DBuf<double> U;         // nK * n * nE   total potentials (secondary + rho_s * primary)
     bool haveU = false;
     bool srState = false;   // X / prim / rhoS / rho hold a singularity-removal forward
@@ -204,6 +205,9 @@ struct Ctx {
     DBuf<int> bypSlot;
     DBuf<double> bypVal;
     int nByp = 0;
+    DBuf<int> cemSlot;           // complete electrode model: constant terms per slot
+    DBuf<double> cemVal;
+    int nCem = 0;
     bool haveUc = false;
 
     // block solver's own stream (graph capture needs a non-default stream),

@@ -15,8 +15,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from .bert import (MeshAnalysis, bypass_marker_resistances, geometric_factors, kwave_list, read_bypass_map,
-                   uncoupled_pairs)
+from .bert import (MeshAnalysis, bypass_marker_resistances, geometric_factors, kwave_list, load_electrode_vector,
+                   read_bypass_map, uncoupled_pairs)
 from .mesh import p2_refinement
 from .data import DataContainerERT
 
@@ -64,8 +64,13 @@ class ERTModelling:
         # :672-674); read here whenever they change
         self.byPassFile = "bypass.map"
         self.bypassResistanceFile = "electrodeBypassResistances.map"
+        self.contactImpedanceFile = "contactImpedance.map"
+        self.contactResistanceFile = "contactResistance.map"
+        self._contactZ = None
         self._bypassSig = None
         self._couplings = ()
+        self._extra = None         # positions of the electrode unknowns (complete electrode model)
+        self._nLib = 0             # nodes the library holds: mesh nodes + electrode unknowns
         self._prim = None          # numerical primary potentials (topography with SR), [nK][n][nE]
         self._primKey = None
         self._complex = False
@@ -119,6 +124,15 @@ class ERTModelling:
         if self._mesh is not None and self._data is not None:
             self._setup_electrodes()
 
+    def setContactImpedances(self, zi):
+        """Contact impedance (Ohm m^2) of every electrode of the complete
+        electrode model (DCMultiElectrodeModelling::setContactImpedances,
+        dcfemmodelling.cpp:776-778), in place of the contactImpedance.map file
+        that is read otherwise (default 1 Ohm m^2)."""
+        self._contactZ = np.asarray(zi, np.float64).copy()
+        if self._mesh is not None and self._data is not None:
+            self._setup_electrodes()
+
     def setVerbose(self, v):
         self.verbose = bool(v)
 
@@ -129,12 +143,15 @@ class ERTModelling:
         if self._data is not None:
             self._setup_electrodes()
 
-    def _upload_mesh(self, couplings):
+    def _upload_mesh(self, couplings, extra=None):
         """Mesh to the library; `couplings`: node pairs sharing no cell that
-        need matrix slots (bypass resistors between distant nodes)."""
+        need matrix slots (bypass resistors between distant nodes, electrode
+        unknowns); `extra`: positions of nodes of no cell appended after the
+        mesh nodes (the electrode unknowns of the complete electrode model)."""
         mesh = self._mesh
         ctx = self._ctx
-        xyz, pxyz = _host_d(mesh.positions())
+        pos = mesh.positions() if extra is None else np.vstack([mesh.positions(), extra])
+        xyz, pxyz = _host_d(pos)
         cells, pc = _host_i(mesh.cells)
         cm, pcm = _host_i(mesh.cellMarkers())
         b, pb = _host_i(mesh.bounds)
@@ -144,13 +161,15 @@ class ERTModelling:
         if couplings:
             ca, pca = _host_i([p[0] for p in couplings])
             cb, pcb = _host_i([p[1] for p in couplings])
-            ctx.check(ctx.L.pgert_set_mesh_coupled(ctx.h, mesh.dim(), mesh.nodeCount(), pxyz, mesh.cellCount(), pc,
+            ctx.check(ctx.L.pgert_set_mesh_coupled(ctx.h, mesh.dim(), len(pos), pxyz, mesh.cellCount(), pc,
                                                    pcm, mesh.boundaryCount(), pb, pbm, plc, len(ca), pca, pcb),
                       "setMesh")
         else:
             ctx.check(ctx.L.pgert_set_mesh(ctx.h, mesh.dim(), mesh.nodeCount(), pxyz, mesh.cellCount(), pc, pcm,
                                            mesh.boundaryCount(), pb, pbm, plc), "setMesh")
         self._couplings = tuple(couplings)
+        self._extra = None if extra is None else np.array(extra, np.float64)
+        self._nLib = len(pos)
         nnz = ctypes.c_int()
         npar = ctypes.c_int()
         ctx.L.pgert_info(ctx.h, ctypes.byref(nnz), ctypes.byref(npar))
@@ -214,10 +233,27 @@ class ERTModelling:
         if not self._sr and os.path.isfile(self.byPassFile):
             fa, fb, fr = read_bypass_map(self.byPassFile, an.elecNode, mesh.nodeMarkers())
             ba, bb, bg = np.concatenate([fa, ba]), np.concatenate([fb, bb]), np.concatenate([1.0 / fr, bg])
+        cem = None
+        if an.cemFaces:
+            # complete electrode model (total field only, as in the reference,
+            # whose SR operator cannot read electrode potentials from it)
+            if self._sr:
+                raise NotImplementedError("the complete electrode model needs ERTModelling(sr=False)")
+            zi = self._contactZ
+            if zi is None and os.path.isfile(self.contactImpedanceFile):
+                zi = load_electrode_vector(self.contactImpedanceFile, len(an.elecNode), 1.0)
+            zr = None
+            if os.path.isfile(self.contactResistanceFile):
+                zr = load_electrode_vector(self.contactResistanceFile, len(an.elecNode), 1.0)
+            cem = an.cem_terms(mesh, zi, zr)
         if not self._sr:
-            need = uncoupled_pairs(mesh.cells, ba, bb)
-            if not set(need) <= set(self._couplings):
-                self._upload_mesh(tuple(sorted(set(need) | set(self._couplings))))
+            need = set(uncoupled_pairs(mesh.cells, ba, bb)) | set(cem[3] if cem else ())
+            extra = an.elecPos if cem else None
+            same_extra = (extra is None and self._extra is None) or (
+                extra is not None and self._extra is not None and np.array_equal(extra, self._extra))
+            if not need <= set(self._couplings) or not same_extra:
+                keep = set(self._couplings) if same_extra else set()
+                self._upload_mesh(tuple(sorted(need | keep)), extra)
         self._analysis = an
         en, pen = _host_i(an.elecNode)
         sc, psc = _host_d(an.sourceCenter)
@@ -233,6 +269,11 @@ class ERTModelling:
         bb, pbb = _host_i(bb)
         bg, pbg = _host_d(bg)
         ctx.check(ctx.L.pgert_set_bypass(ctx.h, len(ba), pba, pbb, pbg), "bypass")
+        if cem:
+            cr, pcr = _host_i(cem[0])
+            cc, pcc = _host_i(cem[1])
+            cv, pcv = _host_d(cem[2])
+            ctx.check(ctx.L.pgert_set_cem(ctx.h, len(cr), pcr, pcc, pcv), "complete electrode model")
         if not self._kUser or self._kv is None:
             self._kv, self._wv = kwave_list(mesh.dim(), an.elecPos)
         self._dirty = True
@@ -242,7 +283,7 @@ class ERTModelling:
         if self._sr:
             return None
         sig = []
-        for f in (self.byPassFile, self.bypassResistanceFile):
+        for f in (self.byPassFile, self.bypassResistanceFile, self.contactImpedanceFile, self.contactResistanceFile):
             try:
                 st = os.stat(f)
                 sig.append((os.path.abspath(f), st.st_mtime_ns, st.st_size))
@@ -351,8 +392,9 @@ class ERTModelling:
         d = self._data
         kf = d["k"]
         if not d.haveData("k") or np.min(np.abs(kf)) < 1e-12:
-            if self._analysis is not None and self._analysis.topography:
-                raise ValueError("data contains no K-factors (needed with topography)")
+            if self._analysis is not None and (self._analysis.topography or self._analysis.cemFaces):
+                raise ValueError("data contains no K-factors (needed with topography or the complete electrode "
+                                 "model, dcfemmodelling.cpp:1088-1098)")
             d.set("k", self.calcGeometricFactor(d))
 
     def _upload_data(self):
@@ -489,7 +531,7 @@ class ERTModelling:
         """Per-wavenumber unweighted total potentials of the last response:
         rows electrode + nE * kIdx, columns nodes (subSolutions_,
         dcfemmodelling.cpp:1681-1682); node_major=True gives [nK, n, nE]."""
-        nK, nE, n = len(self._kv), len(self._analysis.elecNode), self._mesh.nodeCount()
+        nK, nE, n = len(self._kv), len(self._analysis.elecNode), self._nLib
         if node_major:
             out = torch.empty((nK, n, nE), dtype=torch.float64, device=self.device)
             self._ctx.check(self._ctx.L.pgert_potentials_node_major(self._ctx.h, _dptr(out)), "potentials")

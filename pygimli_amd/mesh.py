@@ -128,6 +128,13 @@ class Mesh:
     def cellCenters(self):
         return self.nodes[self.cells].mean(axis=1)
 
+    def boundarySizes(self):
+        """Length (2D) or area (3D) of every boundary (Shape::domainSize)."""
+        p = self.nodes[self.bounds]
+        if self._dim == 2:
+            return np.sqrt(((p[:, 1] - p[:, 0]) ** 2).sum(-1))
+        return 0.5 * np.sqrt((np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]) ** 2).sum(-1))
+
     def boundaryCenters(self):
         return self.nodes[self.bounds].mean(axis=1)
 

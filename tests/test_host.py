@@ -134,6 +134,54 @@ def test_bypass_map_files(tmp_path):
         bert.load_float_map(str(m))
 
 
+def test_cem_electrode_search_and_map_files(tmp_path):
+    """Complete-electrode-model electrodes: the same face groups and matching
+    as the oracle's search (which the reference build pins), electrode
+    unknowns after the nodes; the contact map files read as
+    checkIfMapFileExistAndLoadToVector does (from 1, from 0 when the first key
+    is 0, -1 = last); 2.5D, passive bodies and sensors without faces are
+    refused."""
+    from pygimli_amd.mesh import Mesh
+    mesh, data, _ = workloads.small_3d_cem()
+    an = bert.MeshAnalysis(mesh, data.sensorPositions())
+    st = eo.Setup(mesh, data.sensorPositions())
+    assert sorted(an.cemFaces) == sorted(st.cem)
+    for i in an.cemFaces:
+        np.testing.assert_array_equal(an.cemFaces[i], st.cem[i])
+    np.testing.assert_array_equal(an.elecNode, mesh.nodeCount() + np.arange(data.sensorCount()))
+    rows, cols, vals, pairs = an.cem_terms(mesh)
+    areas = [mesh.boundarySizes()[f].sum() for _, f in sorted(an.cemFaces.items())]
+    diag = vals[rows == cols][np.asarray(rows[rows == cols]) >= mesh.nodeCount()]
+    np.testing.assert_allclose(diag, areas, rtol=1e-15)
+    # each electrode row sums to zero: area/z - sum_f int N / z
+    n = mesh.nodeCount()
+    for e in range(data.sensorCount()):
+        assert abs(vals[rows == n + e].sum()) < 1e-14
+    f = tmp_path / "z.map"
+    f.write_text("1 0.5\n3 2.0\n")
+    np.testing.assert_array_equal(bert.load_electrode_vector(str(f), 4, 1.0), [0.5, 1.0, 2.0, 1.0])
+    f.write_text("0 0.5\n3 2.0\n")
+    np.testing.assert_array_equal(bert.load_electrode_vector(str(f), 4, 1.0), [0.5, 1.0, 1.0, 2.0])
+    f.write_text("-1 7\n2 3\n9 4\n")
+    np.testing.assert_array_equal(bert.load_electrode_vector(str(f), 4, 1.0), [1.0, 3.0, 1.0, 7.0])
+    bm = mesh.boundaryMarkers().copy()
+    bm[bm == -10000] = -9999
+    with pytest.raises(NotImplementedError):
+        bert.MeshAnalysis(Mesh(3, mesh.nodes, mesh.cells, mesh.cellMarkers(), mesh.bounds, bm,
+                               mesh.nodeMarkers()), data.sensorPositions())
+    bm = mesh.boundaryMarkers().copy()
+    bm[bm == -10008] = -1
+    with pytest.raises(NotImplementedError):
+        bert.MeshAnalysis(Mesh(3, mesh.nodes, mesh.cells, mesh.cellMarkers(), mesh.bounds, bm,
+                               mesh.nodeMarkers()), data.sensorPositions())
+    m2, d2, _ = workloads.small_2d()
+    bm2 = m2.boundaryMarkers().copy()
+    bm2[np.nonzero(bm2 == -1)[0][:1]] = -10000
+    with pytest.raises(NotImplementedError):
+        bert.MeshAnalysis(Mesh(2, m2.nodes, m2.cells, m2.cellMarkers(), m2.bounds, bm2, m2.nodeMarkers()),
+                          d2.sensorPositions())
+
+
 @pytest.mark.parametrize("name", ["small_2d", "small_3d"])
 def test_meshes_are_conforming(name):
     mesh, _, _ = getattr(workloads, name)()

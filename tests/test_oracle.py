@@ -722,6 +722,49 @@ def test_sr_topography_oracle_vs_reference_build(name):
     assert np.abs(fa.primaries() - P).max() > 1e-3 * np.abs(P).max()
 
 
+def test_cem_oracle_vs_reference_build(tmp_path, monkeypatch):
+    """Complete electrode model (assembleCompleteElectrodeModel_,
+    dcfemmodelling.cpp:315-410): nine electrodes of surface faces, electrode
+    unknowns after the nodes, potentials read from them.  The oracle against
+    the reference build with the default contact impedance (1 Ohm m^2), with
+    contactImpedance.map (electrodes counted from 1) and with
+    contactResistance.map (z = R * area); point electrodes give another
+    result."""
+    if not ref.available():
+        pytest.skip("oracle/_ref not built")
+    from pygimli_amd.bert import load_electrode_vector
+    mesh, data, rho = workloads.small_3d_cem()
+    monkeypatch.chdir(tmp_path)
+    cases = [(None, None), ("1 0.5\n3 2.0\n9 0.25\n", None), (None, "2 5.0\n-1 0.1\n")]
+    out = []
+    for zfile, rfile in cases:
+        for name, text in (("contactImpedance.map", zfile), ("contactResistance.map", rfile)):
+            if text is None:
+                (tmp_path / name).unlink(missing_ok=True)
+            else:
+                (tmp_path / name).write_text(text)
+        s = ref.Session(mesh, copy.deepcopy(data), nthreads=1, sr=False)
+        k, w = s.kw()
+        r = s.response(rho)
+        J = s.jacobian(rho)
+        f = eo.Forward(mesh, copy.deepcopy(data), k, w, total_field=True)
+        assert len(f.setup.cem) == data.sensorCount()
+        if zfile:
+            f.contactImpedance = load_electrode_vector("contactImpedance.map", 9, 1.0)
+        if rfile:
+            f.contactResistance = load_electrode_vector("contactResistance.map", 9, 1.0)
+        o = eo.response_total(f, rho)
+        oJ = f.jacobian(rho)
+        assert np.abs(o - r).max() <= 1e-12 * np.abs(r).max()
+        assert np.linalg.norm(oJ - J) <= 1e-12 * np.linalg.norm(J)
+        out.append(r)
+    assert np.abs(out[1] - out[0]).max() > 1e-3 * np.abs(out[0]).max()
+    assert np.abs(out[2] - out[0]).max() > 1e-3 * np.abs(out[0]).max()
+    m0, _, _ = workloads.small_3d(spacing=2.0)
+    f0 = eo.Forward(m0, copy.deepcopy(data), k, w, total_field=True)
+    assert np.abs(eo.response_total(f0, rho) - out[0]).max() > 0.1 * np.abs(out[0]).max()
+
+
 def test_neumann_2d_total_field_oracle_vs_reference_build():
     """A 2.5D mesh with only homogeneous Neumann faces: the reference turns
     the Neumann domain off in 2D (dcfemmodelling.cpp:758-764), so the total
