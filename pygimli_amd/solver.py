@@ -62,7 +62,7 @@ def _fill(mesh, a, mass, device=0):
     a = np.broadcast_to(a.astype(np.float64), (mesh.cellCount(),))
     ctx, rowptr, colidx = _context_for(mesh, device)
     dev = torch.device("cuda", device)
-    at = torch.as_tensor(np.ascontiguousarray(a), device=dev)
+    at = torch.as_tensor(np.array(a, np.float64), device=dev)
     vals = torch.empty(len(colidx), dtype=torch.float64, device=dev)
     ctx.check(ctx.L.pgert_fill_matrix(ctx.h, int(mass), ctypes.c_void_p(at.data_ptr()),
                                       ctypes.c_void_p(vals.data_ptr())), "fill_matrix")
