@@ -228,6 +228,17 @@ int pgert_potentials(pgert_ctx * ctx, double * out);
 int pgert_forward_total(pgert_ctx * ctx, const double * rho, const pgert_solve_opts * opts, double * rhoa,
                         pgert_solve_stats * stats);
 
+/* Reference electrode and calibration nodes of the total-field operator
+ * (searchElectrodes_, dcfemmodelling.cpp:1010-1071): ref_node >= 0 (the first
+ * node with marker -999) makes every current pattern a dipole against it
+ * (createCurrentPattern :1517-1523, right-hand side +1 at the electrode, -1
+ * there); the n_cal calibration nodes (marker -1000, or node 0 in a Neumann
+ * domain without one; none outside Neumann domains) get cleared rows and
+ * columns and a unit diagonal after the bypass resistors (:1835), the
+ * right-hand side unchanged.  ref_node = -1 and n_cal = 0 remove them.  The
+ * SR and complex forwards refuse both.  HOST array. */
+int pgert_set_reference(pgert_ctx * ctx, int ref_node, int n_cal, const int * cal_nodes);
+
 /* Complete electrode model of the total-field operator
  * (assembleCompleteElectrodeModel_, dcfemmodelling.cpp:315-410, applied in
  * calculateK_ :1790-1829): n_terms constant entries vals[i] added at

@@ -277,9 +277,24 @@ class Setup:
             yconst = not (np.ptp(ePos[:, 1]) > 0) and np.abs(ePos[:, 1]).max() < 1e-8
             if zvar and yconst:
                 ePos[:, [1, 2]] = ePos[:, [2, 1]]
+        # reference electrode (first node with marker -999, :1010-1015): every
+        # current pattern becomes a dipole against it (createCurrentPattern
+        # :1517-1523); calibration nodes (marker -1000, node 0 when a Neumann
+        # domain has none, :1044-1071) are homogeneous Dirichlet nodes of the
+        # total-field matrix (:1835), ignored outside Neumann domains
+        nm = mesh.nodeMarkers()
+        ref = np.nonzero(nm == -999)[0]
+        self.refNode = int(ref[0]) if len(ref) else -1
+        cal = [int(i) for i in np.nonzero(nm == -1000)[0]]
+        if self.neumannDomain:
+            self.calibration = cal if cal else [0]
+            if self.refNode < 0:
+                raise NotImplementedError("Neumann domain without a reference electrode: the reference takes the "
+                                          "last electrode, whose Jacobian reads past its potential rows")
+        else:
+            self.calibration = []
         # bypass electrodes: nodes with one marker <= -10000 (searchElectrodes_
         # :815-826), matched to a sensor at the group's first node (:886-897)
-        nm = mesh.nodeMarkers()
         self.bypass = {}
         for nid in range(len(nm)):
             if nm[nid] <= -10000:
@@ -299,6 +314,8 @@ class Setup:
         self.cem = {}
         cemIds = sorted({int(-10000 - m) for m in bm[bm <= -9999]})
         if cemIds:
+            if self.refNode >= 0:
+                raise NotImplementedError("a reference electrode with the complete electrode model")
             if cemIds[0] < 0:
                 raise NotImplementedError("passive bodies of the complete electrode model")
             size, _, _ = boundary_geometry(mesh)
@@ -410,6 +427,8 @@ class Setup:
             rhs[self.elecNode[i]] = 1.0
         else:
             rhs[self.mesh.cells[self.elecCell[i]]] = self.elecShape[i]
+        if self.refNode >= 0:
+            rhs[self.refNode] -= 1.0    # dipole against the reference electrode (calculateK_ :1854-1855)
         return rhs
 
     def pot(self, i, sol):
@@ -826,8 +845,8 @@ class Forward:
         self.mesh = mesh
         self.data = data
         self.setup = Setup(mesh, data.sensorPositions())
-        if self.setup.neumannDomain:
-            raise NotImplementedError("3D Neumann-only domains are not part of this path")
+        if self.setup.refNode >= 0 and not total_field:
+            raise NotImplementedError("a reference electrode with singularity removal (dipole primaries)")
         if kvalues is None:
             kvalues, weights = kwave_list(mesh.dim(), self.setup.elecPos)
         self.k = np.asarray(kvalues, np.float64)
@@ -905,6 +924,15 @@ class Forward:
                     S[a2, a2] += g
                     S[a1, a2] -= g
                     S[a2, a1] -= g
+                S = S.tocsr()
+            if self.setup.calibration:
+                # calibration nodes: rows and columns cleared, diagonal one,
+                # after the bypass resistors (:1835); the right-hand side stays
+                S = S.tolil()
+                for c in self.setup.calibration:
+                    S[c, :] = 0.0
+                    S[:, c] = 0.0
+                    S[c, c] = 1.0
                 S = S.tocsr()
             lu = spla.splu(S.tocsc())
             for i in range(self.nE):

@@ -33,7 +33,7 @@ EXPORTS = (
     "pgert_batch_forward", "pgert_batch_jacobian", "pgert_set_constraints", "pgert_cgls", "pgert_cgls_noalloc",
     "pgert_forward_complex", "pgert_jacobian_complex", "pgert_forward_total", "pgert_set_bypass",
     "pgert_trans", "pgert_gn_phi", "pgert_set_electrodes_free", "pgert_set_mesh_coupled", "pgert_set_mesh_p2",
-    "pgert_set_primary_potentials", "pgert_set_cem",
+    "pgert_set_primary_potentials", "pgert_set_cem", "pgert_set_reference",
 )
 
 
@@ -81,6 +81,7 @@ def load():
     L.pgert_set_mesh_p2.argtypes = L.pgert_set_mesh.argtypes
     L.pgert_set_primary_potentials.argtypes = [vp, vp]
     L.pgert_set_cem.argtypes = [vp, ctypes.c_int, _c_int_p, _c_int_p, _c_double_p]
+    L.pgert_set_reference.argtypes = [vp, ctypes.c_int, ctypes.c_int, _c_int_p]
     L.pgert_set_electrodes.argtypes = [vp, ctypes.c_int, _c_int_p, _c_double_p, ctypes.c_double]
     L.pgert_set_electrodes_free.argtypes = [vp, ctypes.c_int, _c_int_p, _c_int_p, _c_double_p, _c_double_p,
                                             ctypes.c_double]

@@ -16,6 +16,8 @@ MAX_DOUBLE = float(np.finfo(np.float64).max)
 TOLERANCE = 1e-12
 MARKER_BOUND_ELECTRODE = -10000        # core/src/bert/bert.h:33
 MARKER_FIXEDVALUE_REGION = -1000000    # core/src/gimli.h:241
+MARKER_NODE_REFERENCEELECTRODE = -999  # core/src/bert/bert.h:31-32
+MARKER_NODE_CALIBRATION = -1000
 BYPASS_RESISTANCE = 1e-6               # dcfemmodelling.cpp:684-688 default
 
 
@@ -184,8 +186,18 @@ class MeshAnalysis:
         if np.any((cm <= MARKER_BOUND_ELECTRODE) & (cm > MARKER_FIXEDVALUE_REGION)):
             raise NotImplementedError("complete electrode model: electrode cells (marker <= -10000) are not "
                                       "supported, electrode faces are")
+        # reference electrode (the first node with marker -999) and calibration
+        # nodes (marker -1000; node 0 in a Neumann domain without one; none
+        # outside Neumann domains), dcfemmodelling.cpp:800-803, :1010-1071
+        nm = np.asarray(mesh.nodeMarkers())
+        ref = np.nonzero(nm == MARKER_NODE_REFERENCEELECTRODE)[0]
+        self.refNode = int(ref[0]) if len(ref) else -1
+        cal = [int(i) for i in np.nonzero(nm == MARKER_NODE_CALIBRATION)[0]]
+        self.calibration = (cal if cal else [0]) if self.neumannDomain else []
         self.cemFaces = {}
         if np.any(bm <= MARKER_BOUND_ELECTRODE + 1):
+            if self.refNode >= 0:
+                raise NotImplementedError("a reference electrode with the complete electrode model")
             self._search_cem(mesh, pos)
             return
         nmk = np.asarray(mesh.nodeMarkers())

@@ -394,8 +394,10 @@ static void mesh_setup(pgert_ctx * c, int elemOrder, int dim, int n_nodes, const
 
         c->hElecNode.clear();
         c->nE = 0;
-        c->nByp = 0;   // bypass and electrode-model slots index the previous pattern
+        c->nByp = 0;   // bypass, electrode-model and calibration slots index the previous pattern
         c->nCem = 0;
+        c->nCalSlot = 0;
+        c->refNode = -1;
         c->s1Valid = c->primValid = c->haveU = c->haveUc = c->srState = false;
         c->primNumeric = false;
         sync(*c);
@@ -765,6 +767,8 @@ void forward_prepare(Ctx & c, const double * rho, const pgert_solve_opts & o) {
     PG_REQUIRE(c.order == 1, "pgert_forward: a P2 mesh serves the total-field primary problem only");
     PG_REQUIRE(!c.virtualElec, "pgert_forward: electrodes on nodes of no cell (complete electrode model) need the "
                                "total-field operator");
+    PG_REQUIRE(c.refNode < 0 && c.nCalSlot == 0,
+               "pgert_forward: a reference electrode or calibration nodes need the total-field operator");
     PG_REQUIRE(rho, "pgert_forward: null pointer");
     hipStream_t s = c.stream;
     const size_t nk = c.nK, nn = c.n, ne = c.nE;
@@ -898,6 +902,8 @@ int pgert_forward_complex(pgert_ctx * c, const double * rho_re, const double * r
                           pgert_solve_stats * stats) {
     return guarded(c, [&] {
         PG_REQUIRE(rho_re && rho_im && resp_re && resp_im, "pgert_forward_complex: null pointer");
+        PG_REQUIRE(c->refNode < 0 && c->nCalSlot == 0,
+                   "pgert_forward_complex: a reference electrode or calibration nodes are not supported here");
         PG_REQUIRE(c->nCem == 0 && !c->virtualElec,
                    "pgert_forward_complex: the complete electrode model is real only (as in the reference, "
                    "dcfemmodelling.cpp:412-417)");
@@ -1004,6 +1010,7 @@ int pgert_forward_total(pgert_ctx * c, const double * rho, const pgert_solve_opt
         C.valsA.alloc(nk * C.nnz);
         for (size_t kk = 0; kk < nk; ++kk) launch_assemble(C, C.rho.p, C.kv[kk], C.valsA.p + kk * C.nnz, false);
         launch_bypass(C, C.valsA.p);
+        launch_calibration(C, C.valsA.p);
         launch_diag_inv(C);
         PG_CHECK(hipEventRecord(C.ev[1], s));
         C.B.alloc(blk);
@@ -1025,6 +1032,43 @@ int pgert_forward_total(pgert_ctx * c, const double * rho, const pgert_solve_opt
         sync(C);
         read_timings(C);
         if (stats) *stats = st;
+    });
+}
+
+int pgert_set_reference(pgert_ctx * c, int ref_node, int n_cal, const int * cal_nodes) {
+    return guarded(c, [&] {
+        Ctx & C = *c;
+        PG_REQUIRE(C.n > 0, "pgert_set_reference: set the mesh first");
+        PG_REQUIRE(ref_node >= -1 && ref_node < C.n, "pgert_set_reference: reference node out of range");
+        PG_REQUIRE(n_cal >= 0 && (n_cal == 0 || cal_nodes), "pgert_set_reference: bad calibration nodes");
+        std::vector<char> isCal(C.n, 0);
+        for (int i = 0; i < n_cal; ++i) {
+            PG_REQUIRE(cal_nodes[i] >= 0 && cal_nodes[i] < C.n, "pgert_set_reference: calibration node out of range");
+            isCal[cal_nodes[i]] = 1;
+        }
+        // every slot of a calibration row or column once, 1 on the diagonal
+        std::vector<std::pair<int, double>> ds;
+        for (int i = 0; i < C.n; ++i) {
+            if (!isCal[i]) continue;
+            for (int q = C.hRowptr[i]; q < C.hRowptr[i + 1]; ++q) {
+                const int j = C.hColidx[q];
+                ds.push_back({q, j == i ? 1.0 : 0.0});
+                if (j != i && !isCal[j]) ds.push_back({find_slot(C, j, i), 0.0});
+            }
+        }
+        std::sort(ds.begin(), ds.end());
+        std::vector<int> slot(ds.size());
+        std::vector<double> val(ds.size());
+        for (size_t q = 0; q < ds.size(); ++q) {
+            slot[q] = ds[q].first;
+            val[q] = ds[q].second;
+        }
+        C.refNode = ref_node;
+        C.nCalSlot = (int)ds.size();
+        C.calSlot.upload(slot, C.stream);
+        C.calVal.upload(val, C.stream);
+        C.haveU = C.haveUc = C.srState = false;
+        sync(C);
     });
 }
 

@@ -300,6 +300,14 @@ void launch_assemble(Ctx & c, const double * rho, double k, double * vals, bool 
     launch_dirichlet(c, vals, false);
 }
 
+void launch_calibration(Ctx & c, double * vals) {
+    if (c.nCalSlot == 0) return;
+    for (int kk = 0; kk < c.nK; ++kk)
+        hipLaunchKernelGGL(k_set_slots, dim3(div_up(c.nCalSlot, 256)), dim3(256), 0, c.stream, c.nCalSlot,
+                           c.calSlot.p, c.calVal.p, vals + (size_t)kk * c.nnz);
+    PG_CHECK(hipGetLastError());
+}
+
 void launch_dirichlet(Ctx & c, double * vals, bool imagPart) {
     if (c.nDirSlot == 0) return;
     hipLaunchKernelGGL(k_set_slots, dim3(div_up(c.nDirSlot, 256)), dim3(256), 0, c.stream, c.nDirSlot,

@@ -95,8 +95,11 @@ __global__ void k_assemble_boundary_c(int nbs, const int * __restrict__ bslot, c
 // (ElectrodeShapeNode::assembleRHS, core/src/bert/electrode.cpp:133-151) or
 // the shape functions at its position on its cell's nodes for a free one
 // (ElectrodeShapeEntity::assembleRHS, :274-287)
+// refNode >= 0: every source is a dipole against the reference electrode
+// there (createCurrentPattern, dcfemmodelling.cpp:1517-1523; calculateK_
+// :1854-1855 adds eB's -1 after eA's +1)
 __global__ void k_point_rhs(long total, int n, int nE, const int * __restrict__ shapeNode,
-                            const double * __restrict__ shapeW, double * __restrict__ B) {
+                            const double * __restrict__ shapeW, int refNode, double * __restrict__ B) {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
     const int e = (int)(t % nE);
@@ -104,6 +107,7 @@ __global__ void k_point_rhs(long total, int n, int nE, const int * __restrict__ 
     double v = 0.0;
     for (int j = 0; j < SHAPE_NODES; ++j)
         if (shapeNode[SHAPE_NODES * e + j] == node) v = shapeW[SHAPE_NODES * e + j];
+    if (node == refNode) v -= 1.0;
     B[t] = v;
 }
 
@@ -279,7 +283,7 @@ void launch_assemble_complex(Ctx & c, const double * rre, const double * rim, do
 void launch_point_rhs(Ctx & c, double * B) {
     const long total = (long)c.nK * c.n * c.nE;
     hipLaunchKernelGGL(k_point_rhs, dim3(div_up(total, 256)), dim3(256), 0, c.stream, total, c.n, c.nE,
-                       c.shapeNode.p, c.shapeW.p, B);
+                       c.shapeNode.p, c.shapeW.p, c.refNode, B);
     PG_CHECK(hipGetLastError());
 }
 

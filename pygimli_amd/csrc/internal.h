@@ -169,6 +169,10 @@ struct Ctx {
     bool primValid = false;
     bool primNumeric = false;    // primaries come from pgert_set_primary_potentials
     bool virtualElec = false;    // an electrode sits on a node of no cell (complete electrode model)
+    int refNode = -1;            // reference electrode node: sources are dipoles against it
+    int nCalSlot = 0;            // calibration nodes: Dirichlet slots applied after the bypass resistors
+    DBuf<int> calSlot;
+    DBuf<double> calVal;
     DBuf<double> U;         // nK * n * nE   total potentials (secondary + rho_s * primary)
     bool haveU = false;
     bool srState = false;   // X / prim / rhoS / rho hold a singularity-removal forward
@@ -268,6 +272,8 @@ void launch_assemble(Ctx & c, const double * rho, double k, double * vals, bool 
 // homogeneous Dirichlet rows / columns of an assembled matrix (imagPart: the
 // imaginary half of a complex one, whose diagonal entries there are 0)
 void launch_dirichlet(Ctx & c, double * vals, bool imagPart);
+// calibration nodes (rows / columns cleared, diagonal one) in every wavenumber's matrix
+void launch_calibration(Ctx & c, double * vals);
 void launch_fill_matrix(Ctx & c, int mass, const double * a, double * vals);
 void launch_map_model(Ctx & c, const double * model, int nModel, double * rho);
 void build_prolongation(Ctx & c, const double * xyz, const int * cells, const int * marker);

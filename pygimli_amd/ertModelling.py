@@ -215,9 +215,13 @@ class ERTModelling:
     def _setup_electrodes(self):
         mesh, data = self._mesh, self._data
         an = MeshAnalysis(mesh, data.sensorPositions())
-        if an.neumannDomain:
-            raise NotImplementedError("3D Neumann-only domains (no mixed or Dirichlet boundary) need a calibration "
-                                      "node and a reference electrode, which this path does not provide")
+        if an.neumannDomain and an.refNode < 0:
+            raise NotImplementedError("a 3D Neumann-only domain without a reference electrode node (marker -999): "
+                                      "the reference then takes the last electrode, and its Jacobian reads past its "
+                                      "potential rows")
+        if (an.refNode >= 0 or an.calibration) and self._sr:
+            raise NotImplementedError("a reference electrode (marker -999) or a Neumann-only domain needs "
+                                      "ERTModelling(sr=False)")
         if an.topography and self._sr:
             # numerical primaries from the P2 total-field problem; its
             # operator would also carry free electrodes (P2 shape functions)
@@ -269,6 +273,8 @@ class ERTModelling:
         bb, pbb = _host_i(bb)
         bg, pbg = _host_d(bg)
         ctx.check(ctx.L.pgert_set_bypass(ctx.h, len(ba), pba, pbb, pbg), "bypass")
+        cal, pcal = _host_i(an.calibration)
+        ctx.check(ctx.L.pgert_set_reference(ctx.h, an.refNode, len(cal), pcal), "reference electrode")
         if cem:
             cr, pcr = _host_i(cem[0])
             cc, pcc = _host_i(cem[1])

@@ -135,9 +135,10 @@ def test_neumann_2d_total_field(method):
 
 
 def test_neumann_3d_and_inhomogeneous_dirichlet_raise():
-    """3D Neumann-only domains (calibration node and reference electrode)
-    and inhomogeneous Dirichlet faces (marker -4, not implemented in the
-    reference either) are refused with an error."""
+    """3D Neumann-only domains without a reference electrode node (the
+    reference takes the last electrode, and its Jacobian then reads past its
+    potential rows) and inhomogeneous Dirichlet faces (marker -4, not
+    implemented in the reference either) are refused with an error."""
     from pygimli_amd.ertModelling import ERTModelling
     from pygimli_amd.mesh import Mesh
     mesh, data, _ = workloads.small_3d()
@@ -153,3 +154,40 @@ def test_neumann_3d_and_inhomogeneous_dirichlet_raise():
     dm = Mesh(3, mesh.nodes, mesh.cells, mesh.cellMarkers(), mesh.bounds, bm, mesh.nodeMarkers())
     with pytest.raises(RuntimeError):
         ERTModelling(sr=False).setMesh(dm)
+
+
+@pytest.mark.parametrize("kw", [dict(neumann=True, calibration=True), dict(neumann=True, calibration=False),
+                                dict(neumann=False)], ids=["neumann-cal", "neumann-node0", "mixed"])
+@pytest.mark.parametrize("method", ["block64", "pcg64"])
+def test_reference_electrode_and_calibration(kw, method):
+    """A reference electrode node (marker -999) makes every source a dipole
+    against it; in a 3D Neumann-only domain the calibration node (marker
+    -1000, else node 0) is a homogeneous Dirichlet node after the bypass
+    resistors (dcfemmodelling.cpp:1010-1071, :1517-1523, :1835).  Against the
+    oracle (equal to the reference build in tests/test_oracle.py), potentials
+    included, and against the reference build; SR refuses it."""
+    from oracle import ref
+    from pygimli_amd.ertModelling import ERTModelling
+    mesh, data, rho = workloads.small_3d_refnode(**kw)
+    fop = _fop(mesh, data, False, method)
+    fop.solverOptions.update(rtol=1e-11)
+    r = fop.response(rho)
+    fop.createJacobian(rho)
+    J = fop.jacobian().cpu().numpy()
+    U = fop.subPotentials().cpu().numpy()
+    f = eo.Forward(mesh, copy.deepcopy(data), fop.kValues(), fop.weights(), total_field=True)
+    o = eo.response_total(f, rho)
+    oJ = f.jacobian(rho)
+    e1 = np.abs(r - o).max() / np.abs(o).max()
+    e2 = np.linalg.norm(J - oJ) / np.linalg.norm(oJ)
+    e3 = np.abs(U - f.subpots).max() / np.abs(f.subpots).max()
+    print("reference electrode", kw, method, e1, e2, e3, fop.lastStats)
+    assert e1 < 1e-8 and e2 < 1e-8 and e3 < 1e-8
+    if ref.available():
+        s = ref.Session(mesh, copy.deepcopy(data), nthreads=1, sr=False)
+        rr = s.response(rho)
+        assert np.abs(r - rr).max() <= 1e-8 * np.abs(rr).max()
+    g = ERTModelling()
+    g.setMesh(mesh)
+    with pytest.raises(NotImplementedError):
+        g.setData(copy.deepcopy(data))

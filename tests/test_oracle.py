@@ -765,6 +765,38 @@ def test_cem_oracle_vs_reference_build(tmp_path, monkeypatch):
     assert np.abs(eo.response_total(f0, rho) - out[0]).max() > 0.1 * np.abs(out[0]).max()
 
 
+@pytest.mark.parametrize("kw", [dict(neumann=True, calibration=True), dict(neumann=True, calibration=False),
+                                dict(neumann=False)], ids=["neumann-cal", "neumann-node0", "mixed"])
+def test_reference_electrode_oracle_vs_reference_build(kw):
+    """A reference electrode node (marker -999): every current pattern is a
+    dipole against it (createCurrentPattern, dcfemmodelling.cpp:1517-1523);
+    in a 3D Neumann-only domain the calibration node (marker -1000, else node
+    0) is a homogeneous Dirichlet node (:1044-1071, :1835).  Oracle = the
+    reference build; without the reference electrode the result differs."""
+    if not ref.available():
+        pytest.skip("oracle/_ref not built")
+    mesh, data, rho = workloads.small_3d_refnode(**kw)
+    s = ref.Session(mesh, copy.deepcopy(data), nthreads=1, sr=False)
+    k, w = s.kw()
+    r = s.response(rho)
+    J = s.jacobian(rho)
+    f = eo.Forward(mesh, copy.deepcopy(data), k, w, total_field=True)
+    assert f.setup.refNode >= 0 and f.setup.neumannDomain == kw["neumann"]
+    o = eo.response_total(f, rho)
+    oJ = f.jacobian(rho)
+    assert np.abs(o - r).max() <= 1e-12 * np.abs(r).max()
+    assert np.linalg.norm(oJ - J) <= 1e-12 * np.linalg.norm(J)
+    if not kw["neumann"]:
+        # four-point data cancel the reference electrode's term by linearity;
+        # the potentials themselves are dipole potentials
+        U = s.subpotentials()
+        np.testing.assert_allclose(f.subpots, U, rtol=0, atol=1e-12 * np.abs(U).max())
+        f0 = eo.Forward(mesh, copy.deepcopy(data), k, w, total_field=True)
+        f0.setup.refNode = -1
+        eo.response_total(f0, rho)
+        assert np.abs(f0.subpots - U).max() > 1e-2 * np.abs(U).max()
+
+
 def test_neumann_2d_total_field_oracle_vs_reference_build():
     """A 2.5D mesh with only homogeneous Neumann faces: the reference turns
     the Neumann domain off in 2D (dcfemmodelling.cpp:758-764), so the total
