@@ -236,7 +236,10 @@ int pgert_forward_total(pgert_ctx * ctx, const double * rho, const pgert_solve_o
  * domain without one; none outside Neumann domains) get cleared rows and
  * columns and a unit diagonal after the bypass resistors (:1835), the
  * right-hand side unchanged.  ref_node = -1 and n_cal = 0 remove them.  The
- * SR and complex forwards refuse both.  HOST array. */
+ * SR forward takes a reference electrode (dipole primaries: its analytic
+ * potential subtracted, rho_s the mean of both geometric means,
+ * checkPrimpotentials_ :2101-2102, calculateK :2219-2244) but refuses
+ * calibration nodes; the complex forward refuses both.  HOST array. */
 int pgert_set_reference(pgert_ctx * ctx, int ref_node, int n_cal, const int * cal_nodes);
 
 /* Complete electrode model of the total-field operator

@@ -293,6 +293,13 @@ class Setup:
                                           "last electrode, whose Jacobian reads past its potential rows")
         else:
             self.calibration = []
+        if self.refNode >= 0:
+            # the reference electrode as an ElectrodeShapeNode: its cells,
+            # nearest-neighbour distance and position (electrode.cpp:98-189)
+            self.refCells = np.nonzero((mesh.cells == self.refNode).any(axis=1))[0]
+            nb = np.unique(mesh.cells[self.refCells].ravel())
+            nb = nb[nb != self.refNode]
+            self.refRmin = np.sqrt(((mesh.nodes[nb] - mesh.nodes[self.refNode]) ** 2).sum(-1)).min()
         # bypass electrodes: nodes with one marker <= -10000 (searchElectrodes_
         # :815-826), matched to a sensor at the group's first node (:886-897)
         self.bypass = {}
@@ -416,6 +423,26 @@ class Setup:
         else:
             u[sn] = 1.0 / (2.0 * PI * r / 2.0)
         return u
+
+    def primary_ref(self, k):
+        """exactDCSolution of the reference electrode (eB of every pattern,
+        checkPrimpotentials_ dcfemmodelling.cpp:2101-2102), singular value at
+        its node as for a node electrode."""
+        u = exact_dc(self.mesh.nodes, self.mesh.nodes[self.refNode], k, self.surfaceZ, 0.0)
+        r = self.refRmin
+        u[self.refNode] = bessel_k0(np.array([r / 6.0 * k]))[0] / PI if k > 0.0 else 1.0 / (2.0 * PI * r / 2.0)
+        return u
+
+    def rho_source(self, i, rho):
+        """rhoSource of pattern i (DCSRMultiElectrodeModelling::calculateK,
+        dcfemmodelling.cpp:2219-2244): the mean of the geometric means of eA
+        and eB (the reference electrode) over those above TOLERANCE."""
+        vals = [self.geom_mean_rho(i, rho)]
+        if self.refNode >= 0:
+            v = rho[self.refCells]
+            vals.append(float(np.exp(np.log(v).sum() / len(v))))
+        vals = [v for v in vals if v > TOLERANCE]
+        return sum(vals) / len(vals)
 
     def source_rhs(self, i, n):
         """Point-source right-hand side of electrode i
@@ -845,8 +872,8 @@ class Forward:
         self.mesh = mesh
         self.data = data
         self.setup = Setup(mesh, data.sensorPositions())
-        if self.setup.refNode >= 0 and not total_field:
-            raise NotImplementedError("a reference electrode with singularity removal (dipole primaries)")
+        if self.setup.calibration and not total_field:
+            raise NotImplementedError("a Neumann domain with singularity removal (numerical dipole primaries)")
         if kvalues is None:
             kvalues, weights = kwave_list(mesh.dim(), self.setup.elecPos)
         self.k = np.asarray(kvalues, np.float64)
@@ -869,8 +896,9 @@ class Forward:
         if self.prim is None:
             P = np.empty((len(self.k) * self.nE, self.mesh.nodeCount()))
             for ki, k in enumerate(self.k):
+                pr = self.setup.primary_ref(k) if self.setup.refNode >= 0 else 0.0
                 for i in range(self.nE):
-                    P[i + ki * self.nE] = self.setup.primary(i, k)
+                    P[i + ki * self.nE] = self.setup.primary(i, k) - pr
             self.prim = P
         return self.prim
 
@@ -887,7 +915,7 @@ class Forward:
             S1 = assemble(self.mesh, ones, k, src, self.geo)
             lu = spla.splu(S.tocsc())
             for i in range(self.nE):
-                rs = self.setup.geom_mean_rho(i, rho)
+                rs = self.setup.rho_source(i, rho)
                 prim = P[i + ki * self.nE] * rs
                 rhs = S1 @ prim / rs - S @ prim
                 U[i + ki * self.nE] = lu.solve(rhs) + prim

@@ -767,8 +767,9 @@ void forward_prepare(Ctx & c, const double * rho, const pgert_solve_opts & o) {
     PG_REQUIRE(c.order == 1, "pgert_forward: a P2 mesh serves the total-field primary problem only");
     PG_REQUIRE(!c.virtualElec, "pgert_forward: electrodes on nodes of no cell (complete electrode model) need the "
                                "total-field operator");
-    PG_REQUIRE(c.refNode < 0 && c.nCalSlot == 0,
-               "pgert_forward: a reference electrode or calibration nodes need the total-field operator");
+    PG_REQUIRE(c.nCalSlot == 0, "pgert_forward: calibration nodes (a Neumann domain) need the total-field operator");
+    PG_REQUIRE(c.refNode < 0 || !c.primNumeric,
+               "pgert_forward: numerical primary potentials with a reference electrode are not supported");
     PG_REQUIRE(rho, "pgert_forward: null pointer");
     hipStream_t s = c.stream;
     const size_t nk = c.nK, nn = c.n, ne = c.nE;
@@ -1063,7 +1064,33 @@ int pgert_set_reference(pgert_ctx * c, int ref_node, int n_cal, const int * cal_
             slot[q] = ds[q].first;
             val[q] = ds[q].second;
         }
+        if (ref_node >= 0) {
+            // the reference electrode's cells and nearest-neighbour distance
+            // (ElectrodeShapeNode, electrode.cpp:98-189)
+            std::vector<int> hc((size_t)C.C * C.nv);
+            PG_CHECK(hipMemcpy(hc.data(), C.cells.p, hc.size() * 4, hipMemcpyDeviceToHost));
+            std::vector<int> rc;
+            double rr = DBL_MAX;
+            for (int cc = 0; cc < C.C; ++cc) {
+                const int * v = hc.data() + (size_t)cc * C.nv;
+                if (std::find(v, v + C.nvert, ref_node) == v + C.nvert) continue;
+                rc.push_back(cc);
+                for (int i = 0; i < C.nvert; ++i) {
+                    if (v[i] == ref_node) continue;
+                    double d2 = 0.0;
+                    for (int q = 0; q < 3; ++q) {
+                        const double dq = C.hXyz[3 * (size_t)v[i] + q] - C.hXyz[3 * (size_t)ref_node + q];
+                        d2 += dq * dq;
+                    }
+                    rr = std::min(rr, std::sqrt(d2));
+                }
+            }
+            PG_REQUIRE(!rc.empty(), "pgert_set_reference: the reference node belongs to no cell");
+            C.refCells.upload(rc, C.stream);
+            C.refRmin = rr;
+        }
         C.refNode = ref_node;
+        C.primValid = false;
         C.nCalSlot = (int)ds.size();
         C.calSlot.upload(slot, C.stream);
         C.calVal.upload(val, C.stream);
@@ -1098,6 +1125,7 @@ int pgert_set_primary_potentials(pgert_ctx * c, const double * prim) {
         PG_REQUIRE(C.order == 1 && C.n > 0 && C.nE > 0 && C.nK > 0,
                    "pgert_set_primary_potentials: set a linear mesh, the electrodes and the wavenumbers first");
         PG_REQUIRE(prim, "pgert_set_primary_potentials: null pointer");
+        PG_REQUIRE(C.refNode < 0, "pgert_set_primary_potentials: not with a reference electrode");
         const size_t total = (size_t)C.nK * C.n * C.nE;
         C.prim.alloc(total);
         PG_CHECK(hipMemcpyAsync(C.prim.p, prim, total * sizeof(double), hipMemcpyDeviceToDevice, C.stream));

@@ -170,6 +170,8 @@ struct Ctx {
     bool primNumeric = false;    // primaries come from pgert_set_primary_potentials
     bool virtualElec = false;    // an electrode sits on a node of no cell (complete electrode model)
     int refNode = -1;            // reference electrode node: sources are dipoles against it
+    double refRmin = 0.0;        // its nearest-neighbour distance (singular primary value)
+    DBuf<int> refCells;          // its cells (rho_s)
     int nCalSlot = 0;            // calibration nodes: Dirichlet slots applied after the bypass resistors
     DBuf<int> calSlot;
     DBuf<double> calVal;

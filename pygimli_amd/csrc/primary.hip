@@ -63,6 +63,44 @@ __global__ void k_primary(int n, int nE, int nK, const double * __restrict__ xyz
     prim[t] = u;
 }
 
+// The reference electrode's primary potential (eB of every current pattern,
+// checkPrimpotentials_ dcfemmodelling.cpp:2101-2102; singular value at its
+// node as for a node electrode) subtracted from every source's: dipole
+// primaries.  One thread per (k, node).
+__global__ void k_primary_ref(int n, int nE, int nK, const double * __restrict__ xyz, int refNode, double refRmin,
+                              const double * __restrict__ kv, double surfaceZ, double * __restrict__ prim) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nK * n) return;
+    const int node = (int)(t % n);
+    const double k = kv[t / n];
+    double u;
+    if (node == refNode)
+        u = k > 0.0 ? bessel_k0(refRmin / 6.0 * k) / PI : 1.0 / (2.0 * PI * refRmin / 2.0);
+    else
+        u = exact_dc(xyz + 3 * (size_t)node, xyz + 3 * (size_t)refNode, k, surfaceZ);
+    double * p = prim + (size_t)t * nE;
+    for (int e = 0; e < nE; ++e) p[e] -= u;
+}
+
+// rhoSource of a dipole pattern (DCSRMultiElectrodeModelling::calculateK,
+// dcfemmodelling.cpp:2219-2244): the mean of the geometric means of eA and of
+// the reference electrode, over those above TOLERANCE.  One thread.
+__global__ void k_rho_ref(int nE, int nCells, const int * __restrict__ cells, const double * __restrict__ rho,
+                          double * __restrict__ rhoS) {
+    if (blockIdx.x * blockDim.x + threadIdx.x != 0) return;
+    double s = 0.0;
+    for (int q = 0; q < nCells; ++q) s += log(rho[cells[q]]);
+    const double rb = exp(s / (double)nCells);
+    for (int e = 0; e < nE; ++e) {
+        const double ra = rhoS[e];
+        double sum = 0.0;
+        int cnt = 0;
+        if (ra > 1e-12) { sum += ra; ++cnt; }
+        if (rb > 1e-12) { sum += rb; ++cnt; }
+        rhoS[e] = sum / cnt;
+    }
+}
+
 // ElectrodeShapeNode::geomMeanCellAttributes (electrode.cpp:102-127)
 __global__ void k_rho_source(int nE, const int * __restrict__ ptr, const int * __restrict__ cells,
                              const double * __restrict__ rho, double * __restrict__ rhoS) {
@@ -203,12 +241,23 @@ void launch_primary(Ctx & c) {
     hipLaunchKernelGGL(k_primary, dim3(div_up(total, 256)), dim3(256), 0, c.stream, c.n, c.nE, c.nK,
                        c.xyz.p, c.elecPos.p, c.singNode.p, c.minRadius.p, c.dk.p, c.surfaceZ, c.prim.p);
     PG_CHECK(hipGetLastError());
+    if (c.refNode >= 0) {
+        const long kn = (long)c.nK * c.n;
+        hipLaunchKernelGGL(k_primary_ref, dim3(div_up(kn, 256)), dim3(256), 0, c.stream, c.n, c.nE, c.nK, c.xyz.p,
+                           c.refNode, c.refRmin, c.dk.p, c.surfaceZ, c.prim.p);
+        PG_CHECK(hipGetLastError());
+    }
 }
 
 void launch_rho_source(Ctx & c, const double * rho) {
     hipLaunchKernelGGL(k_rho_source, dim3(div_up(c.nE, 64)), dim3(64), 0, c.stream, c.nE, c.ecPtr.p,
                        c.ecCell.p, rho, c.rhoS.p);
     PG_CHECK(hipGetLastError());
+    if (c.refNode >= 0) {
+        hipLaunchKernelGGL(k_rho_ref, dim3(1), dim3(64), 0, c.stream, c.nE, (int)c.refCells.n, c.refCells.p, rho,
+                           c.rhoS.p);
+        PG_CHECK(hipGetLastError());
+    }
 }
 
 void launch_sr_rhs(Ctx & c) {

@@ -219,9 +219,11 @@ class ERTModelling:
             raise NotImplementedError("a 3D Neumann-only domain without a reference electrode node (marker -999): "
                                       "the reference then takes the last electrode, and its Jacobian reads past its "
                                       "potential rows")
-        if (an.refNode >= 0 or an.calibration) and self._sr:
-            raise NotImplementedError("a reference electrode (marker -999) or a Neumann-only domain needs "
-                                      "ERTModelling(sr=False)")
+        if an.calibration and self._sr:
+            raise NotImplementedError("a Neumann-only domain (calibration nodes) needs ERTModelling(sr=False)")
+        if an.refNode >= 0 and self._sr and an.topography:
+            raise NotImplementedError("a reference electrode with singularity removal and topography (numerical "
+                                      "dipole primaries) is not supported")
         if an.topography and self._sr:
             # numerical primaries from the P2 total-field problem; its
             # operator would also carry free electrodes (P2 shape functions)

@@ -795,6 +795,15 @@ def test_reference_electrode_oracle_vs_reference_build(kw):
         f0.setup.refNode = -1
         eo.response_total(f0, rho)
         assert np.abs(f0.subpots - U).max() > 1e-2 * np.abs(U).max()
+        # singularity removal: dipole primaries (the reference electrode's
+        # analytic potential subtracted, calculateK :2219-2254)
+        s2 = ref.Session(mesh, copy.deepcopy(data), nthreads=1)
+        r2 = s2.response(rho)
+        J2 = s2.jacobian(rho)
+        f2 = eo.Forward(mesh, copy.deepcopy(data), k, w)
+        assert np.abs(f2.response(rho) - r2).max() <= 1e-12 * np.abs(r2).max()
+        assert np.linalg.norm(f2.jacobian(rho) - J2) <= 1e-12 * np.linalg.norm(J2)
+        np.testing.assert_allclose(f2.subpots, s2.subpotentials(), rtol=0, atol=1e-12 * np.abs(U).max())
 
 
 def test_neumann_2d_total_field_oracle_vs_reference_build():
