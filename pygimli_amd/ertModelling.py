@@ -612,6 +612,39 @@ class ERTModelling:
     def parameterCount(self):
         return self._nParam * (1 + int(self._complex))
 
+    def createStartModel(self, dataVals):
+        """Homogeneous start model (ERTModelling.createStartModel,
+        ertModelling.py:168-186): the median of the data (real); for complex
+        data [re, im] the medians of both parts, the imaginary one flipped to
+        positive (and remembered for flipImagPart) when negative."""
+        v = np.asarray(dataVals, np.float64)
+        if not self._complex:
+            return np.full(self.parameterCount, float(np.median(v)))
+        h = len(v) // 2
+        n = self.parameterCount // 2
+        re = np.full(n, float(np.median(v[:h])))
+        im = np.full(n, float(np.median(v[h:])))
+        self._conjImag = bool(im.min() < 0)
+        return np.concatenate([re, -im if self._conjImag else im])
+
+    def flipImagPart(self, v):
+        """[re, im] with the imaginary part negated when createStartModel
+        flipped it (ertModelling.py:188-202)."""
+        v = np.asarray(v, np.float64)
+        if not getattr(self, "_conjImag", False):
+            return v.copy()
+        h = len(v) // 2
+        return np.concatenate([v[:h], -v[h:]])
+
+    def createConstraints(self, C=None):
+        """Constraint matrix for the inversion (ERTModelling.createConstraints,
+        ertModelling.py:152-166): first-order smoothness between neighbouring
+        parameter cells, or the given C; repeated for both halves of a complex
+        model (RepeatDMatrix).  Returns scipy CSR."""
+        from .inversion import complex_constraints, smoothness_constraints
+        C1 = smoothness_constraints(self._mesh) if C is None else C
+        return complex_constraints(C1) if self._complex else C1
+
     def close(self):
         self._ctx.close()
 

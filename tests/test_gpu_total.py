@@ -246,3 +246,32 @@ def test_learned_warm_start_refuses_total_field():
     rc = f._ctx.L.pgert_mlp_collect(f._ctx.h, 0, ctypes.c_void_p(idx.data_ptr()), 4, ctypes.c_void_p(feat.data_ptr()),
                                     ctypes.c_void_p(tgt.data_ptr()))
     assert rc != 0 and b"singularity-removal" in f._ctx.L.pgert_last_error(f._ctx.h)
+
+
+def test_start_model_constraints_and_imaginary_flip():
+    """createStartModel / createConstraints / flipImagPart of the reference's
+    ERTModelling (ertModelling.py:152-202): the median start model, the
+    smoothness constraints (repeated for a complex model), and the imaginary
+    part flipped to positive and back."""
+    from pygimli_amd.ertModelling import ERTModelling
+    from pygimli_amd.inversion import smoothness_constraints
+    mesh, data, rho = workloads.small_2d()
+    fop = ERTModelling(sr=False)
+    fop.setMesh(mesh)
+    fop.setData(copy.deepcopy(data))
+    r = fop.response(rho)
+    sm = fop.createStartModel(r)
+    assert sm.shape == (fop.parameterCount,) and np.all(sm == np.median(r))
+    C = fop.createConstraints()
+    assert (C != smoothness_constraints(mesh)).nnz == 0
+    fop.setComplex(True)
+    d = np.concatenate([r, -0.01 * r])
+    sm = fop.createStartModel(d)
+    P = fop.parameterCount // 2
+    assert sm.shape == (2 * P,) and np.all(sm[:P] == np.median(r))
+    np.testing.assert_allclose(sm[P:], 0.01 * np.median(r), rtol=1e-14)
+    v = np.concatenate([np.ones(P), 2 * np.ones(P)])
+    assert np.array_equal(fop.flipImagPart(v), np.concatenate([np.ones(P), -2 * np.ones(P)]))
+    C2 = fop.createConstraints()
+    assert C2.shape == (2 * C.shape[0], 2 * C.shape[1])
+
