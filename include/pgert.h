@@ -262,8 +262,11 @@ int pgert_set_cem(pgert_ctx * ctx, int n_terms, const int * rows, const int * co
  * dcfemmodelling.cpp:2009-2056, when the mesh has topography): prim DEVICE
  * [nK][n][nE] (the layout of pgert_potentials_node_major), unit-current
  * potentials of every electrode and wavenumber at the mesh nodes.  They hold
- * until the mesh, the electrodes or the wavenumbers change; pgert_forward
- * then fails until they are set again. */
+ * until the mesh, the electrodes, the reference electrode or the
+ * wavenumbers change; pgert_forward then fails until they are set again.
+ * With a reference electrode they are the dipole primaries (the P2 problem
+ * solved with the same pgert_set_reference); rho_s is then the mean of both
+ * geometric means, as for the analytic ones. */
 int pgert_set_primary_potentials(pgert_ctx * ctx, const double * prim);
 
 /* Bypass resistors of the total-field operator

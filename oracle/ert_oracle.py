@@ -614,6 +614,8 @@ def p2_primaries(mesh, setup, kvalues):
         for i, e in enumerate(setup.elecNode):
             rhs = np.zeros(N)
             rhs[e] = 1.0
+            if setup.refNode >= 0:
+                rhs[setup.refNode] -= 1.0    # dipole against the reference electrode
             P[i + ki * len(setup.elecNode)] = lu.solve(rhs)[:n]
     return P
 

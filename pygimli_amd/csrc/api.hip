@@ -768,8 +768,6 @@ void forward_prepare(Ctx & c, const double * rho, const pgert_solve_opts & o) {
     PG_REQUIRE(!c.virtualElec, "pgert_forward: electrodes on nodes of no cell (complete electrode model) need the "
                                "total-field operator");
     PG_REQUIRE(c.nCalSlot == 0, "pgert_forward: calibration nodes (a Neumann domain) need the total-field operator");
-    PG_REQUIRE(c.refNode < 0 || !c.primNumeric,
-               "pgert_forward: numerical primary potentials with a reference electrode are not supported");
     PG_REQUIRE(rho, "pgert_forward: null pointer");
     hipStream_t s = c.stream;
     const size_t nk = c.nK, nn = c.n, ne = c.nE;
@@ -1125,7 +1123,6 @@ int pgert_set_primary_potentials(pgert_ctx * c, const double * prim) {
         PG_REQUIRE(C.order == 1 && C.n > 0 && C.nE > 0 && C.nK > 0,
                    "pgert_set_primary_potentials: set a linear mesh, the electrodes and the wavenumbers first");
         PG_REQUIRE(prim, "pgert_set_primary_potentials: null pointer");
-        PG_REQUIRE(C.refNode < 0, "pgert_set_primary_potentials: not with a reference electrode");
         const size_t total = (size_t)C.nK * C.n * C.nE;
         C.prim.alloc(total);
         PG_CHECK(hipMemcpyAsync(C.prim.p, prim, total * sizeof(double), hipMemcpyDeviceToDevice, C.stream));

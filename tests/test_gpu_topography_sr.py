@@ -103,3 +103,33 @@ def test_sr_topography_primaries_follow_the_set_up():
     opts = fop._opts()
     assert L.pgert_forward(h, _dptr(rt), ctypes.byref(opts), _dptr(out), None) != 0
     assert b"primary" in L.pgert_last_error(h)
+
+
+def test_sr_topography_with_reference_electrode():
+    """Topography, singularity removal and a reference electrode node: the P2
+    primary problem is solved with the dipole sources as well (the
+    reference's inner operator searches the same -999 node); against the
+    oracle (equal to the reference build, 6e-15 for the primaries) and the
+    reference build."""
+    from oracle import ref
+    mesh, data, rho = workloads.small_3d_topo_ref()
+    fop = _fop(mesh, data, "block64")
+    r = fop.response(rho)
+    fop.createJacobian(rho)
+    J = fop.jacobian().cpu().numpy()
+    f = eo.Forward(mesh, copy.deepcopy(data), fop.kValues(), fop.weights())
+    assert f.setup.refNode >= 0 and f.setup.topography
+    o = f.response(rho)
+    oJ = f.jacobian(rho)
+    U = fop.subPotentials().cpu().numpy()
+    e1 = np.abs(r - o).max() / np.abs(o).max()
+    e2 = np.linalg.norm(J - oJ) / np.linalg.norm(oJ)
+    e3 = np.abs(U - f.subpots).max() / np.abs(f.subpots).max()
+    print("SR topography + reference electrode", e1, e2, e3)
+    assert e1 < 1e-9 and e2 < 1e-9 and e3 < 1e-9
+    if ref.available():
+        s = ref.Session(mesh, copy.deepcopy(data), nthreads=1)
+        s.primary_p2()
+        rr = s.response(rho)
+        assert np.abs(r - rr).max() <= 1e-9 * np.abs(rr).max()
+

@@ -694,7 +694,7 @@ def test_p2_tables_closed_forms():
     assert abs(K[0, 3] + 2 / 3) < 1e-14 and abs(K[0, 4]) < 1e-14
 
 
-@pytest.mark.parametrize("name", ["small_2d_topo", "small_3d_topo"])
+@pytest.mark.parametrize("name", ["small_2d_topo", "small_3d_topo", "small_3d_topo_ref"])
 def test_sr_topography_oracle_vs_reference_build(name):
     """Singularity removal with topography: numerical primary potentials
     from the total-field problem (rho = 1) on the P2 mesh
