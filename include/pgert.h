@@ -236,10 +236,11 @@ int pgert_forward_total(pgert_ctx * ctx, const double * rho, const pgert_solve_o
  * domain without one; none outside Neumann domains) get cleared rows and
  * columns and a unit diagonal after the bypass resistors (:1835), the
  * right-hand side unchanged.  ref_node = -1 and n_cal = 0 remove them.  The
- * SR forward takes a reference electrode (dipole primaries: its analytic
- * potential subtracted, rho_s the mean of both geometric means,
- * checkPrimpotentials_ :2101-2102, calculateK :2219-2244) but refuses
- * calibration nodes; the complex forward refuses both.  HOST array. */
+ * SR forward takes both: dipole primaries (the reference electrode's
+ * analytic potential subtracted, rho_s the mean of both geometric means,
+ * checkPrimpotentials_ :2101-2102, calculateK :2219-2244) and the calibration
+ * nodes in both matrices with their right-hand side zeroed (:2182-2192,
+ * :2280-2283); the complex forward refuses both.  HOST array. */
 int pgert_set_reference(pgert_ctx * ctx, int ref_node, int n_cal, const int * cal_nodes);
 
 /* Complete electrode model of the total-field operator

@@ -604,12 +604,13 @@ def p2_primaries(mesh, setup, kvalues):
             vals.append(Be.ravel())
         A = sp.coo_matrix((np.concatenate(vals), (np.concatenate(r), np.concatenate(cc))), shape=(N, N)).tocsr()
         A.sum_duplicates()
-        if len(hd):
-            A = A.tolil()
-            A[hd, :] = 0.0
-            A[:, hd] = 0.0
-            A[hd, hd] = 1.0
-            A = A.tocsr()
+        for dn in (hd, setup.calibration):
+            if len(dn):
+                A = A.tolil()
+                A[dn, :] = 0.0
+                A[:, dn] = 0.0
+                A[dn, dn] = 1.0
+                A = A.tocsr()
         lu = spla.splu(A.tocsc())
         for i, e in enumerate(setup.elecNode):
             rhs = np.zeros(N)
@@ -874,8 +875,6 @@ class Forward:
         self.mesh = mesh
         self.data = data
         self.setup = Setup(mesh, data.sensorPositions())
-        if self.setup.calibration and not total_field:
-            raise NotImplementedError("a Neumann domain with singularity removal (numerical dipole primaries)")
         if kvalues is None:
             kvalues, weights = kwave_list(mesh.dim(), self.setup.elecPos)
         self.k = np.asarray(kvalues, np.float64)
@@ -912,14 +911,25 @@ class Forward:
         ones = np.ones_like(rho)
         U = np.empty_like(P)
         src = self.setup.sourceCenter
+        cal = self.setup.calibration
         for ki, k in enumerate(self.k):
             S = assemble(self.mesh, rho, k, src, self.geo)
             S1 = assemble(self.mesh, ones, k, src, self.geo)
+            if cal:
+                # calibration nodes in both matrices (:2182, :2192), their
+                # right-hand side zeroed (:2280-2283)
+                S, S1 = S.tolil(), S1.tolil()
+                for M in (S, S1):
+                    M[cal, :] = 0.0
+                    M[:, cal] = 0.0
+                    M[cal, cal] = 1.0
+                S, S1 = S.tocsr(), S1.tocsr()
             lu = spla.splu(S.tocsc())
             for i in range(self.nE):
                 rs = self.setup.rho_source(i, rho)
                 prim = P[i + ki * self.nE] * rs
                 rhs = S1 @ prim / rs - S @ prim
+                rhs[cal] = 0.0
                 U[i + ki * self.nE] = lu.solve(rhs) + prim
         return U
 

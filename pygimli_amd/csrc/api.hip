@@ -397,6 +397,7 @@ static void mesh_setup(pgert_ctx * c, int elemOrder, int dim, int n_nodes, const
         c->nByp = 0;   // bypass, electrode-model and calibration slots index the previous pattern
         c->nCem = 0;
         c->nCalSlot = 0;
+        c->calNodes.release();
         c->refNode = -1;
         c->s1Valid = c->primValid = c->haveU = c->haveUc = c->srState = false;
         c->primNumeric = false;
@@ -767,7 +768,6 @@ void forward_prepare(Ctx & c, const double * rho, const pgert_solve_opts & o) {
     PG_REQUIRE(c.order == 1, "pgert_forward: a P2 mesh serves the total-field primary problem only");
     PG_REQUIRE(!c.virtualElec, "pgert_forward: electrodes on nodes of no cell (complete electrode model) need the "
                                "total-field operator");
-    PG_REQUIRE(c.nCalSlot == 0, "pgert_forward: calibration nodes (a Neumann domain) need the total-field operator");
     PG_REQUIRE(rho, "pgert_forward: null pointer");
     hipStream_t s = c.stream;
     const size_t nk = c.nK, nn = c.n, ne = c.nE;
@@ -777,9 +777,11 @@ void forward_prepare(Ctx & c, const double * rho, const pgert_solve_opts & o) {
     // assembly (A_k for this model, S1_k with rho == 1 once per mesh)
     c.valsA.alloc(nk * c.nnz);
     for (size_t kk = 0; kk < nk; ++kk) launch_assemble(c, c.rho.p, c.kv[kk], c.valsA.p + kk * c.nnz, false);
+    launch_calibration(c, c.valsA.p);   // calibration nodes in S and S1 (dcfemmodelling.cpp:2182, :2192)
     if (!c.s1Valid) {
         c.valsS1.alloc(nk * c.nnz);
         for (size_t kk = 0; kk < nk; ++kk) launch_assemble(c, nullptr, c.kv[kk], c.valsS1.p + kk * c.nnz, true);
+        launch_calibration(c, c.valsS1.p);
         c.s1Valid = true;
     }
     launch_diag_inv(c);
@@ -796,6 +798,7 @@ void forward_prepare(Ctx & c, const double * rho, const pgert_solve_opts & o) {
     launch_rho_source(c, c.rho.p);
     c.B.alloc(nk * nn * ne);
     launch_sr_rhs(c);
+    launch_zero_rows(c, c.B.p);   // the calibration nodes' right-hand side (:2280-2283)
     c.X.alloc(nk * nn * ne);
     if (o.warm_start)
         PG_CHECK(hipMemcpyAsync(c.X.p, o.warm_start, c.X.bytes(), hipMemcpyDeviceToDevice, s));
@@ -1089,6 +1092,8 @@ int pgert_set_reference(pgert_ctx * c, int ref_node, int n_cal, const int * cal_
         }
         C.refNode = ref_node;
         C.primValid = false;
+        C.s1Valid = false;
+        C.calNodes.upload(cal_nodes, n_cal, C.stream);
         C.nCalSlot = (int)ds.size();
         C.calSlot.upload(slot, C.stream);
         C.calVal.upload(val, C.stream);

@@ -300,6 +300,27 @@ void launch_assemble(Ctx & c, const double * rho, double k, double * vals, bool 
     launch_dirichlet(c, vals, false);
 }
 
+// B[k][node][e] = 0 for the listed nodes: one thread per (k, listed node, e)
+__global__ void k_zero_rows(int nK, int count, int n, int nE, const int * __restrict__ nodes,
+                            double * __restrict__ B) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nK * count * nE) return;
+    const int e = (int)(t % nE);
+    const long kq = t / nE;
+    const int q = (int)(kq % count);
+    const long kk = kq / count;
+    B[((size_t)kk * n + nodes[q]) * nE + e] = 0.0;
+}
+
+void launch_zero_rows(Ctx & c, double * B) {
+    const int count = (int)c.calNodes.n;
+    if (count == 0) return;
+    const long total = (long)c.nK * count * c.nE;
+    hipLaunchKernelGGL(k_zero_rows, dim3(div_up(total, 256)), dim3(256), 0, c.stream, c.nK, count, c.n, c.nE,
+                       c.calNodes.p, B);
+    PG_CHECK(hipGetLastError());
+}
+
 void launch_calibration(Ctx & c, double * vals) {
     if (c.nCalSlot == 0) return;
     for (int kk = 0; kk < c.nK; ++kk)

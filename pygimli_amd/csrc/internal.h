@@ -173,6 +173,7 @@ struct Ctx {
     double refRmin = 0.0;        // its nearest-neighbour distance (singular primary value)
     DBuf<int> refCells;          // its cells (rho_s)
     int nCalSlot = 0;            // calibration nodes: Dirichlet slots applied after the bypass resistors
+    DBuf<int> calNodes;          // the calibration nodes (their SR right-hand side is zeroed)
     DBuf<int> calSlot;
     DBuf<double> calVal;
     DBuf<double> U;         // nK * n * nE   total potentials (secondary + rho_s * primary)
@@ -276,6 +277,8 @@ void launch_assemble(Ctx & c, const double * rho, double k, double * vals, bool 
 void launch_dirichlet(Ctx & c, double * vals, bool imagPart);
 // calibration nodes (rows / columns cleared, diagonal one) in every wavenumber's matrix
 void launch_calibration(Ctx & c, double * vals);
+// rows of the calibration nodes of a node-major block [k][node][e] set to zero
+void launch_zero_rows(Ctx & c, double * B);
 void launch_fill_matrix(Ctx & c, int mass, const double * a, double * vals);
 void launch_map_model(Ctx & c, const double * model, int nModel, double * rho);
 void build_prolongation(Ctx & c, const double * xyz, const int * cells, const int * marker);

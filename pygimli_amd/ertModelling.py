@@ -219,8 +219,6 @@ class ERTModelling:
             raise NotImplementedError("a 3D Neumann-only domain without a reference electrode node (marker -999): "
                                       "the reference then takes the last electrode, and its Jacobian reads past its "
                                       "potential rows")
-        if an.calibration and self._sr:
-            raise NotImplementedError("a Neumann-only domain (calibration nodes) needs ERTModelling(sr=False)")
         if an.topography and self._sr:
             # numerical primaries from the P2 total-field problem; its
             # operator would also carry free electrodes (P2 shape functions)
@@ -337,7 +335,7 @@ class ERTModelling:
         equal to these unless setkValues changed them."""
         an = self._analysis
         key = (id(self._mesh), self._mesh.nodeCount(), tuple(an.elecNode.tolist()), tuple(self._kv.tolist()),
-               tuple(an.sourceCenter.tolist()), an.refNode)
+               tuple(an.sourceCenter.tolist()), an.refNode, tuple(an.calibration))
         if self._prim is None or self._primKey != key:
             self._prim = None
             self._prim = self._p2_primaries()
@@ -363,7 +361,8 @@ class ERTModelling:
             sc, psc = _host_d(an.sourceCenter)
             ctx.check(ctx.L.pgert_set_electrodes(ctx.h, len(en), pen, psc, an.surfaceZ), "P2 electrodes")
             # a reference electrode keeps its node id on the P2 mesh: dipole primaries
-            ctx.check(ctx.L.pgert_set_reference(ctx.h, an.refNode, 0, None), "P2 reference electrode")
+            cal, pcal = _host_i(an.calibration)
+            ctx.check(ctx.L.pgert_set_reference(ctx.h, an.refNode, len(cal), pcal), "P2 reference electrode")
             k, pk = _host_d(self._kv)
             w, pw = _host_d(self._wv)
             ctx.check(ctx.L.pgert_set_wavenumbers(ctx.h, len(k), pk, pw), "P2 wavenumbers")

@@ -786,6 +786,17 @@ def test_reference_electrode_oracle_vs_reference_build(kw):
     oJ = f.jacobian(rho)
     assert np.abs(o - r).max() <= 1e-12 * np.abs(r).max()
     assert np.linalg.norm(oJ - J) <= 1e-12 * np.linalg.norm(J)
+    if kw["neumann"]:
+        # singularity removal: numerical P2 dipole primaries with the
+        # calibration node, both matrices calibrated, its right-hand side zero
+        s2 = ref.Session(mesh, copy.deepcopy(data), nthreads=1)
+        P = s2.primary_p2()
+        r2 = s2.response(rho)
+        J2 = s2.jacobian(rho)
+        f2 = eo.Forward(mesh, copy.deepcopy(data), k, w)
+        assert np.abs(f2.primaries() - P).max() <= 1e-11 * np.abs(P).max()
+        assert np.abs(f2.response(rho) - r2).max() <= 1e-12 * np.abs(r2).max()
+        assert np.linalg.norm(f2.jacobian(rho) - J2) <= 1e-12 * np.linalg.norm(J2)
     if not kw["neumann"]:
         # four-point data cancel the reference electrode's term by linearity;
         # the potentials themselves are dipole potentials
