@@ -240,7 +240,8 @@ int pgert_forward_total(pgert_ctx * ctx, const double * rho, const pgert_solve_o
  * analytic potential subtracted, rho_s the mean of both geometric means,
  * checkPrimpotentials_ :2101-2102, calculateK :2219-2244) and the calibration
  * nodes in both matrices with their right-hand side zeroed (:2182-2192,
- * :2280-2283); the complex forward refuses both.  HOST array. */
+ * :2280-2283); the complex forward takes both (a complex unit diagonal).
+ * HOST array. */
 int pgert_set_reference(pgert_ctx * ctx, int ref_node, int n_cal, const int * cal_nodes);
 
 /* Complete electrode model of the total-field operator

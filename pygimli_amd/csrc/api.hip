@@ -904,8 +904,6 @@ int pgert_forward_complex(pgert_ctx * c, const double * rho_re, const double * r
                           pgert_solve_stats * stats) {
     return guarded(c, [&] {
         PG_REQUIRE(rho_re && rho_im && resp_re && resp_im, "pgert_forward_complex: null pointer");
-        PG_REQUIRE(c->refNode < 0 && c->nCalSlot == 0,
-                   "pgert_forward_complex: a reference electrode or calibration nodes are not supported here");
         PG_REQUIRE(c->nCem == 0 && !c->virtualElec,
                    "pgert_forward_complex: the complete electrode model is real only (as in the reference, "
                    "dcfemmodelling.cpp:412-417)");
@@ -922,6 +920,9 @@ int pgert_forward_complex(pgert_ctx * c, const double * rho_re, const double * r
         C.valsAi.alloc(nk * C.nnz);
         for (size_t kk = 0; kk < nk; ++kk)
             launch_assemble_complex(C, rho_re, rho_im, C.kv[kk], C.valsA.p + kk * C.nnz, C.valsAi.p + kk * C.nnz);
+        // calibration nodes: a complex unit diagonal (dcfemmodelling.cpp:1835)
+        launch_calibration(C, C.valsA.p);
+        launch_calibration(C, C.valsAi.p, true);
         launch_diag_inv(C);
         PG_CHECK(hipEventRecord(C.ev[1], s));
         C.Bpt.alloc(blk);

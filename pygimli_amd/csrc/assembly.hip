@@ -321,11 +321,11 @@ void launch_zero_rows(Ctx & c, double * B) {
     PG_CHECK(hipGetLastError());
 }
 
-void launch_calibration(Ctx & c, double * vals) {
+void launch_calibration(Ctx & c, double * vals, bool imagPart) {
     if (c.nCalSlot == 0) return;
     for (int kk = 0; kk < c.nK; ++kk)
         hipLaunchKernelGGL(k_set_slots, dim3(div_up(c.nCalSlot, 256)), dim3(256), 0, c.stream, c.nCalSlot,
-                           c.calSlot.p, c.calVal.p, vals + (size_t)kk * c.nnz);
+                           c.calSlot.p, imagPart ? nullptr : c.calVal.p, vals + (size_t)kk * c.nnz);
     PG_CHECK(hipGetLastError());
 }
 

@@ -276,7 +276,7 @@ void launch_assemble(Ctx & c, const double * rho, double k, double * vals, bool 
 // imaginary half of a complex one, whose diagonal entries there are 0)
 void launch_dirichlet(Ctx & c, double * vals, bool imagPart);
 // calibration nodes (rows / columns cleared, diagonal one) in every wavenumber's matrix
-void launch_calibration(Ctx & c, double * vals);
+void launch_calibration(Ctx & c, double * vals, bool imagPart = false);
 // rows of the calibration nodes of a node-major block [k][node][e] set to zero
 void launch_zero_rows(Ctx & c, double * B);
 void launch_fill_matrix(Ctx & c, int mass, const double * a, double * vals);

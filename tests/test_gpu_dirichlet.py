@@ -214,3 +214,28 @@ def test_reference_electrode_and_calibration(kw, method):
         s2.response(rho)
         U2 = s2.subpotentials()
         assert np.abs(Us - U2).max() <= 1e-8 * np.abs(U2).max()
+
+
+@pytest.mark.parametrize("neumann", [False, True])
+def test_reference_electrode_complex(neumann):
+    """Complex resistivities with a reference electrode node (and, in a 3D
+    Neumann domain, the calibration node with a complex unit diagonal):
+    against the oracle and the reference build's complex branch."""
+    from oracle import ref
+    mesh, data, rho = workloads.small_3d_refnode(neumann=neumann)
+    rng = np.random.default_rng(5)
+    rim = -rho * rng.uniform(0.005, 0.03, len(rho))
+    fop = _fop(mesh, data, False, "block64", cplx=True)
+    resp = fop.response(np.concatenate([rho, rim]))
+    D = data.size()
+    f = eo.Forward(mesh, copy.deepcopy(data), fop.kValues(), fop.weights(), total_field=True)
+    orr, ori = eo.response_complex(f, rho, rim)
+    e = max(np.abs(resp[:D] - orr).max() / np.abs(orr).max(), np.abs(resp[D:] - ori).max() / np.abs(ori).max())
+    print("complex reference electrode, neumann", neumann, e)
+    assert e < 1e-8
+    if ref.available():
+        s = ref.Session(mesh, copy.deepcopy(data), fop.kValues(), fop.weights(), nthreads=1, complex_=True)
+        rr, ri = s.response_complex(rho, rim)
+        assert np.abs(resp[:D] - rr).max() <= 1e-8 * np.abs(rr).max()
+        assert np.abs(resp[D:] - ri).max() <= 1e-8 * np.abs(ri).max()
+

@@ -786,6 +786,12 @@ def test_reference_electrode_oracle_vs_reference_build(kw):
     oJ = f.jacobian(rho)
     assert np.abs(o - r).max() <= 1e-12 * np.abs(r).max()
     assert np.linalg.norm(oJ - J) <= 1e-12 * np.linalg.norm(J)
+    # complex resistivities: the same dipole sources and calibration
+    rim = -rho * np.random.default_rng(5).uniform(0.005, 0.03, len(rho))
+    sc = ref.Session(mesh, copy.deepcopy(data), k, w, nthreads=1, complex_=True)
+    rr, ri = sc.response_complex(rho, rim)
+    orr, ori = eo.response_complex(eo.Forward(mesh, copy.deepcopy(data), k, w, total_field=True), rho, rim)
+    assert np.abs(orr - rr).max() <= 1e-12 * np.abs(rr).max() and np.abs(ori - ri).max() <= 1e-12 * np.abs(ri).max()
     if kw["neumann"]:
         # singularity removal: numerical P2 dipole primaries with the
         # calibration node, both matrices calibrated, its right-hand side zero
