@@ -320,12 +320,16 @@ class Setup:
             raise NotImplementedError("electrode cells of the complete electrode model")
         self.cem = {}
         cemIds = sorted({int(-10000 - m) for m in bm[bm <= -9999]})
+        self.cemPassive = None
         if cemIds:
             if self.refNode >= 0:
                 raise NotImplementedError("a reference electrode with the complete electrode model")
-            if cemIds[0] < 0:
-                raise NotImplementedError("passive bodies of the complete electrode model")
             size, _, _ = boundary_geometry(mesh)
+            if cemIds[0] < 0:
+                # the passive body (marker -9999, id -1): an unknown after the
+                # electrodes' with its contact terms and no current (:838-847)
+                self.cemPassive = np.nonzero(bm == -9999)[0]
+                cemIds = cemIds[1:]
             domains = []
             for eid in cemIds:
                 faces = np.nonzero(bm == -10000 - eid)[0]
@@ -944,7 +948,7 @@ class Forward:
         if cem:
             if dt != np.float64:
                 raise NotImplementedError("the complete electrode model is real only (dcfemmodelling.cpp:412-417)")
-            n = n + len(self.setup.elecNode)     # the electrode unknowns after the nodes
+            n = n + len(self.setup.elecNode) + (self.setup.cemPassive is not None)   # unknowns after the nodes
         U = np.empty((len(self.k) * self.nE, n), dt)
         for ki, k in enumerate(self.k):
             S = assemble(self.mesh, rho, k, self.setup.sourceCenter, self.geo)
@@ -987,11 +991,14 @@ class Forward:
         S[i, j] += int_f N_i N_j / z, S[v, v] = area / z; z = contactImpedance
         (default 1 Ohm m^2), or contactResistance * area when one is given."""
         n = self.mesh.nodeCount()
-        nE = len(self.setup.elecNode)
+        groups = dict(self.setup.cem)
+        if self.setup.cemPassive is not None:
+            groups[len(self.setup.elecNode)] = self.setup.cemPassive
+        nE = len(groups)
         d = self.mesh.dim()
         size, _, _ = boundary_geometry(self.mesh)
         S = sp.bmat([[S, None], [None, sp.csr_matrix((nE, nE))]]).tolil()
-        for e, faces in self.setup.cem.items():
+        for e, faces in groups.items():
             area = size[faces].sum()
             z = self.contactImpedance[e] if self.contactImpedance is not None else 1.0
             if self.contactResistance is not None:

@@ -195,6 +195,7 @@ class MeshAnalysis:
         cal = [int(i) for i in np.nonzero(nm == MARKER_NODE_CALIBRATION)[0]]
         self.calibration = (cal if cal else [0]) if self.neumannDomain else []
         self.cemFaces = {}
+        self.cemPassive = None
         if np.any(bm <= MARKER_BOUND_ELECTRODE + 1):
             if self.refNode >= 0:
                 raise NotImplementedError("a reference electrode with the complete electrode model")
@@ -255,10 +256,18 @@ class MeshAnalysis:
                                       "wavenumber's electrode potentials (dcfemmodelling.cpp:1910-1913)")
         bm = mesh.boundaryMarkers()
         ids = sorted({int(MARKER_BOUND_ELECTRODE - m) for m in bm[bm <= MARKER_BOUND_ELECTRODE + 1]})
-        if ids[0] < 0:
-            raise NotImplementedError("complete electrode model: passive bodies (marker -9999) are not supported")
         size = mesh.boundarySizes()
         nodes = mesh.positions()
+        self.cemPassive = None
+        if ids[0] < 0:
+            # the passive body (faces with marker -9999): an unknown after the
+            # electrodes' with its contact terms and no current (:838-847)
+            self.cemPassive = np.nonzero(bm == MARKER_BOUND_ELECTRODE + 1)[0]
+            self.cemPassivePos = nodes[np.unique(mesh.bounds[self.cemPassive])].mean(0)
+            ids = ids[1:]
+            if not ids:
+                raise NotImplementedError("complete electrode model: a passive body without electrode faces (the "
+                                          "reference then gives node electrodes unknowns no current reaches)")
         domains = []
         for eid in ids:
             faces = np.nonzero(bm == MARKER_BOUND_ELECTRODE - eid)[0]
@@ -288,13 +297,18 @@ class MeshAnalysis:
         area A = sum |f| and contact impedance z (impedances[e], default
         1 Ohm m^2; z = resistances[e] * A when contact resistances are given):
         (i, v), (v, i): -|f| / d / z per face node i (int_f N_i); (i, j):
-        |f| (1 + d_ij) / (d (d + 1)) / z (int_f N_i N_j); (v, v): A / z.
-        Returns rows, cols, vals and the node pairs (v, i) that need slots."""
+        |f| (1 + d_ij) / (d (d + 1)) / z (int_f N_i N_j); (v, v): A / z.  The
+        passive body follows the electrodes (unknown n + nE, impedance entry
+        nE).  Returns rows, cols, vals and the node pairs (v, i) that need
+        slots."""
         d = mesh.dim()
         n = mesh.nodeCount()
         size = mesh.boundarySizes()
         rows, cols, vals, pairs = [], [], [], []
-        for e, faces in sorted(self.cemFaces.items()):
+        groups = dict(self.cemFaces)
+        if self.cemPassive is not None:
+            groups[len(self.cemFaces)] = self.cemPassive
+        for e, faces in sorted(groups.items()):
             area = float(size[faces].sum())
             z = 1.0 if impedances is None else float(impedances[e])
             if resistances is not None:

@@ -240,16 +240,21 @@ class ERTModelling:
             # whose SR operator cannot read electrode potentials from it)
             if self._sr:
                 raise NotImplementedError("the complete electrode model needs ERTModelling(sr=False)")
+            nz = len(an.elecNode) + (an.cemPassive is not None)   # the passive body after the electrodes
             zi = self._contactZ
+            if zi is not None and len(zi) < nz:
+                zi = np.concatenate([zi, np.ones(nz - len(zi))])
             if zi is None and os.path.isfile(self.contactImpedanceFile):
-                zi = load_electrode_vector(self.contactImpedanceFile, len(an.elecNode), 1.0)
+                zi = load_electrode_vector(self.contactImpedanceFile, nz, 1.0)
             zr = None
             if os.path.isfile(self.contactResistanceFile):
-                zr = load_electrode_vector(self.contactResistanceFile, len(an.elecNode), 1.0)
+                zr = load_electrode_vector(self.contactResistanceFile, nz, 1.0)
             cem = an.cem_terms(mesh, zi, zr)
         if not self._sr:
             need = set(uncoupled_pairs(mesh.cells, ba, bb)) | set(cem[3] if cem else ())
-            extra = an.elecPos if cem else None
+            extra = None
+            if cem:
+                extra = an.elecPos if an.cemPassive is None else np.vstack([an.elecPos, an.cemPassivePos])
             same_extra = (extra is None and self._extra is None) or (
                 extra is not None and self._extra is not None and np.array_equal(extra, self._extra))
             if not need <= set(self._couplings) or not same_extra:

@@ -91,3 +91,32 @@ def test_cem_refusals():
     c.setData(copy.deepcopy(data))
     with pytest.raises(RuntimeError):
         c.response(np.concatenate([rho, -0.01 * rho]))
+
+
+def test_cem_passive_body():
+    """A passive body of the complete electrode model (faces with marker
+    -9999: an unknown with contact terms and no current, after the
+    electrodes' unknowns, dcfemmodelling.cpp:838-847, :1811): against the
+    oracle and the reference build; the plate changes the result."""
+    from oracle import ref
+    mesh, data, rho = workloads.small_3d_cem_passive()
+    fop = _fop(mesh, data, "block64")
+    r = fop.response(rho)
+    fop.createJacobian(rho)
+    J = fop.jacobian().cpu().numpy()
+    f = eo.Forward(mesh, copy.deepcopy(data), fop.kValues(), fop.weights(), total_field=True)
+    o = eo.response_total(f, rho)
+    oJ = f.jacobian(rho)
+    e1 = np.abs(r - o).max() / np.abs(o).max()
+    e2 = np.linalg.norm(J - oJ) / np.linalg.norm(oJ)
+    print("CEM passive body", e1, e2)
+    assert e1 < 1e-9 and e2 < 1e-9
+    assert fop.subPotentials().shape[1] == mesh.nodeCount() + data.sensorCount() + 1
+    if ref.available():
+        s = ref.Session(mesh, copy.deepcopy(data), nthreads=1, sr=False)
+        rr = s.response(rho)
+        assert np.abs(r - rr).max() <= 1e-9 * np.abs(rr).max()
+    m0, _, _ = workloads.small_3d_cem()
+    g = _fop(m0, data, "block64")
+    assert np.abs(g.response(rho) - r).max() > 0.1 * np.abs(r).max()
+

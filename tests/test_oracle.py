@@ -763,6 +763,18 @@ def test_cem_oracle_vs_reference_build(tmp_path, monkeypatch):
     m0, _, _ = workloads.small_3d(spacing=2.0)
     f0 = eo.Forward(m0, copy.deepcopy(data), k, w, total_field=True)
     assert np.abs(eo.response_total(f0, rho) - out[0]).max() > 0.1 * np.abs(out[0]).max()
+    # a passive body (marker -9999): an unknown with contact terms, no current
+    for name in ("contactImpedance.map", "contactResistance.map"):
+        (tmp_path / name).unlink(missing_ok=True)
+    mp, dp, _ = workloads.small_3d_cem_passive()
+    sp_ = ref.Session(mp, copy.deepcopy(dp), nthreads=1, sr=False)
+    rp = sp_.response(rho)
+    Jp = sp_.jacobian(rho)
+    fp = eo.Forward(mp, copy.deepcopy(dp), k, w, total_field=True)
+    assert fp.setup.cemPassive is not None and len(fp.setup.cemPassive) > 1
+    assert np.abs(eo.response_total(fp, rho) - rp).max() <= 1e-12 * np.abs(rp).max()
+    assert np.linalg.norm(fp.jacobian(rho) - Jp) <= 1e-12 * np.linalg.norm(Jp)
+    assert np.abs(rp - out[0]).max() > 0.1 * np.abs(out[0]).max()
 
 
 @pytest.mark.parametrize("kw", [dict(neumann=True, calibration=True), dict(neumann=True, calibration=False),
